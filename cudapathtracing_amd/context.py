@@ -1,0 +1,214 @@
+"""Host-side object over the C ABI: one ``cpt_ctx`` (HIP stream + device
+buffers) per :class:`Context`.
+
+Mirrors the reference's call sequence: ``Scene::init`` uploads primitives in
+insertion order and calls ``Device::buildBVH`` (Scene.cuh:72-145,
+Device.cuh:169-195); rendering launches ``render`` (Render.cu:16-261).
+Errors raise :class:`CptError` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib as L
+from .scenes import SceneData
+
+
+class CptError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cpt error {code}: {msg}")
+        self.code = code
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(a) -> int:
+    if a is None:
+        return None
+    if _is_torch(a):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        rc = self._lib.cpt_create(C.byref(h), device)
+        if rc:
+            raise CptError(rc, f"cpt_create(device={device}) failed (no usable HIP device?)")
+        self._h = h
+        self.n_prims = 0
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cpt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise CptError(rc, self._lib.cpt_last_error(self._h).decode())
+
+    def set_stream(self, stream_handle: Optional[int]):
+        self._check(self._lib.cpt_set_stream(self._h, stream_handle))
+
+    def synchronize(self):
+        self._check(self._lib.cpt_synchronize(self._h))
+
+    def _torch_stream(self, *tensors):
+        """Runs the library on torch's current stream when given device tensors."""
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    # -- scene --------------------------------------------------------------
+    def upload(self, s: SceneData):
+        mats = (L.Material * max(1, len(s.materials)))(*s.materials)
+        lights = (L.Light * max(1, len(s.lights)))(*s.lights)
+        sph = (L.Sphere * max(1, len(s.spheres)))(*s.spheres)
+        tri = np.ascontiguousarray(s.tri, np.float64)
+        tm = np.ascontiguousarray(s.tri_mat, np.int32)
+        self._check(self._lib.cpt_scene_upload(self._h, mats, len(s.materials), lights, len(s.lights),
+                                               sph, len(s.spheres), tri.ctypes.data, tm.ctypes.data,
+                                               tri.shape[0]))
+        self.n_prims = s.n_prims
+        self.scene = s
+
+    def load_ini(self, path: str):
+        mn, mx = (C.c_double * 3)(), (C.c_double * 3)()
+        cam = L.CameraDesc()
+        self._check(self._lib.cpt_scene_load_ini(self._h, path.encode(), mn, mx, C.byref(cam)))
+        self.n_prims = int(self._lib.cpt_scene_count(self._h))
+        return np.array(mn[:]), np.array(mx[:]), cam
+
+    # -- LBVH ---------------------------------------------------------------
+    def build(self, bounds_min, bounds_max) -> L.BuildStats:
+        st = L.BuildStats()
+        self._check(self._lib.cpt_bvh_build(self._h, L.d3(bounds_min), L.d3(bounds_max), C.byref(st)))
+        return st
+
+    def export(self) -> dict:
+        n = self.n_prims
+        keys = np.empty(n, np.uint64)
+        order = np.empty(n, np.uint32)
+        ca = np.empty(max(n - 1, 0), np.int32)
+        cb = np.empty(max(n - 1, 0), np.int32)
+        boxes = np.empty((max(n - 1, 0), 12), np.float32)
+        root = np.empty(6, np.float32)
+        self._check(self._lib.cpt_bvh_export(self._h, keys.ctypes.data, order.ctypes.data,
+                                             ca.ctypes.data, cb.ctypes.data, boxes.ctypes.data,
+                                             root.ctypes.data))
+        return dict(keys=keys, order=order, child_a=ca, child_b=cb, boxes=boxes, root_box=root)
+
+    def morton(self, bounds_min, bounds_max) -> np.ndarray:
+        keys = np.empty(self.n_prims, np.uint64)
+        self._check(self._lib.cpt_bvh_morton(self._h, L.d3(bounds_min), L.d3(bounds_max),
+                                             keys.ctypes.data, 0))
+        return keys
+
+    def sort_pairs(self, keys, vals, bits: int = 32):
+        """Stable radix sort in place (numpy arrays or device tensors)."""
+        dev = _is_torch(keys) and keys.is_cuda
+        if dev:
+            self._torch_stream()
+        n = int(keys.shape[0])
+        self._check(self._lib.cpt_radix_sort_pairs(self._h, _ptr(keys), _ptr(vals), n, bits,
+                                                   L.CPT_DEVICE_PTRS if dev else 0))
+
+    # -- traversal ------------------------------------------------------------
+    def trace(self, rays, t_out=None, id_out=None) -> Tuple:
+        """Closest hit for rays (n x 8 float32: o, tmin, d, tmax)."""
+        n = int(rays.shape[0])
+        dev = _is_torch(rays) and rays.is_cuda
+        if dev:
+            import torch
+            self._torch_stream()
+            if t_out is None:
+                t_out = torch.empty(n, dtype=torch.float32, device=rays.device)
+            if id_out is None:
+                id_out = torch.empty(n, dtype=torch.int32, device=rays.device)
+        else:
+            rays = np.ascontiguousarray(rays, np.float32)
+            if t_out is None:
+                t_out = np.empty(n, np.float32)
+            if id_out is None:
+                id_out = np.empty(n, np.int32)
+        self._check(self._lib.cpt_trace_closest(self._h, _ptr(rays), n, _ptr(t_out), _ptr(id_out),
+                                                L.CPT_DEVICE_PTRS if dev else 0))
+        return t_out, id_out
+
+    def trace_count(self, rays):
+        n = int(rays.shape[0])
+        dev = _is_torch(rays) and rays.is_cuda
+        if dev:
+            import torch
+            self._torch_stream()
+            a = torch.empty(n, dtype=torch.int32, device=rays.device)
+            b = torch.empty(n, dtype=torch.int32, device=rays.device)
+        else:
+            rays = np.ascontiguousarray(rays, np.float32)
+            a = np.empty(n, np.int32)
+            b = np.empty(n, np.int32)
+        self._check(self._lib.cpt_trace_count(self._h, _ptr(rays), n, _ptr(a), _ptr(b),
+                                              L.CPT_DEVICE_PTRS if dev else 0))
+        return a, b
+
+    def last_trace_ms(self) -> float:
+        return float(self._lib.cpt_last_trace_ms(self._h))
+
+    # -- path tracing -----------------------------------------------------------
+    def render(self, camera_desc: L.CameraDesc, spp: int, seed: int = 3317,
+               profile: int = L.PROFILE_CONFIG, max_depth: int = 8, rr_depth: int = 3,
+               rr_prob: float = 0.7, sample_base: int = 0, aov: bool = False,
+               paths_per_batch: int = 0, out=None, aov_out=None):
+        cam = camera(camera_desc)
+        p = L.RenderParams(seed, spp, max_depth, rr_depth, profile, rr_prob, sample_base,
+                           1 if aov else 0, paths_per_batch, 0)
+        h, w = cam.height, cam.width
+        dev = out is not None and _is_torch(out) and out.is_cuda
+        if dev:
+            self._torch_stream()
+        if out is None:
+            out = np.empty((h, w, 3), np.float32)
+        if aov and aov_out is None:
+            aov_out = np.empty((h, w, 7), np.float32)
+        st = L.RenderStats()
+        self._check(self._lib.cpt_render(self._h, C.byref(cam), C.byref(p), _ptr(out),
+                                         _ptr(aov_out) if aov else None, C.byref(st),
+                                         L.CPT_DEVICE_PTRS if dev else 0))
+        return out, (aov_out if aov else None), st
+
+
+def camera(desc: L.CameraDesc) -> L.Camera:
+    cam = L.Camera()
+    rc = L.lib().cpt_camera_from_desc(C.byref(desc), C.byref(cam))
+    if rc:
+        raise CptError(rc, "invalid camera")
+    return cam
+
+
+def tonemap_rgba8(rgb: np.ndarray) -> np.ndarray:
+    h, w = rgb.shape[:2]
+    out = np.empty((h, w, 4), np.uint8)
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    rc = L.lib().cpt_tonemap_rgba8(rgb.ctypes.data, w, h, out.ctypes.data)
+    if rc:
+        raise CptError(rc, "tonemap")
+    return out

@@ -1,0 +1,561 @@
+// bvh.hip - LBVH build for gfx950: primitive packing, FP64 Morton codes,
+// LDS-staged wave-level LSD radix sort, Karras emission and an atomic
+// bottom-up refit.
+//
+// Replaces Device::buildBVH (CudaPathTracing/Device.cuh:169-195):
+//   generateMortonCodes (Device.cuh:134-142, morton3D BVH.cuh:23-41)
+//     -> k_morton: FP64 normalisation (bit-exact codes), writes the 30-bit
+//        code and the tile histogram of the first radix digit.
+//   thrust::sort_by_key on (code<<32|index) (Device.cuh:187-189)
+//     -> 4 passes of 8-bit stable LSD sort on the code with the index as
+//        payload.  Keys are unique and the payload starts as the identity,
+//        so a stable sort of the code yields the same permutation as the
+//        reference's 64-bit key sort.  The last pass writes the 64-bit keys.
+//   generateHierarchy (BVH.cuh:174-241)
+//     -> k_emit: determineRange/findSplit on the full 64-bit key, child refs
+//        into the 64 B node, parent links for the refit.
+//   constructAABB<<<1,1>>> (BVH.cuh:243-280)
+//     -> k_refit: one thread per leaf climbs while it is the second child to
+//        arrive (agent-scope acq_rel counter per node); min/max union is exact
+//        so the result equals the reference's serial post-order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "cpt_device.hpp"
+#include "internal.hpp"
+
+namespace cpt {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kTile = kSortThreads * kSortItems;  // 4096 keys per tile
+constexpr int kWaveKeys = 64 * kSortItems;         // 1024 keys per wave
+
+// ---- FP64 helpers (no contraction: matches the reference's FP64 order) ---
+__device__ __forceinline__ float roundDown(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
+__device__ __forceinline__ float roundUp(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// Per triangle: FP64 centre ((p0+p1)+p2)/3 (Hittable.cuh:459), FP64 padded
+// box (Hittable.cuh:460-461, AABB.cuh:30-40) rounded outward to FP32, and
+// the 48 B leaf record.
+__global__ void k_pack_tris(const double *__restrict__ tri, const int32_t *__restrict__ mat,
+                            int64_t nt, int64_t base, double *__restrict__ center,
+                            float *__restrict__ box, float4 *__restrict__ rec) {
+#pragma clang fp contract(off)
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double *p = tri + 9 * i;
+  double x0 = p[0], y0 = p[1], z0 = p[2], x1 = p[3], y1 = p[4], z1 = p[5], x2 = p[6], y2 = p[7],
+         z2 = p[8];
+  int64_t id = base + i;
+  center[3 * id + 0] = ((x0 + x1) + x2) / 3.0;
+  center[3 * id + 1] = ((y0 + y1) + y2) / 3.0;
+  center[3 * id + 2] = ((z0 + z1) + z2) / 3.0;
+  double mnx = dmin(dmin(x0, x1), x2), mny = dmin(dmin(y0, y1), y2), mnz = dmin(dmin(z0, z1), z2);
+  double mxx = dmax(dmax(x0, x1), x2), mxy = dmax(dmax(y0, y1), y2), mxz = dmax(dmax(z0, z1), z2);
+  const double pad = 0.0001;
+  mnx -= pad; mny -= pad; mnz -= pad;
+  mxx += pad; mxy += pad; mxz += pad;
+  float *b = box + 6 * id;
+  b[0] = roundDown(dmin(mnx, mxx)); b[1] = roundDown(dmin(mny, mxy)); b[2] = roundDown(dmin(mnz, mxz));
+  b[3] = roundUp(dmax(mnx, mxx)); b[4] = roundUp(dmax(mny, mxy)); b[5] = roundUp(dmax(mnz, mxz));
+  int m = mat ? mat[i] : 0;
+  rec[3 * id + 0] = make_float4((float)x0, (float)y0, (float)z0, __int_as_float((int)id));
+  rec[3 * id + 1] = make_float4((float)x1, (float)y1, (float)z1, __int_as_float(CPT_OBJ_TRIANGLE));
+  rec[3 * id + 2] = make_float4((float)x2, (float)y2, (float)z2, __int_as_float(m));
+}
+
+__device__ __forceinline__ uint32_t expandBits(uint32_t v) {  // BVH.cuh:12-19
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t mortonCode(double cx, double cy, double cz, double3 smin,
+                                               double3 smax) {
+#pragma clang fp contract(off)
+  double nx = (cx - smin.x) / (smax.x - smin.x);
+  double ny = (cy - smin.y) / (smax.y - smin.y);
+  double nz = (cz - smin.z) / (smax.z - smin.z);
+  double x = dmin(dmax(nx * 1024.0, 0.0), 1023.0);
+  double y = dmin(dmax(ny * 1024.0, 0.0), 1023.0);
+  double z = dmin(dmax(nz * 1024.0, 0.0), 1023.0);
+  return expandBits((uint32_t)x) * 4u + expandBits((uint32_t)y) * 2u + expandBits((uint32_t)z);
+}
+
+// Morton code per primitive; tile histogram of digit 0 (hist is digit-major:
+// hist[d * nTiles + tile]).  keys64 (optional) receives code<<32|index.
+__global__ __launch_bounds__(kSortThreads) void k_morton(const double *__restrict__ center, int64_t n,
+                                                         double3 smin, double3 smax,
+                                                         uint32_t *__restrict__ codes,
+                                                         uint32_t *__restrict__ hist, int nTiles,
+                                                         uint64_t *__restrict__ keys64) {
+  __shared__ uint32_t sh[256];
+  const int tid = threadIdx.x;
+  sh[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll 4
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = base + j * kSortThreads + tid;
+    if (i < n) {
+      uint32_t c = mortonCode(center[3 * i], center[3 * i + 1], center[3 * i + 2], smin, smax);
+      if (codes) codes[i] = c;
+      if (keys64) keys64[i] = ((uint64_t)c << 32) | (uint64_t)(uint32_t)i;
+      atomicAdd(&sh[c & 255u], 1u);
+    }
+  }
+  __syncthreads();
+  if (hist) hist[(int64_t)tid * nTiles + blockIdx.x] = sh[tid];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
+                                                       int shift, uint32_t *__restrict__ hist,
+                                                       int nTiles) {
+  __shared__ uint32_t sh[256];
+  const int tid = threadIdx.x;
+  sh[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile;
+#pragma unroll 4
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = base + j * kSortThreads + tid;
+    if (i < n) atomicAdd(&sh[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)tid * nTiles + blockIdx.x] = sh[tid];
+}
+
+// Exclusive scan of 256 values held one per thread of a 256-thread block.
+__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t *scratch4,
+                                                          uint32_t &total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) scratch4[wave] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    uint32_t s = scratch4[w];
+    if (w < wave) pre += s;
+    tot += s;
+  }
+  total = tot;
+  __syncthreads();
+  return pre + x - v;
+}
+
+// Row d of the digit-major histogram -> exclusive prefix over tiles;
+// rowTotal[d] = count of digit d.
+__global__ __launch_bounds__(kSortThreads) void k_scan_rows(uint32_t *__restrict__ hist, int nTiles,
+                                                            uint32_t *__restrict__ rowTotal) {
+  __shared__ uint32_t scratch[4];
+  const int d = blockIdx.x, tid = threadIdx.x;
+  uint32_t *row = hist + (int64_t)d * nTiles;
+  const int chunk = (nTiles + kSortThreads - 1) / kSortThreads;
+  const int beg = tid * chunk, end = min(beg + chunk, nTiles);
+  uint32_t s = 0;
+  for (int i = beg; i < end; i++) s += row[i];
+  uint32_t total;
+  uint32_t run = blockExclusiveScan256(s, scratch, total);
+  for (int i = beg; i < end; i++) {
+    uint32_t v = row[i];
+    row[i] = run;
+    run += v;
+  }
+  if (tid == 0) rowTotal[d] = total;
+}
+
+// Stable scatter of one 8-bit digit.  Wave w of the block ranks keys
+// [tile + w*1024, +1024) in 16 rounds of 64 (lane order inside a round), so
+// (wave, round, lane) is the input order.  Ranks come from a ballot-based
+// match over the 8 digit bits and per-wave LDS counters; the tile is then
+// reordered in LDS and written out digit run by digit run.
+__global__ __launch_bounds__(kSortThreads) void k_scatter(
+    const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
+    uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut, uint64_t *__restrict__ keys64Out,
+    int64_t n, int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowTotal,
+    int nTiles) {
+  __shared__ uint32_t sWave[4][256];
+  __shared__ uint32_t sStart[256];
+  __shared__ uint32_t sGlobal[256];
+  __shared__ uint32_t sKeys[kTile];
+  __shared__ uint32_t sVals[kTile];
+  __shared__ uint32_t scratch[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t tileBase = (int64_t)blockIdx.x * kTile;
+  const int64_t waveBase = tileBase + (int64_t)wave * kWaveKeys;
+  const int tileCount = (int)min((int64_t)kTile, n - tileBase);
+
+#pragma unroll
+  for (int w = 0; w < 4; w++) sWave[w][tid] = 0;
+
+  uint32_t key[kSortItems], val[kSortItems], pos[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = waveBase + j * 64 + lane;
+    key[j] = i < n ? keysIn[i] : 0u;
+    val[j] = i < n ? (valsIn ? valsIn[i] : (uint32_t)i) : 0u;
+  }
+  __syncthreads();
+  const uint64_t ltMask = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = waveBase + j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (key[j] >> shift) & 255u;
+      uint64_t peers = __ballot(1);
+#pragma unroll
+      for (int bit = 0; bit < 8; bit++) {
+        const bool b = (d >> bit) & 1u;
+        const uint64_t m = __ballot(b);
+        peers &= b ? m : ~m;
+      }
+      const uint32_t rank = __popcll(peers & ltMask);
+      const uint32_t base = sWave[wave][d];
+      const int leader = __ffsll((unsigned long long)peers) - 1;
+      __builtin_amdgcn_wave_barrier();
+      if (lane == leader) sWave[wave][d] = base + (uint32_t)__popcll(peers);
+      __builtin_amdgcn_wave_barrier();
+      pos[j] = base + rank;
+    }
+  }
+  __syncthreads();
+  // per-digit prefix across waves, tile-local digit starts, global bases
+  uint32_t run = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    uint32_t c = sWave[w][tid];
+    sWave[w][tid] = run;
+    run += c;
+  }
+  uint32_t tot;
+  const uint32_t start = blockExclusiveScan256(run, scratch, tot);
+  const uint32_t digitStart = blockExclusiveScan256(rowTotal[tid], scratch, tot);
+  sStart[tid] = start;
+  sGlobal[tid] = digitStart + hist[(int64_t)tid * nTiles + blockIdx.x] - start;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortItems; j++) {
+    int64_t i = waveBase + j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (key[j] >> shift) & 255u;
+      const uint32_t p = sStart[d] + sWave[wave][d] + pos[j];
+      if (p < (uint32_t)kTile) {  // always true for consistent ranks
+        sKeys[p] = key[j];
+        sVals[p] = val[j];
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < tileCount; i += kSortThreads) {
+    const uint32_t k = sKeys[i];
+    const uint32_t v = sVals[i];
+    const uint32_t g = sGlobal[(k >> shift) & 255u] + (uint32_t)i;
+    if ((int64_t)g >= n) continue;  // always false for a consistent histogram
+    if (keysOut) keysOut[g] = k;
+    if (valsOut) valsOut[g] = v;
+    if (keys64Out) keys64Out[g] = ((uint64_t)k << 32) | (uint64_t)v;
+  }
+}
+
+// ---- Karras emission (BVH.cuh:81-241) --------------------------------------
+__device__ __forceinline__ int deltaKey(const uint64_t *__restrict__ k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = k[i], b = k[j];
+  if (a == b) return 64;
+  return __clzll(a ^ b);
+}
+
+__global__ void k_emit(const uint64_t *__restrict__ keys, int n, float4 *__restrict__ nodes,
+                       int32_t *__restrict__ parentInt, int32_t *__restrict__ parentLeaf,
+                       int32_t *__restrict__ flags) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n - 1) return;
+  // determineRange, BVH.cuh:130-172
+  const int dNext = deltaKey(keys, n, idx, idx + 1);
+  const int dPrev = deltaKey(keys, n, idx, idx - 1);
+  const int d = dNext > dPrev ? 1 : -1;
+  const int dMin = deltaKey(keys, n, idx, idx - d);
+  int64_t lMax = 2;
+  while (true) {
+    int64_t j = (int64_t)idx + lMax * d;
+    if (j < 0 || j >= n) break;
+    if (deltaKey(keys, n, idx, (int)j) <= dMin) break;
+    lMax *= 2;
+  }
+  int64_t l = 0;
+  for (int64_t t = lMax / 2; t >= 1; t /= 2) {
+    int64_t j = (int64_t)idx + (l + t) * d;
+    if (j >= 0 && j < n && deltaKey(keys, n, idx, (int)j) > dMin) l += t;
+  }
+  const int jEnd = (int)(idx + l * d);
+  const int first = min(idx, jEnd), last = max(idx, jEnd);
+  // findSplit, BVH.cuh:81-118
+  int split;
+  const uint64_t fc = keys[first], lc = keys[last];
+  if (fc == lc) {
+    split = (first + last) >> 1;
+  } else {
+    const int common = __clzll(fc ^ lc);
+    split = first;
+    int step = last - first;
+    do {
+      step = (step + 1) >> 1;
+      const int ns = split + step;
+      if (ns < last && __clzll(fc ^ keys[ns]) > common) split = ns;
+    } while (step > 1);
+  }
+  const int ca = (split == first) ? ~split : split;
+  const int cb = (split + 1 == last) ? ~(split + 1) : (split + 1);
+  nodes[4 * (int64_t)idx + 3] = make_float4(__int_as_float(ca), __int_as_float(cb), 0.f, 0.f);
+  if (ca < 0) parentLeaf[~ca] = idx << 1; else parentInt[ca] = idx << 1;
+  if (cb < 0) parentLeaf[~cb] = (idx << 1) | 1; else parentInt[cb] = (idx << 1) | 1;
+  flags[idx] = 0;
+  if (idx == 0) parentInt[0] = -1;
+}
+
+// Writes a child box into slot (0 = A, 1 = B) of node `node`.
+__device__ __forceinline__ void storeChildBox(float *__restrict__ nf, int64_t node, int slot,
+                                              const float bx[6]) {
+  float *p = nf + 16 * node;
+  const int o = slot * 4;
+  p[o + 0] = bx[0];
+  p[o + 1] = bx[3];
+  p[o + 2] = bx[1];
+  p[o + 3] = bx[4];
+  p[8 + slot * 2 + 0] = bx[2];
+  p[8 + slot * 2 + 1] = bx[5];
+}
+
+__global__ void k_refit(int n, const uint32_t *__restrict__ order, const float4 *__restrict__ rec,
+                        const float *__restrict__ box32, float4 *__restrict__ nodes,
+                        const int32_t *__restrict__ parentInt,
+                        const int32_t *__restrict__ parentLeaf, int32_t *__restrict__ flags,
+                        float4 *__restrict__ leaves, float *__restrict__ rootBox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = order[i];
+  leaves[3 * (int64_t)i + 0] = rec[3 * (int64_t)p + 0];
+  leaves[3 * (int64_t)i + 1] = rec[3 * (int64_t)p + 1];
+  leaves[3 * (int64_t)i + 2] = rec[3 * (int64_t)p + 2];
+  float bx[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) bx[k] = box32[6 * (int64_t)p + k];
+  if (n == 1) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+    return;
+  }
+  float *nf = (float *)nodes;
+  int par = parentLeaf[i];
+  int node = par >> 1;
+  storeChildBox(nf, node, par & 1, bx);
+  while (true) {
+    const int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;  // sibling subtree not finished: it will continue
+    const float4 a = nodes[4 * (int64_t)node + 0];
+    const float4 b = nodes[4 * (int64_t)node + 1];
+    const float4 c = nodes[4 * (int64_t)node + 2];
+    bx[0] = fminf(a.x, b.x);
+    bx[3] = fmaxf(a.y, b.y);
+    bx[1] = fminf(a.z, b.z);
+    bx[4] = fmaxf(a.w, b.w);
+    bx[2] = fminf(c.x, c.z);
+    bx[5] = fmaxf(c.y, c.w);
+    if (node == 0) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+      return;
+    }
+    par = parentInt[node];
+    node = par >> 1;
+    storeChildBox(nf, node, par & 1, bx);
+  }
+}
+
+// ---- host launchers ---------------------------------------------------------
+
+void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]) {
+  // Device.cuh:174-181: +-1 expansion; +-20 when the box is empty (x tested).
+  volatile double one = 1.0;
+  for (int k = 0; k < 3; k++) {
+    smin[k] = bmin[k] - one;
+    smax[k] = bmax[k] + one;
+  }
+  if (smin[0] > smax[0]) {
+    for (int k = 0; k < 3; k++) {
+      smin[k] = -20.0;
+      smax[k] = 20.0;
+    }
+  }
+}
+
+int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, int64_t nt,
+                  int64_t base) {
+  if (nt <= 0) return CPT_OK;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_pack_tris, dim3((unsigned)((nt + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                     tri64Dev, matDev, nt, base, ctx->center64.as<double>(), ctx->box32.as<float>(),
+                     ctx->rec.as<float4>());
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  return CPT_OK;
+}
+
+int launchMorton(cpt_ctx *ctx, const double bmin[3], const double bmax[3], uint64_t *keysDev) {
+  double smin[3], smax[3];
+  expandBounds(bmin, bmax, smin, smax);
+  const int64_t n = ctx->nPrims;
+  if (n <= 0) return CPT_OK;
+  const int nTiles = (int)((n + kTile - 1) / kTile);
+  hipLaunchKernelGGL(k_morton, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream,
+                     ctx->center64.as<double>(), n, make_double3(smin[0], smin[1], smin[2]),
+                     make_double3(smax[0], smax[1], smax[2]), (uint32_t *)nullptr,
+                     (uint32_t *)nullptr, nTiles, keysDev);
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  return CPT_OK;
+}
+
+// Sorts (keys, vals) by the low `bits` bits.  Result ends in (keys, vals);
+// keysTmp/valsTmp are scratch of the same size.  identityVals: vals are
+// generated as the index on the first pass (vals content ignored).
+// If `keys64Out` is set (internal use) the last pass writes 64-bit keys.
+static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
+                      uint32_t *valsTmp, int64_t n, int bits, bool identityVals,
+                      bool firstHistDone, uint64_t *keys64Out, uint32_t *orderOut) {
+  const int nTiles = (int)((n + kTile - 1) / kTile);
+  const int passes = (bits + 7) / 8;
+  uint32_t *hist = ctx->hist.as<uint32_t>();
+  uint32_t *rowTotal = ctx->rowTotal.as<uint32_t>();
+  uint32_t *kin = keys, *vin = vals, *kout = keysTmp, *vout = valsTmp;
+  for (int p = 0; p < passes; p++) {
+    const int shift = 8 * p;
+    if (!(p == 0 && firstHistDone))
+      hipLaunchKernelGGL(k_hist, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream, kin, n, shift,
+                         hist, nTiles);
+    hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kSortThreads), 0, ctx->stream, hist, nTiles,
+                       rowTotal);
+    const bool last = p == passes - 1;
+    uint32_t *ko = kout, *vo = vout;
+    uint64_t *k64 = nullptr;
+    if (last && keys64Out) {
+      ko = nullptr;
+      vo = orderOut;
+      k64 = keys64Out;
+    }
+    hipLaunchKernelGGL(k_scatter, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream, kin,
+                       (p == 0 && identityVals) ? (const uint32_t *)nullptr : vin, ko, vo, k64, n,
+                       shift, hist, rowTotal, nTiles);
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  if (!keys64Out && kin != keys) {
+    CPT_TRY_HIP(ctx, hipMemcpyAsync(keys, kin, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    CPT_TRY_HIP(ctx, hipMemcpyAsync(vals, vin, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return CPT_OK;
+}
+
+int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals) {
+  if (n <= 0) return CPT_OK;
+  const int nTiles = (int)((n + kTile - 1) / kTile);
+  int rc;
+  if ((rc = ensure(ctx, ctx->hist, (size_t)nTiles * 256 * 4))) return rc;
+  if ((rc = ensure(ctx, ctx->rowTotal, 256 * 4))) return rc;
+  return sortPasses(ctx, keys, vals, keysTmp, valsTmp, n, bits, identityVals, false, nullptr,
+                    nullptr);
+}
+
+int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st) {
+  const int64_t n = ctx->nPrims;
+  if (n > (int64_t)0x7ffffffe) return fail(ctx, CPT_ERR_ARG, "cpt_bvh_build", "too many primitives");
+  ctx->built = false;
+  if (n == 0) {
+    ctx->built = true;
+    if (st) std::memset(st, 0, sizeof(*st));
+    return CPT_OK;
+  }
+  const int nTiles = (int)((n + kTile - 1) / kTile);
+  int rc;
+  if ((rc = ensure(ctx, ctx->codesA, n * 4)) || (rc = ensure(ctx, ctx->codesB, n * 4)) ||
+      (rc = ensure(ctx, ctx->valsA, n * 4)) || (rc = ensure(ctx, ctx->valsB, n * 4)) ||
+      (rc = ensure(ctx, ctx->hist, (size_t)nTiles * 256 * 4)) ||
+      (rc = ensure(ctx, ctx->rowTotal, 256 * 4)) || (rc = ensure(ctx, ctx->keys64, n * 8)) ||
+      (rc = ensure(ctx, ctx->order, n * 4)) ||
+      (rc = ensure(ctx, ctx->nodes, (size_t)(n > 1 ? n - 1 : 1) * 64)) ||
+      (rc = ensure(ctx, ctx->parentInt, (size_t)n * 4)) ||
+      (rc = ensure(ctx, ctx->parentLeaf, (size_t)n * 4)) ||
+      (rc = ensure(ctx, ctx->flags, (size_t)n * 4)) || (rc = ensure(ctx, ctx->leaves, (size_t)n * 48)) ||
+      (rc = ensure(ctx, ctx->rootBox, 64)))
+    return rc;
+  double smin[3], smax[3];
+  expandBounds(bmin, bmax, smin, smax);
+  hipEvent_t *ev = ctx->ev;
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+  hipLaunchKernelGGL(k_morton, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream,
+                     ctx->center64.as<double>(), n, make_double3(smin[0], smin[1], smin[2]),
+                     make_double3(smax[0], smax[1], smax[2]), ctx->codesA.as<uint32_t>(),
+                     ctx->hist.as<uint32_t>(), nTiles, (uint64_t *)nullptr);
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+  if ((rc = sortPasses(ctx, ctx->codesA.as<uint32_t>(), ctx->valsA.as<uint32_t>(),
+                       ctx->codesB.as<uint32_t>(), ctx->valsB.as<uint32_t>(), n, 30, true, true,
+                       ctx->keys64.as<uint64_t>(), ctx->order.as<uint32_t>())))
+    return rc;
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+  const int bs = 256;
+  if (n > 1) {
+    hipLaunchKernelGGL(k_emit, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                       ctx->keys64.as<uint64_t>(), (int)n, ctx->nodes.as<float4>(),
+                       ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>(),
+                       ctx->flags.as<int32_t>());
+    CPT_TRY_HIP(ctx, hipGetLastError());
+  }
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+  hipLaunchKernelGGL(k_refit, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
+                     ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
+                     ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
+                     ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
+                     ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+  if (st) {
+    CPT_TRY_HIP(ctx, hipEventSynchronize(ev[4]));
+    float a = 0, b = 0, c = 0, d = 0;
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&a, ev[0], ev[1]));
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&b, ev[1], ev[2]));
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&c, ev[2], ev[3]));
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&d, ev[3], ev[4]));
+    st->morton_ms = a;
+    st->sort_ms = b;
+    st->emit_ms = c;
+    st->refit_ms = d;
+    st->total_ms = a + b + c + d;
+    st->n_prims = (int32_t)n;
+  }
+  ctx->built = true;
+  return CPT_OK;
+}
+
+}  // namespace cpt

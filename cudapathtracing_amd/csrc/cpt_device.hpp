@@ -1,0 +1,308 @@
+// cpt_device.hpp - device-side data layout and the closest-hit traversal.
+//
+// HBM layout of a built scene (N primitives):
+//   nodes   : (N-1) x 64 B, Karras numbering (node 0 = root).  float4 x4:
+//             [0] = A.min.x A.max.x A.min.y A.max.y
+//             [1] = B.min.x B.max.x B.min.y B.max.y
+//             [2] = A.min.z A.max.z B.min.z B.max.z
+//             [3] = refA refB 0 0 (int bits; ref >= 0 internal, ~k = leaf k)
+//   leaves  : N x 48 B in leaf (sorted Morton) order.  float4 x3:
+//             triangle: {v0, id} {v1, type} {v2, material}
+//             sphere  : {centre, id} {radius 0 0, type} {0 0 0, material}
+//             light   : {centre, id} {normal, type} {width height visible, mat}
+// Child boxes are FP32, rounded outward from the reference's FP64 padded
+// boxes (Hittable.cuh:421,453,460-461), so every FP32 box contains the FP64
+// box the reference tests.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cpt.h"
+
+namespace cpt {
+
+constexpr int kStackLds = 16;    // traversal stack entries kept in LDS per lane
+constexpr int kStackTotal = 64;  // Karras depth <= 62 for unique 64-bit keys
+
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mkf3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mkf3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mkf3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mkf3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mkf3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return mkf3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mkf3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return __builtin_fmaf(a.x, b.x, __builtin_fmaf(a.y, b.y, a.z * b.z)); }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return mkf3(__builtin_fmaf(a.y, b.z, -a.z * b.y), __builtin_fmaf(a.z, b.x, -a.x * b.z),
+              __builtin_fmaf(a.x, b.y, -a.y * b.x));
+}
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 unit3(f3 a) { return a * (1.0f / len3(a)); }
+__device__ __forceinline__ f3 xyz(float4 v) { return mkf3(v.x, v.y, v.z); }
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) --------------------
+struct u4 {
+  uint32_t x, y, z, w;
+};
+__host__ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+  const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)M0 * c.x;
+    uint64_t p1 = (uint64_t)M1 * c.z;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += W0;
+    k1 += W1;
+  }
+  return c;
+}
+
+// Sample stream: draw k of (pixel p, sample s) is word k&3 of
+// philox(ctr = {k>>2, p, s, 0x43505431}, key = seed), mapped to
+// ((x >> 8) + 1) * 2^-24 in (0, 1] -- exact in both FP32 and FP64.
+struct Rng {
+  uint32_t k0, k1, pix, smp, cnt;
+  uint32_t blk;
+  u4 cache;
+  __device__ __forceinline__ void init(uint64_t seed, uint32_t p, uint32_t s, uint32_t c) {
+    k0 = (uint32_t)seed;
+    k1 = (uint32_t)(seed >> 32);
+    pix = p;
+    smp = s;
+    cnt = c;
+    blk = 0xffffffffu;
+  }
+  __device__ __forceinline__ float next() {
+    uint32_t b = cnt >> 2;
+    if (b != blk) {
+      cache = philox4x32_10(u4{b, pix, smp, 0x43505431u}, k0, k1);
+      blk = b;
+    }
+    uint32_t w = cnt & 3;
+    uint32_t x = w == 0 ? cache.x : (w == 1 ? cache.y : (w == 2 ? cache.z : cache.w));
+    cnt++;
+    return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+  }
+};
+
+// ---- FP32 primitive tests --------------------------------------------------
+
+// Triangle::hit (Hittable.cuh:85-140) in FP32 with the reference's accept
+// rules: |det| < 1e-8 misses, u in [0,1], v >= 0, u+v <= 1, t in [tmin,tmax]
+// inclusive.
+__device__ __forceinline__ bool hitTri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float tmin, float tmax,
+                                       float &t) {
+  f3 e1 = v1 - v0, e2 = v2 - v0;
+  f3 h = cross3(d, e2);
+  float a = dot3(e1, h);
+  if (fabsf(a) < 1e-8f) return false;
+  float f = __builtin_amdgcn_rcpf(a);
+  f3 s = o - v0;
+  float u = f * dot3(s, h);
+  if (u < 0.0f || u > 1.0f) return false;
+  f3 q = cross3(s, e1);
+  float v = f * dot3(d, q);
+  if (v < 0.0f || u + v > 1.0f) return false;
+  float tt = f * dot3(e2, q);
+  if (tt < tmin || tt > tmax) return false;
+  t = tt;
+  return true;
+}
+
+// Sphere::hit (Hittable.cuh:27-54).
+__device__ __forceinline__ bool hitSph(f3 c, float r, f3 o, f3 d, float tmin, float tmax, float &t) {
+  f3 oc = o - c;
+  float a = dot3(d, d);
+  float b = dot3(d, oc);
+  float cc = dot3(oc, oc) - r * r;
+  float disc = b * b - a * cc;
+  if (disc < 0.0f) return false;
+  float sq = sqrtf(disc);
+  float tt = (-b - sq) / a;
+  if (tt < tmin || tt > tmax) tt = (-b + sq) / a;
+  if (tt < tmin || tt > tmax) return false;
+  t = tt;
+  return true;
+}
+
+// Light frame: edgeU as in the Light constructor (Hittable.cuh:363-364).
+__device__ __forceinline__ void lightFrame(f3 n, f3 &eu, f3 &ev) {
+  eu = fabsf(n.x) > 0.9f ? unit3(cross3(mkf3(0, 1, 0), n)) : unit3(cross3(mkf3(1, 0, 0), n));
+  ev = unit3(cross3(n, eu));
+}
+
+// Light::hit (Hittable.cuh:367-398): one-sided, invisible lights never hit.
+__device__ __forceinline__ bool hitQuad(f3 c, f3 n, float w, float hgt, bool vis, f3 o, f3 d,
+                                        float tmin, float tmax, float &t) {
+  if (!vis) return false;
+  float denom = dot3(d, n);
+  if (fabsf(denom) < 1e-6f || denom > 0.0f) return false;
+  float tt = dot3(c - o, n) / denom;
+  if (tt < tmin || tt > tmax) return false;
+  f3 eu, ev;
+  lightFrame(n, eu, ev);
+  f3 diff = (o + tt * d) - c;
+  float u = dot3(diff, eu), v = dot3(diff, ev);
+  if (fabsf(u) <= w * 0.5f && fabsf(v) <= hgt * 0.5f) {
+    t = tt;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool hitLeaf(const float4 *__restrict__ leaves, int li, f3 o, f3 d,
+                                        float tmin, float tmax, float &t) {
+  const float4 r0 = leaves[3 * li + 0];
+  const float4 r1 = leaves[3 * li + 1];
+  const float4 r2 = leaves[3 * li + 2];
+  const int type = __float_as_int(r1.w);
+  if (type == CPT_OBJ_TRIANGLE) return hitTri(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_SPHERE) return hitSph(xyz(r0), r1.x, o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_LIGHT)
+    return hitQuad(xyz(r0), xyz(r1), r2.x, r2.y, r2.z != 0.0f, o, d, tmin, tmax, t);
+  return false;
+}
+
+struct TraceCounters {
+  int nodes = 0, prims = 0;
+};
+
+// Closest hit over the LBVH.  Starts at the root's children (the root box
+// is not tested, as in traverseIterative, BVH.cuh:293-301), tests both child
+// boxes against [tmin, best t], tests leaf children at once, descends into
+// the nearer internal child and pushes the farther.  Returns the leaf index
+// of the closest hit (-1 for none); tbest holds its FP32 distance.
+// stk points at this lane's LDS column; entries are stk[i * stride].
+template <bool kCount>
+__device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
+                                            const float4 *__restrict__ leaves, int n, f3 o, f3 d,
+                                            float tmin, float &tbest, int *stk, int stride,
+                                            TraceCounters &cnt) {
+  int hit = -1;
+  if (n <= 0) return -1;
+  if (n == 1) {
+    float t;
+    if (kCount) cnt.prims++;
+    if (hitLeaf(leaves, 0, o, d, tmin, tbest, t)) {
+      tbest = t;
+      hit = 0;
+    }
+    return hit;
+  }
+  // Zero direction components are replaced by a tiny value so that the
+  // slab products stay finite (the triangle tests use the true direction).
+  const float eps = 1e-20f;
+  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
+               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
+  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  int ovf[kStackTotal - kStackLds];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    if (kCount) cnt.nodes++;
+    const float4 *np = nodes + 4 * node;
+    const float4 a = np[0];
+    const float4 b = np[1];
+    const float4 c = np[2];
+    const float4 r = np[3];
+    const int ca = __float_as_int(r.x), cb = __float_as_int(r.y);
+    float ax0 = __builtin_fmaf(a.x, inv.x, -ood.x), ax1 = __builtin_fmaf(a.y, inv.x, -ood.x);
+    float ay0 = __builtin_fmaf(a.z, inv.y, -ood.y), ay1 = __builtin_fmaf(a.w, inv.y, -ood.y);
+    float az0 = __builtin_fmaf(c.x, inv.z, -ood.z), az1 = __builtin_fmaf(c.y, inv.z, -ood.z);
+    float bx0 = __builtin_fmaf(b.x, inv.x, -ood.x), bx1 = __builtin_fmaf(b.y, inv.x, -ood.x);
+    float by0 = __builtin_fmaf(b.z, inv.y, -ood.y), by1 = __builtin_fmaf(b.w, inv.y, -ood.y);
+    float bz0 = __builtin_fmaf(c.z, inv.z, -ood.z), bz1 = __builtin_fmaf(c.w, inv.z, -ood.z);
+    float nearA = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fmaxf(fminf(az0, az1), tmin));
+    float farA = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fminf(fmaxf(az0, az1), tbest));
+    float nearB = fmaxf(fmaxf(fminf(bx0, bx1), fminf(by0, by1)), fmaxf(fminf(bz0, bz1), tmin));
+    float farB = fminf(fminf(fmaxf(bx0, bx1), fmaxf(by0, by1)), fminf(fmaxf(bz0, bz1), tbest));
+    bool hA = nearA <= farA;
+    bool hB = nearB <= farB;
+    if (hA && ca < 0) {
+      float t;
+      if (kCount) cnt.prims++;
+      if (hitLeaf(leaves, ~ca, o, d, tmin, tbest, t)) {
+        tbest = t;
+        hit = ~ca;
+      }
+      hA = false;
+    }
+    if (hB && cb < 0) {
+      float t;
+      if (kCount) cnt.prims++;
+      if (hitLeaf(leaves, ~cb, o, d, tmin, tbest, t)) {
+        tbest = t;
+        hit = ~cb;
+      }
+      hB = false;
+    }
+    if (hA && hB) {
+      int first = nearA <= nearB ? ca : cb;
+      int second = nearA <= nearB ? cb : ca;
+      if (sp < kStackLds) stk[sp * stride] = second;
+      else if (sp < kStackTotal) ovf[sp - kStackLds] = second;
+      sp = sp < kStackTotal ? sp + 1 : sp;
+      node = first;
+    } else if (hA) {
+      node = ca;
+    } else if (hB) {
+      node = cb;
+    } else {
+      if (sp == 0) break;
+      sp--;
+      node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
+    }
+  }
+  return hit;
+}
+
+// ---- FP64 refinement of the winning hit ------------------------------------
+// Recomputes the distance to primitive `id` with the reference's FP64
+// formulas (Hittable.cuh:27-54, 85-140, 367-398) on the FP64 scene copy.
+struct Scene64 {
+  const double *tri;    // n_tri x 9, insertion order
+  const double *sph;    // n_sph x 4 (c, r)
+  const double *lgt;    // n_lgt x 6 (c, n unit)
+  int32_t nLights, nSpheres;
+};
+
+__device__ __forceinline__ double refineT(const Scene64 &s, int id, double ox, double oy, double oz,
+                                          double dx, double dy, double dz, double tmin, double tmax) {
+#pragma clang fp contract(off)
+  if (id < s.nLights) {
+    const double *L = s.lgt + 6 * id;
+    double denom = dx * L[3] + dy * L[4] + dz * L[5];
+    double num = (L[0] - ox) * L[3] + (L[1] - oy) * L[4] + (L[2] - oz) * L[5];
+    return num / denom;
+  }
+  if (id < s.nLights + s.nSpheres) {
+    const double *S = s.sph + 4 * (id - s.nLights);
+    double ocx = ox - S[0], ocy = oy - S[1], ocz = oz - S[2];
+    double a = dx * dx + dy * dy + dz * dz;
+    double b = dx * ocx + dy * ocy + dz * ocz;
+    double c = (ocx * ocx + ocy * ocy + ocz * ocz) - S[3] * S[3];
+    double disc = b * b - a * c;
+    double sq = sqrt(disc > 0.0 ? disc : 0.0);
+    double t = (-b - sq) / a;
+    if (t < tmin || t > tmax) t = (-b + sq) / a;
+    return t;
+  }
+  const double *T = s.tri + 9 * (int64_t)(id - s.nLights - s.nSpheres);
+  double e1x = T[3] - T[0], e1y = T[4] - T[1], e1z = T[5] - T[2];
+  double e2x = T[6] - T[0], e2y = T[7] - T[1], e2z = T[8] - T[2];
+  double hx = dy * e2z - dz * e2y, hy = dz * e2x - dx * e2z, hz = dx * e2y - dy * e2x;
+  double a = e1x * hx + e1y * hy + e1z * hz;
+  double f = 1.0 / a;
+  double sx = ox - T[0], sy = oy - T[1], sz = oz - T[2];
+  double qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+  return f * (e2x * qx + e2y * qy + e2z * qz);
+}
+
+}  // namespace cpt

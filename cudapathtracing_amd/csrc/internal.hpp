@@ -1,0 +1,91 @@
+// internal.hpp - host-side context shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cpt.h"
+
+#define CPT_TRY_HIP(ctx, expr)                                                      \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) return cpt::fail((ctx), CPT_ERR_HIP, #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace cpt {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  template <class T> T *as() const { return (T *)p; }
+};
+
+// Device material (32 B): colour, alpha_x, alpha_y, eta, type.
+struct MatDev {
+  float cr, cg, cb, ax, ay, eta;
+  int type, pad;
+};
+
+struct RenderBufs;
+
+}  // namespace cpt
+
+struct cpt_ctx {
+  int device = 0;
+  hipStream_t ownStream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<cpt::DevBuf *> bufs;
+
+  // scene (insertion order)
+  int64_t nPrims = 0, nTri = 0;
+  int32_t nLights = 0, nSpheres = 0, nMats = 0;
+  cpt::DevBuf center64;  // N x 3 double
+  cpt::DevBuf box32;     // N x 6 float, outward-rounded padded boxes
+  cpt::DevBuf rec;       // N x 3 float4, leaf records in insertion order
+  cpt::DevBuf tri64;     // nTri x 9 double
+  cpt::DevBuf sph64;     // nSpheres x 4 double
+  cpt::DevBuf lgt64;     // nLights x 6 double
+  cpt::DevBuf mats;      // (nMats + nLights) x MatDev
+  cpt::DevBuf lightIdx;  // nLights x int (primitive ids of the lights)
+  std::vector<cpt_light> hostLights;
+
+  // LBVH
+  bool built = false;
+  cpt::DevBuf codesA, codesB, valsA, valsB, hist, rowTotal;
+  cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
+
+  // traversal I/O staging
+  cpt::DevBuf rayBuf, tBuf, idBuf, cntA, cntB;
+  float lastTraceMs = 0.f;
+  hipEvent_t ev[16] = {};
+
+  cpt::RenderBufs *render = nullptr;
+};
+
+namespace cpt {
+
+int fail(cpt_ctx *ctx, int code, const char *what, const char *detail);
+// Ensures buffer b holds at least `bytes` bytes (contents not preserved).
+int ensure(cpt_ctx *ctx, DevBuf &b, size_t bytes);
+void releaseAll(cpt_ctx *ctx);
+
+// bvh.hip
+int launchMorton(cpt_ctx *ctx, const double smin[3], const double smax[3], uint64_t *keysDev);
+int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals);
+int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
+int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, int64_t nt,
+                  int64_t base);
+void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]);
+
+// trace.hip
+int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
+                int32_t *nodesDev, int32_t *primsDev);
+
+// render.hip
+void renderRelease(cpt_ctx *ctx);
+
+}  // namespace cpt

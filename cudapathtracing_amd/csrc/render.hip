@@ -1,0 +1,531 @@
+// render.hip - wavefront path tracer for gfx950.
+//
+// Replaces the megakernel render<<<>>> (CudaPathTracing/Render.cu:16-261):
+// one path per (pixel, sample), state in HBM (SoA), index queues between
+//   k_gen     camera ray (Camera::getRandomSampleRay, Camera.cuh:37-43) in FP64
+//   k_extend  closest hit (traverseIterative) for the active queue
+//   k_shade   miss/sky, emitter, homogeneous volume (Render.cu:64-135), NEE to
+//             every quad light (Render.cu:161-187), Russian roulette
+//             (Render.cu:189-204), BSDF sampling (Render.cu:206-218); pushes
+//             survivors to the next queue and shadow rays to the shadow queue
+//   k_shadow  closest hit of each shadow ray; adds its contribution when the
+//             light itself is the closest hit (Render.cu:180-185)
+// Radiance is summed per pixel with FP32 atomics and divided by spp at the
+// end (Render.cu:220-222).  Random numbers: the counter-based stream of
+// cpt_device.hpp, consumed in the reference's order per sample (jitter x,y;
+// per vertex: 2 per light for NEE, 1 for roulette, 3 for the BSDF).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "bsdf.hpp"
+#include "cpt_device.hpp"
+#include "internal.hpp"
+
+namespace cpt {
+
+constexpr int kRenderThreads = 256;
+constexpr float kRayTMin = 0.001f;        // Render.cu tMin (BVH.cuh:282 default)
+constexpr float kRayTMax = 999999.0f;     // INF, rtmath.cuh:8
+constexpr uint32_t kFlagNee = 1u << 16;   // hasSampleLight (Render.cu:56)
+constexpr uint32_t kFlagAov = 1u << 17;   // first-vertex AOV written
+constexpr uint32_t kDepthMask = 0xffffu;
+
+struct RenderBufs {
+  int64_t cap = 0, shadowCap = 0;
+  int64_t npix = 0;
+  DevBuf ray, thr, meta, hit, qA, qB, shadow, counters, fb, aov;
+};
+
+struct CamDev {
+  double lf[3], pc[3], dh[3], dv[3];
+  float bg[3];
+  int w, h;
+};
+
+struct RenderArgs {
+  const float4 *nodes, *leaves, *rec;
+  const MatDev *mats;
+  int nPrims, nLights;
+  CamDev cam;
+  uint64_t seed;
+  int spp, sampleBase, profile, maxDepth, rrDepth, wantAov;
+  float rrProb;
+  int64_t npix;
+};
+
+__device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+// Wave-aggregated append: returns the slot for lanes with pred, ~0u otherwise.
+__device__ __forceinline__ uint32_t waveAppend(uint32_t *counter, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return ~0u;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)laneId() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  const uint64_t lt = (1ull << laneId()) - 1ull;
+  return pred ? base + (uint32_t)__popcll(m & lt) : ~0u;
+}
+
+__global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t base, int64_t count,
+                                                         float4 *__restrict__ ray, float4 *__restrict__ thr,
+                                                         uint4 *__restrict__ meta, uint32_t *__restrict__ q) {
+#pragma clang fp contract(off)
+  const int64_t p = (int64_t)blockIdx.x * kRenderThreads + threadIdx.x;
+  if (p >= count) return;
+  const int64_t gi = base + p;
+  const uint32_t pixel = (uint32_t)(gi % a.npix);
+  const uint32_t sample = (uint32_t)(a.sampleBase + gi / a.npix);
+  Rng rng;
+  rng.init(a.seed, pixel, sample, 0);
+  const double rx = rng.next(), ry = rng.next();
+  const int i = (int)(pixel % (uint32_t)a.cam.w), j = (int)(pixel / (uint32_t)a.cam.w);
+  const double fx = rx - 0.5 + i, fy = ry - 0.5 + j;
+  double d[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double pix = (a.cam.pc[k] + fx * a.cam.dh[k]) + fy * a.cam.dv[k];
+    d[k] = pix - a.cam.lf[k];
+  }
+  ray[2 * p] = make_float4((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2], kRayTMin);
+  ray[2 * p + 1] = make_float4((float)d[0], (float)d[1], (float)d[2], kRayTMax);
+  thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+  meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
+  q[p] = (uint32_t)p;
+}
+
+__global__ __launch_bounds__(kRenderThreads) void k_extend(const uint32_t *__restrict__ q, uint32_t count,
+                                                            const float4 *__restrict__ ray,
+                                                            const float4 *__restrict__ nodes,
+                                                            const float4 *__restrict__ leaves, int nPrims,
+                                                            float2 *__restrict__ hit) {
+  __shared__ int stack[kStackLds * kRenderThreads];
+  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t p = q[k];
+  const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
+  float tb = r1.w;
+  TraceCounters cnt;
+  const int leaf = traceClosest<false>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
+                                       stack + threadIdx.x, kRenderThreads, cnt);
+  hit[p] = make_float2(__int_as_float(leaf), tb);
+}
+
+__device__ __forceinline__ ShadeMat loadMat(const MatDev *mats, int m) {
+  const MatDev md = mats[m];
+  ShadeMat s;
+  s.color = mkf3(md.cr, md.cg, md.cb);
+  s.ax = md.ax;
+  s.ay = md.ay;
+  s.eta = md.eta;
+  s.type = md.type;
+  return s;
+}
+
+// Hit geometry from a leaf record: outward normal (Hittable.cuh:51,128,393).
+__device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
+  if (type == CPT_OBJ_TRIANGLE) {
+    f3 v0 = xyz(r0);
+    return unit3(cross3(xyz(r1) - v0, xyz(r2) - v0));
+  }
+  if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
+  return xyz(r1);
+}
+
+__global__ __launch_bounds__(kRenderThreads) void k_shade(
+    RenderArgs a, const uint32_t *__restrict__ q, uint32_t count, float4 *__restrict__ ray,
+    float4 *__restrict__ thr, uint4 *__restrict__ meta, const float2 *__restrict__ hit,
+    uint32_t *__restrict__ qNext, float4 *__restrict__ shadow, int64_t shadowCap,
+    uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov) {
+  __shared__ int stack[kStackLds * kRenderThreads];
+  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  const bool active = k < count;
+  bool alive = false;
+  uint32_t p = 0;
+  if (active) p = q[k];
+  float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, t4 = o4;
+  uint4 mt = make_uint4(0, 0, 0, 0);
+  int leaf = -1;
+  float th = 0.f;
+  if (active) {
+    o4 = ray[2 * (int64_t)p];
+    d4 = ray[2 * (int64_t)p + 1];
+    t4 = thr[p];
+    mt = meta[p];
+    const float2 h = hit[p];
+    leaf = __float_as_int(h.x);
+    th = h.y;
+  }
+  Rng rng;
+  rng.init(a.seed, mt.x, mt.y, mt.z);
+  uint32_t flags = mt.w;
+  f3 T = mkf3(t4.x, t4.y, t4.z);
+  f3 o = xyz(o4), d = xyz(d4);
+  float *pix = fb + 3 * (int64_t)mt.x;
+  float *pa = a.wantAov ? aov + 7 * (int64_t)mt.x : nullptr;
+  // NEE state for this lane
+  int neeLight = -1;
+  f3 neeO = mkf3(0, 0, 0), neeD = neeO, neeC = neeO;
+  bool neeWant = false;
+
+  if (active) {
+    if (leaf < 0) {
+      // miss: sky colour (Render.cu:138-142)
+      const f3 bg = mkf3(a.cam.bg[0], a.cam.bg[1], a.cam.bg[2]);
+      atomicAdd(pix + 0, T.x * bg.x);
+      atomicAdd(pix + 1, T.y * bg.y);
+      atomicAdd(pix + 2, T.z * bg.z);
+      if (pa && !(flags & kFlagAov)) {
+        atomicAdd(pa + 0, bg.x);
+        atomicAdd(pa + 1, bg.y);
+        atomicAdd(pa + 2, bg.z);
+      }
+    } else {
+      const float4 r0 = a.leaves[3 * (int64_t)leaf], r1 = a.leaves[3 * (int64_t)leaf + 1],
+                   r2 = a.leaves[3 * (int64_t)leaf + 2];
+      const int type = __float_as_int(r1.w);
+      const int mi = __float_as_int(r2.w);
+      const ShadeMat m = loadMat(a.mats, mi);
+      f3 hp = o + d * th;
+      f3 outw = outwardNormal(r0, r1, r2, type, hp);
+      const bool front = dot3(d, outw) < 0.0f;
+      f3 n = front ? outw : -outw;
+      if (m.type == CPT_M_VOLUME) {
+        // homogeneous medium, sigma_maj = 25 (Render.cu:64-135)
+        const float sig = 25.0f;
+        TraceCounters cnt;
+        float tb = kRayTMax;
+        int l2 = traceClosest<false>(a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb,
+                                     stack + threadIdx.x, kRenderThreads, cnt);
+        o = hp;
+        if (l2 >= 0) {
+          float dMax = tb;
+          float t = -__logf(1.0f - rng.next()) / sig;
+          f3 pos = hp + d * t;
+          o = pos;
+          while (t < dMax) {
+            if (rng.next() > 0.8f) break;
+            T = T * (__expf(-t * sig) / 0.8f);
+            float z = 1.0f - 2.0f * rng.next();
+            float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            float ph = 2.0f * kPi * rng.next();
+            float s, c;
+            __sincosf(ph, &s, &c);
+            d = unit3(mkf3(rr * c, rr * s, z));
+            o = pos;
+            pos = hp + d * t;
+            float tb2 = kRayTMax;
+            int l3 = traceClosest<false>(a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
+                                         stack + threadIdx.x, kRenderThreads, cnt);
+            if (l3 >= 0) dMax = tb2;
+            t = -__logf(1.0f - rng.next()) / sig;
+            if (t > dMax || l3 < 0) break;
+          }
+        }
+        alive = true;
+      } else if (m.type == CPT_M_LIGHT) {
+        // emitter (Render.cu:144-149)
+        if (!(flags & kFlagNee)) {
+          atomicAdd(pix + 0, T.x * m.color.x);
+          atomicAdd(pix + 1, T.y * m.color.y);
+          atomicAdd(pix + 2, T.z * m.color.z);
+        }
+        if (pa && !(flags & kFlagAov)) {
+          atomicAdd(pa + 0, 1.0f);
+          atomicAdd(pa + 1, 1.0f);
+          atomicAdd(pa + 2, 1.0f);
+          atomicAdd(pa + 3, n.x);
+          atomicAdd(pa + 4, n.y);
+          atomicAdd(pa + 5, n.z);
+          atomicAdd(pa + 6, th * len3(d));
+        }
+      } else {
+        if (pa && !(flags & kFlagAov)) {
+          atomicAdd(pa + 0, m.color.x);
+          atomicAdd(pa + 1, m.color.y);
+          atomicAdd(pa + 2, m.color.z);
+          atomicAdd(pa + 3, n.x);
+          atomicAdd(pa + 4, n.y);
+          atomicAdd(pa + 5, n.z);
+          atomicAdd(pa + 6, len3(mkf3((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2]) - hp));
+        }
+        flags |= kFlagAov;
+        const f3 ud = unit3(d);
+        const f3 V = -ud;
+        // NEE to every light (only for opaque-type surfaces, Render.cu:166)
+        const bool neeMat = m.type == CPT_M_OPAQUE || m.type == CPT_M_LAMBERT || m.type == CPT_M_GGX;
+        if (neeMat && a.nLights > 0) {
+          for (int li = 0; li < a.nLights; li++) {
+            const float4 l0 = a.rec[3 * li], l1 = a.rec[3 * li + 1], l2 = a.rec[3 * li + 2];
+            const f3 lc = xyz(l0), ln = xyz(l1);
+            const float lw = l2.x;
+            // the reference uses the width for both extents (Render.cu:170-171)
+            const float lh = a.profile == CPT_PROFILE_REFERENCE ? l2.x : l2.y;
+            f3 eu, ev;
+            lightFrame(ln, eu, ev);
+            const float u1 = rng.next(), u2 = rng.next();
+            const f3 lp = lc + eu * ((u1 - 0.5f) * lw) + ev * ((u2 - 0.5f) * lh);
+            const f3 toL = lp - hp;
+            const f3 dir = unit3(toL);
+            if (dot3(dir, n) > 0.0f && dot3(-dir, ln) > 0.0f && l2.z != 0.0f) {
+              const ShadeMat lm = loadMat(a.mats, __float_as_int(l2.w));
+              const f3 fr = bsdfEval(m, V, dir, n, mkf3(0, 0, 0), front);
+              const float g = dot3(dir, n) * dot3(-dir, ln) / dot3(toL, toL) * (lw * lh);
+              neeC = T * fr * lm.color * g;
+              neeO = hp;
+              neeD = dir;
+              neeLight = li;
+              neeWant = true;
+            }
+            flags |= kFlagNee;
+            // append this light's shadow ray (ballot respects the active lanes)
+            const uint32_t slot = waveAppend(counters + 1, neeWant);
+            if (neeWant && slot < shadowCap) {
+              shadow[4 * (int64_t)slot + 0] = make_float4(neeO.x, neeO.y, neeO.z, kRayTMin);
+              shadow[4 * (int64_t)slot + 1] = make_float4(neeD.x, neeD.y, neeD.z, kRayTMax);
+              shadow[4 * (int64_t)slot + 2] = make_float4(neeC.x, neeC.y, neeC.z, __int_as_float((int)mt.x));
+              shadow[4 * (int64_t)slot + 3] = make_float4(__int_as_float(neeLight), 0.f, 0.f, 0.f);
+            }
+            neeWant = false;
+          }
+        }
+        // Russian roulette / depth cap (Render.cu:189-204)
+        const uint32_t depth = flags & kDepthMask;
+        float prr = 1.0f;
+        bool cont = true;
+        if (a.profile == CPT_PROFILE_REFERENCE) {
+          prr = 0.7f;
+          if (rng.next() > prr) cont = false;
+        } else {
+          if ((int)depth + 1 >= a.maxDepth) cont = false;
+          else if ((int)depth >= a.rrDepth) {
+            prr = a.rrProb;
+            if (rng.next() > prr) cont = false;
+          }
+        }
+        if (cont) {
+          flags = (flags & ~kDepthMask) | ((depth + 1) & kDepthMask);
+          const float r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
+          f3 dir = mkf3(0, 0, 1), wm = mkf3(0, 0, 0);
+          float pdf = -1.0f;
+          bsdfSample(m, ud, n, front, r1, r2, r3, dir, pdf, wm);
+          if (pdf > 0.0f) {
+            const f3 fr = bsdfEval(m, V, unit3(dir), n, wm, front);
+            T = T * fr * (fabsf(dot3(dir, n)) / pdf / prr);
+            o = hp;
+            d = dir;
+            alive = true;
+          }
+        }
+      }
+    }
+  }
+  const uint32_t slot = waveAppend(counters + 0, alive);
+  if (alive) {
+    ray[2 * (int64_t)p] = make_float4(o.x, o.y, o.z, kRayTMin);
+    ray[2 * (int64_t)p + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
+    thr[p] = make_float4(T.x, T.y, T.z, 0.f);
+    meta[p] = make_uint4(mt.x, mt.y, rng.cnt, flags);
+    qNext[slot] = p;
+  }
+}
+
+__global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow,
+                                                            const uint32_t *__restrict__ counters,
+                                                            const float4 *__restrict__ nodes,
+                                                            const float4 *__restrict__ leaves, int nPrims,
+                                                            float *__restrict__ fb) {
+  __shared__ int stack[kStackLds * kRenderThreads];
+  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  if (k >= counters[1]) return;
+  const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
+  const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
+  float tb = r1.w;
+  TraceCounters cnt;
+  const int leaf = traceClosest<false>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
+                                       stack + threadIdx.x, kRenderThreads, cnt);
+  if (leaf < 0) return;
+  const int id = __float_as_int(leaves[3 * (int64_t)leaf].w);
+  if (id != __float_as_int(l.x)) return;
+  float *pix = fb + 3 * (int64_t)__float_as_int(c.w);
+  atomicAdd(pix + 0, c.x);
+  atomicAdd(pix + 1, c.y);
+  atomicAdd(pix + 2, c.z);
+}
+
+__global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ out, int64_t n, float scale) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = acc[i] * scale;
+}
+
+void renderRelease(cpt_ctx *ctx) {
+  if (!ctx->render) return;
+  RenderBufs *r = ctx->render;
+  DevBuf *all[] = {&r->ray, &r->thr, &r->meta, &r->hit, &r->qA, &r->qB, &r->shadow, &r->counters, &r->fb, &r->aov};
+  for (DevBuf *b : all)
+    if (b->p) hipFree(b->p);
+  delete r;
+  ctx->render = nullptr;
+}
+
+static unsigned blocksFor(int64_t n) { return (unsigned)((n + kRenderThreads - 1) / kRenderThreads); }
+
+}  // namespace cpt
+
+using namespace cpt;
+
+extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_params *prm, float *rgb,
+                          float *aovOut, cpt_render_stats *stats, int flags) {
+  if (!ctx || !cam || !prm || !rgb) return CPT_ERR_ARG;
+  if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_render", "no BVH built");
+  if (cam->width <= 0 || cam->height <= 0 || prm->spp <= 0 || prm->sample_base < 0)
+    return fail(ctx, CPT_ERR_ARG, "cpt_render", "bad image size or spp");
+  if (prm->profile == CPT_PROFILE_CONFIG && (prm->max_depth <= 0 || !(prm->rr_prob > 0.0)))
+    return fail(ctx, CPT_ERR_ARG, "cpt_render", "max_depth and rr_prob must be positive");
+  const int64_t npix = (int64_t)cam->width * cam->height;
+  if (npix > (int64_t)1 << 30) return fail(ctx, CPT_ERR_ARG, "cpt_render", "image too large");
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  if (!ctx->render) ctx->render = new RenderBufs();
+  RenderBufs &R = *ctx->render;
+  const int64_t total = npix * prm->spp;
+  const int nl = std::max(1, (int)ctx->nLights);
+  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 22);
+  cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 23) / nl));
+  cap = std::min(cap, total);
+  const int64_t shadowCap = cap * nl;
+  int rc;
+  if ((rc = ensure(ctx, R.ray, cap * 32)) || (rc = ensure(ctx, R.thr, cap * 16)) ||
+      (rc = ensure(ctx, R.meta, cap * 16)) || (rc = ensure(ctx, R.hit, cap * 8)) ||
+      (rc = ensure(ctx, R.qA, cap * 4)) || (rc = ensure(ctx, R.qB, cap * 4)) ||
+      (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
+      (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
+    return rc;
+  R.cap = cap;
+  R.shadowCap = shadowCap;
+  hipStream_t s = ctx->stream;
+
+  RenderArgs a;
+  a.nodes = ctx->nodes.as<float4>();
+  a.leaves = ctx->leaves.as<float4>();
+  a.rec = ctx->rec.as<float4>();
+  a.mats = ctx->mats.as<MatDev>();
+  a.nPrims = (int)ctx->nPrims;
+  a.nLights = ctx->nLights;
+  for (int k = 0; k < 3; k++) {
+    a.cam.lf[k] = cam->look_from[k];
+    a.cam.pc[k] = cam->pixel_center[k];
+    a.cam.dh[k] = cam->delta_h[k];
+    a.cam.dv[k] = cam->delta_v[k];
+    a.cam.bg[k] = (float)cam->background[k];
+  }
+  a.cam.w = cam->width;
+  a.cam.h = cam->height;
+  a.seed = prm->seed;
+  a.spp = prm->spp;
+  a.sampleBase = prm->sample_base;
+  a.profile = prm->profile;
+  a.maxDepth = prm->max_depth;
+  a.rrDepth = prm->rr_depth;
+  a.rrProb = (float)prm->rr_prob;
+  a.wantAov = aovOut ? 1 : 0;
+  a.npix = npix;
+
+  hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), npix * 3, 0.f);
+  if (aovOut)
+    hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(), npix * 7, 0.f);
+  CPT_TRY_HIP(ctx, hipGetLastError());
+
+  double rays = 0, extMs = 0, shMs = 0, sdMs = 0;
+  int bounces = 0;
+  uint32_t *cnt = R.counters.as<uint32_t>();
+  hipEvent_t *ev = ctx->ev;
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[12], s));
+  for (int64_t base = 0; base < total; base += cap) {
+    const int64_t count = std::min(cap, total - base);
+    hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
+                       R.ray.as<float4>(), R.thr.as<float4>(), R.meta.as<uint4>(), R.qA.as<uint32_t>());
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    uint32_t qn = (uint32_t)count;
+    uint32_t *qa = R.qA.as<uint32_t>(), *qb = R.qB.as<uint32_t>();
+    while (qn > 0) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
+      hipLaunchKernelGGL(k_extend, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
+                         (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>());
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
+      hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qa, qn, R.ray.as<float4>(),
+                         R.thr.as<float4>(), R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb,
+                         R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(), R.aov.as<float>());
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
+      const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
+      hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
+                         (const float4 *)R.shadow.as<float4>(), (const uint32_t *)cnt, a.nodes, a.leaves,
+                         a.nPrims, R.fb.as<float>());
+      CPT_TRY_HIP(ctx, hipGetLastError());
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
+      uint32_t hc[2] = {0, 0};
+      CPT_TRY_HIP(ctx, hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, s));
+      CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+      float e1 = 0, e2 = 0, e3 = 0;
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e1, ev[13], ev[14]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e2, ev[14], ev[15]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[15], ev[11]));
+      extMs += e1;
+      shMs += e2;
+      sdMs += e3;
+      rays += (double)qn + (double)std::min<int64_t>(hc[1], shadowCap);
+      bounces++;
+      qn = hc[0];
+      std::swap(qa, qb);
+    }
+  }
+  // mean over samples
+  const float inv = 1.0f / (float)prm->spp;
+  float *dstRgb = rgb, *dstAov = aovOut;
+  DevBuf tmp;
+  if (!(flags & CPT_DEVICE_PTRS)) {
+    if ((rc = ensure(ctx, tmp, npix * 40))) return rc;
+    dstRgb = tmp.as<float>();
+    dstAov = tmp.as<float>() + npix * 3;
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), dstRgb,
+                     npix * 3, inv);
+  if (aovOut)
+    hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(),
+                       dstAov, npix * 7, inv);
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[10], s));
+  if (!(flags & CPT_DEVICE_PTRS)) {
+    hipMemcpyAsync(rgb, dstRgb, npix * 12, hipMemcpyDeviceToHost, s);
+    if (aovOut) hipMemcpyAsync(aovOut, dstAov, npix * 28, hipMemcpyDeviceToHost, s);
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (tmp.p) hipFree(tmp.p);
+  if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_render", hipGetErrorString(e));
+  if (stats) {
+    float ms = 0;
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[12], ev[10]));
+    stats->ms = ms;
+    stats->rays = rays;
+    stats->paths = (double)total;
+    stats->extend_ms = extMs;
+    stats->shade_ms = shMs;
+    stats->shadow_ms = sdMs;
+    stats->bounces = bounces;
+  }
+  return CPT_OK;
+}
+
+extern "C" int cpt_scene_load_ini(cpt_ctx *ctx, const char *, double *, double *, cpt_camera_desc *) {
+  return cpt::fail(ctx, CPT_ERR_IO, "cpt_scene_load_ini", "not built yet");
+}

@@ -1,0 +1,281 @@
+/*
+ * cpt.h - C ABI of the MI355X (gfx950) LBVH build, closest-hit traversal and
+ * path-trace integrator.
+ *
+ * The library replaces the device side of Baoste/CudaPathTracing:
+ *   - Device::buildBVH            CudaPathTracing/Device.cuh:169-195
+ *       (generateMortonCodes Device.cuh:134, thrust::sort_by_key Device.cuh:187,
+ *        generateBVH/generateHierarchy BVH.cuh:174, constructAABB BVH.cuh:243)
+ *   - traverseIterative           CudaPathTracing/BVH.cuh:282-332
+ *   - render<<<>>> (STATIC mode)  CudaPathTracing/Render.cu:16-261
+ *   - the per-object construction kernels allocate*OnDevice
+ *                                 CudaPathTracing/Device.cuh:29-79
+ * Host interfaces kept from the reference (IniParser, Scene::init, Camera)
+ * sit on top of this ABI (cpt_scene_load_ini, cpt_camera_from_desc).
+ *
+ * Conventions
+ *   - Every entry point returns CPT_OK (0) or a negative CPT_ERR_* code and
+ *     never exits the process; cpt_last_error() gives the message.
+ *   - One context is used by one host thread at a time.  All device work is
+ *     ordered on the context's stream (cpt_set_stream may replace it).
+ *   - Pointers are host pointers unless CPT_DEVICE_PTRS is passed in flags;
+ *     with host pointers the call is synchronous.
+ *   - Primitive IDs are insertion indices in the reference's order
+ *     (Scene.cuh:74-89): lights, then spheres, then triangles (floors, which
+ *     split into (lt,lb,rt),(rt,lb,rb) per Device.cuh:42-43, then meshes).
+ */
+#ifndef CPT_H
+#define CPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CPT_OK 0
+#define CPT_ERR_ARG (-1)     /* invalid argument or size                   */
+#define CPT_ERR_HIP (-2)     /* HIP runtime error                          */
+#define CPT_ERR_STATE (-3)   /* call order (e.g. trace before build)       */
+#define CPT_ERR_OOM (-4)     /* device allocation failed                   */
+#define CPT_ERR_IO (-5)      /* file could not be read / parsed            */
+#define CPT_ERR_ABORTED (-6) /* host abort flag stopped a persistent kernel */
+
+#define CPT_DEVICE_PTRS 1 /* flags bit: buffers are device pointers */
+
+/* ---- material / object types ------------------------------------------ */
+/* MaterialType, CudaPathTracing/rtmath.cuh:15, plus two new entries. */
+#define CPT_M_NONE 0
+#define CPT_M_LIGHT 1
+#define CPT_M_OPAQUE 2               /* GGX conductor, n=2.63 k=9.08    */
+#define CPT_M_SPECULAR_DIELECTRIC 3  /* smooth glass, eta 1.51          */
+#define CPT_M_ROUGH_DIELECTRIC 4     /* GGX glass                       */
+#define CPT_M_VOLUME 5               /* homogeneous sigma=25 medium     */
+#define CPT_M_LAMBERT 6              /* new: cosine-sampled diffuse     */
+#define CPT_M_GGX 7                  /* new: isotropic GGX, Schlick F0  */
+
+/* ObjectType, CudaPathTracing/Hittable.cuh:8. */
+#define CPT_OBJ_NONE 0
+#define CPT_OBJ_SPHERE 1
+#define CPT_OBJ_LIGHT 2
+#define CPT_OBJ_TRIANGLE 3
+
+/* Material, CudaPathTracing/Material.cuh:5-22.  For CPT_M_GGX `color` is F0
+ * and alpha_x is the roughness alpha; for CPT_M_LAMBERT `color` is albedo. */
+typedef struct cpt_material {
+  double color[3];
+  double alpha_x;
+  double alpha_y;
+  double eta; /* 1.51 in the reference */
+  int32_t type;
+  int32_t pad_;
+} cpt_material;
+
+/* Light quad, CudaPathTracing/Hittable.cuh:346-399 and IniParser.h:26-34. */
+typedef struct cpt_light {
+  double center[3];
+  double width;
+  double height;
+  double normal[3];
+  double color[3]; /* emitted radiance (ini colour / 255) */
+  int32_t visible;
+  int32_t pad_;
+} cpt_light;
+
+/* Sphere, CudaPathTracing/Hittable.cuh:10-55. */
+typedef struct cpt_sphere {
+  double center[3];
+  double radius;
+  int32_t material; /* index into the material table */
+  int32_t pad_;
+} cpt_sphere;
+
+/* Camera as constructed by Camera::Camera, CudaPathTracing/Camera.cu:5-31. */
+typedef struct cpt_camera_desc {
+  double background[3];
+  double look_from[3];
+  double look_at[3];
+  double vfov_deg;
+  double aspect;
+  int32_t width;
+  int32_t pad_;
+} cpt_camera_desc;
+
+/* Derived camera (Camera.cu:12-30): height = max(1, (int)(width/aspect)),
+ * pixel(i,j) = pixel_center + (i + rx - 0.5) delta_h + (j + ry - 0.5) delta_v,
+ * ray direction = pixel - look_from (not normalised), row j = 0 is the
+ * bottom of the image. */
+typedef struct cpt_camera {
+  double look_from[3];
+  double pixel_center[3];
+  double delta_h[3];
+  double delta_v[3];
+  double background[3];
+  int32_t width;
+  int32_t height;
+} cpt_camera;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct cpt_ctx cpt_ctx;
+
+int cpt_create(cpt_ctx **out, int hip_device);
+void cpt_destroy(cpt_ctx *ctx);
+const char *cpt_last_error(const cpt_ctx *ctx);
+/* Use an external hipStream_t (e.g. torch's current stream); NULL restores
+ * the context's own stream. */
+int cpt_set_stream(cpt_ctx *ctx, void *hip_stream);
+int cpt_synchronize(cpt_ctx *ctx);
+/* Library build string (arch, version). */
+const char *cpt_version(void);
+
+/* ---- scene upload ------------------------------------------------------- */
+/* Replaces Scene::add* + allocate*OnDevice (Scene.cuh:175-270,
+ * Device.cuh:29-79): one bulk upload, IDs = insertion order
+ * [lights | spheres | triangles].  tri_xyz holds n_tri*9 doubles
+ * (p0,p1,p2); tri_mat holds n_tri material indices (NULL = material 0).
+ * Lights receive implicit CPT_M_LIGHT materials.  Host pointers only. */
+int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials,
+                     int32_t n_materials, const cpt_light *lights,
+                     int32_t n_lights, const cpt_sphere *spheres,
+                     int32_t n_spheres, const double *tri_xyz,
+                     const int32_t *tri_mat, int64_t n_tri);
+
+/* Number of primitives of the current scene. */
+int64_t cpt_scene_count(const cpt_ctx *ctx);
+
+/* Parses a reference scene_ini file (IniParser::Parse, IniParser.cpp:3-96)
+ * and runs Scene::init (Scene.cuh:72-145): adds lights, spheres, floors and
+ * [Mesh] OBJ files in that order, uploads the scene and writes the
+ * reference's Morton bounds (untranslated mesh-vertex box, Device.cuh:174-181
+ * is applied later by cpt_bvh_build) and camera.  Any out pointer may be
+ * NULL.  A mesh file that cannot be read adds no triangles (ObjLoader.cuh:
+ * 65-70). */
+int cpt_scene_load_ini(cpt_ctx *ctx, const char *path, double scene_min[3],
+                       double scene_max[3], cpt_camera_desc *camera_out);
+
+/* ---- LBVH --------------------------------------------------------------- */
+typedef struct cpt_build_stats {
+  float morton_ms;
+  float sort_ms;
+  float emit_ms;
+  float refit_ms;
+  float total_ms;
+  int32_t n_prims;
+  int32_t pad_;
+} cpt_build_stats;
+
+/* Device::buildBVH (Device.cuh:169-195).  scene_min/max are the reference's
+ * minBoundary/maxBoundary; the +-1 expansion and the +-20 fallback
+ * (Device.cuh:174-181) are applied inside.  stats may be NULL. */
+int cpt_bvh_build(cpt_ctx *ctx, const double scene_min[3],
+                  const double scene_max[3], cpt_build_stats *stats);
+
+/* Copies the built hierarchy to host memory (any pointer may be NULL):
+ *   keys[n]      sorted 64-bit keys (code<<32 | primitive id), BVH.cuh:39-40
+ *   order[n]     sorted primitive ids (d_objsIdx after the sort)
+ *   child_a/b[n-1] children of internal node i in Karras numbering;
+ *                a leaf is encoded as ~leaf_index (negative)
+ *   boxes[(n-1)*12] for internal node i: child A box (min xyz, max xyz) then
+ *                child B box, FP32 rounded outward from the FP64 box
+ *   root_box[6]  FP32 box of the whole tree */
+int cpt_bvh_export(cpt_ctx *ctx, uint64_t *keys, uint32_t *order,
+                   int32_t *child_a, int32_t *child_b, float *boxes,
+                   float *root_box);
+
+/* Morton codes only (generateMortonCodes, Device.cuh:134-142): writes the
+ * unsorted 64-bit keys of the current scene for the given bounds
+ * (expansion rule applied as in cpt_bvh_build). */
+int cpt_bvh_morton(cpt_ctx *ctx, const double scene_min[3],
+                   const double scene_max[3], uint64_t *keys_out, int flags);
+
+/* Stable radix sort of 32-bit keys with 32-bit values (the sort stage of
+ * cpt_bvh_build, exposed for testing): sorts n (key,value) pairs by the low
+ * `bits` key bits (1..32).  In place on host or device buffers. */
+int cpt_radix_sort_pairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *values,
+                         int64_t n, int32_t bits, int flags);
+
+/* ---- traversal ---------------------------------------------------------- */
+/* Closest hit for n rays (traverseIterative, BVH.cuh:282-332, with
+ * Triangle/Sphere/Light::hit, Hittable.cuh:27-140,367-398).
+ * rays: n x 8 floats {ox,oy,oz,tmin, dx,dy,dz,tmax}.  t_out: hit distance
+ * (recomputed in FP64 for the winning primitive, then rounded), id_out:
+ * primitive id or -1.  Either output may be NULL. */
+int cpt_trace_closest(cpt_ctx *ctx, const float *rays, int64_t n,
+                      float *t_out, int32_t *id_out, int flags);
+
+/* Same traversal with per-ray work counters (nodes visited, primitives
+ * tested), used for the algorithmic-bytes model. */
+int cpt_trace_count(cpt_ctx *ctx, const float *rays, int64_t n,
+                    int32_t *nodes_out, int32_t *prims_out, int flags);
+
+/* Average duration of the last cpt_trace_closest kernel (ms), measured with
+ * HIP events on the launching stream. */
+float cpt_last_trace_ms(const cpt_ctx *ctx);
+
+/* ---- path tracing ------------------------------------------------------- */
+#define CPT_PROFILE_REFERENCE 0 /* Render.cu STATIC: RR 0.7 every bounce,   */
+                                /* no depth cap, width-for-both NEE quirk   */
+#define CPT_PROFILE_CONFIG 1    /* max_depth cap, RR from rr_depth on       */
+
+typedef struct cpt_render_params {
+  uint64_t seed;
+  int32_t spp;
+  int32_t max_depth; /* bounces (CONFIG profile); ignored for REFERENCE */
+  int32_t rr_depth;  /* first bounce with Russian roulette (CONFIG)    */
+  int32_t profile;
+  double rr_prob;       /* survival probability, 0.7 in Render.cu:202 */
+  int32_t sample_base;  /* index of the first sample (progressive)    */
+  int32_t want_aov;     /* fill albedo/normal/depth outputs           */
+  int32_t paths_per_batch; /* 0 = automatic                          */
+  int32_t pad_;
+} cpt_render_params;
+
+typedef struct cpt_render_stats {
+  double ms;
+  double rays; /* extension + shadow rays traced */
+  double paths;
+  double extend_ms; /* sum of extension-kernel durations */
+  double shadow_ms;
+  double shade_ms;
+  int32_t bounces;
+  int32_t pad_;
+} cpt_render_stats;
+
+/* render<<<>>> (Render.cu:16-261) as a wavefront pipeline.  rgb: h*w*3
+ * floats, mean radiance per pixel, row 0 = bottom (Render.cu:220-222).
+ * aov (may be NULL): h*w*7 floats (albedo rgb, normal xyz, depth) averaged
+ * over samples at the first non-emitter hit. */
+int cpt_render(cpt_ctx *ctx, const cpt_camera *camera,
+               const cpt_render_params *params, float *rgb, float *aov,
+               cpt_render_stats *stats, int flags);
+
+/* Camera::Camera (Camera.cu:5-31) in FP64 on the host. */
+int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
+
+/* pic2RGBW / STATIC output (Render.cu:232-260, 462-481): sqrt gamma and
+ * uchar(254.99*clamp01(x)); rgba8 gets h*w*4 bytes.  Host pointers. */
+int cpt_tonemap_rgba8(const float *rgb, int32_t width, int32_t height,
+                      uint8_t *rgba8);
+
+/* ---- procedural scenes (host, no reference counterpart) ---------------- */
+/* Icosphere with 20*4^level triangles of radius r around c. */
+int64_t cpt_gen_icosphere_count(int32_t level);
+int cpt_gen_icosphere(int32_t level, const double c[3], double r,
+                      double amp, uint64_t noise_seed, double *tri_xyz);
+/* n triangles: centroids U[0,1]^3, equilateral, side exp(mu + sigma z),
+ * uniform random orientation. */
+int cpt_gen_triangle_soup(int64_t n, uint64_t seed, double mu, double sigma,
+                          double *tri_xyz);
+/* n rays, origins U[lo,hi]^3, directions uniform on the sphere. */
+int cpt_gen_random_rays(int64_t n, uint64_t seed, double lo, double hi,
+                        float tmin, float tmax, float *rays);
+/* Vertex bounding box of n triangles (the reference's mesh bounds). */
+int cpt_tri_bounds(const double *tri_xyz, int64_t n, double mn[3],
+                   double mx[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPT_H */

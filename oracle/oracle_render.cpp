@@ -1,0 +1,540 @@
+// oracle_render.cpp - FP64 CPU restatement of the reference integrator and
+// materials.  TEST INFRASTRUCTURE ONLY (see oracle_common.hpp).
+//
+// Follows:
+//   render (STATIC mode)   CudaPathTracing/Render.cu:16-261 (per sample: jitter,
+//                          bounce loop, homogeneous volume 64-135, miss 138-142,
+//                          emitter 144-149, NEE 161-187, roulette 189-204,
+//                          BSDF 206-218, mean 220-222)
+//   Camera ray             Camera.cuh:37-43
+//   Material::fr           Material.cuh:24-93
+//   Material::sampleLight  Material.cuh:95-193, brdf 201-235
+//   GGX D / lambda / G     Material.cuh:238-285
+//   Fresnel                Material.cuh:287-318
+// Deviations, all in cases where the reference returns NaN: sin(theta) = 0 in
+// DistributionGGX/lambda uses cos(phi) = 1 (the phi term is multiplied by
+// tan(theta) = 0 there), and 1 - z^2 < 0 from rounding is clamped to 0.
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11); draw k of
+// (pixel p, sample s) is word k&3 of philox({k>>2, p, s, 0x43505431}, seed),
+// mapped to ((x>>8)+1)*2^-24.  Draw order per sample follows Render.cu.
+// The new material types (CPT_M_LAMBERT, CPT_M_GGX) and the CONFIG profile
+// (max depth, roulette from a given depth) have no reference counterpart.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+#include "../include/cpt.h"
+#include "oracle_common.hpp"
+#include "oracle_scene.hpp"
+
+namespace orc {
+
+struct PhiloxOut {
+  uint32_t v[4];
+};
+
+PhiloxOut philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return PhiloxOut{{c0, c1, c2, c3}};
+}
+
+struct Stream {
+  uint32_t k0, k1, pix, smp, cnt = 0;
+  double next() {
+    uint32_t c[4] = {cnt >> 2, pix, smp, 0x43505431u};
+    PhiloxOut o = philox4x32_10(c, k0, k1);
+    uint32_t x = o.v[cnt & 3];
+    cnt++;
+    return (double)((x >> 8) + 1u) * (1.0 / 16777216.0);
+  }
+};
+
+struct OMat {
+  d3 color;
+  double ax, ay, eta;
+  int type;
+};
+
+// worldToLocal, Material.cuh:238-248
+static d3 worldToLocal(const d3 &N, const d3 &W) {
+  d3 z = N;
+  d3 up = mk(0.0, 0.0, 1.0);
+  if (std::fabs(Dot(z, up)) > 0.99999) up = mk(1.0, 0.0, 0.0);
+  d3 x = Unit(Cross(up, z));
+  d3 y = Cross(z, x);
+  return mk(Dot(W, x), Dot(W, y), Dot(W, z));
+}
+
+// DistributionGGX, Material.cuh:250-265
+static double DistributionGGX(const OMat &m, const d3 &N, const d3 &H) {
+  d3 w = worldToLocal(N, H);
+  double sinT = std::sqrt(std::fmax(0.0, 1.0 - w.z * w.z));
+  double cosP = sinT > 0.0 ? w.x / sinT : 1.0;
+  double sinP = sinT > 0.0 ? w.y / sinT : 0.0;
+  double cos2 = w.z * w.z;
+  double sin2 = 1.0 - cos2;
+  double tan2 = std::fmax(0.0, sin2) / cos2;
+  double cos4 = cos2 * cos2;
+  double cDa = cosP / m.ax;
+  double sDa = sinP / m.ay;
+  double e = tan2 * (cDa * cDa + sDa * sDa);
+  return 1.0 / (PI * m.ax * m.ay * cos4 * (1 + e) * (1 + e));
+}
+
+// lambda, Material.cuh:266-277
+static double lambda(const OMat &m, const d3 &N, const d3 &R) {
+  d3 w = worldToLocal(N, R);
+  double sinT = std::sqrt(std::fmax(0.0, 1 - w.z * w.z));
+  double cosP = sinT > 0.0 ? w.x / sinT : 1.0;
+  double sinP = sinT > 0.0 ? w.y / sinT : 0.0;
+  double alpha2 = m.ax * m.ax * cosP * cosP + m.ay * m.ay * sinP * sinP;
+  double alpha = std::sqrt(alpha2);
+  double tanT = std::tan(std::acos(std::fmin(mMax(Dot(N, R), 0.0), 1.0)));
+  return (std::sqrt(1.0 + alpha * alpha * tanT * tanT) - 1.0) / 2.0;
+}
+static double GeometrySmith(const OMat &m, const d3 &N, const d3 &V, const d3 &L) {
+  return 1.0 / (1.0 + lambda(m, N, V) + lambda(m, N, L));
+}
+static double GeometrySmith1(const OMat &m, const d3 &N, const d3 &W) { return 1.0 / (1.0 + lambda(m, N, W)); }
+
+// FresnelConductor, Material.cuh:287-295 (complex arithmetic of rtmath.cuh)
+struct cx {
+  double x, y;
+};
+static cx cdiv(cx a, cx b) {
+  double den = b.x * b.x + b.y * b.y;
+  return cx{(a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den};
+}
+static double FresnelConductor(const d3 &N, const d3 &V, const d3 &L, double n, double k) {
+  cx eta{n, k};
+  double ci = mMax(Dot(N, V), 0.0);
+  double ct = mMax(Dot(N, L), 0.0);
+  cx ec = cx{ci * eta.x, ci * eta.y};
+  cx rpa = cdiv(cx{ec.x - ct, ec.y}, cx{ct + ec.x, ec.y});
+  cx et = cx{ct * eta.x, ct * eta.y};
+  cx rpe = cdiv(cx{ci - et.x, -et.y}, cx{ci + et.x, et.y});
+  return ((rpa.x * rpa.x + rpa.y * rpa.y) + (rpe.x * rpe.x + rpe.y * rpe.y)) / 2.0;
+}
+
+// FresnelDielectric, Material.cuh:305-318
+static double FresnelDielectric(const d3 &N, const d3 &V, double eta) {
+  double ci = std::fabs(Dot(N, V));
+  double si = std::sqrt(mMax(0.0, 1.0 - ci * ci));
+  double st = si / eta;
+  if (st >= 1.0) return 1.0;
+  double ct = std::sqrt(mMax(0.0, 1.0 - st * st));
+  double rpa = (eta * ci - ct) / (eta * ci + ct);
+  double rpe = (ci - eta * ct) / (ci + eta * ct);
+  return (rpa * rpa + rpe * rpe) * 0.5;
+}
+
+// Material::fr, Material.cuh:24-93 (+ LAMBERT, GGX)
+static d3 fr(const OMat &m, const d3 &hitColor, const d3 &rayDir, const d3 &normal, const d3 &direction,
+             const d3 &wm, bool frontFace) {
+  d3 V = Unit(-rayDir);
+  d3 L = Unit(direction);
+  if (m.type == CPT_M_LAMBERT) return Dot(L, normal) > 0.0 ? hitColor / PI : mk(0, 0, 0);
+  d3 H = Unit(V + L);
+  bool reflect = Dot(V, normal) * Dot(L, normal) > 0.0;
+  if (m.type == CPT_M_SPECULAR_DIELECTRIC) {
+    double rr = frontFace ? m.eta : (1.0 / m.eta);
+    double ct = std::fabs(Dot(normal, L));
+    double F = FresnelDielectric(normal, V, rr);
+    if (reflect) return F * hitColor / ct;
+    return (1.0 - F) * hitColor / ct;
+  }
+  if (m.type == CPT_M_GGX) {
+    double nl = Dot(normal, L), nv = Dot(normal, V);
+    if (!(nl > 0.0) || !(nv > 0.0)) return mk(0, 0, 0);
+    double D = DistributionGGX(m, normal, H);
+    double G = GeometrySmith(m, normal, V, L);
+    double s = 1.0 - mMax(Dot(V, H), 0.0);
+    double s5 = s * s * s * s * s;
+    d3 F = mk(m.color.x + (1.0 - m.color.x) * s5, m.color.y + (1.0 - m.color.y) * s5,
+              m.color.z + (1.0 - m.color.z) * s5);
+    return F * (D * G / (4.0 * nv * nl));
+  }
+  if (!reflect) {
+    double rr = frontFace ? m.eta : (1.0 / m.eta);
+    double D = DistributionGGX(m, normal, wm);
+    if (Dot(wm, V) < 0.0) V = -V;
+    if (Dot(wm, L) < 0.0) L = -L;
+    double G = GeometrySmith(m, normal, V, L);
+    double F = FresnelDielectric(wm, V, rr);
+    double ci = std::fabs(Dot(normal, V));
+    double ct = std::fabs(Dot(normal, L));
+    double mbase = std::fabs(Dot(L, wm)) + std::fabs(Dot(V, wm)) / rr;
+    double denom = mbase * mbase;
+    double spec = (1.0 - F) * D * G * std::fabs(Dot(wm, L) * Dot(wm, V) / (ci * ct * denom));
+    return spec * hitColor;
+  }
+  double n = 2.63, k = 9.08;
+  if (m.type == CPT_M_ROUGH_DIELECTRIC) {
+    n = 1.58;
+    k = 0.001;
+  }
+  double D = DistributionGGX(m, normal, H);
+  double G = GeometrySmith(m, normal, V, L);
+  double F = FresnelConductor(normal, V, L, n, k);
+  double NdotL = mMax(Dot(normal, L), 0.0);
+  double NdotV = mMax(Dot(normal, V), 0.0);
+  double denominator = 4.0 * NdotV * NdotL + 0.001;
+  double spec = (D * G * F) / denominator;
+  return spec * hitColor;
+}
+
+static const double kOneEm6f = (double)1e-6f;  // mMax(1e-6f, nh.z), Material.cuh:140,222
+
+// brdf(), Material.cuh:201-235
+static void brdfSample(const OMat &m, const d3 &rayDir, const d3 &normal, d3 &direction, double &pdf,
+                       double r1, double r2) {
+  d3 w = normal;
+  d3 a = (std::fabs(w.z) > 0.99999) ? mk(1.0, 0.0, 0.0) : mk(0.0, 0.0, 1.0);
+  d3 u = Unit(Cross(a, w));
+  d3 v = Cross(w, u);
+  d3 uo = -Unit(rayDir);
+  d3 Wo = mk(Dot(uo, u), Dot(uo, v), Dot(uo, w));
+  d3 Wh = Unit(mk(m.ax * Wo.x, m.ay * Wo.y, Wo.z));
+  d3 T1 = (Wh.z < 0.99999) ? Unit(Cross(mk(0, 0, 1), Wh)) : Unit(Cross(mk(1, 0, 0), Wh));
+  d3 T2 = Cross(Wh, T1);
+  double r = std::sqrt(r1);
+  double theta = 2 * PI * r2;
+  double px = r * std::cos(theta), py = r * std::sin(theta);
+  double h = std::sqrt(1 - px * px);
+  double lerpX = (1 + Wh.z) / 2.0;
+  py = (1.0 - lerpX) * h + lerpX * py;
+  double pz = std::sqrt(mMax(0.0, 1.0 - (px * px + py * py)));
+  d3 nh = px * T1 + py * T2 + pz * Wh;
+  d3 Wm = Unit(mk(m.ax * nh.x, m.ay * nh.y, mMax(kOneEm6f, nh.z)));
+  d3 Wi = -Wo + 2 * Dot(Wo, Wm) * Wm;
+  direction = Unit(Wi.x * u + Wi.y * v + Wi.z * w);
+  if (Dot(direction, normal) < 0.0) {
+    pdf = -1.0;
+    return;
+  }
+  d3 H = Unit(direction + uo);
+  pdf = GeometrySmith1(m, normal, uo) / Wo.z * DistributionGGX(m, normal, H) / 4.0;
+}
+
+// Material::sampleLight, Material.cuh:95-193 (+ LAMBERT)
+static void sampleLight(const OMat &m, const d3 &rayDir, const d3 &normal, d3 &direction, d3 &wm,
+                        double &pdf, double r1, double r2, double r3, bool frontFace) {
+  if (m.type == CPT_M_SPECULAR_DIELECTRIC) {
+    double rr = frontFace ? m.eta : (1.0 / m.eta);
+    d3 ud = Unit(rayDir);
+    double R = FresnelDielectric(normal, -ud, rr);
+    if (r3 < R) {
+      direction = ud - 2.0 * Dot(ud, normal) * normal;
+      pdf = R;
+    } else {
+      d3 vPerp = 1.0 / rr * (ud + Dot(-ud, normal) * normal);
+      d3 vPar = -std::sqrt(std::fabs(1.0 - SquaredLength(vPerp))) * normal;
+      direction = Unit(vPerp + vPar);
+      pdf = 1.0 - R;
+    }
+  } else if (m.type == CPT_M_ROUGH_DIELECTRIC) {
+    d3 w = normal;
+    d3 a = (std::fabs(w.x) > 0.9) ? mk(0.0, 1.0, 0.0) : mk(1.0, 0.0, 0.0);
+    d3 u = Unit(Cross(w, a));
+    d3 v = Cross(w, u);
+    d3 uo = -Unit(rayDir);
+    d3 Wo = mk(Dot(uo, u), Dot(uo, v), Dot(uo, w));
+    d3 Wh = Unit(mk(m.ax * Wo.x, m.ay * Wo.y, Wo.z));
+    d3 T1 = (Wh.z < 0.99999) ? Unit(Cross(mk(0, 0, 1), Wh)) : mk(1, 0, 0);
+    d3 T2 = Cross(Wh, T1);
+    double r = std::sqrt(r1);
+    double theta = 2 * PI * r2;
+    double px = r * std::cos(theta), py = r * std::sin(theta);
+    double h = std::sqrt(1 - px * px);
+    double lerpX = (1 + Wh.z) / 2.0;
+    py = (1.0 - lerpX) * h + lerpX * py;
+    double pz = std::sqrt(mMax(0.0, 1.0 - (px * px + py * py)));
+    d3 nh = px * T1 + py * T2 + pz * Wh;
+    d3 Wm = Unit(mk(m.ax * nh.x, m.ay * nh.y, mMax(kOneEm6f, nh.z)));
+    d3 sn = Unit(Wm.x * u + Wm.y * v + Wm.z * w);
+    wm = sn;
+    double rr = frontFace ? m.eta : (1.0 / m.eta);
+    double R = FresnelDielectric(sn, -uo, rr);
+    if (r3 < R) {
+      brdfSample(m, rayDir, normal, direction, pdf, r1, r2);
+      pdf *= R;
+    } else {
+      d3 vPerp = 1.0 / rr * (uo + Dot(-uo, sn) * sn);
+      d3 vPar = -std::sqrt(std::fabs(1.0 - SquaredLength(vPerp))) * sn;
+      d3 rd = Unit(vPerp + vPar);
+      d3 Wi = mk(Dot(rd, u), Dot(rd, v), Dot(rd, w));
+      direction = rd;
+      if (Dot(direction, normal) > 0.0) {
+        pdf = -1.0;
+        return;
+      }
+      double mbase = Dot(-Wi, Wm) + Dot(Wo, Wm) / rr;
+      double denom = mbase * mbase;
+      double dwm = std::fabs(Dot(Wi, Wm)) / denom;
+      pdf = GeometrySmith1(m, normal, uo) / Wo.z * DistributionGGX(m, normal, sn) * Dot(Wo, Wm) * dwm *
+            (1.0 - R);
+    }
+  } else if (m.type == CPT_M_LAMBERT) {
+    // the reference's commented cosine sampler, Material.cuh:177-188
+    double sinT = std::sqrt(1.0 - r1);
+    double cosT = std::sqrt(r1);
+    d3 w = normal;
+    d3 a = (std::fabs(w.x) > 0.9) ? mk(0.0, 1.0, 0.0) : mk(1.0, 0.0, 0.0);
+    d3 u = Unit(Cross(w, a));
+    d3 v = Cross(w, u);
+    direction = std::cos(2.0 * PI * r2) * sinT * u + std::sin(2.0 * PI * r2) * sinT * v + cosT * w;
+    pdf = cosT / PI;
+  } else {  // OPAQUE, GGX
+    brdfSample(m, rayDir, normal, direction, pdf, r1, r2);
+  }
+}
+
+struct Cam {
+  d3 lf, pc, dh, dv, bg;
+  int w, h;
+};
+
+struct Params {
+  uint64_t seed;
+  int spp, maxDepth, rrDepth, profile, sampleBase;
+  double rrProb;
+};
+
+struct Hit {
+  int id = -1;
+  double t = INF;
+};
+
+static Hit trace(const Scene &s, const d3 &o, const d3 &d) {
+  Hit h;
+  double t;
+  h.id = s.traceRef(o, d, 0.001, INF, t, nullptr);
+  h.t = t;
+  return h;
+}
+
+static OMat matOf(const Scene &s, const Prim &p) {
+  OMat m;
+  if (p.type == CPT_OBJ_LIGHT) {
+    m.color = p.le;
+    m.ax = m.ay = 1.0;
+    m.eta = 1.51;
+    m.type = CPT_M_LIGHT;
+    return m;
+  }
+  const cpt_material &c = s.materials[p.mat];
+  m.color = mk(c.color[0], c.color[1], c.color[2]);
+  m.ax = c.alpha_x;
+  m.ay = c.alpha_y;
+  m.eta = c.eta;
+  m.type = c.type;
+  return m;
+}
+
+// One sample of the bounce loop, Render.cu:42-219.  Returns the radiance.
+d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, uint32_t sample) {
+  Stream rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)(j * cam.w + i), sample, 0};
+  double rx = rng.next(), ry = rng.next();
+  d3 pixel = cam.pc + (rx - 0.5 + i) * cam.dh + (ry - 0.5 + j) * cam.dv;
+  d3 ro = cam.lf, rd = pixel - cam.lf;
+  d3 T = mk(1, 1, 1), L = mk(0, 0, 0);
+  int depth = 0;
+  bool hasSampleLight = false;
+  const int nl = s.nLights;
+  while (true) {
+    Hit hr = trace(s, ro, rd);
+    if (hr.id >= 0 && s.prims[hr.id].type != CPT_OBJ_LIGHT &&
+        s.materials[s.prims[hr.id].mat].type == CPT_M_VOLUME) {
+      const double sigmaMaj = 25.0;
+      d3 hp = ro + hr.t * rd;
+      ro = hp;
+      Hit vr = trace(s, ro, rd);
+      if (vr.id < 0) continue;
+      double dMax = vr.t;
+      double t = -std::log(1.0 - rng.next()) / sigmaMaj;
+      d3 pos = hp + t * rd;
+      ro = pos;
+      while (t < dMax) {
+        const double P_RR = 0.8;
+        if (rng.next() > P_RR) break;
+        T = T * (std::exp(-t * sigmaMaj) / P_RR);
+        double z = 1.0 - 2.0 * rng.next();
+        double r = std::sqrt(1.0 - z * z);
+        double phi = 2.0 * PI * rng.next();
+        d3 dir = Unit(mk(r * std::cos(phi), r * std::sin(phi), z));
+        ro = pos;
+        rd = dir;
+        pos = hp + t * dir;
+        Hit v2 = trace(s, ro, rd);
+        if (v2.id >= 0) dMax = v2.t;
+        t = -std::log(1.0 - rng.next()) / sigmaMaj;
+        if (t > dMax || v2.id < 0) break;
+      }
+      continue;
+    }
+    if (hr.id < 0) {
+      L = L + T * cam.bg;
+      break;
+    }
+    const Prim &p = s.prims[hr.id];
+    const OMat m = matOf(s, p);
+    d3 hp = ro + hr.t * rd;
+    if (m.type == CPT_M_LIGHT) {
+      if (!hasSampleLight) L = L + T * m.color;
+      break;
+    }
+    d3 outw;
+    if (p.type == CPT_OBJ_TRIANGLE) outw = Unit(Cross(p.p1 - p.p0, p.p2 - p.p0));
+    else outw = (hp - p.c) / p.r;
+    bool front = Dot(rd, outw) < 0;
+    d3 n = front ? outw : -outw;
+    const d3 zero = mk(0, 0, 0);
+    if (m.type == CPT_M_OPAQUE || m.type == CPT_M_LAMBERT || m.type == CPT_M_GGX) {
+      for (int k = 0; k < nl; k++) {
+        const Prim &lp = s.prims[k];
+        double lw = lp.w;
+        double lh = P.profile == CPT_PROFILE_REFERENCE ? lp.w : lp.h;  // Render.cu:170-171
+        double pdfLight = 1.0 / (lw * lh);
+        d3 lightPos = lp.c + (rng.next() - 0.5) * lw * lp.eu + (rng.next() - 0.5) * lh * lp.ev;
+        d3 dir = Unit(lightPos - hp);
+        Hit sh = trace(s, hp, dir);
+        if (Dot(dir, n) > 0.0 && Dot(-dir, lp.n) > 0.0 && sh.id == k) {
+          d3 c = lp.le * fr(m, m.color, rd, n, dir, zero, front) * Dot(dir, n) * Dot(-dir, lp.n) /
+                 SquaredLength(lightPos - hp) / pdfLight;
+          L = L + T * c;
+        }
+        hasSampleLight = true;
+      }
+    }
+    double P_RR = 1.0;
+    if (P.profile == CPT_PROFILE_REFERENCE) {
+      P_RR = 0.7;
+      if (rng.next() > P_RR) break;
+    } else {
+      if (depth + 1 >= P.maxDepth) break;
+      if (depth >= P.rrDepth) {
+        P_RR = P.rrProb;
+        if (rng.next() > P_RR) break;
+      }
+    }
+    depth++;
+    double r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
+    d3 dir, wm = zero;
+    double pdf = -1.0;
+    sampleLight(m, rd, n, dir, wm, pdf, r1, r2, r3, front);
+    if (pdf <= 0.0) break;
+    T = T * fr(m, m.color, rd, n, dir, wm, front) * std::fabs(Dot(dir, n)) / pdf / P_RR;
+    ro = hp;
+    rd = dir;
+  }
+  return L;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+void orc_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+  PhiloxOut o = philox4x32_10(ctr, k0, k1);
+  std::memcpy(out, o.v, 16);
+}
+
+// Renders the pixels [px0, px1) of the camera image (all when px1 <= px0):
+// rgb gets the mean radiance per pixel (h*w*3 doubles, row 0 = bottom);
+// per_sample (optional) gets each sample's radiance (npix*spp*3).
+int orc_render(void *sp, const cpt_camera *cam, const cpt_render_params *prm, double *rgb, double *perSample,
+               int nthreads) {
+  Scene *s = (Scene *)sp;
+  if (!s->built) return -1;
+  Cam c;
+  c.lf = mk(cam->look_from[0], cam->look_from[1], cam->look_from[2]);
+  c.pc = mk(cam->pixel_center[0], cam->pixel_center[1], cam->pixel_center[2]);
+  c.dh = mk(cam->delta_h[0], cam->delta_h[1], cam->delta_h[2]);
+  c.dv = mk(cam->delta_v[0], cam->delta_v[1], cam->delta_v[2]);
+  c.bg = mk(cam->background[0], cam->background[1], cam->background[2]);
+  c.w = cam->width;
+  c.h = cam->height;
+  Params P{prm->seed, prm->spp, prm->max_depth, prm->rr_depth, prm->profile, prm->sample_base, prm->rr_prob};
+  const int64_t npix = (int64_t)c.w * c.h;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t px = 0; px < npix; px++) {
+    int i = (int)(px % c.w), j = (int)(px / c.w);
+    d3 acc = mk(0, 0, 0);
+    for (int k = 0; k < P.spp; k++) {
+      d3 r = renderSample(*s, c, P, i, j, (uint32_t)(P.sampleBase + k));
+      if (perSample) {
+        double *o = perSample + 3 * (px * P.spp + k);
+        o[0] = r.x; o[1] = r.y; o[2] = r.z;
+      }
+      acc = acc + r;
+    }
+    acc = acc / (double)P.spp;
+    rgb[3 * px] = acc.x;
+    rgb[3 * px + 1] = acc.y;
+    rgb[3 * px + 2] = acc.z;
+  }
+  return 0;
+}
+
+// BSDF probes for closed-form tests: evaluates fr and one sample.
+int orc_bsdf(const cpt_material *mat, const double rayDir[3], const double normal[3], int frontFace,
+             const double r[3], double dirOut[3], double *pdfOut, double frOut[3]) {
+  OMat m;
+  m.color = mk(mat->color[0], mat->color[1], mat->color[2]);
+  m.ax = mat->alpha_x;
+  m.ay = mat->alpha_y;
+  m.eta = mat->eta;
+  m.type = mat->type;
+  d3 rd = mk(rayDir[0], rayDir[1], rayDir[2]);
+  d3 n = mk(normal[0], normal[1], normal[2]);
+  d3 dir, wm = mk(0, 0, 0);
+  double pdf = -1;
+  sampleLight(m, rd, n, dir, wm, pdf, r[0], r[1], r[2], frontFace != 0);
+  d3 f = fr(m, m.color, rd, n, dir, wm, frontFace != 0);
+  dirOut[0] = dir.x; dirOut[1] = dir.y; dirOut[2] = dir.z;
+  *pdfOut = pdf;
+  frOut[0] = f.x; frOut[1] = f.y; frOut[2] = f.z;
+  return 0;
+}
+
+// fr for a given outgoing direction (wm = 0).
+int orc_bsdf_eval(const cpt_material *mat, const double rayDir[3], const double normal[3], int frontFace,
+                  const double dir[3], double frOut[3]) {
+  OMat m;
+  m.color = mk(mat->color[0], mat->color[1], mat->color[2]);
+  m.ax = mat->alpha_x;
+  m.ay = mat->alpha_y;
+  m.eta = mat->eta;
+  m.type = mat->type;
+  d3 f = fr(m, m.color, mk(rayDir[0], rayDir[1], rayDir[2]), mk(normal[0], normal[1], normal[2]),
+            mk(dir[0], dir[1], dir[2]), mk(0, 0, 0), frontFace != 0);
+  frOut[0] = f.x; frOut[1] = f.y; frOut[2] = f.z;
+  return 0;
+}
+
+// GGX normal distribution D(h) for normal n (the pdf normalisation test).
+double orc_ggx_d(const cpt_material *mat, const double n[3], const double h[3]) {
+  OMat m;
+  m.ax = mat->alpha_x;
+  m.ay = mat->alpha_y;
+  return DistributionGGX(m, mk(n[0], n[1], n[2]), mk(h[0], h[1], h[2]));
+}
+
+}  // extern "C"

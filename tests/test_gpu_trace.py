@@ -1,0 +1,171 @@
+"""GPU closest-hit parity (traverseIterative, CudaPathTracing/BVH.cuh:282-332).
+
+* small scenes: brute-force FP64 oracle; on certified rays (every primitive's
+  accept/reject decision robust by a margin of 1e-4, runner-up hit farther
+  by more than 1e-4 relative) the hit ids are bit-exact, and the distance,
+  recomputed in FP64 for the winning primitive, is within 1e-6 relative.
+* large scenes: the oracle's reference-order traversal; ids equal except
+  near-ties, whose FP64 distances then agree to 1e-6.
+"""
+import numpy as np
+import pytest
+
+from cudapathtracing_amd import _lib as L
+from cudapathtracing_amd import scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+T_REL = 1e-6
+
+
+def camera_rays(desc, w, h, jitter_seed=0):
+    from cudapathtracing_amd import camera
+    cam = camera(desc)
+    rng = np.random.default_rng(jitter_seed)
+    i, j = np.meshgrid(np.arange(cam.width), np.arange(cam.height))
+    i, j = i.ravel(), j.ravel()
+    rx, ry = rng.random(i.size), rng.random(i.size)
+    pc, dh, dv, lf = (np.array(getattr(cam, f)[:]) for f in ("pixel_center", "delta_h", "delta_v", "look_from"))
+    pix = pc + (rx - 0.5 + i)[:, None] * dh + (ry - 0.5 + j)[:, None] * dv
+    d = pix - lf
+    r = np.zeros((i.size, 8), np.float32)
+    r[:, :3] = lf
+    r[:, 3] = 0.001
+    r[:, 4:7] = d
+    r[:, 7] = 999999.0
+    return r
+
+
+def box_rays(n, seed):
+    r = scenes.random_rays(n, seed=seed, lo=-0.45, hi=0.45)
+    return r
+
+
+def check_small(ctx, s, rays):
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    t, ids = ctx.trace(rays)
+    o = O.OracleScene(s)
+    tb, idb, cert = o.trace_brute(rays, eb=1e-4, et=1e-4)
+    print(f"  certified {cert.mean():.5f} of {len(rays)} rays, hits {np.mean(idb >= 0):.3f}")
+    assert cert.mean() > 0.95
+    bad = np.where(cert & (ids != idb))[0]
+    assert bad.size == 0, (bad[:10], ids[bad[:10]], idb[bad[:10]])
+    h = cert & (idb >= 0)
+    rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
+    assert rel.max() <= T_REL, rel.max()
+    miss = cert & (idb < 0)
+    assert np.all(t[miss] == rays[miss, 7])
+    return cert.mean()
+
+
+def test_cornell_camera_and_interior_rays(ctx):
+    print("[gpu] trace cornell", flush=True)
+    s = scenes.cornell_icosphere()
+    rays = np.concatenate([camera_rays(scenes.default_camera(128, 128), 128, 128), box_rays(20000, 3)])
+    check_small(ctx, s, rays)
+
+
+def test_scene3_spheres_floors_light(ctx):
+    from test_gpu_bvh import scene3_like
+    print("[gpu] trace scene3", flush=True)
+    s = scene3_like()
+    d = L.CameraDesc(L.d3((0.8, 0.8, 0.8)), L.d3((0.0, 5.0, 10.0)), L.d3((0.0, 2.0, 0.0)), 45.0, 16 / 9, 160, 0)
+    rays = camera_rays(d, 160, 90)
+    # rays towards the light from the floor
+    rng = np.random.default_rng(1)
+    n = 5000
+    o = np.stack([rng.uniform(-5, 5, n), np.zeros(n) + 0.01, rng.uniform(-5, 8, n)], 1)
+    tgt = np.stack([rng.uniform(-1.5, 1.5, n), rng.uniform(-1.5, 1.5, n), np.full(n, 10.0)], 1)
+    r2 = np.zeros((n, 8), np.float32)
+    r2[:, :3], r2[:, 3], r2[:, 4:7], r2[:, 7] = o, 0.001, tgt - o, 999999.0
+    check_small(ctx, s, np.concatenate([rays, r2]))
+
+
+def test_soup_brute_force(ctx):
+    print("[gpu] trace soup 20k brute force", flush=True)
+    s = scenes.SceneData()
+    m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    s.add_mesh(scenes.triangle_soup(20000, seed=5, mu=-4.0), m)
+    rays = scenes.random_rays(20000, seed=6)
+    check_small(ctx, s, rays)
+
+
+def test_axis_aligned_and_degenerate_rays(ctx):
+    print("[gpu] trace axis-aligned", flush=True)
+    s = scenes.cornell_icosphere()
+    rng = np.random.default_rng(2)
+    n = 6000
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = rng.uniform(-0.45, 0.45, (n, 3))
+    axis = rng.integers(0, 3, n)
+    sign = rng.choice([-1.0, 1.0], n)
+    r[np.arange(n), 4 + axis] = sign
+    r[:, 3], r[:, 7] = 0.001, 999999.0
+    r[:100, :3] = 0.0  # origin exactly at 0 with zero direction components
+    r[100:200, 7] = 0.01  # short tmax
+    check_small(ctx, s, r)
+
+
+def test_empty_and_tiny_scenes(ctx):
+    print("[gpu] trace empty/tiny", flush=True)
+    s = scenes.SceneData()
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    rays = scenes.random_rays(100, seed=1)
+    t, ids = ctx.trace(rays)
+    assert np.all(ids == -1) and np.all(t == rays[:, 7])
+    for n in (1, 2, 3):
+        s = scenes.SceneData()
+        m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+        s.add_mesh(scenes.triangle_soup(n, seed=9, mu=-1.0), m)
+        tgt = s.tri.reshape(-1, 3, 3).mean(1)
+        o = np.array([[-1.0, -1.0, -1.0]])
+        r = np.zeros((n, 8), np.float32)
+        r[:, :3], r[:, 3], r[:, 4:7], r[:, 7] = o, 0.001, tgt - o, 999999.0
+        check_small(ctx, s, r)
+
+
+def test_soup_1m_reference_traversal(ctx):
+    print("[gpu] trace soup 1M vs reference-order traversal", flush=True)
+    s = scenes.SceneData()
+    m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    s.add_mesh(scenes.triangle_soup(1_000_000, seed=1, mu=-5.0), m)
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    rays = scenes.random_rays(200_000, seed=2)
+    t, ids = ctx.trace(rays)
+    o = O.OracleScene(s)
+    o.build(s.bounds_min, s.bounds_max)
+    tr, idr, _ = o.trace_ref(rays)
+    same = ids == idr
+    print(f"  id agreement {same.mean():.6f}, hits {np.mean(idr >= 0):.3f}")
+    assert same.mean() > 0.999
+    h = same & (idr >= 0)
+    rel = np.abs(t[h].astype(np.float64) - tr[h]) / np.abs(tr[h])
+    assert rel.max() <= T_REL
+    # mismatches must be near-ties: the FP64 distance to the GPU's primitive
+    # equals the oracle's closest distance within 1e-6
+    mm = np.where(~same)[0]
+    if mm.size:
+        tg = o.prim_t(rays[mm], ids[mm])
+        ok = (ids[mm] >= 0) & (idr[mm] >= 0) & (np.abs(tg - tr[mm]) <= 1e-6 * np.abs(tr[mm]))
+        assert ok.all(), (mm[~ok][:5], ids[mm][~ok][:5], idr[mm][~ok][:5])
+
+
+def test_device_pointer_trace(ctx):
+    import torch
+    print("[gpu] trace with device pointers", flush=True)
+    s = scenes.cornell_icosphere()
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    rays = box_rays(10000, 11)
+    t0, i0 = ctx.trace(rays)
+    rt = torch.from_numpy(rays).cuda()
+    t1, i1 = ctx.trace(rt)
+    torch.cuda.synchronize()
+    ctx.set_stream(None)
+    assert np.array_equal(i1.cpu().numpy(), i0) and np.array_equal(t1.cpu().numpy(), t0)
+    nodes, prims = ctx.trace_count(rays)
+    assert nodes.min() >= 1 and prims.max() >= 1
