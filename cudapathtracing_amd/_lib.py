@@ -90,6 +90,8 @@ SIGNATURES = [
                                    C.POINTER(Sphere), C.c_int32, P, P, C.c_int64]),
     ("cpt_scene_count", C.c_int64, [P]),
     ("cpt_scene_load_ini", C.c_int, [P, C.c_char_p, dp, dp, C.POINTER(CameraDesc)]),
+    ("cpt_ini_counts", C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
+    ("cpt_ini_read", C.c_int, [C.c_char_p, P, P, P, P, P, dp, dp, C.POINTER(CameraDesc)]),
     ("cpt_bvh_build", C.c_int, [P, dp, dp, C.POINTER(BuildStats)]),
     ("cpt_bvh_export", C.c_int, [P, P, P, P, P, P, P]),
     ("cpt_bvh_morton", C.c_int, [P, dp, dp, P, C.c_int]),
@@ -101,6 +103,16 @@ SIGNATURES = [
                              C.POINTER(RenderStats), C.c_int]),
     ("cpt_camera_from_desc", C.c_int, [C.POINTER(CameraDesc), C.POINTER(Camera)]),
     ("cpt_tonemap_rgba8", C.c_int, [P, C.c_int32, C.c_int32, P]),
+    ("cpt_denoiser_create", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]),
+    ("cpt_denoiser_destroy", None, [P]),
+    ("cpt_denoiser_num_params", C.c_int64, [P]),
+    ("cpt_denoiser_get_params", C.c_int, [P, P]),
+    ("cpt_denoiser_set_params", C.c_int, [P, P]),
+    ("cpt_denoiser_layers", C.c_int, [P, P, P, P, P]),
+    ("cpt_denoiser_train_step", C.c_int, [P, P, P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P, P, C.c_int]),
+    ("cpt_denoise", C.c_int, [P, P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int]),
+    ("cpt_denoiser_stats", C.c_int, [P, P, P, P]),
+    ("cpt_conv3x3", C.c_int, [P, C.c_int, P, P, P, P, P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     ("cpt_gen_icosphere_count", C.c_int64, [C.c_int32]),
     ("cpt_gen_icosphere", C.c_int, [C.c_int32, dp, C.c_double, C.c_double, C.c_uint64, P]),
     ("cpt_gen_triangle_soup", C.c_int, [C.c_int64, C.c_uint64, C.c_double, C.c_double, P]),

@@ -1,0 +1,768 @@
+// denoise.hip - the learned denoiser: a 5-level convolutional U-Net
+// (channels 32, 64, 64, 96, 128; 3x3 convolutions, ReLU, 2x2 average
+// pooling, nearest upsampling with skip concatenation) that maps a
+// low-spp radiance image plus albedo, normal and depth to a denoised image.
+// Weights are BF16 (FP32 master copy and Adam moments), activations BF16,
+// accumulation FP32 on MFMA.  The network predicts log(1 + radiance) as a
+// residual on the noisy input's log(1 + radiance) and is trained with an L1
+// loss in that space.
+//
+// No reference counterpart: the reference's only filter is the joint
+// bilateral + temporal blend of Render.cu:353-460, which this replaces as
+// the learned method.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cpt_device.hpp"
+#include "denoise_kernels.hpp"
+#include "internal.hpp"
+
+namespace cpt {
+
+constexpr int kLevels = 5;
+constexpr int kChan[kLevels] = {32, 64, 64, 96, 128};
+constexpr int kInChan = 32;   // 10 used: log radiance (3), albedo (3), normal (3), log depth
+constexpr int kOutChan = 32;  // 3 used
+constexpr int kLayers = 19;
+
+struct LayerDesc {
+  int cin, cout, relu;
+};
+
+static void layerTable(LayerDesc L[kLayers]) {
+  int k = 0;
+  int prev = kInChan;
+  for (int l = 0; l < kLevels; l++) {
+    L[k++] = {prev, kChan[l], 1};
+    L[k++] = {kChan[l], kChan[l], 1};
+    prev = kChan[l];
+  }
+  for (int l = kLevels - 2; l >= 0; l--) {
+    L[k++] = {prev + kChan[l], kChan[l], 1};
+    L[k++] = {kChan[l], kChan[l], 1};
+    prev = kChan[l];
+  }
+  L[k++] = {kChan[0], kOutChan, 0};
+}
+
+// ---- elementwise kernels ----------------------------------------------------
+
+__global__ void k_prep_input(const float *__restrict__ in10, int h, int w, int H, int W, int64_t n,
+                             uint16_t *__restrict__ X, float *__restrict__ logRad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // padded pixel index
+  if (i >= n) return;
+  const int64_t per = (int64_t)H * W;
+  const int b = (int)(i / per);
+  const int rem = (int)(i % per);
+  const int y = rem / W, x = rem % W;
+  float v[kInChan];
+#pragma unroll
+  for (int c = 0; c < kInChan; c++) v[c] = 0.f;
+  if (y < h && x < w) {
+    const float *s = in10 + (((int64_t)b * h + y) * w + x) * 10;
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = log1pf(fmaxf(s[c], 0.f));
+#pragma unroll
+    for (int c = 3; c < 9; c++) v[c] = s[c];
+    v[9] = log1pf(fmaxf(s[9], 0.f));
+  }
+  uint16_t *d = X + i * kInChan;
+#pragma unroll
+  for (int c = 0; c < kInChan; c += 8) {
+    uint4 pk;
+    uint16_t *ph = (uint16_t *)&pk;
+#pragma unroll
+    for (int j = 0; j < 8; j++) ph[j] = f2bf(v[c + j]);
+    *(uint4 *)(d + c) = pk;
+  }
+  logRad[i * 3 + 0] = v[0];
+  logRad[i * 3 + 1] = v[1];
+  logRad[i * 3 + 2] = v[2];
+}
+
+__global__ void k_pool2(const uint16_t *__restrict__ X, int H, int W, int C, int64_t n, uint16_t *__restrict__ Y) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*(H/2)*(W/2)*C
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  int64_t p = i / C;
+  const int w2 = W / 2, h2 = H / 2;
+  const int x = (int)(p % w2);
+  p /= w2;
+  const int y = (int)(p % h2);
+  const int64_t b = p / h2;
+  const int64_t base = ((b * H + 2 * y) * W + 2 * x) * C + c;
+  float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (int64_t)W * C]) + bf2f(X[base + (int64_t)W * C + C]);
+  Y[i] = f2bf(0.25f * s);
+}
+
+// dX[2y+i][2x+j] (+)= dY[y][x] / 4
+__global__ void k_pool2_bwd(const uint16_t *__restrict__ dY, int H, int W, int C, int64_t n, uint16_t *__restrict__ dX,
+                            int accumulate) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*H*W*C (full res)
+  if (i >= n) return;
+  const int c = (int)(i % C);
+  int64_t p = i / C;
+  const int x = (int)(p % W);
+  p /= W;
+  const int y = (int)(p % H);
+  const int64_t b = p / H;
+  const float g = 0.25f * bf2f(dY[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]);
+  dX[i] = f2bf(accumulate ? bf2f(dX[i]) + g : g);
+}
+
+// Out[b,y,x] = [Low[b,y/2,x/2, 0:C1] | Skip[b,y,x, 0:C2]]
+__global__ void k_up_concat(const uint16_t *__restrict__ low, int C1, const uint16_t *__restrict__ skip, int C2, int H,
+                            int W, int64_t n, uint16_t *__restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*H*W*(C1+C2)
+  if (i >= n) return;
+  const int C = C1 + C2;
+  const int c = (int)(i % C);
+  int64_t p = i / C;
+  const int x = (int)(p % W);
+  int64_t q = p / W;
+  const int y = (int)(q % H);
+  const int64_t b = q / H;
+  out[i] = c < C1 ? low[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c] : skip[p * C2 + (c - C1)];
+}
+
+// dLow[y][x] = sum over the 2x2 block of dOut[..., 0:C1]; dSkip = dOut[..., C1:]
+__global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C2, int H, int W, int64_t nLow,
+                                uint16_t *__restrict__ dLow, uint16_t *__restrict__ dSkip) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*(H/2)*(W/2)*(C1 + 4*C2)
+  const int C = C1 + C2;
+  const int per = C1 + 4 * C2;
+  if (i >= nLow * per) return;
+  const int64_t lp = i / per;  // low-res pixel
+  const int k = (int)(i % per);
+  const int w2 = W / 2, h2 = H / 2;
+  const int x = (int)(lp % w2);
+  const int64_t q = lp / w2;
+  const int y = (int)(q % h2);
+  const int64_t b = q / h2;
+  if (k < C1) {
+    float s = 0.f;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) s += bf2f(dOut[((b * H + 2 * y + dy) * W + 2 * x + dx) * C + k]);
+    dLow[lp * C1 + k] = f2bf(s);
+  } else {
+    const int kk = k - C1;
+    const int sub = kk / C2, c = kk % C2;
+    const int64_t fp = (b * H + 2 * y + (sub >> 1)) * W + 2 * x + (sub & 1);
+    dSkip[fp * C2 + c] = dOut[fp * C + C1 + c];
+  }
+}
+
+__global__ void k_relu_mask(uint16_t *__restrict__ dY, const uint16_t *__restrict__ Y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!(bf2f(Y[i]) > 0.f)) dY[i] = 0;
+}
+
+// L1 loss on channels 0..2 of pred (ldp channels) against log(1+target);
+// dPred (bf16, kOutChan channels) = sign(pred - t) / count.
+__global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ target, int h, int w, int H,
+                     int W, int64_t n, float invCount, uint16_t *__restrict__ dPred, float *__restrict__ loss) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // padded pixel
+  float l = 0.f;
+  if (i < n) {
+    const int64_t per = (int64_t)H * W;
+    const int b = (int)(i / per), rem = (int)(i % per), y = rem / W, x = rem % W;
+    uint16_t g[kOutChan];
+#pragma unroll
+    for (int c = 0; c < kOutChan; c++) g[c] = 0;
+    if (y < h && x < w) {
+      const float *t = target + (((int64_t)b * h + y) * w + x) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const float d = pred[i * ldp + c] - log1pf(fmaxf(t[c], 0.f));
+        l += fabsf(d);
+        g[c] = f2bf(d > 0.f ? invCount : (d < 0.f ? -invCount : 0.f));
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kOutChan; c += 8) *(uint4 *)(dPred + i * kOutChan + c) = *(uint4 *)(g + c);
+  }
+  // block reduction of the loss
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] * invCount);
+}
+
+// db[c] += sum over pixels of dY[p][c]; block = 256 threads over a pixel range
+__global__ void k_bias_grad(const uint16_t *__restrict__ dY, int C, int64_t nPix, int pixPerBlock,
+                            float *__restrict__ db) {
+  __shared__ float s[256];
+  const int64_t p0 = (int64_t)blockIdx.x * pixPerBlock;
+  const int64_t p1 = p0 + pixPerBlock < nPix ? p0 + pixPerBlock : nPix;
+  for (int c0 = 0; c0 < C; c0 += 256 / 8) {
+    // thread t: channel c0 + t%32, pixels strided by 8
+    const int c = c0 + (threadIdx.x & 31);
+    float acc = 0.f;
+    if (c < C)
+      for (int64_t p = p0 + (threadIdx.x >> 5); p < p1; p += 8) acc += bf2f(dY[p * C + c]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 32 && c < C) {
+      float t = 0.f;
+      for (int k = 0; k < 8; k++) t += s[threadIdx.x + 32 * k];
+      atomicAdd(db + c, t);
+    }
+    __syncthreads();
+  }
+}
+
+// Adam on FP32 master weights; writes the BF16 weights (OHWI) and, for the
+// backward-data pass, the flipped transpose Wt[ci][2-ky][2-kx][co].
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       int update) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !update) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void k_pack_weights(const float *__restrict__ p, int cin, int cout, uint16_t *__restrict__ w,
+                               uint16_t *__restrict__ wt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over cout*9*cin
+  const int64_t n = (int64_t)cout * 9 * cin;
+  if (i >= n) return;
+  const int ci = (int)(i % cin);
+  const int t = (int)((i / cin) % 9);
+  const int co = (int)(i / ((int64_t)cin * 9));
+  const uint16_t h = f2bf(p[i]);
+  w[i] = h;
+  const int ky = t / 3, kx = t % 3;
+  wt[(((int64_t)ci * 3 + (2 - ky)) * 3 + (2 - kx)) * cout + co] = h;
+}
+
+__global__ void k_output(const float *__restrict__ pred, int ldp, int h, int w, int H, int W, int64_t n,
+                         float *__restrict__ rgb) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*h*w
+  if (i >= n) return;
+  const int64_t per = (int64_t)h * w;
+  const int b = (int)(i / per), rem = (int)(i % per), y = rem / w, x = rem % w;
+  const int64_t p = ((int64_t)b * H + y) * W + x;
+#pragma unroll
+  for (int c = 0; c < 3; c++) rgb[i * 3 + c] = fmaxf(expm1f(pred[p * ldp + c]), 0.f);
+}
+
+__global__ void k_zero_f32(float *__restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// ---- host-side network ------------------------------------------------------
+
+struct Tensor {
+  DevBuf buf;
+  int c = 0, h = 0, w = 0;  // per image
+  uint16_t *p() const { return buf.as<uint16_t>(); }
+};
+
+}  // namespace cpt
+
+struct cpt_denoiser {
+  cpt_ctx *ctx = nullptr;
+  cpt::LayerDesc L[cpt::kLayers];
+  int64_t wOff[cpt::kLayers], bOff[cpt::kLayers], nParams = 0;
+  int64_t wbOff[cpt::kLayers];  // bf16 weight offsets
+  cpt::DevBuf params, grads, m, v, wbf, wtbf;
+  int maxB = 0, maxH = 0, maxW = 0;
+  int B = 0, H = 0, W = 0;  // current padded shape
+  // activations (and their gradients) per image batch
+  cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], cat[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
+  cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
+  cpt::DevBuf pred, logRad, dPred, loss, io;
+  int64_t step = 0;
+  float lastFwdMs = 0.f, lastStepMs = 0.f;
+  double flopsFwd = 0;
+};
+
+namespace cpt {
+
+static int ensureT(cpt_ctx *ctx, Tensor &t, int B, int h, int w, int c) {
+  t.c = c;
+  t.h = h;
+  t.w = w;
+  return ensure(ctx, t.buf, (size_t)B * h * w * c * 2);
+}
+
+static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
+  cpt_ctx *ctx = dn->ctx;
+  int rc;
+  if ((rc = ensureT(ctx, dn->x0, B, H, W, kInChan)) || (rc = ensureT(ctx, dn->gx0, 1, 1, 1, 8))) return rc;
+  for (int l = 0; l < kLevels; l++) {
+    const int h = H >> l, w = W >> l;
+    for (int k = 0; k < 2; k++) {
+      if ((rc = ensureT(ctx, dn->e[l][k], B, h, w, kChan[l])) || (rc = ensureT(ctx, dn->ge[l][k], B, h, w, kChan[l])))
+        return rc;
+    }
+    if (l < kLevels - 1) {
+      if ((rc = ensureT(ctx, dn->pool[l], B, h / 2, w / 2, kChan[l])) ||
+          (rc = ensureT(ctx, dn->gpool[l], B, h / 2, w / 2, kChan[l])))
+        return rc;
+      const int below = l == kLevels - 2 ? kChan[kLevels - 1] : kChan[l + 1];
+      if ((rc = ensureT(ctx, dn->cat[l], B, h, w, below + kChan[l])) ||
+          (rc = ensureT(ctx, dn->gcat[l], B, h, w, below + kChan[l])))
+        return rc;
+      for (int k = 0; k < 2; k++)
+        if ((rc = ensureT(ctx, dn->d[l][k], B, h, w, kChan[l])) || (rc = ensureT(ctx, dn->gd[l][k], B, h, w, kChan[l])))
+          return rc;
+    }
+  }
+  const int64_t np = (int64_t)B * H * W;
+  if ((rc = ensure(ctx, dn->pred, np * kOutChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
+      (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)))
+    return rc;
+  return CPT_OK;
+}
+
+static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, int B, float *outF, uint16_t *outB, int mode,
+                   const float *res) {
+  const LayerDesc &Ld = dn->L[layer];
+  hipStream_t s = dn->ctx->stream;
+  const uint16_t *w = dn->wbf.as<uint16_t>() + dn->wbOff[layer];
+  const float *bias = dn->params.as<float>() + dn->bOff[layer];
+  const int H = in.h, W = in.w;
+  const int NT = Ld.cout >= 64 && Ld.cout % 64 == 0 ? 64 : (Ld.cout % 32 == 0 ? 32 : 16);
+  dim3 grid((W + kConvPix - 1) / kConvPix, H, B * ((Ld.cout + NT - 1) / NT));
+  if (NT == 64)
+    hipLaunchKernelGGL(k_conv_fwd<64>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
+                       outF, Ld.cout, mode, res, 3);
+  else if (NT == 32)
+    hipLaunchKernelGGL(k_conv_fwd<32>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
+                       outF, Ld.cout, mode, res, 3);
+  else
+    hipLaunchKernelGGL(k_conv_fwd<16>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
+                       outF, Ld.cout, mode, res, 3);
+  dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
+  return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+// Backward of one conv: dOut (bf16, already ReLU-masked) -> dW, db, dIn.
+static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const uint16_t *dOut, int B, uint16_t *dIn) {
+  const LayerDesc &Ld = dn->L[layer];
+  hipStream_t s = dn->ctx->stream;
+  const int H = in.h, W = in.w;
+  float *gw = dn->grads.as<float>() + dn->wOff[layer];
+  float *gb = dn->grads.as<float>() + dn->bOff[layer];
+  const int rows = H >= 64 ? 16 : (H >= 16 ? 8 : 4);
+  const int groups = (H + rows - 1) / rows;
+  hipLaunchKernelGGL(k_conv_bwd_w, dim3((Ld.cout + 63) / 64, Ld.cin / 32, B * groups), dim3(kConvThreads), 0, s, dOut,
+                     Ld.cout, in.p(), Ld.cin, H, W, rows, gw);
+  const int64_t nPix = (int64_t)B * H * W;
+  const int ppb = 2048;
+  hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((nPix + ppb - 1) / ppb)), dim3(256), 0, s, dOut, Ld.cout, nPix, ppb, gb);
+  if (dIn) {
+    // backward data: forward kernel on dOut with the flipped transposed weights
+    const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
+    const int cout = Ld.cin;  // gradient channels
+    const int NT = cout >= 64 && cout % 64 == 0 ? 64 : (cout % 32 == 0 ? 32 : 16);
+    dim3 grid((W + kConvPix - 1) / kConvPix, H, B * ((cout + NT - 1) / NT));
+    if (NT == 64)
+      hipLaunchKernelGGL(k_conv_fwd<64>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
+                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
+    else if (NT == 32)
+      hipLaunchKernelGGL(k_conv_fwd<32>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
+                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
+    else
+      hipLaunchKernelGGL(k_conv_fwd<16>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
+                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
+  }
+  return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+static int forward(cpt_denoiser *dn, int B) {
+  int rc = 0;
+  hipStream_t s = dn->ctx->stream;
+  int layer = 0;
+  const Tensor *cur = &dn->x0;
+  for (int l = 0; l < kLevels; l++) {
+    if (l > 0) {
+      const Tensor &src = dn->e[l - 1][1];
+      Tensor &dst = dn->pool[l - 1];
+      const int64_t n = (int64_t)B * dst.h * dst.w * dst.c;
+      hipLaunchKernelGGL(k_pool2, dim3(nb256(n)), dim3(256), 0, s, src.p(), src.h, src.w, src.c, n, dst.p());
+      cur = &dst;
+    }
+    if ((rc = convFwd(dn, layer++, *cur, B, nullptr, dn->e[l][0].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->e[l][0], B, nullptr, dn->e[l][1].p(), 1, nullptr))) return rc;
+    cur = &dn->e[l][1];
+  }
+  for (int l = kLevels - 2; l >= 0; l--) {
+    Tensor &cat = dn->cat[l];
+    const int64_t n = (int64_t)B * cat.h * cat.w * cat.c;
+    hipLaunchKernelGGL(k_up_concat, dim3(nb256(n)), dim3(256), 0, s, cur->p(), cur->c, dn->e[l][1].p(), kChan[l], cat.h,
+                       cat.w, n, cat.p());
+    if ((rc = convFwd(dn, layer++, cat, B, nullptr, dn->d[l][0].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->d[l][0], B, nullptr, dn->d[l][1].p(), 1, nullptr))) return rc;
+    cur = &dn->d[l][1];
+  }
+  return convFwd(dn, layer, *cur, B, dn->pred.as<float>(), nullptr, 2, dn->logRad.as<float>());
+}
+
+static void reluMask(hipStream_t s, Tensor &g, const Tensor &y, int B) {
+  const int64_t n = (int64_t)B * y.h * y.w * y.c;
+  hipLaunchKernelGGL(k_relu_mask, dim3(nb256(n)), dim3(256), 0, s, g.p(), y.p(), n);
+}
+
+static int backward(cpt_denoiser *dn, int B) {
+  int rc;
+  hipStream_t s = dn->ctx->stream;
+  int layer = kLayers - 1;
+  // output conv: input d[0][1], gradient dPred (already d loss / d conv output)
+  if ((rc = convBwd(dn, layer--, dn->d[0][1], dn->dPred.as<uint16_t>(), B, dn->gd[0][1].p()))) return rc;
+  for (int l = 0; l <= kLevels - 2; l++) {
+    // decoder level l: d[l][1] = conv(d[l][0]), d[l][0] = conv(cat[l])
+    reluMask(s, dn->gd[l][1], dn->d[l][1], B);
+    if ((rc = convBwd(dn, layer--, dn->d[l][0], dn->gd[l][1].p(), B, dn->gd[l][0].p()))) return rc;
+    reluMask(s, dn->gd[l][0], dn->d[l][0], B);
+    if ((rc = convBwd(dn, layer--, dn->cat[l], dn->gd[l][0].p(), B, dn->gcat[l].p()))) return rc;
+    // split the concat gradient: upsampled part -> gradient of the level below
+    Tensor &low = l == kLevels - 2 ? dn->ge[kLevels - 1][1] : dn->gd[l + 1][1];
+    const int C1 = low.c, C2 = kChan[l];
+    const int64_t nLow = (int64_t)B * low.h * low.w;
+    hipLaunchKernelGGL(k_up_concat_bwd, dim3(nb256(nLow * (C1 + 4 * C2))), dim3(256), 0, s, dn->gcat[l].p(), C1, C2,
+                       dn->cat[l].h, dn->cat[l].w, nLow, low.p(), dn->ge[l][1].p());
+  }
+  // encoder, deepest first; layer index now points at enc level 4 conv 1
+  for (int l = kLevels - 1; l >= 0; l--) {
+    reluMask(s, dn->ge[l][1], dn->e[l][1], B);
+    if ((rc = convBwd(dn, layer--, dn->e[l][0], dn->ge[l][1].p(), B, dn->ge[l][0].p()))) return rc;
+    reluMask(s, dn->ge[l][0], dn->e[l][0], B);
+    const Tensor &in = l == 0 ? dn->x0 : dn->pool[l - 1];
+    uint16_t *gin = l == 0 ? nullptr : dn->gpool[l - 1].p();
+    if ((rc = convBwd(dn, layer--, in, dn->ge[l][0].p(), B, gin))) return rc;
+    if (l > 0) {
+      // average-pool backward adds into the skip gradient of level l-1
+      Tensor &g = dn->ge[l - 1][1];
+      const int64_t n = (int64_t)B * g.h * g.w * g.c;
+      hipLaunchKernelGGL(k_pool2_bwd, dim3(nb256(n)), dim3(256), 0, s, dn->gpool[l - 1].p(), g.h, g.w, g.c, n, g.p(), 1);
+    }
+  }
+  return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+static int packAll(cpt_denoiser *dn) {
+  hipStream_t s = dn->ctx->stream;
+  for (int l = 0; l < kLayers; l++) {
+    const int64_t n = (int64_t)dn->L[l].cout * 9 * dn->L[l].cin;
+    hipLaunchKernelGGL(k_pack_weights, dim3(nb256(n)), dim3(256), 0, s, dn->params.as<float>() + dn->wOff[l], dn->L[l].cin,
+                       dn->L[l].cout, dn->wbf.as<uint16_t>() + dn->wbOff[l], dn->wtbf.as<uint16_t>() + dn->wbOff[l]);
+  }
+  return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+static int prepare(cpt_denoiser *dn, const float *in10Dev, int b, int h, int w) {
+  const int H = (h + 15) / 16 * 16, W = (w + 15) / 16 * 16;
+  if (b > dn->maxB || H > dn->maxH || W > dn->maxW) return fail(dn->ctx, CPT_ERR_ARG, "denoiser", "batch or image larger than created");
+  int rc = allocActs(dn, b, H, W);
+  if (rc) return rc;
+  dn->B = b;
+  dn->H = H;
+  dn->W = W;
+  const int64_t n = (int64_t)b * H * W;
+  hipLaunchKernelGGL(k_prep_input, dim3(nb256(n)), dim3(256), 0, dn->ctx->stream, in10Dev, h, w, H, W, n, dn->x0.p(),
+                     dn->logRad.as<float>());
+  return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+// Stages host inputs in the denoiser's io buffer; returns device pointers.
+static int stage(cpt_denoiser *dn, const float *a, size_t na, const float *b, size_t nb, int flags, const float **da,
+                 const float **db, float **out, size_t nout) {
+  if (flags & CPT_DEVICE_PTRS) {
+    *da = a;
+    if (db) *db = b;
+    return CPT_OK;
+  }
+  int rc = ensure(dn->ctx, dn->io, (na + nb + nout) * 4);
+  if (rc) return rc;
+  float *base = dn->io.as<float>();
+  CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(base, a, na * 4, hipMemcpyHostToDevice, dn->ctx->stream));
+  *da = base;
+  if (b && db) {
+    CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(base + na, b, nb * 4, hipMemcpyHostToDevice, dn->ctx->stream));
+    *db = base + na;
+  }
+  if (out) *out = base + na + nb;
+  return CPT_OK;
+}
+
+}  // namespace cpt
+
+using namespace cpt;
+
+extern "C" {
+
+int cpt_denoiser_create(cpt_ctx *ctx, int32_t maxB, int32_t maxH, int32_t maxW, uint64_t seed, cpt_denoiser **out) {
+  if (!ctx || !out || maxB <= 0 || maxH <= 0 || maxW <= 0) return CPT_ERR_ARG;
+  *out = nullptr;
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  cpt_denoiser *dn = new cpt_denoiser();
+  dn->ctx = ctx;
+  dn->maxB = maxB;
+  dn->maxH = (maxH + 15) / 16 * 16;
+  dn->maxW = (maxW + 15) / 16 * 16;
+  layerTable(dn->L);
+  int64_t off = 0, boff = 0;
+  for (int l = 0; l < kLayers; l++) {
+    dn->wOff[l] = off;
+    off += (int64_t)dn->L[l].cout * 9 * dn->L[l].cin;
+    dn->bOff[l] = off;
+    off += dn->L[l].cout;
+    dn->wbOff[l] = boff;
+    boff += (int64_t)dn->L[l].cout * 9 * dn->L[l].cin;
+  }
+  dn->nParams = off;
+  int rc;
+  if ((rc = ensure(ctx, dn->params, off * 4)) || (rc = ensure(ctx, dn->grads, off * 4)) || (rc = ensure(ctx, dn->m, off * 4)) ||
+      (rc = ensure(ctx, dn->v, off * 4)) || (rc = ensure(ctx, dn->wbf, boff * 2)) || (rc = ensure(ctx, dn->wtbf, boff * 2))) {
+    cpt_denoiser_destroy(dn);
+    return rc;
+  }
+  // He-normal initialisation from a counter-based stream (Box-Muller)
+  std::vector<float> p(off, 0.f);
+  for (int l = 0; l < kLayers; l++) {
+    const int64_t n = (int64_t)dn->L[l].cout * 9 * dn->L[l].cin;
+    const double sd = std::sqrt(2.0 / (9.0 * dn->L[l].cin)) * (l == kLayers - 1 ? 0.1 : 1.0);
+    for (int64_t i = 0; i < n; i += 2) {
+      u4 r = philox4x32_10(u4{(uint32_t)i, (uint32_t)(i >> 32), (uint32_t)l, 0x574e4954u}, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const double u1 = ((r.x >> 8) + 1) * (1.0 / 16777216.0), u2 = (r.y >> 8) * (1.0 / 16777216.0);
+      const double rad = std::sqrt(-2.0 * std::log(u1));
+      p[dn->wOff[l] + i] = (float)(sd * rad * std::cos(6.283185307179586 * u2));
+      if (i + 1 < n) p[dn->wOff[l] + i + 1] = (float)(sd * rad * std::sin(6.283185307179586 * u2));
+    }
+  }
+  hipStream_t s = ctx->stream;
+  if (hipMemcpyAsync(dn->params.p, p.data(), off * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemsetAsync(dn->m.p, 0, off * 4, s) != hipSuccess || hipMemsetAsync(dn->v.p, 0, off * 4, s) != hipSuccess ||
+      packAll(dn) != CPT_OK || hipStreamSynchronize(s) != hipSuccess) {
+    cpt_denoiser_destroy(dn);
+    return fail(ctx, CPT_ERR_HIP, "cpt_denoiser_create", "upload failed");
+  }
+  *out = dn;
+  return CPT_OK;
+}
+
+void cpt_denoiser_destroy(cpt_denoiser *dn) {
+  if (!dn) return;
+  cpt_ctx *ctx = dn->ctx;
+  hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
+                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf};
+  for (DevBuf *b : bufs)
+    if (b->p) hipFree(b->p);
+  for (int l = 0; l < kLevels; l++)
+    for (int k = 0; k < 2; k++) {
+      if (dn->e[l][k].buf.p) hipFree(dn->e[l][k].buf.p);
+      if (dn->ge[l][k].buf.p) hipFree(dn->ge[l][k].buf.p);
+    }
+  for (int l = 0; l < kLevels - 1; l++) {
+    Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->cat[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
+    for (Tensor *t : ts)
+      if (t->buf.p) hipFree(t->buf.p);
+  }
+  delete dn;
+}
+
+int64_t cpt_denoiser_num_params(const cpt_denoiser *dn) { return dn ? dn->nParams : -1; }
+
+int cpt_denoiser_get_params(cpt_denoiser *dn, float *out) {
+  if (!dn || !out) return CPT_ERR_ARG;
+  CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(out, dn->params.p, dn->nParams * 4, hipMemcpyDeviceToHost, dn->ctx->stream));
+  CPT_TRY_HIP(dn->ctx, hipStreamSynchronize(dn->ctx->stream));
+  return CPT_OK;
+}
+
+int cpt_denoiser_set_params(cpt_denoiser *dn, const float *in) {
+  if (!dn || !in) return CPT_ERR_ARG;
+  CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(dn->params.p, in, dn->nParams * 4, hipMemcpyHostToDevice, dn->ctx->stream));
+  int rc = packAll(dn);
+  if (rc) return fail(dn->ctx, rc, "cpt_denoiser_set_params", "pack");
+  CPT_TRY_HIP(dn->ctx, hipStreamSynchronize(dn->ctx->stream));
+  return CPT_OK;
+}
+
+int cpt_denoiser_layers(const cpt_denoiser *dn, int32_t *cin, int32_t *cout, int64_t *woff, int64_t *boff) {
+  if (!dn) return -1;
+  for (int l = 0; l < kLayers; l++) {
+    if (cin) cin[l] = dn->L[l].cin;
+    if (cout) cout[l] = dn->L[l].cout;
+    if (woff) woff[l] = dn->wOff[l];
+    if (boff) boff[l] = dn->bOff[l];
+  }
+  return kLayers;
+}
+
+// Forward + loss + backward; grads_out (optional, host or device) receives the
+// gradient of every parameter; with lr > 0 an Adam step follows.
+int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *target, int32_t b, int32_t h, int32_t w,
+                            float lr, float *loss_out, float *grads_out, int flags) {
+  if (!dn || !in10 || !target || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
+  cpt_ctx *ctx = dn->ctx;
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t npx = (size_t)b * h * w;
+  const float *dIn = nullptr, *dT = nullptr;
+  int rc = stage(dn, in10, npx * 10, target, npx * 3, flags, &dIn, &dT, nullptr, 0);
+  if (rc) return rc;
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], s));
+  if ((rc = prepare(dn, dIn, b, h, w))) return rc;
+  dn->flopsFwd = 0;
+  if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
+  hipLaunchKernelGGL(k_zero_f32, dim3(nb256(dn->nParams)), dim3(256), 0, s, dn->grads.as<float>(), dn->nParams);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)1);
+  const int64_t np = (int64_t)b * dn->H * dn->W;
+  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kOutChan, dT, h, w, dn->H, dn->W, np,
+                     1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->loss.as<float>());
+  if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
+  if (lr > 0.f) {
+    dn->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)dn->step), c2 = 1.f - powf(b2, (float)dn->step);
+    hipLaunchKernelGGL(k_adam, dim3(nb256(dn->nParams)), dim3(256), 0, s, dn->params.as<float>(), dn->grads.as<float>(),
+                       dn->m.as<float>(), dn->v.as<float>(), dn->nParams, lr, b1, b2, 1e-8f, c1, c2, 1);
+    if ((rc = packAll(dn))) return fail(ctx, rc, "denoiser adam", "launch");
+  }
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
+  if (grads_out) {
+    if (flags & CPT_DEVICE_PTRS)
+      CPT_TRY_HIP(ctx, hipMemcpyAsync(grads_out, dn->grads.p, dn->nParams * 4, hipMemcpyDeviceToDevice, s));
+    else
+      CPT_TRY_HIP(ctx, hipMemcpyAsync(grads_out, dn->grads.p, dn->nParams * 4, hipMemcpyDeviceToHost, s));
+  }
+  if (loss_out) {
+    CPT_TRY_HIP(ctx, hipMemcpyAsync(loss_out, dn->loss.p, 4, hipMemcpyDeviceToHost, s));
+    CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&dn->lastStepMs, ctx->ev[6], ctx->ev[7]));
+  }
+  return CPT_OK;
+}
+
+// Denoised radiance for b images: in10 = B*h*w*10 floats (rgb, albedo,
+// normal, depth), out_rgb = B*h*w*3.
+int cpt_denoise(cpt_denoiser *dn, const float *in10, float *outRgb, int32_t b, int32_t h, int32_t w, int flags) {
+  if (!dn || !in10 || !outRgb || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
+  cpt_ctx *ctx = dn->ctx;
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t npx = (size_t)b * h * w;
+  const float *dIn = nullptr;
+  float *dOut = nullptr;
+  int rc = stage(dn, in10, npx * 10, nullptr, 0, flags, &dIn, nullptr, &dOut, npx * 3);
+  if (rc) return rc;
+  if (flags & CPT_DEVICE_PTRS) dOut = outRgb;
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], s));
+  if ((rc = prepare(dn, dIn, b, h, w))) return rc;
+  dn->flopsFwd = 0;
+  if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
+  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kOutChan, h, w, dn->H, dn->W,
+                     (int64_t)npx, dOut);
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
+  if (!(flags & CPT_DEVICE_PTRS)) CPT_TRY_HIP(ctx, hipMemcpyAsync(outRgb, dOut, npx * 12, hipMemcpyDeviceToHost, s));
+  CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+  CPT_TRY_HIP(ctx, hipEventElapsedTime(&dn->lastFwdMs, ctx->ev[6], ctx->ev[7]));
+  return CPT_OK;
+}
+
+// Timing and work of the last call: ms of the last cpt_denoise / train step
+// and the forward FLOPs of one pass.
+int cpt_denoiser_stats(const cpt_denoiser *dn, float *fwd_ms, float *step_ms, double *fwd_flops) {
+  if (!dn) return CPT_ERR_ARG;
+  if (fwd_ms) *fwd_ms = dn->lastFwdMs;
+  if (step_ms) *step_ms = dn->lastStepMs;
+  if (fwd_flops) *fwd_flops = dn->flopsFwd;
+  return CPT_OK;
+}
+
+// One 3x3 convolution on caller data (testing / reuse): mode 0 forward
+// (y = conv(x, w) + bias, fp32 out [B,H,W,Cout]), mode 1 backward-weight
+// (dw[Cout*9*Cin] fp32 = sum dy x), mode 2 backward-data (dx fp32
+// [B,H,W,Cin] from dy and w).  x, w, dy are bf16 bit patterns.
+int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, const float *bias, const uint16_t *dy,
+                float *out, int32_t B, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
+  if (!ctx || B <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || Cin % 32 || Cout % 32 || mode < 0 || mode > 2)
+    return fail(ctx, CPT_ERR_ARG, "cpt_conv3x3", "channels must be multiples of 32");
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const size_t nx = (size_t)B * H * W * Cin, ny = (size_t)B * H * W * Cout, nw = (size_t)Cout * 9 * Cin;
+  DevBuf dx, dw, dyb, db, dout, wt;
+  int rc = 0;
+  auto cleanup = [&]() {
+    DevBuf *all[] = {&dx, &dw, &dyb, &db, &dout, &wt};
+    for (DevBuf *b : all)
+      if (b->p) hipFree(b->p);
+  };
+  size_t nout = mode == 0 ? ny : (mode == 1 ? nw : nx);
+  if ((rc = ensure(ctx, dx, nx * 2)) || (rc = ensure(ctx, dw, nw * 2)) || (rc = ensure(ctx, dyb, ny * 2)) ||
+      (rc = ensure(ctx, db, Cout * 4)) || (rc = ensure(ctx, dout, nout * 4)) || (rc = ensure(ctx, wt, nw * 2))) {
+    cleanup();
+    return rc;
+  }
+  if (x) hipMemcpyAsync(dx.p, x, nx * 2, hipMemcpyHostToDevice, s);
+  if (w) hipMemcpyAsync(dw.p, w, nw * 2, hipMemcpyHostToDevice, s);
+  if (dy) hipMemcpyAsync(dyb.p, dy, ny * 2, hipMemcpyHostToDevice, s);
+  if (bias) hipMemcpyAsync(db.p, bias, Cout * 4, hipMemcpyHostToDevice, s);
+  else hipMemsetAsync(db.p, 0, Cout * 4, s);
+  const unsigned gx = (W + kConvPix - 1) / kConvPix;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cout / 32)), dim3(kConvThreads), 0, s, dx.as<uint16_t>(), Cin,
+                       dw.as<uint16_t>(), db.as<float>(), Cout, H, W, (uint16_t *)nullptr, dout.as<float>(), Cout, 0,
+                       (const float *)nullptr, 3);
+  } else if (mode == 1) {
+    hipMemsetAsync(dout.p, 0, nout * 4, s);
+    const int rows = 4, groups = (H + rows - 1) / rows;
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3((Cout + 63) / 64, Cin / 32, B * groups), dim3(kConvThreads), 0, s,
+                       dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, H, W, rows, dout.as<float>());
+  } else {
+    // flipped transpose on the host side of the device: reuse k_pack_weights
+    // through an fp32 staging copy of w
+    DevBuf wf;
+    if ((rc = ensure(ctx, wf, nw * 4))) {
+      cleanup();
+      return rc;
+    }
+    std::vector<uint16_t> hw(nw);
+    std::vector<float> hf(nw);
+    if (w) std::memcpy(hw.data(), w, nw * 2);
+    for (size_t i = 0; i < nw; i++) {
+      uint32_t u = ((uint32_t)hw[i]) << 16;
+      std::memcpy(&hf[i], &u, 4);
+    }
+    hipMemcpyAsync(wf.p, hf.data(), nw * 4, hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(k_pack_weights, dim3(nb256(nw)), dim3(256), 0, s, wf.as<float>(), Cin, Cout, dw.as<uint16_t>(),
+                       wt.as<uint16_t>());
+    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cin / 32)), dim3(kConvThreads), 0, s, dyb.as<uint16_t>(), Cout,
+                       wt.as<uint16_t>(), (const float *)nullptr, Cin, H, W, (uint16_t *)nullptr, dout.as<float>(), Cin,
+                       0, (const float *)nullptr, 3);
+    hipStreamSynchronize(s);
+    hipFree(wf.p);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout.p, nout * 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  cleanup();
+  if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_conv3x3", hipGetErrorString(e));
+  return CPT_OK;
+}
+
+}  // extern "C"

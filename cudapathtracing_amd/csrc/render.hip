@@ -31,6 +31,7 @@ constexpr float kRayTMax = 999999.0f;     // INF, rtmath.cuh:8
 constexpr uint32_t kFlagNee = 1u << 16;   // hasSampleLight (Render.cu:56)
 constexpr uint32_t kFlagAov = 1u << 17;   // first-vertex AOV written
 constexpr uint32_t kDepthMask = 0xffffu;
+constexpr int kMaxBounces = 4096;
 
 struct RenderBufs {
   int64_t cap = 0, shadowCap = 0;
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
           float t = -__logf(1.0f - rng.next()) / sig;
           f3 pos = hp + d * t;
           o = pos;
-          while (t < dMax) {
+          for (int guard = 0; t < dMax && guard < 100000; guard++) {
             if (rng.next() > 0.8f) break;
             T = T * (__expf(-t * sig) / 0.8f);
             float z = 1.0f - 2.0f * rng.next();
@@ -341,14 +342,14 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   }
 }
 
-__global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow,
+__global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow, int64_t shadowCap,
                                                             const uint32_t *__restrict__ counters,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
                                                             float *__restrict__ fb) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
-  if (k >= counters[1]) return;
+  if ((int64_t)k >= shadowCap || k >= counters[1]) return;
   const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
   float tb = r1.w;
@@ -458,7 +459,9 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     CPT_TRY_HIP(ctx, hipGetLastError());
     uint32_t qn = (uint32_t)count;
     uint32_t *qa = R.qA.as<uint32_t>(), *qb = R.qB.as<uint32_t>();
-    while (qn > 0) {
+    // paths end by roulette, depth cap, miss or absorption; the cap only
+    // bounds pathological inputs (NaN geometry) so the loop always ends
+    for (int pass = 0; qn > 0 && pass < kMaxBounces; pass++) {
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
       hipLaunchKernelGGL(k_extend, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
@@ -470,7 +473,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
-                         (const float4 *)R.shadow.as<float4>(), (const uint32_t *)cnt, a.nodes, a.leaves,
+                         (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, a.nodes, a.leaves,
                          a.nPrims, R.fb.as<float>());
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
@@ -524,8 +527,4 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     stats->bounces = bounces;
   }
   return CPT_OK;
-}
-
-extern "C" int cpt_scene_load_ini(cpt_ctx *ctx, const char *, double *, double *, cpt_camera_desc *) {
-  return cpt::fail(ctx, CPT_ERR_IO, "cpt_scene_load_ini", "not built yet");
 }

@@ -80,6 +80,37 @@ class SceneData:
         self.tri_mat = np.ascontiguousarray(np.concatenate([self.tri_mat, m]))
 
 
+def load_ini(path: str) -> SceneData:
+    """IniParser::Parse + Scene::init (IniParser.cpp:3-96, Scene.cuh:72-145)
+    on the host: objects in the reference's order, [Mesh] OBJ files loaded
+    with the reference's rotate/scale/bounds/translate rule."""
+    lib = L.lib()
+    cnt = (C.c_int64 * 4)()
+    rc = lib.cpt_ini_counts(path.encode(), cnt)
+    if rc:
+        raise OSError(f"cannot read scene file {path} ({rc})")
+    nm, nl, ns, nt = (int(x) for x in cnt)
+    mats = (L.Material * max(nm, 1))()
+    lights = (L.Light * max(nl, 1))()
+    sph = (L.Sphere * max(ns, 1))()
+    tri = np.empty((nt, 9), np.float64)
+    tm = np.empty(nt, np.int32)
+    mn, mx = (C.c_double * 3)(), (C.c_double * 3)()
+    cam = L.CameraDesc()
+    rc = lib.cpt_ini_read(path.encode(), mats, lights, sph, tri.ctypes.data, tm.ctypes.data, mn, mx,
+                          C.byref(cam))
+    if rc:
+        raise OSError(f"cannot read scene file {path} ({rc})")
+    s = SceneData()
+    s.materials = list(mats[:nm])
+    s.lights = list(lights[:nl])
+    s.spheres = list(sph[:ns])
+    s.tri, s.tri_mat = tri, tm
+    s.bounds_min, s.bounds_max = np.array(mn[:]), np.array(mx[:])
+    s.camera = cam
+    return s
+
+
 def icosphere(level: int, center=(0.0, 0.0, 0.0), radius=1.0, amp=0.0, seed=0) -> np.ndarray:
     lib = L.lib()
     n = lib.cpt_gen_icosphere_count(level)

@@ -155,6 +155,17 @@ int64_t cpt_scene_count(const cpt_ctx *ctx);
 int cpt_scene_load_ini(cpt_ctx *ctx, const char *path, double scene_min[3],
                        double scene_max[3], cpt_camera_desc *camera_out);
 
+/* Host-only parse of a scene_ini file into caller buffers (no device
+ * work).  cpt_ini_counts gives {materials, lights, spheres, triangles};
+ * cpt_ini_read fills arrays of those sizes (any pointer may be NULL) plus the
+ * Morton bounds and the camera.  Returns CPT_ERR_IO for an unreadable or
+ * malformed file. */
+int cpt_ini_counts(const char *path, int64_t counts[4]);
+int cpt_ini_read(const char *path, cpt_material *materials, cpt_light *lights,
+                 cpt_sphere *spheres, double *tri_xyz, int32_t *tri_mat,
+                 double scene_min[3], double scene_max[3],
+                 cpt_camera_desc *camera_out);
+
 /* ---- LBVH --------------------------------------------------------------- */
 typedef struct cpt_build_stats {
   float morton_ms;
@@ -258,6 +269,44 @@ int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
  * uchar(254.99*clamp01(x)); rgba8 gets h*w*4 bytes.  Host pointers. */
 int cpt_tonemap_rgba8(const float *rgb, int32_t width, int32_t height,
                       uint8_t *rgba8);
+
+/* ---- learned denoiser (no reference counterpart; replaces the role of the
+ * joint bilateral filter of Render.cu:353-460) ---------------------------- */
+/* 5-level U-Net (channels 32,64,64,96,128; 3x3 conv + ReLU, 2x2 average
+ * pooling, nearest upsampling + skip concatenation), BF16 weights and
+ * activations, FP32 accumulation on MFMA, FP32 master weights + Adam.
+ * Input per pixel: 10 floats (radiance rgb, albedo rgb, normal xyz, depth);
+ * output: denoised radiance rgb.  Images are padded to multiples of 16. */
+typedef struct cpt_denoiser cpt_denoiser;
+int cpt_denoiser_create(cpt_ctx *ctx, int32_t max_batch, int32_t max_h,
+                        int32_t max_w, uint64_t seed, cpt_denoiser **out);
+void cpt_denoiser_destroy(cpt_denoiser *dn);
+int64_t cpt_denoiser_num_params(const cpt_denoiser *dn);
+int cpt_denoiser_get_params(cpt_denoiser *dn, float *out);
+int cpt_denoiser_set_params(cpt_denoiser *dn, const float *in);
+/* Layer table: per layer Cin, Cout and the offsets of its weights
+ * ([Cout][3][3][Cin]) and biases in the packed parameter vector; returns the
+ * number of layers. */
+int cpt_denoiser_layers(const cpt_denoiser *dn, int32_t *cin, int32_t *cout,
+                        int64_t *w_off, int64_t *b_off);
+/* Forward, L1 loss on log(1+radiance), backward; Adam step when lr > 0.
+ * loss_out / grads_out (packed like the parameters) may be NULL. */
+int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10,
+                            const float *target_rgb, int32_t b, int32_t h,
+                            int32_t w, float lr, float *loss_out,
+                            float *grads_out, int flags);
+int cpt_denoise(cpt_denoiser *dn, const float *in10, float *out_rgb, int32_t b,
+                int32_t h, int32_t w, int flags);
+/* ms of the last cpt_denoise and train step; forward FLOPs of one pass. */
+int cpt_denoiser_stats(const cpt_denoiser *dn, float *fwd_ms, float *step_ms,
+                       double *fwd_flops);
+/* One 3x3 'same' convolution on host bf16 data (bit patterns), Cin and Cout
+ * multiples of 32.  mode 0: out[B,H,W,Cout] = conv(x, w) + bias; mode 1:
+ * out[Cout,3,3,Cin] = weight gradient from dy and x; mode 2: out[B,H,W,Cin]
+ * = input gradient from dy and w. */
+int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w,
+                const float *bias, const uint16_t *dy, float *out, int32_t B,
+                int32_t H, int32_t W, int32_t Cin, int32_t Cout);
 
 /* ---- procedural scenes (host, no reference counterpart) ---------------- */
 /* Icosphere with 20*4^level triangles of radius r around c. */

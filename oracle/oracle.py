@@ -67,7 +67,56 @@ RENDER_SIGS = {
     "orc_bsdf": (C.c_int, [P, dp, dp, C.c_int, dp, dp, dp, dp]),
     "orc_bsdf_eval": (C.c_int, [P, dp, dp, C.c_int, dp, dp]),
     "orc_ggx_d": (C.c_double, [P, dp, dp]),
+    "orc_conv3x3_fwd": (None, [P, P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_conv3x3_bwd_w": (None, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_conv3x3_bwd_x": (None, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "orc_unet_num_params": (C.c_int64, []),
+    "orc_unet_grad": (C.c_double, [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P]),
 }
+
+
+def conv3x3_fwd(x, w, bias):
+    B, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    x = np.ascontiguousarray(x, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    b = np.ascontiguousarray(bias, np.float64)
+    y = np.empty((B, H, W, Cout), np.float64)
+    lib().orc_conv3x3_fwd(x.ctypes.data, w.ctypes.data, b.ctypes.data, y.ctypes.data, B, H, W, Cin, Cout)
+    return y
+
+
+def conv3x3_bwd_w(dy, x):
+    B, H, W, Cin = x.shape
+    Cout = dy.shape[-1]
+    dw = np.empty((Cout, 3, 3, Cin), np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().orc_conv3x3_bwd_w(dy.ctypes.data, x.ctypes.data, dw.ctypes.data, B, H, W, Cin, Cout)
+    return dw
+
+
+def conv3x3_bwd_x(dy, w):
+    B, H, W, Cout = dy.shape
+    Cin = w.shape[-1]
+    dx = np.empty((B, H, W, Cin), np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    lib().orc_conv3x3_bwd_x(dy.ctypes.data, w.ctypes.data, dx.ctypes.data, B, H, W, Cin, Cout)
+    return dx
+
+
+def unet_grad(params, in10, target, emulate=True):
+    """FP64 loss and parameter gradient of the denoiser network."""
+    params = np.ascontiguousarray(params, np.float32)
+    in10 = np.ascontiguousarray(in10, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    B, h, w, _ = in10.shape
+    g = np.empty(params.shape[0], np.float64)
+    pred = np.empty((B, h, w, 32), np.float32)
+    loss = lib().orc_unet_grad(params.ctypes.data, in10.ctypes.data, target.ctypes.data, B, h, w,
+                               1 if emulate else 0, g.ctypes.data, pred.ctypes.data)
+    return loss, g, pred
 
 
 def philox(ctr, key):
