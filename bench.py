@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 25)
     ap.add_argument("--cpu-rays", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the path-tracing workload (1.31M-triangle displaced icosphere, 1080p 64 spp)")
+    ap.add_argument("--render-spp", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -113,6 +116,11 @@ def main():
                          f"LBVH; oracle LBVH build {cpu_build_s * 1e3:.0f} ms"}
         log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads, build {cpu_build_s:.2f}s")
 
+    render = None
+    if not args.no_render:
+        render = bench_render(cpt, scenes, args.render_spp)
+        log(f"render: {render}")
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -139,9 +147,33 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_trace"},
         "cpu_baseline": cpu,
+        "path_tracer": render,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_render(cpt, scenes, spp):
+    """Plain path tracer on the displaced level-8 icosphere (1,310,720 GGX
+    triangles in the Cornell box), 1920x1080, `spp` samples, depth 8,
+    roulette after depth 3.  Returns Mrays/s (extension + shadow rays),
+    Mpix/s (pixels per second at this spp) and the kernel time split."""
+    import time
+    t0 = time.time()
+    s = scenes.displaced_icosphere(1920, 1080)
+    with cpt.Context(0) as ctx:
+        ctx.upload(s)
+        st_b = ctx.build(s.bounds_min, s.bounds_max)
+        ctx.render(s.camera, 1, seed=1)  # warm-up
+        img, _, st = ctx.render(s.camera, spp, seed=3317)
+    return {"workload": "displaced_icosphere_1080p", "triangles": int(s.tri.shape[0]), "spp": spp,
+            "max_depth": 8, "rr_depth": 3, "ms": round(st.ms, 2), "rays": st.rays,
+            "mrays_per_s": round(st.rays / (st.ms * 1e-3) / 1e6, 1),
+            "mpix_per_s": round(1920 * 1080 / (st.ms * 1e-3) / 1e6, 3),
+            "extend_ms": round(st.extend_ms, 2), "shade_ms": round(st.shade_ms, 2),
+            "shadow_ms": round(st.shadow_ms, 2), "bounces": st.bounces,
+            "lbvh_build_ms": round(st_b.total_ms, 3), "mean_radiance": float(img.mean()),
+            "wall_s": round(time.time() - t0, 1)}
 
 
 if __name__ == "__main__":

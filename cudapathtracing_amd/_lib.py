@@ -103,6 +103,10 @@ SIGNATURES = [
                              C.POINTER(RenderStats), C.c_int]),
     ("cpt_camera_from_desc", C.c_int, [C.POINTER(CameraDesc), C.POINTER(Camera)]),
     ("cpt_tonemap_rgba8", C.c_int, [P, C.c_int32, C.c_int32, P]),
+    ("cpt_bilateral", C.c_int, [P, P, P, P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_int]),
+    ("cpt_scene_update_triangles", C.c_int, [P, C.c_int64, P, C.c_int64]),
+    ("cpt_bvh_refit", C.c_int, [P, C.POINTER(BuildStats)]),
     ("cpt_denoiser_create", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]),
     ("cpt_denoiser_destroy", None, [P]),
     ("cpt_denoiser_num_params", C.c_int64, [P]),

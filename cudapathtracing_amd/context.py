@@ -104,6 +104,26 @@ class Context:
         self._check(self._lib.cpt_bvh_build(self._h, L.d3(bounds_min), L.d3(bounds_max), C.byref(st)))
         return st
 
+    def update_triangles(self, first: int, tri: np.ndarray):
+        """Replaces triangles [first, first+n) (indices after lights and spheres)."""
+        tri = np.ascontiguousarray(tri, np.float64).reshape(-1, 9)
+        self._check(self._lib.cpt_scene_update_triangles(self._h, first, tri.ctypes.data, tri.shape[0]))
+
+    def refit(self) -> L.BuildStats:
+        st = L.BuildStats()
+        self._check(self._lib.cpt_bvh_refit(self._h, C.byref(st)))
+        return st
+
+    def bilateral(self, rgb, aov, sigma_g=8.0, sigma_r=5.0, sigma_n=0.1, sigma_d=0.2):
+        """Joint bilateral filter of Render.cu:353-398 (host arrays)."""
+        h, w = rgb.shape[:2]
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        aov = np.ascontiguousarray(aov, np.float32)
+        out = np.empty((h, w, 3), np.float32)
+        self._check(self._lib.cpt_bilateral(self._h, rgb.ctypes.data, aov.ctypes.data, out.ctypes.data, w, h,
+                                            sigma_g, sigma_r, sigma_n, sigma_d, 0))
+        return out
+
     def export(self) -> dict:
         n = self.n_prims
         keys = np.empty(n, np.uint64)

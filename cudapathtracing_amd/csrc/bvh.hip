@@ -393,6 +393,11 @@ __global__ void k_refit(int n, const uint32_t *__restrict__ order, const float4 
   }
 }
 
+__global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ---- host launchers ---------------------------------------------------------
 
 void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]) {
@@ -485,6 +490,37 @@ int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysT
   if ((rc = ensure(ctx, ctx->rowTotal, 256 * 4))) return rc;
   return sortPasses(ctx, keys, vals, keysTmp, valsTmp, n, bits, identityVals, false, nullptr,
                     nullptr);
+}
+
+// Node boxes from the current primitives with the topology, parents and
+// leaf order of the last build.
+int refitBvh(cpt_ctx *ctx, cpt_build_stats *st) {
+  const int64_t n = ctx->nPrims;
+  if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_bvh_refit", "no BVH built");
+  if (n == 0) return CPT_OK;
+  const int bs = 256;
+  hipEvent_t *ev = ctx->ev;
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+  if (n > 1)
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
+                       ctx->flags.as<int32_t>(), n - 1);
+  hipLaunchKernelGGL(k_refit, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
+                     ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
+                     ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
+                     ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
+                     ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+  CPT_TRY_HIP(ctx, hipEventSynchronize(ev[1]));
+  if (st) {
+    float ms = 0;
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    std::memset(st, 0, sizeof(*st));
+    st->refit_ms = ms;
+    st->total_ms = ms;
+    st->n_prims = (int32_t)n;
+  }
+  return CPT_OK;
 }
 
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st) {

@@ -268,6 +268,41 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
   return CPT_OK;
 }
 
+int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri, int64_t n) {
+  if (!ctx || first < 0 || n < 0 || (n > 0 && !tri) || first + n > ctx->nTri)
+    return fail(ctx, CPT_ERR_ARG, "cpt_scene_update_triangles", "range outside the scene's triangles");
+  if (n == 0) return CPT_OK;
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  double *dst = ctx->tri64.as<double>() + 9 * first;
+  CPT_TRY_HIP(ctx, hipMemcpyAsync(dst, tri, n * 72, hipMemcpyHostToDevice, s));
+  // materials are kept: the pack kernel rewrites them from the old records
+  DevBuf mat;
+  int rc = ensure(ctx, mat, n * 4);
+  if (rc) return rc;
+  const int64_t base = (int64_t)ctx->nLights + ctx->nSpheres;
+  // extract the material ids (word 11 of each 48 B record) on the host side
+  std::vector<float> recs(n * 12);
+  CPT_TRY_HIP(ctx, hipMemcpyAsync(recs.data(), ctx->rec.as<float>() + 12 * (base + first), n * 48,
+                                  hipMemcpyDeviceToHost, s));
+  CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+  std::vector<int32_t> mids(n);
+  for (int64_t i = 0; i < n; i++) std::memcpy(&mids[i], &recs[12 * i + 11], 4);
+  CPT_TRY_HIP(ctx, hipMemcpyAsync(mat.p, mids.data(), n * 4, hipMemcpyHostToDevice, s));
+  rc = packTriangles(ctx, dst, mat.as<int32_t>(), n, base + first);
+  hipError_t e = hipStreamSynchronize(s);
+  hipFree(mat.p);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_update_triangles", hipGetErrorString(e));
+  return CPT_OK;
+}
+
+int cpt_bvh_refit(cpt_ctx *ctx, cpt_build_stats *st) {
+  if (!ctx) return CPT_ERR_ARG;
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  return refitBvh(ctx, st);
+}
+
 int cpt_bvh_build(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st) {
   if (!ctx || !bmin || !bmax) return CPT_ERR_ARG;
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));

@@ -77,6 +77,7 @@ int launchMorton(cpt_ctx *ctx, const double smin[3], const double smax[3], uint6
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
                    uint32_t *valsTmp, int64_t n, int bits, bool identityVals);
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
+int refitBvh(cpt_ctx *ctx, cpt_build_stats *st);
 int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, int64_t nt,
                   int64_t base);
 void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]);

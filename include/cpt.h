@@ -270,6 +270,28 @@ int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
 int cpt_tonemap_rgba8(const float *rgb, int32_t width, int32_t height,
                       uint8_t *rgba8);
 
+/* ---- non-learned denoiser ---------------------------------------------- */
+/* Separable joint bilateral filter, gaussianSeparate (Render.cu:353-398)
+ * horizontal then vertical (main.cu:100-104), radius 16.  rgb: w*h*3,
+ * aov: w*h*7 (albedo, normal, depth) as written by cpt_render, out: w*h*3.
+ * The reference's defaults are sigmaG 8, sigmaR 5, sigmaN 0.1, sigmaD 0.2
+ * (Window.cu:11-14). */
+int cpt_bilateral(cpt_ctx *ctx, const float *rgb, const float *aov, float *out,
+                  int32_t w, int32_t h, float sigma_g, float sigma_r,
+                  float sigma_n, float sigma_d, int flags);
+
+/* ---- dynamic scenes ------------------------------------------------------ */
+/* Replaces n triangles starting at triangle index `first` (0 = first
+ * triangle after the lights and spheres) with new vertices, keeping their
+ * materials; the primitive count is unchanged.  Host pointer. */
+int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri_xyz,
+                               int64_t n);
+/* Recomputes every node box of the last built hierarchy from the current
+ * primitives without re-sorting (the per-frame rebuild of Scene::Update,
+ * Scene.cuh:147-173, reduced to a refit).  The topology keeps the Morton
+ * order of the last cpt_bvh_build. */
+int cpt_bvh_refit(cpt_ctx *ctx, cpt_build_stats *stats);
+
 /* ---- learned denoiser (no reference counterpart; replaces the role of the
  * joint bilateral filter of Render.cu:353-460) ---------------------------- */
 /* 5-level U-Net (channels 32,64,64,96,128; 3x3 conv + ReLU, 2x2 average

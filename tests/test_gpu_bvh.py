@@ -117,3 +117,61 @@ def test_device_pointer_sort(ctx):
     ctx.set_stream(None)
     perm = np.argsort(keys, kind="stable")
     assert np.array_equal(vt.cpu().numpy(), perm.astype(np.int32))
+
+
+def leaf_ranges(ca, cb, n):
+    """[first, last] leaf range of every internal node from the child arrays."""
+    lo = np.zeros(n - 1, np.int64)
+    hi = np.zeros(n - 1, np.int64)
+    order = []
+    st = [0]
+    while st:
+        i = st.pop()
+        order.append(i)
+        for c in (ca[i], cb[i]):
+            if c >= 0:
+                st.append(c)
+    for i in reversed(order):
+        r = []
+        for c in (ca[i], cb[i]):
+            r.append((~c, ~c) if c < 0 else (lo[c], hi[c]))
+        lo[i] = min(r[0][0], r[1][0])
+        hi[i] = max(r[0][1], r[1][1])
+    return lo, hi
+
+
+def test_dynamic_refit(ctx):
+    print("[gpu] dynamic refit", flush=True)
+    s = soup(30000, seed=8)
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    g0 = ctx.export()
+    rng = np.random.default_rng(9)
+    moved = s.tri + rng.normal(0, 0.01, s.tri.shape)
+    moved = moved.astype(np.float32).astype(np.float64)
+    ctx.update_triangles(0, moved[:15000])
+    ctx.update_triangles(15000, moved[15000:])
+    st = ctx.refit()
+    assert st.refit_ms > 0
+    g = ctx.export()
+    assert np.array_equal(g["child_a"], g0["child_a"]) and np.array_equal(g["order"], g0["order"])
+    s2 = scenes.SceneData()
+    s2.materials = s.materials
+    s2.add_mesh(moved, 0)
+    _, pbox = O.OracleScene(s2).prims()
+    n = s2.n_prims
+    lo, hi = leaf_ranges(g["child_a"], g["child_b"], n)
+    order = g["order"]
+    for i in rng.choice(n - 1, 400, replace=False):
+        for c, ref in enumerate((g["child_a"][i], g["child_b"][i])):
+            f, l = (~ref, ~ref) if ref < 0 else (lo[ref], hi[ref])
+            ids = order[f:l + 1]
+            want = np.concatenate([pbox[ids, :3].min(0), pbox[ids, 3:].max(0)])
+            a, b = round_outward(want)
+            assert np.array_equal(g["boxes"][i, 6 * c:6 * c + 3], a[:3])
+            assert np.array_equal(g["boxes"][i, 6 * c + 3:6 * c + 6], b[3:])
+    # traversal on the refitted tree still finds the brute-force closest hits
+    rays = scenes.random_rays(3000, seed=12)
+    t, ids = ctx.trace(rays)
+    tb, idb, cert = O.OracleScene(s2).trace_brute(rays, eb=1e-4, et=1e-4)
+    assert np.array_equal(ids[cert], idb[cert])
