@@ -1,18 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark of the LBVH build + closest-hit traversal path on one MI355X.
+"""Benchmarks of the MI355X path on one GPU; prints ONE JSON line.
 
-Workload (one step): LBVH build of a 10M-triangle soup (centroids U[0,1]^3,
-lognormal side mu=-6 sigma=0.5, random orientation, seed 1; 30-bit Morton
-codes) followed by closest-hit traversal of 2^25 rays (origins
-U[-0.5,1.5]^3, uniform directions, seed 2; tmin 0.001, tmax 999999).
-Inputs are resident in HBM before the timed region.
+Primary workload (`value`, one step = one frame): the plain path tracer on
+the displaced level-8 icosphere (1,310,720 triangles, GGX with roughness
+U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
+1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
+Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
+the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
+per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 64 per node visited + 48 per
+primitive tested, node/primitive counts from a counting run of the same
+frame, divided by the summed HIP-event durations of the extension launches.
 
-Prints one JSON line (rank 0).  `value` is traversal throughput in Mrays/s;
-`lbvh_build_ms` is the build time of the same step; `roofline` covers the
-traversal kernel (algorithmic bytes: 32 B ray + 8 B result + 64 B per node
-visited + 48 B per primitive tested, counts from the counting variant of the
-same kernel); `cpu_baseline` is the FP64 CPU oracle (reference traversal
-order, OpenMP) on a 2^18-ray sample of the same rays.
+Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
+U[0,1]^3, lognormal side mu=-6 sigma=0.5, random orientation, seed 1) and
+closest-hit traversal of 2^25 random rays (origins U[-0.5,1.5]^3, seed 2),
+with its own roofline.
+
+`cpu_baseline`: the FP64 CPU oracle (reference semantics, OpenMP on the host
+cores) on a bounded sample of the primary workload (a centred pixel window at
+the same spp); `lbvh.cpu_baseline` the same for traversal.
 """
 import argparse
 import json
@@ -33,41 +39,131 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tris", type=int, default=10_000_000)
-    ap.add_argument("--rays", type=int, default=1 << 25)
-    ap.add_argument("--cpu-rays", type=int, default=1 << 18)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-render", action="store_true",
-                    help="skip the path-tracing workload (1.31M-triangle displaced icosphere, 1080p 64 spp)")
-    ap.add_argument("--render-spp", type=int, default=64)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
+def cores():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
+
+def bench_render(cpt, L, scenes, args):
+    s = scenes.displaced_icosphere(1920, 1080)
+    ctx = cpt.Context(0)
+    ctx.upload(s)
+    st_b = ctx.build(s.bounds_min, s.bounds_max)
+    kw = dict(seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
+    for _ in range(args.warmup):
+        ctx.render(s.camera, 1, **kw)
+    frames = []
+    for k in range(args.steps):
+        img, _, st = ctx.render(s.camera, args.spp, sample_base=k * args.spp, **kw)
+        frames.append(st)
+        log(f"frame {k}: {st.ms:.1f} ms, {st.rays / 1e6:.1f} Mrays, {st.bounces} bounces")
+    # counting run (untimed) for the algorithmic bytes of the extension kernel
+    _, _, stc = ctx.render(s.camera, args.spp, sample_base=0, count_work=True, **kw)
+    ms = float(np.mean([f.ms for f in frames]))
+    rays = float(np.mean([f.rays for f in frames]))
+    ext_ms = float(np.mean([f.extend_ms for f in frames]))
+    ext_rays = float(np.mean([f.extend_rays for f in frames]))
+    ext_bytes = 44.0 * stc.extend_rays + 64.0 * stc.extend_nodes + 48.0 * stc.extend_prims
+    ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
+    achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
+    res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
+               shade_ms=float(np.mean([f.shade_ms for f in frames])),
+               shadow_ms=float(np.mean([f.shadow_ms for f in frames])),
+               bounces=int(frames[0].bounces), launches=int(frames[0].bounces) * 4,
+               nodes_per_ray=stc.extend_nodes / max(stc.extend_rays, 1),
+               prims_per_ray=stc.extend_prims / max(stc.extend_rays, 1),
+               achieved=achieved, build_ms=st_b.total_ms, mean=float(img.mean()))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        o = O.OracleScene(s)
+        o.build(s.bounds_min, s.bounds_max)
+        cam = cpt.camera(s.camera)
+        # centred window of the frame, same camera and spp
+        win = args.cpu_window
+        sub = L.Camera()
+        C = __import__("ctypes")
+        C.memmove(C.byref(sub), C.byref(cam), C.sizeof(L.Camera))
+        i0, j0 = (cam.width - win) // 2, (cam.height - win) // 2
+        for k in range(3):
+            sub.pixel_center[k] = cam.pixel_center[k] + i0 * cam.delta_h[k] + j0 * cam.delta_v[k]
+        sub.width = sub.height = win
+        prm = L.RenderParams(3317, args.spp, 8, 3, L.PROFILE_CONFIG, 0.7, 0, 0, 0, 0)
+        t0 = time.time()
+        o.render(sub, prm, nthreads=cores())
+        dt = time.time() - t0
+        # ray count of the same window, traced on the GPU
+        gw, _, gst = ctx.render(sub, args.spp, seed=3317, profile=L.PROFILE_CONFIG)
+        cpu = {"value": round(gst.rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores(), "kind": "port",
+               "sample": f"FP64 oracle (reference-order unculled traversal, FP64 BSDFs, OpenMP) rendering the "
+                         f"centred {win}x{win} window of the same frame at {args.spp} spp in {dt:.1f} s; "
+                         f"rays = the GPU's ray count for that window"}
+        log(f"cpu baseline {cpu}")
+    ctx.close()
+    return res, cpu, int(s.tri.shape[0])
+
+
+WEIGHTS = os.path.join(ROOT, "cudapathtracing_amd", "weights", "unet_cornell.f32")
+
+
+def bench_equal_quality(cpt, L, scenes, args):
+    """Denoised 4-spp vs plain path tracing at equal relMSE.
+
+    Scene: the displaced-icosphere workload at args.eq_width x eq_height.
+    Reference: args.eq_ref_spp spp.  Denoised: 4 spp + albedo/normal/depth,
+    then the U-Net (time = render + inference).  Plain: spp doubled from 4
+    until relMSE falls below the denoised relMSE, time interpolated in
+    log(spp).  Mpix/s = pixels / time for each method."""
+    if not os.path.exists(WEIGHTS):
+        return None
+    from cudapathtracing_amd.denoiser import Denoiser, pack_input
+    from cudapathtracing_amd.training import equal_quality, relmse
+    w, h = args.eq_width, args.eq_height
+    s = scenes.displaced_icosphere(w, h)
+    kw = dict(profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
+    with cpt.Context(0) as ctx:
+        ctx.upload(s)
+        ctx.build(s.bounds_min, s.bounds_max)
+        t0 = time.time()
+        ref, _, st_ref = ctx.render(s.camera, args.eq_ref_spp, seed=99991, **kw)
+        log(f"equal-quality reference {args.eq_ref_spp} spp: {time.time() - t0:.1f}s")
+        dn = Denoiser(ctx, 1, h, w)
+        dn.set_params(np.fromfile(WEIGHTS, dtype="<f4"))
+        ctx.render(s.camera, 4, seed=5, aov=True, **kw)  # warm-up
+        dn(pack_input(*ctx.render(s.camera, 4, seed=5, aov=True, **kw)[:2])[None])
+        rgb, aov, st4 = ctx.render(s.camera, 4, seed=7, aov=True, **kw)
+        den = dn(pack_input(rgb, aov)[None])[0]
+        dstat = dn.stats()
+        e_den = relmse(den, ref)
+        e_noisy = relmse(rgb, ref)
+        bil = ctx.bilateral(rgb, aov)
+        e_bil = relmse(bil, ref)
+        t_den = st4.ms + dstat["fwd_ms"]
+        eq = equal_quality(ctx, s.camera, ref, e_den, seed=11, spp_start=4, spp_max=args.eq_ref_spp // 2)
+        dn.close()
+    npix = w * h
+    return {"width": w, "height": h, "reference_spp": args.eq_ref_spp, "denoised_spp": 4,
+            "relmse_noisy_4spp": e_noisy, "relmse_bilateral_4spp": e_bil, "relmse_denoised_4spp": e_den,
+            "denoised_ms": round(t_den, 2), "render_4spp_ms": round(st4.ms, 2),
+            "denoise_ms": round(dstat["fwd_ms"], 2),
+            "denoise_tflops": round(dstat["fwd_flops"] / (dstat["fwd_ms"] * 1e-3) / 1e12, 1),
+            "plain_spp_at_equal_relmse": round(eq["spp"], 1), "plain_ms_at_equal_relmse": round(eq["ms"], 2),
+            "reached": eq["reached"],
+            "mpix_per_s_denoised": round(npix / (t_den * 1e-3) / 1e6, 3),
+            "mpix_per_s_plain": round(npix / (eq["ms"] * 1e-3) / 1e6, 3),
+            "speedup": round(eq["ms"] / t_den, 2),
+            "plain_points": [(p[0], round(p[1], 6), round(p[2], 2)) for p in eq["points"]]}
+
+
+def bench_lbvh(cpt, scenes, args):
     import torch
-    import cudapathtracing_amd as cpt
-    from cudapathtracing_amd import scenes
-
-    t0 = time.time()
     scene = scenes.triangle_soup_scene(args.tris, seed=1)
     rays_np = scenes.random_rays(args.rays, seed=2)
-    log(f"generated {args.tris} triangles, {args.rays} rays in {time.time() - t0:.1f}s")
-
     ctx = cpt.Context(0)
     ctx.upload(scene)
     rays = torch.from_numpy(rays_np).cuda()
     t_out = torch.empty(args.rays, dtype=torch.float32, device="cuda")
     id_out = torch.empty(args.rays, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
-    log("scene and rays resident in HBM")
-
     build_ms, trace_ms, stage = [], [], []
     for it in range(args.warmup + args.steps):
         torch.cuda.synchronize()
@@ -79,101 +175,103 @@ def main():
             build_ms.append(st.total_ms)
             trace_ms.append(tr)
             stage.append((st.morton_ms, st.sort_ms, st.emit_ms, st.refit_ms))
-        log(f"step {it}: build {st.total_ms:.3f} ms, trace {tr:.3f} ms")
-    ctx.set_stream(None)
-
-    # algorithmic bytes of the traversal kernel (counting variant, untimed)
+        log(f"lbvh step {it}: build {st.total_ms:.3f} ms, trace {tr:.3f} ms")
     nodes, prims = ctx.trace_count(rays)
     torch.cuda.synchronize()
     ctx.set_stream(None)
     n_nodes = float(nodes.double().sum().item())
     n_prims = float(prims.double().sum().item())
-    alg_bytes = 40.0 * args.rays + 64.0 * n_nodes + 48.0 * n_prims
     hits = float((id_out >= 0).double().mean().item())
-
-    tr_avg = float(np.mean(trace_ms))
-    bd_avg = float(np.mean(build_ms))
-    mrays = args.rays / (tr_avg * 1e-3) / 1e6
-    achieved = alg_bytes / (tr_avg * 1e-3) / 1e9
-    stage_avg = np.mean(np.array(stage), axis=0).tolist()
-
-    cpu = None
+    alg = 40.0 * args.rays + 64.0 * n_nodes + 48.0 * n_prims
+    tr_avg, bd_avg = float(np.mean(trace_ms)), float(np.mean(build_ms))
+    sa = np.mean(np.array(stage), axis=0).tolist()
+    achieved = alg / (tr_avg * 1e-3) / 1e9
+    out = {"triangles": args.tris, "rays": args.rays, "build_ms": round(bd_avg, 4),
+           "build_stage_ms": {"morton": round(sa[0], 4), "sort": round(sa[1], 4), "emit": round(sa[2], 4),
+                              "refit": round(sa[3], 4)},
+           "trace_ms": round(tr_avg, 4), "mrays_per_s": round(args.rays / (tr_avg * 1e-3) / 1e6, 1),
+           "hit_fraction": round(hits, 4), "nodes_per_ray": round(n_nodes / args.rays, 2),
+           "prims_per_ray": round(n_prims / args.rays, 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace"}}
     if not args.no_cpu:
         from oracle import oracle as O
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         o = O.OracleScene(scene)
         tb = time.time()
         o.build(scene.bounds_min, scene.bounds_max)
-        cpu_build_s = time.time() - tb
+        cb = time.time() - tb
         sample = rays_np[:args.cpu_rays]
         tc = time.time()
         o.trace_ref(sample)
-        cpu_s = time.time() - tc
-        cpu = {"value": round(sample.shape[0] / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": cores,
-               "kind": "port",
-               "sample": f"FP64 oracle, reference traversal order (no t-culling), first "
-                         f"{sample.shape[0]} of the {args.rays} rays on the same {args.tris}-triangle "
-                         f"LBVH; oracle LBVH build {cpu_build_s * 1e3:.0f} ms"}
-        log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads, build {cpu_build_s:.2f}s")
+        dt = time.time() - tc
+        out["cpu_baseline"] = {"value": round(sample.shape[0] / dt / 1e6, 4), "unit": "Mrays/s",
+                               "cores": cores(), "kind": "port",
+                               "sample": f"FP64 oracle, reference traversal order (no t-culling), first "
+                                         f"{sample.shape[0]} rays; oracle LBVH build of the same "
+                                         f"{args.tris} triangles {cb * 1e3:.0f} ms"}
+    ctx.close()
+    return out
 
-    render = None
-    if not args.no_render:
-        render = bench_render(cpt, scenes, args.render_spp)
-        log(f"render: {render}")
 
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--tris", type=int, default=10_000_000)
+    ap.add_argument("--rays", type=int, default=1 << 25)
+    ap.add_argument("--cpu-rays", type=int, default=1 << 17)
+    ap.add_argument("--cpu-window", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lbvh", action="store_true")
+    ap.add_argument("--no-eq", action="store_true")
+    ap.add_argument("--eq-width", type=int, default=960)
+    ap.add_argument("--eq-height", type=int, default=540)
+    ap.add_argument("--eq-ref-spp", type=int, default=2048)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import cudapathtracing_amd as cpt
+    from cudapathtracing_amd import _lib as L
+    from cudapathtracing_amd import scenes
+
+    r, cpu, ntri = bench_render(cpt, L, scenes, args)
+    lb = None if args.no_lbvh else bench_lbvh(cpt, scenes, args)
+    eq = None if args.no_eq else bench_equal_quality(cpt, L, scenes, args)
     out = {
         "metric": METRIC,
-        "value": round(mrays, 2),
+        "value": round(r["mrays"], 1),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(tr_avg + bd_avg, 4),
+        "ms_per_step": round(r["ms"], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "lbvh_build_plus_closest_hit", "triangles": args.tris,
-                   "rays": args.rays, "morton_bits": 30, "tmin": 0.001, "tmax": 999999.0},
-        "lbvh_build_ms": round(bd_avg, 4),
-        "lbvh_stage_ms": {"morton": round(stage_avg[0], 4), "sort": round(stage_avg[1], 4),
-                          "emit": round(stage_avg[2], 4), "refit": round(stage_avg[3], 4)},
-        "trace_ms": round(tr_avg, 4),
-        "hit_fraction": round(hits, 4),
-        "nodes_per_ray": round(n_nodes / args.rays, 3),
-        "prims_per_ray": round(n_prims / args.rays, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_trace"},
+        "config": {"workload": "plain_path_tracer_displaced_icosphere", "triangles": ntri, "width": 1920,
+                   "height": 1080, "spp": args.spp, "max_depth": 8, "rr_depth": 3},
+        "mpix_per_s": round(1920 * 1080 / (r["ms"] * 1e-3) / 1e6, 3),
+        "rays_per_frame": r["rays"],
+        "kernel_ms": {"extend": round(r["ext_ms"], 2), "shade": round(r["shade_ms"], 2),
+                      "shadow": round(r["shadow_ms"], 2)},
+        "bounces": r["bounces"],
+        "extend_nodes_per_ray": round(r["nodes_per_ray"], 2),
+        "extend_prims_per_ray": round(r["prims_per_ray"], 2),
+        "scene_lbvh_build_ms": round(r["build_ms"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_extend",
+                     "avg_launch_ms": round(r["ext_ms"] / max(1, r["bounces"]), 4)},
         "cpu_baseline": cpu,
-        "path_tracer": render,
+        "lbvh": lb,
+        "equal_relmse": eq,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
-
-
-def bench_render(cpt, scenes, spp):
-    """Plain path tracer on the displaced level-8 icosphere (1,310,720 GGX
-    triangles in the Cornell box), 1920x1080, `spp` samples, depth 8,
-    roulette after depth 3.  Returns Mrays/s (extension + shadow rays),
-    Mpix/s (pixels per second at this spp) and the kernel time split."""
-    import time
-    t0 = time.time()
-    s = scenes.displaced_icosphere(1920, 1080)
-    with cpt.Context(0) as ctx:
-        ctx.upload(s)
-        st_b = ctx.build(s.bounds_min, s.bounds_max)
-        ctx.render(s.camera, 1, seed=1)  # warm-up
-        img, _, st = ctx.render(s.camera, spp, seed=3317)
-    return {"workload": "displaced_icosphere_1080p", "triangles": int(s.tri.shape[0]), "spp": spp,
-            "max_depth": 8, "rr_depth": 3, "ms": round(st.ms, 2), "rays": st.rays,
-            "mrays_per_s": round(st.rays / (st.ms * 1e-3) / 1e6, 1),
-            "mpix_per_s": round(1920 * 1080 / (st.ms * 1e-3) / 1e6, 3),
-            "extend_ms": round(st.extend_ms, 2), "shade_ms": round(st.shade_ms, 2),
-            "shadow_ms": round(st.shadow_ms, 2), "bounces": st.bounces,
-            "lbvh_build_ms": round(st_b.total_ms, 3), "mean_radiance": float(img.mean()),
-            "wall_s": round(time.time() - t0, 1)}
 
 
 if __name__ == "__main__":

@@ -28,6 +28,7 @@ OBJ_NONE, OBJ_SPHERE, OBJ_LIGHT, OBJ_TRIANGLE = range(4)
 
 PROFILE_REFERENCE = 0
 PROFILE_CONFIG = 1
+RENDER_COUNT_WORK = 1
 
 D3 = C.c_double * 3
 
@@ -66,13 +67,14 @@ class RenderParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("rr_depth", C.c_int32), ("profile", C.c_int32), ("rr_prob", C.c_double),
                 ("sample_base", C.c_int32), ("want_aov", C.c_int32),
-                ("paths_per_batch", C.c_int32), ("pad_", C.c_int32)]
+                ("paths_per_batch", C.c_int32), ("flags", C.c_int32)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("rays", C.c_double), ("paths", C.c_double),
                 ("extend_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
-                ("bounces", C.c_int32), ("pad_", C.c_int32)]
+                ("bounces", C.c_int32), ("pad_", C.c_int32), ("extend_rays", C.c_double),
+                ("extend_nodes", C.c_double), ("extend_prims", C.c_double)]
 
 
 P = C.c_void_p
@@ -86,6 +88,7 @@ SIGNATURES = [
     ("cpt_set_stream", C.c_int, [P, P]),
     ("cpt_synchronize", C.c_int, [P]),
     ("cpt_version", C.c_char_p, []),
+    ("cpt_set_option", C.c_int, [P, C.c_int32, C.c_int32]),
     ("cpt_scene_upload", C.c_int, [P, C.POINTER(Material), C.c_int32, C.POINTER(Light), C.c_int32,
                                    C.POINTER(Sphere), C.c_int32, P, P, C.c_int64]),
     ("cpt_scene_count", C.c_int64, [P]),

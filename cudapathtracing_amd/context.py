@@ -70,6 +70,10 @@ class Context:
     def set_stream(self, stream_handle: Optional[int]):
         self._check(self._lib.cpt_set_stream(self._h, stream_handle))
 
+    def set_traversal_width(self, width: int):
+        """4 (default): four-wide records; 2: binary LBVH nodes."""
+        self._check(self._lib.cpt_set_option(self._h, 1, width))
+
     def synchronize(self):
         self._check(self._lib.cpt_synchronize(self._h))
 
@@ -197,10 +201,10 @@ class Context:
     def render(self, camera_desc: L.CameraDesc, spp: int, seed: int = 3317,
                profile: int = L.PROFILE_CONFIG, max_depth: int = 8, rr_depth: int = 3,
                rr_prob: float = 0.7, sample_base: int = 0, aov: bool = False,
-               paths_per_batch: int = 0, out=None, aov_out=None):
-        cam = camera(camera_desc)
+               paths_per_batch: int = 0, out=None, aov_out=None, count_work: bool = False):
+        cam = camera_desc if isinstance(camera_desc, L.Camera) else camera(camera_desc)
         p = L.RenderParams(seed, spp, max_depth, rr_depth, profile, rr_prob, sample_base,
-                           1 if aov else 0, paths_per_batch, 0)
+                           1 if aov else 0, paths_per_batch, L.RENDER_COUNT_WORK if count_work else 0)
         h, w = cam.height, cam.width
         dev = out is not None and _is_torch(out) and out.is_cuda
         if dev:

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kSortThreads) void k_morton(const double *__restric
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_hist(const uint32_t *__restrict__ keys, int64_t n,
-                                                       int shift, uint32_t *__restrict__ hist,
+                                                       int shift, uint32_t dmask, uint32_t *__restrict__ hist,
                                                        int nTiles) {
   __shared__ uint32_t sh[256];
   const int tid = threadIdx.x;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kSortThreads) void k_hist(const uint32_t *__restric
 #pragma unroll 4
   for (int j = 0; j < kSortItems; j++) {
     int64_t i = base + j * kSortThreads + tid;
-    if (i < n) atomicAdd(&sh[(keys[i] >> shift) & 255u], 1u);
+    if (i < n) atomicAdd(&sh[(keys[i] >> shift) & dmask], 1u);
   }
   __syncthreads();
   hist[(int64_t)tid * nTiles + blockIdx.x] = sh[tid];
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scan_rows(uint32_t *__restrict
 __global__ __launch_bounds__(kSortThreads) void k_scatter(
     const uint32_t *__restrict__ keysIn, const uint32_t *__restrict__ valsIn,
     uint32_t *__restrict__ keysOut, uint32_t *__restrict__ valsOut, uint64_t *__restrict__ keys64Out,
-    int64_t n, int shift, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowTotal,
+    int64_t n, int shift, uint32_t dmask, const uint32_t *__restrict__ hist, const uint32_t *__restrict__ rowTotal,
     int nTiles) {
   __shared__ uint32_t sWave[4][256];
   __shared__ uint32_t sStart[256];
@@ -222,7 +222,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   for (int j = 0; j < kSortItems; j++) {
     int64_t i = waveBase + j * 64 + lane;
     if (i < n) {
-      const uint32_t d = (key[j] >> shift) & 255u;
+      const uint32_t d = (key[j] >> shift) & dmask;
       uint64_t peers = __ballot(1);
 #pragma unroll
       for (int bit = 0; bit < 8; bit++) {
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   for (int j = 0; j < kSortItems; j++) {
     int64_t i = waveBase + j * 64 + lane;
     if (i < n) {
-      const uint32_t d = (key[j] >> shift) & 255u;
+      const uint32_t d = (key[j] >> shift) & dmask;
       const uint32_t p = sStart[d] + sWave[wave][d] + pos[j];
       if (p < (uint32_t)kTile) {  // always true for consistent ranks
         sKeys[p] = key[j];
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter(
   for (int i = tid; i < tileCount; i += kSortThreads) {
     const uint32_t k = sKeys[i];
     const uint32_t v = sVals[i];
-    const uint32_t g = sGlobal[(k >> shift) & 255u] + (uint32_t)i;
+    const uint32_t g = sGlobal[(k >> shift) & dmask] + (uint32_t)i;
     if ((int64_t)g >= n) continue;  // always false for a consistent histogram
     if (keysOut) keysOut[g] = k;
     if (valsOut) valsOut[g] = v;
@@ -393,6 +393,55 @@ __global__ void k_refit(int n, const uint32_t *__restrict__ order, const float4 
   }
 }
 
+// Four-wide record of every internal node (layout in cpt_device.hpp): the
+// node's children with each internal child replaced by its two children.
+__global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  float b[6][4];
+  int ref[4];
+  int k = 0;
+  const float4 a0 = nodes[4 * (int64_t)i], a1 = nodes[4 * (int64_t)i + 1], a2 = nodes[4 * (int64_t)i + 2],
+               a3 = nodes[4 * (int64_t)i + 3];
+  const int cr[2] = {__float_as_int(a3.x), __float_as_int(a3.y)};
+  // child boxes of a node: slot 0 = A, 1 = B (see storeChildBox)
+  const float cb[2][6] = {{a0.x, a0.z, a2.x, a0.y, a0.w, a2.y}, {a1.x, a1.z, a2.z, a1.y, a1.w, a2.w}};
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    if (cr[c] < 0) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) b[q][k] = cb[c][q];
+      ref[k++] = cr[c];
+    } else {
+      const int64_t g = cr[c];
+      const float4 g0 = nodes[4 * g], g1 = nodes[4 * g + 1], g2 = nodes[4 * g + 2], g3 = nodes[4 * g + 3];
+      const float gb[2][6] = {{g0.x, g0.z, g2.x, g0.y, g0.w, g2.y}, {g1.x, g1.z, g2.z, g1.y, g1.w, g2.w}};
+      const int gr[2] = {__float_as_int(g3.x), __float_as_int(g3.y)};
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) b[q][k] = gb[e][q];
+        ref[k++] = gr[e];
+      }
+    }
+  }
+  for (; k < 4; k++) {
+#pragma unroll
+    for (int q = 0; q < 6; q++) b[q][k] = 0.f;
+    ref[k] = (int)0x80000000;
+  }
+  float4 *o = nodes4 + 8 * (int64_t)i;
+  // SoA: minx maxx miny maxy minz maxz refs
+  o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+  o[1] = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
+  o[2] = make_float4(b[1][0], b[1][1], b[1][2], b[1][3]);
+  o[3] = make_float4(b[4][0], b[4][1], b[4][2], b[4][3]);
+  o[4] = make_float4(b[2][0], b[2][1], b[2][2], b[2][3]);
+  o[5] = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
+  o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -454,9 +503,11 @@ static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *ke
   uint32_t *kin = keys, *vin = vals, *kout = keysTmp, *vout = valsTmp;
   for (int p = 0; p < passes; p++) {
     const int shift = 8 * p;
+    // the last pass may cover fewer than 8 key bits
+    const uint32_t dmask = bits - shift >= 8 ? 255u : ((1u << (bits - shift)) - 1u);
     if (!(p == 0 && firstHistDone))
       hipLaunchKernelGGL(k_hist, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream, kin, n, shift,
-                         hist, nTiles);
+                         dmask, hist, nTiles);
     hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(kSortThreads), 0, ctx->stream, hist, nTiles,
                        rowTotal);
     const bool last = p == passes - 1;
@@ -469,7 +520,7 @@ static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *ke
     }
     hipLaunchKernelGGL(k_scatter, dim3(nTiles), dim3(kSortThreads), 0, ctx->stream, kin,
                        (p == 0 && identityVals) ? (const uint32_t *)nullptr : vin, ko, vo, k64, n,
-                       shift, hist, rowTotal, nTiles);
+                       shift, dmask, hist, rowTotal, nTiles);
     CPT_TRY_HIP(ctx, hipGetLastError());
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -509,6 +560,9 @@ int refitBvh(cpt_ctx *ctx, cpt_build_stats *st) {
                      ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
                      ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
                      ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
+  if (n > 1)
+    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
+                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
   CPT_TRY_HIP(ctx, hipEventSynchronize(ev[1]));
@@ -543,7 +597,8 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
       (rc = ensure(ctx, ctx->parentInt, (size_t)n * 4)) ||
       (rc = ensure(ctx, ctx->parentLeaf, (size_t)n * 4)) ||
       (rc = ensure(ctx, ctx->flags, (size_t)n * 4)) || (rc = ensure(ctx, ctx->leaves, (size_t)n * 48)) ||
-      (rc = ensure(ctx, ctx->rootBox, 64)))
+      (rc = ensure(ctx, ctx->rootBox, 64)) ||
+      (rc = ensure(ctx, ctx->nodes4, (size_t)(n > 1 ? n - 1 : 1) * 128)))
     return rc;
   double smin[3], smax[3];
   expandBounds(bmin, bmax, smin, smax);
@@ -574,6 +629,9 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
                      ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
                      ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
                      ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
+  if (n > 1)
+    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
+                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
   if (st) {

@@ -57,7 +57,7 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->valsA,    &ctx->valsB,  &ctx->hist,      &ctx->rowTotal,   &ctx->keys64,
                    &ctx->order,    &ctx->nodes,  &ctx->parentInt, &ctx->parentLeaf, &ctx->flags,
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
-                   &ctx->cntA,     &ctx->cntB};
+                   &ctx->cntA,     &ctx->cntB,    &ctx->nodes4};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -146,6 +146,15 @@ int cpt_synchronize(cpt_ctx *ctx) {
 }
 
 int64_t cpt_scene_count(const cpt_ctx *ctx) { return ctx ? ctx->nPrims : -1; }
+
+int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
+  if (!ctx) return CPT_ERR_ARG;
+  if (option == CPT_OPT_TRAVERSAL_WIDTH && (value == 2 || value == 4)) {
+    ctx->traversal = value;
+    return CPT_OK;
+  }
+  return fail(ctx, CPT_ERR_ARG, "cpt_set_option", "unknown option or value");
+}
 
 int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, const cpt_light *lights,
                      int32_t nl, const cpt_sphere *spheres, int32_t ns, const double *tri,

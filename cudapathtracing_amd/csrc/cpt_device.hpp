@@ -263,6 +263,101 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
   return hit;
 }
 
+// ---- four-wide traversal ------------------------------------------------------
+// Record of internal node i (128 B, 8 x float4): the boxes of i's children,
+// each internal child replaced by its own two children (so a record spans
+// two binary levels), as SoA {minx[4]} {maxx[4]} {miny[4]} {maxy[4]}
+// {minz[4]} {maxz[4]} {ref[4]} {unused}; ref >= 0 is a binary node index
+// (whose record continues the walk), ~k is leaf k, kEmpty4 an unused slot.
+// Walking from the root touches only even-depth records.
+constexpr int kEmpty4 = (int)0x80000000;
+constexpr int kStack4Total = 96;  // <= 3 pushes per record, <= 31 record levels
+
+template <bool kCount>
+__device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
+                                             const float4 *__restrict__ leaves, int n, f3 o, f3 d,
+                                             float tmin, float &tbest, int *stk, int stride,
+                                             TraceCounters &cnt) {
+  int hit = -1;
+  if (n <= 0) return -1;
+  if (n == 1) {
+    float t;
+    if (kCount) cnt.prims++;
+    if (hitLeaf(leaves, 0, o, d, tmin, tbest, t)) {
+      tbest = t;
+      hit = 0;
+    }
+    return hit;
+  }
+  const float eps = 1e-20f;
+  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
+               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
+  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  int ovf[kStack4Total - kStackLds];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    if (kCount) cnt.nodes++;
+    const float4 *np = nodes4 + 8 * (int64_t)node;
+    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+    const float4 rf = np[6];
+    const float bx0[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bx1[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+    const float by0[4] = {mny.x, mny.y, mny.z, mny.w}, by1[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
+    const float bz0[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bz1[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+    const int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+    float tn[4];
+    bool h[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const float x0 = __builtin_fmaf(bx0[k], inv.x, -ood.x), x1 = __builtin_fmaf(bx1[k], inv.x, -ood.x);
+      const float y0 = __builtin_fmaf(by0[k], inv.y, -ood.y), y1 = __builtin_fmaf(by1[k], inv.y, -ood.y);
+      const float z0 = __builtin_fmaf(bz0[k], inv.z, -ood.z), z1 = __builtin_fmaf(bz1[k], inv.z, -ood.z);
+      tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
+      const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbest));
+      h[k] = tn[k] <= tf && ref[k] != kEmpty4;
+    }
+    // leaf children first, in slot order
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (h[k] && ref[k] < 0) {
+        float t;
+        if (kCount) cnt.prims++;
+        if (hitLeaf(leaves, ~ref[k], o, d, tmin, tbest, t)) {
+          tbest = t;
+          hit = ~ref[k];
+        }
+        h[k] = false;
+      }
+    }
+    // internal children: continue with the nearest, push the others
+    int best = -1;
+    float bt = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (h[k] && tn[k] < bt) {
+        bt = tn[k];
+        best = k;
+      }
+    if (best < 0) {
+      if (sp == 0) break;
+      sp--;
+      node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
+      continue;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (h[k] && k != best) {
+        if (sp < kStackLds) stk[sp * stride] = ref[k];
+        else if (sp < kStack4Total) ovf[sp - kStackLds] = ref[k];
+        sp = sp < kStack4Total ? sp + 1 : sp;
+      }
+    }
+    node = ref[best];
+  }
+  return hit;
+}
+
 // ---- FP64 refinement of the winning hit ------------------------------------
 // Recomputes the distance to primitive `id` with the reference's FP64
 // formulas (Hittable.cuh:27-54, 85-140, 367-398) on the FP64 scene copy.

@@ -56,6 +56,8 @@ struct cpt_ctx {
   bool built = false;
   cpt::DevBuf codesA, codesB, valsA, valsB, hist, rowTotal;
   cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
+  cpt::DevBuf nodes4;   // (N-1) x 128 B four-wide records (grandchildren of each node)
+  int traversal = 4;    // 2: binary node walk, 4: four-wide records
 
   // traversal I/O staging
   cpt::DevBuf rayBuf, tBuf, idBuf, cntA, cntB;

@@ -46,7 +46,8 @@ struct CamDev {
 };
 
 struct RenderArgs {
-  const float4 *nodes, *leaves, *rec;
+  const float4 *nodes, *leaves, *rec;  // nodes: binary nodes or four-wide records
+  int wide;
   const MatDev *mats;
   int nPrims, nLights;
   CamDev cam;
@@ -57,6 +58,15 @@ struct RenderArgs {
 };
 
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+// Closest hit over the binary nodes or the four-wide records (`wide`).
+template <bool kCount>
+__device__ __forceinline__ int traceAny(int wide, const float4 *__restrict__ nodes, const float4 *__restrict__ leaves,
+                                        int n, f3 o, f3 d, float tmin, float &tbest, int *stk, int stride,
+                                        TraceCounters &cnt) {
+  return wide ? traceClosest4<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
+              : traceClosest<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+}
 
 // Wave-aggregated append: returns the slot for lanes with pred, ~0u otherwise.
 __device__ __forceinline__ uint32_t waveAppend(uint32_t *counter, bool pred) {
@@ -106,21 +116,36 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   q[p] = (uint32_t)p;
 }
 
+// kCount: also sums node visits and primitive tests into work[0], work[1]
+template <bool kCount>
 __global__ __launch_bounds__(kRenderThreads) void k_extend(const uint32_t *__restrict__ q, uint32_t count,
                                                             const float4 *__restrict__ ray,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
-                                                            float2 *__restrict__ hit) {
+                                                            float2 *__restrict__ hit,
+                                                            unsigned long long *__restrict__ work, int wide) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
-  if (k >= count) return;
-  const uint32_t p = q[k];
-  const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
-  float tb = r1.w;
   TraceCounters cnt;
-  const int leaf = traceClosest<false>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
-                                       stack + threadIdx.x, kRenderThreads, cnt);
-  hit[p] = make_float2(__int_as_float(leaf), tb);
+  if (k < count) {
+    const uint32_t p = q[k];
+    const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
+    float tb = r1.w;
+    const int leaf = traceAny<kCount>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
+                                          stack + threadIdx.x, kRenderThreads, cnt);
+    hit[p] = make_float2(__int_as_float(leaf), tb);
+  }
+  if (kCount) {
+    unsigned long long a = (unsigned long long)cnt.nodes, b = (unsigned long long)cnt.prims;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if (__lane_id() == 0) {
+      atomicAdd(work + 0, a);
+      atomicAdd(work + 1, b);
+    }
+  }
 }
 
 __device__ __forceinline__ ShadeMat loadMat(const MatDev *mats, int m) {
@@ -207,7 +232,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         const float sig = 25.0f;
         TraceCounters cnt;
         float tb = kRayTMax;
-        int l2 = traceClosest<false>(a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb,
+        int l2 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb,
                                      stack + threadIdx.x, kRenderThreads, cnt);
         o = hp;
         if (l2 >= 0) {
@@ -227,7 +252,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
             o = pos;
             pos = hp + d * t;
             float tb2 = kRayTMax;
-            int l3 = traceClosest<false>(a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
+            int l3 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
                                          stack + threadIdx.x, kRenderThreads, cnt);
             if (l3 >= 0) dMax = tb2;
             t = -__logf(1.0f - rng.next()) / sig;
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
                                                             const uint32_t *__restrict__ counters,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
-                                                            float *__restrict__ fb) {
+                                                            float *__restrict__ fb, int wide) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   if ((int64_t)k >= shadowCap || k >= counters[1]) return;
@@ -354,7 +379,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
   float tb = r1.w;
   TraceCounters cnt;
-  const int leaf = traceClosest<false>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
+  const int leaf = traceAny<false>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
                                        stack + threadIdx.x, kRenderThreads, cnt);
   if (leaf < 0) return;
   const int id = __float_as_int(leaves[3 * (int64_t)leaf].w);
@@ -417,7 +442,8 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   hipStream_t s = ctx->stream;
 
   RenderArgs a;
-  a.nodes = ctx->nodes.as<float4>();
+  a.wide = ctx->traversal == 4 ? 1 : 0;
+  a.nodes = a.wide ? ctx->nodes4.as<float4>() : ctx->nodes.as<float4>();
   a.leaves = ctx->leaves.as<float4>();
   a.rec = ctx->rec.as<float4>();
   a.mats = ctx->mats.as<MatDev>();
@@ -447,9 +473,13 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(), npix * 7, 0.f);
   CPT_TRY_HIP(ctx, hipGetLastError());
 
-  double rays = 0, extMs = 0, shMs = 0, sdMs = 0;
+  double rays = 0, extMs = 0, shMs = 0, sdMs = 0, extRays = 0;
   int bounces = 0;
   uint32_t *cnt = R.counters.as<uint32_t>();
+  // counters: [0] next queue, [1] shadow queue; [8..11] two u64 work sums
+  unsigned long long *work = (unsigned long long *)(cnt + 8);
+  const bool countWork = (prm->flags & CPT_RENDER_COUNT_WORK) != 0;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
   hipEvent_t *ev = ctx->ev;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[12], s));
   for (int64_t base = 0; base < total; base += cap) {
@@ -462,10 +492,17 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     // paths end by roulette, depth cap, miss or absorption; the cap only
     // bounds pathological inputs (NaN geometry) so the loop always ends
     for (int pass = 0; qn > 0 && pass < kMaxBounces; pass++) {
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
-      hipLaunchKernelGGL(k_extend, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
-                         (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>());
+      if (countWork)
+        hipLaunchKernelGGL(k_extend<true>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
+                           (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>(),
+                           work, a.wide);
+      else
+        hipLaunchKernelGGL(k_extend<false>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
+                           (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>(),
+                           work, a.wide);
+      extRays += qn;
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
       hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qa, qn, R.ray.as<float4>(),
                          R.thr.as<float4>(), R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb,
@@ -474,7 +511,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
                          (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, a.nodes, a.leaves,
-                         a.nPrims, R.fb.as<float>());
+                         a.nPrims, R.fb.as<float>(), a.wide);
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
       uint32_t hc[2] = {0, 0};
@@ -525,6 +562,11 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     stats->shade_ms = shMs;
     stats->shadow_ms = sdMs;
     stats->bounces = bounces;
+    stats->extend_rays = extRays;
+    unsigned long long wk[2] = {0, 0};
+    if (countWork) CPT_TRY_HIP(ctx, hipMemcpy(wk, work, 16, hipMemcpyDeviceToHost));
+    stats->extend_nodes = (double)wk[0];
+    stats->extend_prims = (double)wk[1];
   }
   return CPT_OK;
 }

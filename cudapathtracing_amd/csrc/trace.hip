@@ -11,11 +11,12 @@ namespace cpt {
 
 constexpr int kTraceThreads = 256;
 
-template <bool kCount>
+template <bool kCount, bool kWide>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(
     const float4 *__restrict__ rays, int64_t nRays, const float4 *__restrict__ nodes,
-    const float4 *__restrict__ leaves, int nPrims, Scene64 s64, float *__restrict__ tOut,
-    int32_t *__restrict__ idOut, int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut) {
+    const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, int nPrims, Scene64 s64,
+    float *__restrict__ tOut, int32_t *__restrict__ idOut, int32_t *__restrict__ nodesOut,
+    int32_t *__restrict__ primsOut) {
   __shared__ int stack[kStackLds * kTraceThreads];
   const int64_t i = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
   if (i >= nRays) return;
@@ -23,8 +24,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   const f3 o = xyz(r0), d = xyz(r1);
   float tbest = r1.w;
   TraceCounters cnt;
-  const int leaf = traceClosest<kCount>(nodes, leaves, nPrims, o, d, r0.w, tbest,
-                                        stack + threadIdx.x, kTraceThreads, cnt);
+  const int leaf =
+      kWide ? traceClosest4<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt)
+            : traceClosest<kCount>(nodes, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt);
   if (kCount) {
     nodesOut[i] = cnt.nodes;
     primsOut[i] = cnt.prims;
@@ -46,15 +48,26 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   Scene64 s64{ctx->tri64.as<double>(), ctx->sph64.as<double>(), ctx->lgt64.as<double>(),
               ctx->nLights, ctx->nSpheres};
   const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
+  const float4 *r = (const float4 *)raysDev;
+  const float4 *nd = ctx->nodes.as<float4>(), *n4 = ctx->nodes4.as<float4>(), *lv = ctx->leaves.as<float4>();
+  const int np = (int)ctx->nPrims;
+  const bool wide = ctx->traversal == 4;
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
-  if (nodesDev)
-    hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(kTraceThreads), 0, ctx->stream,
-                       (const float4 *)raysDev, n, ctx->nodes.as<float4>(), ctx->leaves.as<float4>(),
-                       (int)ctx->nPrims, s64, tDev, idDev, nodesDev, primsDev);
-  else
-    hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(kTraceThreads), 0, ctx->stream,
-                       (const float4 *)raysDev, n, ctx->nodes.as<float4>(), ctx->leaves.as<float4>(),
-                       (int)ctx->nPrims, s64, tDev, idDev, nodesDev, primsDev);
+  if (nodesDev) {
+    if (wide)
+      hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv, np,
+                         s64, tDev, idDev, nodesDev, primsDev);
+    else
+      hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
+                         np, s64, tDev, idDev, nodesDev, primsDev);
+  } else {
+    if (wide)
+      hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
+                         np, s64, tDev, idDev, nodesDev, primsDev);
+    else
+      hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
+                         np, s64, tDev, idDev, nodesDev, primsDev);
+  }
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[9], ctx->stream));
   return CPT_OK;

@@ -129,6 +129,11 @@ int cpt_set_stream(cpt_ctx *ctx, void *hip_stream);
 int cpt_synchronize(cpt_ctx *ctx);
 /* Library build string (arch, version). */
 const char *cpt_version(void);
+/* Options.  CPT_OPT_TRAVERSAL_WIDTH: 4 (default) walks four-wide records
+ * built from the binary LBVH (each spans two binary levels); 2 walks the
+ * binary nodes.  Results are identical up to exact-t ties. */
+#define CPT_OPT_TRAVERSAL_WIDTH 1
+int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */
 /* Replaces Scene::add* + allocate*OnDevice (Scene.cuh:175-270,
@@ -240,8 +245,11 @@ typedef struct cpt_render_params {
   int32_t sample_base;  /* index of the first sample (progressive)    */
   int32_t want_aov;     /* fill albedo/normal/depth outputs           */
   int32_t paths_per_batch; /* 0 = automatic                          */
-  int32_t pad_;
+  int32_t flags;           /* CPT_RENDER_COUNT_WORK: count extension-  */
+                           /* kernel node visits and primitive tests   */
 } cpt_render_params;
+
+#define CPT_RENDER_COUNT_WORK 1
 
 typedef struct cpt_render_stats {
   double ms;
@@ -252,6 +260,9 @@ typedef struct cpt_render_stats {
   double shade_ms;
   int32_t bounces;
   int32_t pad_;
+  double extend_rays;  /* extension rays traced                        */
+  double extend_nodes; /* nodes visited by extension rays (COUNT_WORK) */
+  double extend_prims; /* primitives tested by extension rays (ditto)  */
 } cpt_render_stats;
 
 /* render<<<>>> (Render.cu:16-261) as a wavefront pipeline.  rgb: h*w*3

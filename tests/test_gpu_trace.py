@@ -45,18 +45,21 @@ def box_rays(n, seed):
 def check_small(ctx, s, rays):
     ctx.upload(s)
     ctx.build(s.bounds_min, s.bounds_max)
-    t, ids = ctx.trace(rays)
     o = O.OracleScene(s)
     tb, idb, cert = o.trace_brute(rays, eb=1e-4, et=1e-4)
     print(f"  certified {cert.mean():.5f} of {len(rays)} rays, hits {np.mean(idb >= 0):.3f}")
     assert cert.mean() > 0.95
-    bad = np.where(cert & (ids != idb))[0]
-    assert bad.size == 0, (bad[:10], ids[bad[:10]], idb[bad[:10]])
-    h = cert & (idb >= 0)
-    rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
-    assert rel.max() <= T_REL, rel.max()
-    miss = cert & (idb < 0)
-    assert np.all(t[miss] == rays[miss, 7])
+    for width in (2, 4):  # binary nodes and four-wide records
+        ctx.set_traversal_width(width)
+        t, ids = ctx.trace(rays)
+        bad = np.where(cert & (ids != idb))[0]
+        assert bad.size == 0, (width, bad[:10], ids[bad[:10]], idb[bad[:10]])
+        h = cert & (idb >= 0)
+        rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
+        assert rel.max() <= T_REL, (width, rel.max())
+        miss = cert & (idb < 0)
+        assert np.all(t[miss] == rays[miss, 7])
+    ctx.set_traversal_width(4)
     return cert.mean()
 
 
