@@ -222,7 +222,7 @@ def main():
     ap.add_argument("--tris", type=int, default=10_000_000)
     ap.add_argument("--rays", type=int, default=1 << 25)
     ap.add_argument("--cpu-rays", type=int, default=1 << 17)
-    ap.add_argument("--cpu-window", type=int, default=24)
+    ap.add_argument("--cpu-window", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbvh", action="store_true")
     ap.add_argument("--no-eq", action="store_true")

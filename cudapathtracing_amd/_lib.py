@@ -126,6 +126,8 @@ SIGNATURES = [
     ("cpt_gen_random_rays", C.c_int, [C.c_int64, C.c_uint64, C.c_double, C.c_double, C.c_float,
                                       C.c_float, P]),
     ("cpt_tri_bounds", C.c_int, [P, C.c_int64, dp, dp]),
+    ("cpt_scene_set_medium", C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, dp, dp, C.c_double, dp]),
+    ("cpt_gen_fbm_grid", C.c_int, [C.c_int32, C.c_uint64, P]),
 ]
 
 _lib = None
@@ -138,6 +140,11 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
                                "(the HIP library has no CPU fallback)")
+        # torch ships its own HIP runtime.  Loading torch first lets libcpt.so
+        # bind to that same runtime (same soname): two HIP runtimes in one
+        # process cannot both open the GPU, and torch tensors are passed to
+        # this library as device pointers.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

@@ -94,6 +94,18 @@ class Context:
                                                tri.shape[0]))
         self.n_prims = s.n_prims
         self.scene = s
+        if s.medium is not None:
+            self.set_medium(s.medium)
+
+    def set_medium(self, m):
+        """Heterogeneous medium (scenes.Medium) of the uploaded scene; None removes it."""
+        if m is None:
+            self._check(self._lib.cpt_scene_set_medium(self._h, None, 0, 0, 0, None, None, 0.0, None))
+            return
+        dens = np.ascontiguousarray(m.density, np.float32)
+        nz, ny, nx = dens.shape
+        self._check(self._lib.cpt_scene_set_medium(self._h, dens.ctypes.data, nx, ny, nz, L.d3(m.bmin),
+                                                   L.d3(m.bmax), float(m.sigma), L.d3(m.albedo)))
 
     def load_ini(self, path: str):
         mn, mx = (C.c_double * 3)(), (C.c_double * 3)()

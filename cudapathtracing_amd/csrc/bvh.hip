@@ -15,11 +15,13 @@
 //     -> k_emit: determineRange/findSplit on the full 64-bit key, child refs
 //        into the 64 B node, parent links for the refit.
 //   constructAABB<<<1,1>>> (BVH.cuh:243-280)
-//     -> k_refit: one thread per leaf climbs while it is the second child to
-//        arrive (agent-scope acq_rel counter per node); min/max union is exact
+//     -> k_refit_local + k_refit_cross: bottom-up climbs where the second
+//        child to arrive continues (LDS counters inside a chunk of sorted
+//        leaves, agent-scope counters across chunks); min/max union is exact
 //        so the result equals the reference's serial post-order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -287,8 +289,7 @@ __device__ __forceinline__ int deltaKey(const uint64_t *__restrict__ k, int n, i
 }
 
 __global__ void k_emit(const uint64_t *__restrict__ keys, int n, float4 *__restrict__ nodes,
-                       int32_t *__restrict__ parentInt, int32_t *__restrict__ parentLeaf,
-                       int32_t *__restrict__ flags) {
+                       int32_t *__restrict__ parentInt, int32_t *__restrict__ parentLeaf) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n - 1) return;
   // determineRange, BVH.cuh:130-172
@@ -327,10 +328,10 @@ __global__ void k_emit(const uint64_t *__restrict__ keys, int n, float4 *__restr
   }
   const int ca = (split == first) ? ~split : split;
   const int cb = (split + 1 == last) ? ~(split + 1) : (split + 1);
-  nodes[4 * (int64_t)idx + 3] = make_float4(__int_as_float(ca), __int_as_float(cb), 0.f, 0.f);
+  nodes[4 * (int64_t)idx + 3] =
+      make_float4(__int_as_float(ca), __int_as_float(cb), __int_as_float(first), __int_as_float(last));
   if (ca < 0) parentLeaf[~ca] = idx << 1; else parentInt[ca] = idx << 1;
   if (cb < 0) parentLeaf[~cb] = (idx << 1) | 1; else parentInt[cb] = (idx << 1) | 1;
-  flags[idx] = 0;
   if (idx == 0) parentInt[0] = -1;
 }
 
@@ -347,32 +348,97 @@ __device__ __forceinline__ void storeChildBox(float *__restrict__ nf, int64_t no
   p[8 + slot * 2 + 1] = bx[5];
 }
 
-__global__ void k_refit(int n, const uint32_t *__restrict__ order, const float4 *__restrict__ rec,
-                        const float *__restrict__ box32, float4 *__restrict__ nodes,
-                        const int32_t *__restrict__ parentInt,
-                        const int32_t *__restrict__ parentLeaf, int32_t *__restrict__ flags,
-                        float4 *__restrict__ leaves, float *__restrict__ rootBox) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t p = order[i];
-  leaves[3 * (int64_t)i + 0] = rec[3 * (int64_t)p + 0];
-  leaves[3 * (int64_t)i + 1] = rec[3 * (int64_t)p + 1];
-  leaves[3 * (int64_t)i + 2] = rec[3 * (int64_t)p + 2];
-  float bx[6];
-#pragma unroll
-  for (int k = 0; k < 6; k++) bx[k] = box32[6 * (int64_t)p + k];
-  if (n == 1) {
-#pragma unroll
-    for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
-    return;
+// Refit in two passes.  Pass 1 (k_refit_local): a workgroup owns a chunk of
+// kRefitChunk consecutive sorted leaves and finishes, bottom-up in LDS with
+// workgroup-scope counters, every internal node whose leaf range lies inside
+// the chunk.  Each child of a node that crosses a chunk boundary is written
+// to global memory and its parent appended to an escape list.  Pass 2
+// (k_refit_cross) climbs from the escapes through the crossing nodes with
+// agent-scope counters.  Internal node i of a Karras tree is the lowest
+// common ancestor of leaves split(i), split(i)+1, so the crossing nodes are
+// the ancestors of the chunk-boundary splits: at most 64 per boundary
+// (depth <= 64 for unique 64-bit keys), two escapes each.
+constexpr int kRefitThreads = 256;
+constexpr int kRefitChunk = 1024;
+
+int64_t refitEscapeBound(int64_t n) {
+  const int64_t chunks = (n + kRefitChunk - 1) / kRefitChunk;
+  return std::min<int64_t>(2 * n, 128 * chunks + 2);
+}
+
+__global__ __launch_bounds__(kRefitThreads) void k_refit_local(
+    int n, const uint32_t *__restrict__ order, const float4 *__restrict__ rec, const float *__restrict__ box32,
+    float4 *__restrict__ nodes, const int32_t *__restrict__ parentInt, const int32_t *__restrict__ parentLeaf,
+    int32_t *__restrict__ flags, float4 *__restrict__ leaves, float *__restrict__ rootBox,
+    int32_t *__restrict__ esc, uint32_t *__restrict__ escCount, int64_t escCap) {
+  __shared__ float cbox[kRefitChunk * 12];  // [node - c0][slot][minx miny minz maxx maxy maxz]
+  __shared__ int cnt[kRefitChunk];
+  const int c0 = blockIdx.x * kRefitChunk;
+  const int c1 = min(c0 + kRefitChunk, n) - 1;  // last leaf of the chunk
+  for (int k = threadIdx.x; k < kRefitChunk; k += kRefitThreads) {
+    cnt[k] = 0;
+    if (c0 + k < n - 1) flags[c0 + k] = 0;  // counters of pass 2
   }
+  __syncthreads();
   float *nf = (float *)nodes;
-  int par = parentLeaf[i];
-  int node = par >> 1;
-  storeChildBox(nf, node, par & 1, bx);
+  for (int r = 0; r < kRefitChunk / kRefitThreads; r++) {
+    const int i = c0 + r * kRefitThreads + (int)threadIdx.x;
+    if (i > c1) break;
+    const uint32_t p = order[i];
+    leaves[3 * (int64_t)i + 0] = rec[3 * (int64_t)p + 0];
+    leaves[3 * (int64_t)i + 1] = rec[3 * (int64_t)p + 1];
+    leaves[3 * (int64_t)i + 2] = rec[3 * (int64_t)p + 2];
+    float bx[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) bx[k] = box32[6 * (int64_t)p + k];
+    if (n == 1) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+      break;
+    }
+    int par = parentLeaf[i];
+    while (true) {
+      const int node = par >> 1, slot = par & 1;
+      storeChildBox(nf, node, slot, bx);
+      const float4 rg = nodes[4 * (int64_t)node + 3];  // z, w: first and last leaf
+      if (__float_as_int(rg.z) < c0 || __float_as_int(rg.w) > c1) {
+        const uint32_t e = atomicAdd(escCount, 1u);
+        if ((int64_t)e < escCap) esc[e] = node;
+        break;
+      }
+      float *s = cbox + (node - c0) * 12;
+#pragma unroll
+      for (int k = 0; k < 6; k++) s[slot * 6 + k] = bx[k];
+      const int old = __hip_atomic_fetch_add(&cnt[node - c0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;  // the sibling's thread continues
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        bx[k] = fminf(bx[k], s[(slot ^ 1) * 6 + k]);
+        bx[k + 3] = fmaxf(bx[k + 3], s[(slot ^ 1) * 6 + k + 3]);
+      }
+      if (node == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+        break;
+      }
+      par = parentInt[node];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kRefitThreads) void k_refit_cross(const int32_t *__restrict__ esc,
+                                                               const uint32_t *__restrict__ escCount,
+                                                               int64_t escCap, float4 *nodes,
+                                                               const int32_t *__restrict__ parentInt,
+                                                               int32_t *flags, float *__restrict__ rootBox) {
+  const int64_t e = (int64_t)blockIdx.x * kRefitThreads + threadIdx.x;
+  if (e >= escCap || e >= (int64_t)*escCount) return;
+  int node = esc[e];
+  float *nf = (float *)nodes;
+  float bx[6];
   while (true) {
     const int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;  // sibling subtree not finished: it will continue
+    if (old == 0) return;  // the other child is not finished: its walker continues
     const float4 a = nodes[4 * (int64_t)node + 0];
     const float4 b = nodes[4 * (int64_t)node + 1];
     const float4 c = nodes[4 * (int64_t)node + 2];
@@ -387,7 +453,7 @@ __global__ void k_refit(int n, const uint32_t *__restrict__ order, const float4 
       for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
       return;
     }
-    par = parentInt[node];
+    const int par = parentInt[node];
     node = par >> 1;
     storeChildBox(nf, node, par & 1, bx);
   }
@@ -543,27 +609,43 @@ int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysT
                     nullptr);
 }
 
+// Node boxes (two refit passes), sorted leaf records and four-wide records.
+static int launchRefit(cpt_ctx *ctx, int64_t n) {
+  const int64_t cap = refitEscapeBound(n);
+  int rc = ensure(ctx, ctx->esc, (size_t)(cap + 64) * 4);
+  if (rc) return rc;
+  int32_t *esc = ctx->esc.as<int32_t>();
+  uint32_t *escCount = (uint32_t *)(esc + cap);
+  const int bs = 256;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)escCount, (int64_t)1);
+  const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
+  hipLaunchKernelGGL(k_refit_local, dim3(chunks), dim3(kRefitThreads), 0, ctx->stream, (int)n,
+                     ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
+                     ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>(),
+                     ctx->flags.as<int32_t>(), ctx->leaves.as<float4>(), ctx->rootBox.as<float>(), esc, escCount,
+                     cap);
+  if (n > 1) {
+    hipLaunchKernelGGL(k_refit_cross, dim3((unsigned)((cap + kRefitThreads - 1) / kRefitThreads)),
+                       dim3(kRefitThreads), 0, ctx->stream, (const int32_t *)esc, (const uint32_t *)escCount, cap,
+                       ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->flags.as<int32_t>(),
+                       ctx->rootBox.as<float>());
+    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
+                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
+  }
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  return CPT_OK;
+}
+
 // Node boxes from the current primitives with the topology, parents and
 // leaf order of the last build.
 int refitBvh(cpt_ctx *ctx, cpt_build_stats *st) {
   const int64_t n = ctx->nPrims;
   if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_bvh_refit", "no BVH built");
   if (n == 0) return CPT_OK;
-  const int bs = 256;
   hipEvent_t *ev = ctx->ev;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-  if (n > 1)
-    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
-                       ctx->flags.as<int32_t>(), n - 1);
-  hipLaunchKernelGGL(k_refit, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                     ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
-                     ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
-                     ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
-                     ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
-  if (n > 1)
-    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
-  CPT_TRY_HIP(ctx, hipGetLastError());
+  int rc = launchRefit(ctx, n);
+  if (rc) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
   CPT_TRY_HIP(ctx, hipEventSynchronize(ev[1]));
   if (st) {
@@ -619,20 +701,11 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
   if (n > 1) {
     hipLaunchKernelGGL(k_emit, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream,
                        ctx->keys64.as<uint64_t>(), (int)n, ctx->nodes.as<float4>(),
-                       ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>(),
-                       ctx->flags.as<int32_t>());
+                       ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>());
     CPT_TRY_HIP(ctx, hipGetLastError());
   }
   CPT_TRY_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-  hipLaunchKernelGGL(k_refit, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                     ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
-                     ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
-                     ctx->parentLeaf.as<int32_t>(), ctx->flags.as<int32_t>(),
-                     ctx->leaves.as<float4>(), ctx->rootBox.as<float>());
-  if (n > 1)
-    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
-  CPT_TRY_HIP(ctx, hipGetLastError());
+  if ((rc = launchRefit(ctx, n))) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
   if (st) {
     CPT_TRY_HIP(ctx, hipEventSynchronize(ev[4]));

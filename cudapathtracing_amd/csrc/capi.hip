@@ -57,7 +57,7 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->valsA,    &ctx->valsB,  &ctx->hist,      &ctx->rowTotal,   &ctx->keys64,
                    &ctx->order,    &ctx->nodes,  &ctx->parentInt, &ctx->parentLeaf, &ctx->flags,
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
-                   &ctx->cntA,     &ctx->cntB,    &ctx->nodes4};
+                   &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc, &ctx->medDensity};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -174,6 +174,7 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
         return fail(ctx, CPT_ERR_ARG, "cpt_scene_upload", "triangle material index out of range");
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   ctx->built = false;
+  ctx->hasMedium = false;
   int rc;
   if ((rc = ensure(ctx, ctx->center64, n * 24)) || (rc = ensure(ctx, ctx->box32, n * 24)) ||
       (rc = ensure(ctx, ctx->rec, n * 48)) || (rc = ensure(ctx, ctx->tri64, nt * 72)) ||
@@ -303,6 +304,40 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri, i
   hipFree(mat.p);
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_update_triangles", hipGetErrorString(e));
+  return CPT_OK;
+}
+
+int cpt_scene_set_medium(cpt_ctx *ctx, const float *density, int32_t nx, int32_t ny, int32_t nz,
+                         const double bmin[3], const double bmax[3], double sigmaMax, const double albedo[3]) {
+  if (!ctx) return CPT_ERR_ARG;
+  if (!density || nx <= 0 || ny <= 0 || nz <= 0) {
+    ctx->hasMedium = false;
+    return CPT_OK;
+  }
+  if (!bmin || !bmax || !albedo || !(sigmaMax > 0) || nx > 4096 || ny > 4096 || nz > 4096)
+    return fail(ctx, CPT_ERR_ARG, "cpt_scene_set_medium", "bad grid, box, sigma or albedo");
+  for (int k = 0; k < 3; k++)
+    if (!(bmax[k] > bmin[k])) return fail(ctx, CPT_ERR_ARG, "cpt_scene_set_medium", "empty box");
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t n = (size_t)nx * ny * nz;
+  // delta and ratio tracking treat density as a probability of a real collision
+  for (size_t i = 0; i < n; i++)
+    if (!(density[i] >= 0.0f && density[i] <= 1.0f))
+      return fail(ctx, CPT_ERR_ARG, "cpt_scene_set_medium", "density outside [0, 1]");
+  int rc = ensure(ctx, ctx->medDensity, n * 4);
+  if (rc) return rc;
+  CPT_TRY_HIP(ctx, hipMemcpyAsync(ctx->medDensity.p, density, n * 4, hipMemcpyHostToDevice, ctx->stream));
+  CPT_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; k++) {
+    ctx->medBox[k] = (float)bmin[k];
+    ctx->medBox[3 + k] = (float)bmax[k];
+    ctx->medAlbedo[k] = (float)albedo[k];
+  }
+  ctx->medN[0] = nx;
+  ctx->medN[1] = ny;
+  ctx->medN[2] = nz;
+  ctx->medSigma = (float)sigmaMax;
+  ctx->hasMedium = true;
   return CPT_OK;
 }
 

@@ -27,7 +27,7 @@ constexpr int kStackTotal = 64;  // Karras depth <= 62 for unique 64-bit keys
 struct f3 {
   float x, y, z;
 };
-__device__ __forceinline__ f3 mkf3(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ __forceinline__ f3 mkf3(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mkf3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mkf3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ f3 operator-(f3 a) { return mkf3(-a.x, -a.y, -a.z); }
@@ -66,22 +66,26 @@ __host__ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t
 // Sample stream: draw k of (pixel p, sample s) is word k&3 of
 // philox(ctr = {k>>2, p, s, 0x43505431}, key = seed), mapped to
 // ((x >> 8) + 1) * 2^-24 in (0, 1] -- exact in both FP32 and FP64.
+// A shadow ray's transmittance stream uses the tag 0x80000000 | (path draw
+// count when the shadow ray was made) instead of 'CPT1'.
 struct Rng {
-  uint32_t k0, k1, pix, smp, cnt;
+  uint32_t k0, k1, pix, smp, cnt, tag;
   uint32_t blk;
   u4 cache;
-  __device__ __forceinline__ void init(uint64_t seed, uint32_t p, uint32_t s, uint32_t c) {
+  __device__ __forceinline__ void init(uint64_t seed, uint32_t p, uint32_t s, uint32_t c,
+                                       uint32_t t = 0x43505431u) {
     k0 = (uint32_t)seed;
     k1 = (uint32_t)(seed >> 32);
     pix = p;
     smp = s;
     cnt = c;
+    tag = t;
     blk = 0xffffffffu;
   }
   __device__ __forceinline__ float next() {
     uint32_t b = cnt >> 2;
     if (b != blk) {
-      cache = philox4x32_10(u4{b, pix, smp, 0x43505431u}, k0, k1);
+      cache = philox4x32_10(u4{b, pix, smp, tag}, k0, k1);
       blk = b;
     }
     uint32_t w = cnt & 3;

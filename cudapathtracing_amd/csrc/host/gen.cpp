@@ -204,6 +204,19 @@ int cpt_gen_random_rays(int64_t n, uint64_t seed, double lo, double hi, float tm
   return CPT_OK;
 }
 
+int cpt_gen_fbm_grid(int32_t n, uint64_t seed, float *density) {
+  if (n <= 0 || n > 1024 || !density) return CPT_ERR_ARG;
+  const int64_t total = (int64_t)n * n * n;
+  parallelFor(total, [&](int64_t i) {
+    const int x = (int)(i % n), y = (int)((i / n) % n), z = (int)(i / ((int64_t)n * n));
+    V p{(x + 0.5) / n, (y + 0.5) / n, (z + 0.5) / n};
+    // fBm in [-1,1] -> density with contrast, clamped to [0,1]
+    double v = 0.5 + 1.2 * valueNoise(p, seed);
+    density[i] = (float)(v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v));
+  });
+  return CPT_OK;
+}
+
 int cpt_tri_bounds(const double *tri, int64_t n, double mn[3], double mx[3]) {
   if (!mn || !mx || (n > 0 && !tri)) return CPT_ERR_ARG;
   for (int k = 0; k < 3; k++) {

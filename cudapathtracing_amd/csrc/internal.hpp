@@ -57,6 +57,13 @@ struct cpt_ctx {
   cpt::DevBuf codesA, codesB, valsA, valsB, hist, rowTotal;
   cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
   cpt::DevBuf nodes4;   // (N-1) x 128 B four-wide records (grandchildren of each node)
+  cpt::DevBuf esc;      // refit escape list + its counter
+  // heterogeneous medium (density grid), cleared by cpt_scene_upload
+  bool hasMedium = false;
+  cpt::DevBuf medDensity;
+  float medBox[6] = {0, 0, 0, 0, 0, 0};
+  float medSigma = 0.f, medAlbedo[3] = {0, 0, 0};
+  int medN[3] = {0, 0, 0};
   int traversal = 4;    // 2: binary node walk, 4: four-wide records
 
   // traversal I/O staging

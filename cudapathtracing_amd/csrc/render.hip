@@ -45,6 +45,67 @@ struct CamDev {
   int w, h;
 };
 
+// Heterogeneous medium: a density grid in [0,1] (x fastest) over an
+// axis-aligned box; sigma_t(x) = sigma * density(nearest voxel).
+struct MediumDev {
+  const float *density;
+  f3 bmin, bmax, scale;  // scale = voxels per unit length
+  int nx, ny, nz, on;
+  float sigma;
+  f3 albedo;
+};
+
+constexpr int kMaxCollisions = 100000;
+
+// [t0, t1] = [tmin, tmax] clipped to the medium box (slab test); false if empty.
+__device__ __forceinline__ bool clipBox(const MediumDev &md, f3 o, f3 d, float tmin, float tmax, float &t0,
+                                        float &t1) {
+  t0 = tmin;
+  t1 = tmax;
+  const float ox[3] = {o.x, o.y, o.z}, dx[3] = {d.x, d.y, d.z};
+  const float lo[3] = {md.bmin.x, md.bmin.y, md.bmin.z}, hi[3] = {md.bmax.x, md.bmax.y, md.bmax.z};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float inv = 1.0f / dx[k];
+    float ta = (lo[k] - ox[k]) * inv, tb = (hi[k] - ox[k]) * inv;
+    if (ta > tb) {
+      const float tmp = ta;
+      ta = tb;
+      tb = tmp;
+    }
+    t0 = fmaxf(t0, ta);
+    t1 = fminf(t1, tb);
+  }
+  return t0 < t1;
+}
+
+__device__ __forceinline__ float densityAt(const MediumDev &md, f3 p) {
+  const float fx = fminf(fmaxf(floorf((p.x - md.bmin.x) * md.scale.x), 0.0f), (float)(md.nx - 1));
+  const float fy = fminf(fmaxf(floorf((p.y - md.bmin.y) * md.scale.y), 0.0f), (float)(md.ny - 1));
+  const float fz = fminf(fmaxf(floorf((p.z - md.bmin.z) * md.scale.z), 0.0f), (float)(md.nz - 1));
+  return md.density[((int64_t)(int)fz * md.ny + (int)fy) * md.nx + (int)fx];
+}
+
+// Ratio tracking (Novak et al. 2014) of the transmittance over the part of
+// [tmin, tmax] inside the box: the product of (1 - density) at tentative
+// collisions.  Stream: (pixel, sample) with tag 0x80000000 | path draw count
+// when the shadow ray was made.
+__device__ __forceinline__ float ratioTrack(const MediumDev &md, uint64_t seed, uint32_t pixel, uint32_t sample,
+                                            uint32_t pathCnt, f3 o, f3 d, float tmin, float tmax) {
+  float t0, t1;
+  if (!clipBox(md, o, d, tmin, tmax, t0, t1)) return 1.0f;
+  Rng rng;
+  rng.init(seed, pixel, sample, 0, 0x80000000u | pathCnt);
+  const float inv = 1.0f / (md.sigma * len3(d));
+  float t = t0, tr = 1.0f;
+  for (int guard = 0; guard < kMaxCollisions; guard++) {
+    t -= __logf(1.0f - rng.next()) * inv;
+    if (!(t < t1)) break;
+    tr *= 1.0f - densityAt(md, o + d * t);
+  }
+  return tr;
+}
+
 struct RenderArgs {
   const float4 *nodes, *leaves, *rec;  // nodes: binary nodes or four-wide records
   int wide;
@@ -55,6 +116,7 @@ struct RenderArgs {
   int spp, sampleBase, profile, maxDepth, rrDepth, wantAov;
   float rrProb;
   int64_t npix;
+  MediumDev med;
 };
 
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
@@ -169,6 +231,24 @@ __device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int
   return xyz(r1);
 }
 
+// Delta tracking of the heterogeneous medium along [t0, t1] (Woodcock):
+// tentative collisions at exponential steps of the majorant, each real with
+// probability density(x).  Draws: one per step, one per acceptance test.
+__device__ __forceinline__ bool deltaTrack(const MediumDev &md, Rng &rng, f3 o, f3 d, float t0, float t1, f3 &x) {
+  const float inv = 1.0f / (md.sigma * len3(d));
+  float t = t0;
+  for (int guard = 0; guard < kMaxCollisions; guard++) {
+    t -= __logf(1.0f - rng.next()) * inv;
+    if (!(t < t1)) return false;
+    const f3 p = o + d * t;
+    if (rng.next() < densityAt(md, p)) {
+      x = p;
+      return true;
+    }
+  }
+  return false;
+}
+
 __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, const uint32_t *__restrict__ q, uint32_t count, float4 *__restrict__ ray,
     float4 *__restrict__ thr, uint4 *__restrict__ meta, const float2 *__restrict__ hit,
@@ -200,13 +280,24 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   f3 o = xyz(o4), d = xyz(d4);
   float *pix = fb + 3 * (int64_t)mt.x;
   float *pa = a.wantAov ? aov + 7 * (int64_t)mt.x : nullptr;
-  // NEE state for this lane
-  int neeLight = -1;
-  f3 neeO = mkf3(0, 0, 0), neeD = neeO, neeC = neeO;
-  bool neeWant = false;
 
+  // scattering vertex: a surface with a BSDF, or a real collision in the medium
+  bool vtx = false, inMedium = false;
+  f3 hp = mkf3(0, 0, 0), n = hp;
+  ShadeMat m;
+  m.type = CPT_M_NONE;
+  bool front = true;
   if (active) {
-    if (leaf < 0) {
+    if (a.med.on) {
+      float t0, t1;
+      if (clipBox(a.med, o, d, o4.w, leaf >= 0 ? th : d4.w, t0, t1) &&
+          deltaTrack(a.med, rng, o, d, t0, t1, hp)) {
+        vtx = true;
+        inMedium = true;
+      }
+    }
+    if (inMedium) {
+    } else if (leaf < 0) {
       // miss: sky colour (Render.cu:138-142)
       const f3 bg = mkf3(a.cam.bg[0], a.cam.bg[1], a.cam.bg[2]);
       atomicAdd(pix + 0, T.x * bg.x);
@@ -222,25 +313,25 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
                    r2 = a.leaves[3 * (int64_t)leaf + 2];
       const int type = __float_as_int(r1.w);
       const int mi = __float_as_int(r2.w);
-      const ShadeMat m = loadMat(a.mats, mi);
-      f3 hp = o + d * th;
+      m = loadMat(a.mats, mi);
+      hp = o + d * th;
       f3 outw = outwardNormal(r0, r1, r2, type, hp);
-      const bool front = dot3(d, outw) < 0.0f;
-      f3 n = front ? outw : -outw;
+      front = dot3(d, outw) < 0.0f;
+      n = front ? outw : -outw;
       if (m.type == CPT_M_VOLUME) {
         // homogeneous medium, sigma_maj = 25 (Render.cu:64-135)
         const float sig = 25.0f;
         TraceCounters cnt;
         float tb = kRayTMax;
         int l2 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb,
-                                     stack + threadIdx.x, kRenderThreads, cnt);
+                                 stack + threadIdx.x, kRenderThreads, cnt);
         o = hp;
         if (l2 >= 0) {
           float dMax = tb;
           float t = -__logf(1.0f - rng.next()) / sig;
           f3 pos = hp + d * t;
           o = pos;
-          for (int guard = 0; t < dMax && guard < 100000; guard++) {
+          for (int guard = 0; t < dMax && guard < kMaxCollisions; guard++) {
             if (rng.next() > 0.8f) break;
             T = T * (__expf(-t * sig) / 0.8f);
             float z = 1.0f - 2.0f * rng.next();
@@ -253,7 +344,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
             pos = hp + d * t;
             float tb2 = kRayTMax;
             int l3 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
-                                         stack + threadIdx.x, kRenderThreads, cnt);
+                                     stack + threadIdx.x, kRenderThreads, cnt);
             if (l3 >= 0) dMax = tb2;
             t = -__logf(1.0f - rng.next()) / sig;
             if (t > dMax || l3 < 0) break;
@@ -277,82 +368,106 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
           atomicAdd(pa + 6, th * len3(d));
         }
       } else {
-        if (pa && !(flags & kFlagAov)) {
-          atomicAdd(pa + 0, m.color.x);
-          atomicAdd(pa + 1, m.color.y);
-          atomicAdd(pa + 2, m.color.z);
-          atomicAdd(pa + 3, n.x);
-          atomicAdd(pa + 4, n.y);
-          atomicAdd(pa + 5, n.z);
-          atomicAdd(pa + 6, len3(mkf3((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2]) - hp));
+        vtx = true;
+      }
+    }
+  }
+  const f3 alb = inMedium ? a.med.albedo : m.color;
+  if (vtx) {
+    if (pa && !(flags & kFlagAov)) {
+      atomicAdd(pa + 0, alb.x);
+      atomicAdd(pa + 1, alb.y);
+      atomicAdd(pa + 2, alb.z);
+      atomicAdd(pa + 3, n.x);
+      atomicAdd(pa + 4, n.y);
+      atomicAdd(pa + 5, n.z);
+      atomicAdd(pa + 6, len3(mkf3((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2]) - hp));
+    }
+    flags |= kFlagAov;
+  }
+  const f3 ud = unit3(d);
+  const f3 V = -ud;
+  // NEE to every light (Render.cu:161-187): opaque-type surfaces (Render.cu:166)
+  // and medium collisions.  The ballots in waveAppend need every lane of the
+  // wave, so the loop runs on all lanes with a per-lane predicate.
+  const bool neeHere = vtx && (inMedium || m.type == CPT_M_OPAQUE || m.type == CPT_M_LAMBERT ||
+                               m.type == CPT_M_GGX);
+  const int nLoop = __builtin_amdgcn_readfirstlane(a.nLights);
+  if (__ballot(neeHere) != 0) {
+    for (int li = 0; li < nLoop; li++) {
+      bool want = false;
+      f3 dir = mkf3(0, 0, 0), c = dir;
+      if (neeHere) {
+        const float4 l0 = a.rec[3 * li], l1 = a.rec[3 * li + 1], l2 = a.rec[3 * li + 2];
+        const f3 lc = xyz(l0), ln = xyz(l1);
+        const float lw = l2.x;
+        // the reference uses the width for both extents (Render.cu:170-171)
+        const float lh = a.profile == CPT_PROFILE_REFERENCE ? l2.x : l2.y;
+        f3 eu, ev;
+        lightFrame(ln, eu, ev);
+        const float u1 = rng.next(), u2 = rng.next();
+        const f3 lp = lc + eu * ((u1 - 0.5f) * lw) + ev * ((u2 - 0.5f) * lh);
+        const f3 toL = lp - hp;
+        dir = unit3(toL);
+        const float cl = dot3(-dir, ln);
+        if ((inMedium || dot3(dir, n) > 0.0f) && cl > 0.0f && l2.z != 0.0f) {
+          const ShadeMat lm = loadMat(a.mats, __float_as_int(l2.w));
+          // surface: fr cos; medium: albedo times the isotropic phase 1/(4 pi)
+          const f3 f = inMedium ? alb * (0.25f / kPi) : bsdfEval(m, V, dir, n, mkf3(0, 0, 0), front) * dot3(dir, n);
+          c = T * f * lm.color * (cl / dot3(toL, toL) * (lw * lh));
+          want = true;
         }
-        flags |= kFlagAov;
-        const f3 ud = unit3(d);
-        const f3 V = -ud;
-        // NEE to every light (only for opaque-type surfaces, Render.cu:166)
-        const bool neeMat = m.type == CPT_M_OPAQUE || m.type == CPT_M_LAMBERT || m.type == CPT_M_GGX;
-        if (neeMat && a.nLights > 0) {
-          for (int li = 0; li < a.nLights; li++) {
-            const float4 l0 = a.rec[3 * li], l1 = a.rec[3 * li + 1], l2 = a.rec[3 * li + 2];
-            const f3 lc = xyz(l0), ln = xyz(l1);
-            const float lw = l2.x;
-            // the reference uses the width for both extents (Render.cu:170-171)
-            const float lh = a.profile == CPT_PROFILE_REFERENCE ? l2.x : l2.y;
-            f3 eu, ev;
-            lightFrame(ln, eu, ev);
-            const float u1 = rng.next(), u2 = rng.next();
-            const f3 lp = lc + eu * ((u1 - 0.5f) * lw) + ev * ((u2 - 0.5f) * lh);
-            const f3 toL = lp - hp;
-            const f3 dir = unit3(toL);
-            if (dot3(dir, n) > 0.0f && dot3(-dir, ln) > 0.0f && l2.z != 0.0f) {
-              const ShadeMat lm = loadMat(a.mats, __float_as_int(l2.w));
-              const f3 fr = bsdfEval(m, V, dir, n, mkf3(0, 0, 0), front);
-              const float g = dot3(dir, n) * dot3(-dir, ln) / dot3(toL, toL) * (lw * lh);
-              neeC = T * fr * lm.color * g;
-              neeO = hp;
-              neeD = dir;
-              neeLight = li;
-              neeWant = true;
-            }
-            flags |= kFlagNee;
-            // append this light's shadow ray (ballot respects the active lanes)
-            const uint32_t slot = waveAppend(counters + 1, neeWant);
-            if (neeWant && slot < shadowCap) {
-              shadow[4 * (int64_t)slot + 0] = make_float4(neeO.x, neeO.y, neeO.z, kRayTMin);
-              shadow[4 * (int64_t)slot + 1] = make_float4(neeD.x, neeD.y, neeD.z, kRayTMax);
-              shadow[4 * (int64_t)slot + 2] = make_float4(neeC.x, neeC.y, neeC.z, __int_as_float((int)mt.x));
-              shadow[4 * (int64_t)slot + 3] = make_float4(__int_as_float(neeLight), 0.f, 0.f, 0.f);
-            }
-            neeWant = false;
-          }
-        }
-        // Russian roulette / depth cap (Render.cu:189-204)
-        const uint32_t depth = flags & kDepthMask;
-        float prr = 1.0f;
-        bool cont = true;
-        if (a.profile == CPT_PROFILE_REFERENCE) {
-          prr = 0.7f;
-          if (rng.next() > prr) cont = false;
-        } else {
-          if ((int)depth + 1 >= a.maxDepth) cont = false;
-          else if ((int)depth >= a.rrDepth) {
-            prr = a.rrProb;
-            if (rng.next() > prr) cont = false;
-          }
-        }
-        if (cont) {
-          flags = (flags & ~kDepthMask) | ((depth + 1) & kDepthMask);
-          const float r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
-          f3 dir = mkf3(0, 0, 1), wm = mkf3(0, 0, 0);
-          float pdf = -1.0f;
-          bsdfSample(m, ud, n, front, r1, r2, r3, dir, pdf, wm);
-          if (pdf > 0.0f) {
-            const f3 fr = bsdfEval(m, V, unit3(dir), n, wm, front);
-            T = T * fr * (fabsf(dot3(dir, n)) / pdf / prr);
-            o = hp;
-            d = dir;
-            alive = true;
-          }
+        flags |= kFlagNee;
+      }
+      const uint32_t slot = waveAppend(counters + 1, want);
+      if (want && slot < shadowCap) {
+        shadow[4 * (int64_t)slot + 0] = make_float4(hp.x, hp.y, hp.z, kRayTMin);
+        shadow[4 * (int64_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, kRayTMax);
+        shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)mt.x));
+        // light id, and the stream of the transmittance estimate
+        shadow[4 * (int64_t)slot + 3] =
+            make_float4(__int_as_float(li), __uint_as_float(mt.y), __uint_as_float(rng.cnt), 0.f);
+      }
+    }
+  }
+  if (vtx) {
+    // Russian roulette / depth cap (Render.cu:189-204)
+    const uint32_t depth = flags & kDepthMask;
+    float prr = 1.0f;
+    bool cont = true;
+    if (a.profile == CPT_PROFILE_REFERENCE) {
+      prr = 0.7f;
+      if (rng.next() > prr) cont = false;
+    } else {
+      if ((int)depth + 1 >= a.maxDepth) cont = false;
+      else if ((int)depth >= a.rrDepth) {
+        prr = a.rrProb;
+        if (rng.next() > prr) cont = false;
+      }
+    }
+    if (cont) {
+      flags = (flags & ~kDepthMask) | ((depth + 1) & kDepthMask);
+      const float r1 = rng.next(), r2 = rng.next(), r3 = rng.next();
+      if (inMedium) {
+        // isotropic phase function: weight albedo (phase / pdf = 1)
+        const float z = 1.0f - 2.0f * r1;
+        const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float s, cph;
+        __sincosf(2.0f * kPi * r2, &s, &cph);
+        T = T * alb * (1.0f / prr);
+        o = hp;
+        d = mkf3(rr * cph, rr * s, z);
+        alive = true;
+      } else {
+        f3 dir = mkf3(0, 0, 1), wm = mkf3(0, 0, 0);
+        float pdf = -1.0f;
+        bsdfSample(m, ud, n, front, r1, r2, r3, dir, pdf, wm);
+        if (pdf > 0.0f) {
+          const f3 fr = bsdfEval(m, V, unit3(dir), n, wm, front);
+          T = T * fr * (fabsf(dot3(dir, n)) / pdf / prr);
+          o = hp;
+          d = dir;
+          alive = true;
         }
       }
     }
@@ -367,11 +482,15 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   }
 }
 
+// Shadow rays: the light's contribution counts when the light itself is the
+// closest hit (Render.cu:180-185), times the medium's transmittance from a
+// ratio-tracking estimate when a medium is present.
 __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow, int64_t shadowCap,
                                                             const uint32_t *__restrict__ counters,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
-                                                            float *__restrict__ fb, int wide) {
+                                                            float *__restrict__ fb, int wide, MediumDev med,
+                                                            uint64_t seed) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   if ((int64_t)k >= shadowCap || k >= counters[1]) return;
@@ -380,14 +499,18 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
   float tb = r1.w;
   TraceCounters cnt;
   const int leaf = traceAny<false>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
-                                       stack + threadIdx.x, kRenderThreads, cnt);
+                                   stack + threadIdx.x, kRenderThreads, cnt);
   if (leaf < 0) return;
   const int id = __float_as_int(leaves[3 * (int64_t)leaf].w);
   if (id != __float_as_int(l.x)) return;
-  float *pix = fb + 3 * (int64_t)__float_as_int(c.w);
-  atomicAdd(pix + 0, c.x);
-  atomicAdd(pix + 1, c.y);
-  atomicAdd(pix + 2, c.z);
+  const uint32_t pixel = (uint32_t)__float_as_int(c.w);
+  float tr = 1.0f;
+  if (med.on) tr = ratioTrack(med, seed, pixel, __float_as_uint(l.y), __float_as_uint(l.z), xyz(r0), xyz(r1),
+                              r0.w, tb);
+  float *pix = fb + 3 * (int64_t)pixel;
+  atomicAdd(pix + 0, c.x * tr);
+  atomicAdd(pix + 1, c.y * tr);
+  atomicAdd(pix + 2, c.z * tr);
 }
 
 __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ out, int64_t n, float scale) {
@@ -467,6 +590,18 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   a.rrProb = (float)prm->rr_prob;
   a.wantAov = aovOut ? 1 : 0;
   a.npix = npix;
+  a.med.on = ctx->hasMedium ? 1 : 0;
+  a.med.density = ctx->medDensity.as<float>();
+  a.med.bmin = mkf3(ctx->medBox[0], ctx->medBox[1], ctx->medBox[2]);
+  a.med.bmax = mkf3(ctx->medBox[3], ctx->medBox[4], ctx->medBox[5]);
+  a.med.scale = mkf3((float)ctx->medN[0] / (ctx->medBox[3] - ctx->medBox[0]),
+                     (float)ctx->medN[1] / (ctx->medBox[4] - ctx->medBox[1]),
+                     (float)ctx->medN[2] / (ctx->medBox[5] - ctx->medBox[2]));
+  a.med.nx = ctx->medN[0];
+  a.med.ny = ctx->medN[1];
+  a.med.nz = ctx->medN[2];
+  a.med.sigma = ctx->medSigma;
+  a.med.albedo = mkf3(ctx->medAlbedo[0], ctx->medAlbedo[1], ctx->medAlbedo[2]);
 
   hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), npix * 3, 0.f);
   if (aovOut)
@@ -511,7 +646,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
                          (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, a.nodes, a.leaves,
-                         a.nPrims, R.fb.as<float>(), a.wide);
+                         a.nPrims, R.fb.as<float>(), a.wide, a.med, a.seed);
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
       uint32_t hc[2] = {0, 0};

@@ -31,6 +31,26 @@ from . import _lib as L
 
 
 @dataclass
+class Medium:
+    """Heterogeneous medium: density grid in [0,1] of shape (nz, ny, nx),
+    sigma_t = sigma * density of the nearest voxel, over [bmin, bmax]."""
+    density: np.ndarray
+    bmin: tuple = (-0.5, -0.5, -0.5)
+    bmax: tuple = (0.5, 0.5, 0.5)
+    sigma: float = 8.0
+    albedo: tuple = (0.8, 0.8, 0.8)
+
+
+def fbm_grid(n: int, seed: int = 0) -> np.ndarray:
+    """Seeded 4-octave value-noise fBm density grid, shape (n, n, n)."""
+    out = np.empty((n, n, n), np.float32)
+    rc = L.lib().cpt_gen_fbm_grid(n, seed, out.ctypes.data)
+    if rc:
+        raise ValueError("cpt_gen_fbm_grid: bad size")
+    return out
+
+
+@dataclass
 class SceneData:
     materials: List[L.Material] = field(default_factory=list)
     lights: List[L.Light] = field(default_factory=list)
@@ -40,6 +60,7 @@ class SceneData:
     bounds_min: np.ndarray = field(default_factory=lambda: np.full(3, 999999.0))
     bounds_max: np.ndarray = field(default_factory=lambda: np.full(3, -999999.0))
     camera: Optional[L.CameraDesc] = None
+    medium: Optional[Medium] = None
 
     @property
     def n_prims(self) -> int:
@@ -185,6 +206,14 @@ def displaced_icosphere(width=1920, height=1080, seed=0, level=8) -> SceneData:
     mats = base + (np.arange(n) // patch)
     s.add_mesh(tri, mats.astype(np.int32))
     s.camera = default_camera(width, height)
+    return s
+
+
+def displaced_icosphere_medium(width=1920, height=1080, seed=0, level=8, grid=256) -> SceneData:
+    """The displaced-icosphere scene with a grid^3 fBm density medium
+    (sigma_t max 8, albedo 0.8) filling the box."""
+    s = displaced_icosphere(width, height, seed, level)
+    s.medium = Medium(fbm_grid(grid, seed))
     return s
 
 

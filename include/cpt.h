@@ -281,6 +281,21 @@ int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
 int cpt_tonemap_rgba8(const float *rgb, int32_t width, int32_t height,
                       uint8_t *rgba8);
 
+/* ---- heterogeneous medium (no reference counterpart; the reference has only
+ * the homogeneous sigma=25 sphere hack of Render.cu:64-135) -------------- */
+/* A density grid density[ix + nx*(iy + ny*iz)] in [0,1] over the box
+ * [bmin, bmax]; sigma_t(x) = sigma_max * density(voxel of x), nearest-voxel
+ * lookup; isotropic phase; single-scattering albedo `albedo`.  Free flights
+ * are sampled by delta tracking with majorant sigma_max, shadow-ray
+ * transmittance by ratio tracking.  n = 0 (or density NULL) removes the
+ * medium.  Call after cpt_scene_upload. */
+int cpt_scene_set_medium(cpt_ctx *ctx, const float *density, int32_t nx,
+                         int32_t ny, int32_t nz, const double bmin[3],
+                         const double bmax[3], double sigma_max,
+                         const double albedo[3]);
+/* Seeded 4-octave fBm value-noise density grid (n^3 floats in [0,1]). */
+int cpt_gen_fbm_grid(int32_t n, uint64_t seed, float *density);
+
 /* ---- non-learned denoiser ---------------------------------------------- */
 /* Separable joint bilateral filter, gaussianSeparate (Render.cu:353-398)
  * horizontal then vertical (main.cu:100-104), radius 16.  rgb: w*h*3,

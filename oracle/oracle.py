@@ -64,6 +64,9 @@ def _register_render(L):
 RENDER_SIGS = {
     "orc_philox4x32_10": (None, [P, C.c_uint32, C.c_uint32, P]),
     "orc_render": (C.c_int, [P, P, P, P, P, C.c_int]),
+    "orc_scene_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_double, P]),
+    "orc_medium_transmittance": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
+    "orc_medium_collide": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
     "orc_bsdf": (C.c_int, [P, dp, dp, C.c_int, dp, dp, dp, dp]),
     "orc_bsdf_eval": (C.c_int, [P, dp, dp, C.c_int, dp, dp]),
     "orc_ggx_d": (C.c_double, [P, dp, dp]),
@@ -177,6 +180,14 @@ class OracleScene:
                                      tri.ctypes.data, tm.ctypes.data, tri.shape[0])
         self.n = int(L.orc_scene_count(self._h))
         self.scene = s
+        m = getattr(s, "medium", None)
+        if m is not None:
+            dens = np.ascontiguousarray(m.density, np.float32)
+            self._keep.append(dens)
+            nz, ny, nx = dens.shape
+            d3 = C.c_double * 3
+            L.orc_scene_set_medium(self._h, dens.ctypes.data, nx, ny, nz, d3(*m.bmin), d3(*m.bmax),
+                                   float(m.sigma), d3(*m.albedo))
 
     def __del__(self):
         try:
@@ -238,6 +249,22 @@ class OracleScene:
         if rc:
             raise RuntimeError("orc_render: build the oracle BVH first")
         return (rgb, ps) if per_sample else rgb
+
+    def medium_transmittance(self, o, d, tmin, tmax, seed, n):
+        """n ratio-tracking transmittance estimates along o + t d."""
+        out = np.empty(n, np.float64)
+        d3 = C.c_double * 3
+        if lib().orc_medium_transmittance(self._h, d3(*o), d3(*d), tmin, tmax, seed, n, out.ctypes.data):
+            raise RuntimeError("no medium")
+        return out
+
+    def medium_collide(self, o, d, tmin, tmax, seed, n):
+        """n delta-tracking runs: collision t or -1."""
+        out = np.empty(n, np.float64)
+        d3 = C.c_double * 3
+        if lib().orc_medium_collide(self._h, d3(*o), d3(*d), tmin, tmax, seed, n, out.ctypes.data):
+            raise RuntimeError("no medium")
+        return out
 
     def prim_t(self, rays: np.ndarray, ids: np.ndarray):
         rays = np.ascontiguousarray(rays, np.float32)

@@ -419,8 +419,11 @@ static int classify(const Prim &p, const d3 &o, const d3 &d, double tmin, double
     if (!p.visible) return -1;
     double denom = Dot(d, p.n);
     double dl = Length(d);
-    if (denom > 1e-6 + eb * dl || std::fabs(denom) < 1e-6 - eb * dl) return -1;
-    if (denom > -(1e-6 + eb * dl)) return 0;
+    // the 1e-6 threshold is absolute, so its margin is the FP32 rounding of
+    // the dot product (a few ulp of |d|), not the relative margin eb
+    const double de = 5e-7 * dl;
+    if (denom > -(1e-6 - de)) return -1;  // back face, parallel or near-parallel
+    if (denom >= -(1e-6 + de)) return 0;
     double t = Dot(p.c - o, p.n) / denom;
     tOut = t;
     d3 diff = (o + t * d) - p.c;

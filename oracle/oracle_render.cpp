@@ -19,6 +19,7 @@
 // mapped to ((x>>8)+1)*2^-24.  Draw order per sample follows Render.cu.
 // The new material types (CPT_M_LAMBERT, CPT_M_GGX) and the CONFIG profile
 // (max depth, roulette from a given depth) have no reference counterpart.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -52,10 +53,12 @@ PhiloxOut philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1) {
   return PhiloxOut{{c0, c1, c2, c3}};
 }
 
+// tag: 'CPT1' for a path; 0x80000000 | (path draw count when the shadow ray
+// was made) for the transmittance estimate of a shadow ray.
 struct Stream {
-  uint32_t k0, k1, pix, smp, cnt = 0;
+  uint32_t k0, k1, pix, smp, cnt = 0, tag = 0x43505431u;
   double next() {
-    uint32_t c[4] = {cnt >> 2, pix, smp, 0x43505431u};
+    uint32_t c[4] = {cnt >> 2, pix, smp, tag};
     PhiloxOut o = philox4x32_10(c, k0, k1);
     uint32_t x = o.v[cnt & 3];
     cnt++;
@@ -345,6 +348,67 @@ static OMat matOf(const Scene &s, const Prim &p) {
   return m;
 }
 
+// ---- heterogeneous medium (no reference counterpart) ----------------------
+static const int kMaxCollisions = 100000;
+
+// [t0, t1] = [tmin, tmax] clipped to the medium box by the slab test (NaN
+// slab ends are dropped, as fmax/fmin do); false when empty.
+static bool clipBox(const Medium &m, const d3 &o, const d3 &d, double tmin, double tmax, double &t0, double &t1) {
+  t0 = tmin;
+  t1 = tmax;
+  const double ox[3] = {o.x, o.y, o.z}, dx[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; k++) {
+    const double inv = 1.0 / dx[k];
+    double ta = (m.bmin[k] - ox[k]) * inv, tb = (m.bmax[k] - ox[k]) * inv;
+    if (ta > tb) std::swap(ta, tb);
+    t0 = std::fmax(t0, ta);
+    t1 = std::fmin(t1, tb);
+  }
+  return t0 < t1;
+}
+
+static double densityAt(const Medium &m, const d3 &p) {
+  const double px[3] = {p.x, p.y, p.z};
+  int64_t ix[3];
+  for (int k = 0; k < 3; k++) {
+    double f = std::floor((px[k] - m.bmin[k]) / (m.bmax[k] - m.bmin[k]) * m.n[k]);
+    f = std::fmin(std::fmax(f, 0.0), (double)(m.n[k] - 1));
+    ix[k] = (int64_t)f;
+  }
+  return m.density[(ix[2] * m.n[1] + ix[1]) * m.n[0] + ix[0]];
+}
+
+// Delta tracking: exponential steps of the majorant; a tentative collision
+// is real with probability density(x).  One draw per step, one per test.
+static bool deltaTrack(const Medium &m, Stream &rng, const d3 &o, const d3 &d, double t0, double t1, d3 &x) {
+  const double inv = 1.0 / (m.sigma * Length(d));
+  double t = t0;
+  for (int guard = 0; guard < kMaxCollisions; guard++) {
+    t -= std::log(1.0 - rng.next()) * inv;
+    if (!(t < t1)) return false;
+    const d3 p = o + t * d;
+    if (rng.next() < densityAt(m, p)) {
+      x = p;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Ratio tracking: product of (1 - density) at the tentative collisions.
+static double ratioTrack(const Medium &m, Stream rng, const d3 &o, const d3 &d, double tmin, double tmax) {
+  double t0, t1;
+  if (!clipBox(m, o, d, tmin, tmax, t0, t1)) return 1.0;
+  const double inv = 1.0 / (m.sigma * Length(d));
+  double t = t0, tr = 1.0;
+  for (int guard = 0; guard < kMaxCollisions; guard++) {
+    t -= std::log(1.0 - rng.next()) * inv;
+    if (!(t < t1)) break;
+    tr *= 1.0 - densityAt(m, o + t * d);
+  }
+  return tr;
+}
+
 // One sample of the bounce loop, Render.cu:42-219.  Returns the radiance.
 d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, uint32_t sample) {
   Stream rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)(j * cam.w + i), sample, 0};
@@ -357,6 +421,55 @@ d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, u
   const int nl = s.nLights;
   while (true) {
     Hit hr = trace(s, ro, rd);
+    if (s.med.on) {
+      double t0, t1;
+      d3 x;
+      if (clipBox(s.med, ro, rd, 0.001, hr.id >= 0 ? hr.t : INF, t0, t1) &&
+          deltaTrack(s.med, rng, ro, rd, t0, t1, x)) {
+        // real collision: NEE with the isotropic phase function, roulette,
+        // new direction; weight albedo
+        const d3 alb = s.med.albedo;
+        for (int k = 0; k < nl; k++) {
+          const Prim &lp = s.prims[k];
+          double lw = lp.w;
+          double lh = P.profile == CPT_PROFILE_REFERENCE ? lp.w : lp.h;
+          const double u1 = rng.next();
+          const double u2 = rng.next();
+          d3 lightPos = lp.c + (u1 - 0.5) * lw * lp.eu + (u2 - 0.5) * lh * lp.ev;
+          d3 dir = Unit(lightPos - x);
+          Hit sh = trace(s, x, dir);
+          if (Dot(-dir, lp.n) > 0.0 && sh.id == k) {
+            Stream srng{rng.k0, rng.k1, rng.pix, rng.smp, 0, 0x80000000u | rng.cnt};
+            const double tr = ratioTrack(s.med, srng, x, dir, 0.001, sh.t);
+            d3 c = lp.le * alb * (0.25 / PI) * Dot(-dir, lp.n) / SquaredLength(lightPos - x) * (lw * lh);
+            L = L + T * c * tr;
+          }
+          hasSampleLight = true;
+        }
+        double P_RR = 1.0;
+        if (P.profile == CPT_PROFILE_REFERENCE) {
+          P_RR = 0.7;
+          if (rng.next() > P_RR) break;
+        } else {
+          if (depth + 1 >= P.maxDepth) break;
+          if (depth >= P.rrDepth) {
+            P_RR = P.rrProb;
+            if (rng.next() > P_RR) break;
+          }
+        }
+        depth++;
+        const double r1 = rng.next();
+        const double r2 = rng.next();
+        rng.next();  // third draw of a vertex, unused by the phase function
+        const double z = 1.0 - 2.0 * r1;
+        const double r = std::sqrt(std::fmax(0.0, 1.0 - z * z));
+        const double phi = 2.0 * PI * r2;
+        T = T * alb / P_RR;
+        ro = x;
+        rd = mk(r * std::cos(phi), r * std::sin(phi), z);
+        continue;
+      }
+    }
     if (hr.id >= 0 && s.prims[hr.id].type != CPT_OBJ_LIGHT &&
         s.materials[s.prims[hr.id].mat].type == CPT_M_VOLUME) {
       const double sigmaMaj = 25.0;
@@ -409,12 +522,20 @@ d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, u
         double lw = lp.w;
         double lh = P.profile == CPT_PROFILE_REFERENCE ? lp.w : lp.h;  // Render.cu:170-171
         double pdfLight = 1.0 / (lw * lh);
-        d3 lightPos = lp.c + (rng.next() - 0.5) * lw * lp.eu + (rng.next() - 0.5) * lh * lp.ev;
+        // two draws in a fixed order: u1 along edgeU, then u2 along edgeV (the
+        // operands of one expression have no defined evaluation order in C++)
+        const double u1 = rng.next();
+        const double u2 = rng.next();
+        d3 lightPos = lp.c + (u1 - 0.5) * lw * lp.eu + (u2 - 0.5) * lh * lp.ev;
         d3 dir = Unit(lightPos - hp);
         Hit sh = trace(s, hp, dir);
         if (Dot(dir, n) > 0.0 && Dot(-dir, lp.n) > 0.0 && sh.id == k) {
           d3 c = lp.le * fr(m, m.color, rd, n, dir, zero, front) * Dot(dir, n) * Dot(-dir, lp.n) /
                  SquaredLength(lightPos - hp) / pdfLight;
+          if (s.med.on) {
+            Stream srng{rng.k0, rng.k1, rng.pix, rng.smp, 0, 0x80000000u | rng.cnt};
+            c = c * ratioTrack(s.med, srng, hp, dir, 0.001, sh.t);
+          }
           L = L + T * c;
         }
         hasSampleLight = true;
@@ -449,6 +570,61 @@ d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, u
 using namespace orc;
 
 extern "C" {
+
+// Medium of the scene (density nx*ny*nz floats in [0,1], x fastest); a
+// null density removes it.
+int orc_scene_set_medium(void *sp, const float *density, int nx, int ny, int nz, const double bmin[3],
+                         const double bmax[3], double sigma, const double albedo[3]) {
+  Scene *s = (Scene *)sp;
+  Medium &m = s->med;
+  if (!density || nx <= 0 || ny <= 0 || nz <= 0) {
+    m.on = false;
+    return 0;
+  }
+  m.density.assign(density, density + (size_t)nx * ny * nz);
+  m.n[0] = nx;
+  m.n[1] = ny;
+  m.n[2] = nz;
+  for (int k = 0; k < 3; k++) {
+    m.bmin[k] = (double)(float)bmin[k];
+    m.bmax[k] = (double)(float)bmax[k];
+  }
+  m.sigma = (double)(float)sigma;
+  m.albedo = mk((double)(float)albedo[0], (double)(float)albedo[1], (double)(float)albedo[2]);
+  m.on = true;
+  return 0;
+}
+
+// n ratio-tracking estimates of the transmittance over [tmin, tmax] along
+// o + t d (estimate i uses pixel i, sample 0 of the shadow-ray stream).
+int orc_medium_transmittance(void *sp, const double o[3], const double d[3], double tmin, double tmax,
+                             uint64_t seed, int n, double *out) {
+  Scene *s = (Scene *)sp;
+  if (!s->med.on) return -1;
+  const d3 O = mk(o[0], o[1], o[2]), D = mk(d[0], d[1], d[2]);
+  for (int i = 0; i < n; i++) {
+    Stream r{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)i, 0, 0, 0x80000000u};
+    out[i] = ratioTrack(s->med, r, O, D, tmin, tmax);
+  }
+  return 0;
+}
+
+// n delta-tracking runs over [tmin, tmax]: the collision t, or -1 for none.
+int orc_medium_collide(void *sp, const double o[3], const double d[3], double tmin, double tmax, uint64_t seed,
+                       int n, double *out) {
+  Scene *s = (Scene *)sp;
+  if (!s->med.on) return -1;
+  const d3 O = mk(o[0], o[1], o[2]), D = mk(d[0], d[1], d[2]);
+  for (int i = 0; i < n; i++) {
+    Stream r{(uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)i, 0, 0};
+    double t0, t1;
+    d3 x;
+    out[i] = -1.0;
+    if (clipBox(s->med, O, D, tmin, tmax, t0, t1) && deltaTrack(s->med, r, O, D, t0, t1, x))
+      out[i] = Dot(x - O, D) / Dot(D, D);
+  }
+  return 0;
+}
 
 void orc_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
   PhiloxOut o = philox4x32_10(ctr, k0, k1);

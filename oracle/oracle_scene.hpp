@@ -24,7 +24,20 @@ struct Prim {
   int mat = 0;  // material index; lights use -1-lightIndex
 };
 
+// Heterogeneous medium: density grid in [0,1] (x fastest) over a box,
+// sigma_t = sigma * density of the nearest voxel.  Box, sigma and albedo are
+// held at the FP32 values the GPU path stores.
+struct Medium {
+  bool on = false;
+  std::vector<float> density;
+  int n[3] = {0, 0, 0};
+  double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  double sigma = 0;
+  d3 albedo{0, 0, 0};
+};
+
 struct Scene {
+  Medium med;
   std::vector<cpt_material> materials;
   std::vector<Prim> prims;
   int nLights = 0, nSpheres = 0;

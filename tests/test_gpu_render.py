@@ -114,9 +114,53 @@ def test_converged_image_statistics(ctx):
     rel = np.abs(g - r) / np.maximum(r, 1e-3)
     assert np.mean(rel <= REL) >= 0.95
     assert abs(g.mean() / r.mean() - 1) < 1e-3
-    # AOV: albedo in [0,1], unit normals, positive depth on the first hit
+    # AOV (means over the pixel's samples): albedo in [0,1], positive depth;
+    # normals have length <= 1 (the sky adds none, silhouettes average two),
+    # and the mean normal of a pixel inside the box is close to unit length
     alb, nrm, dep = aov[..., :3], aov[..., 3:6], aov[..., 6]
     assert alb.min() >= 0 and alb.max() <= 1.0 + 1e-5
     assert np.all(dep > 0)
-    assert np.all(np.abs(np.linalg.norm(nrm, axis=-1) - 1) < 0.2)
+    nl = np.linalg.norm(nrm, axis=-1)
+    assert nl.max() <= 1.0 + 1e-4
+    assert np.median(nl[8:24, 8:24]) > 0.95
     assert st.rays >= 32 * 32 * 32
+
+
+def test_medium_paths(ctx):
+    """Heterogeneous fBm medium (delta tracking, ratio-tracked shadow rays)
+    in the Cornell box: path-by-path agreement with the oracle."""
+    print("[gpu] render cornell + fBm medium paths", flush=True)
+    s = scenes.cornell_icosphere()
+    s.medium = scenes.Medium(scenes.fbm_grid(32, 1), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 8.0, (0.8, 0.7, 0.6))
+    d = scenes.default_camera(48, 48)
+    f = path_match(ctx, s, d, L.PROFILE_CONFIG)
+    print(f"  medium path agreement {f:.5f}")
+    assert f >= 0.98
+    ctx.set_medium(None)
+
+
+def test_absorbing_slab_closed_form(ctx):
+    """A light behind a purely absorbing homogeneous slab: E[pixel] =
+    Le exp(-sigma rho thickness sec(theta))."""
+    import math
+    print("[gpu] render absorbing slab", flush=True)
+    s = scenes.SceneData()
+    s.add_light((0.0, 0.0, -1.0), 4.0, 4.0, (0.0, 0.0, 1.0), (2.0, 2.0, 2.0))
+    s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    s.medium = scenes.Medium(np.full((1, 1, 1), 0.6, np.float32), (-3, -3, -0.25), (3, 3, 0.25), 2.0,
+                             (0.0, 0.0, 0.0))
+    ctx.upload(s)
+    ctx.build(np.array([-3.0, -3, -1]), np.array([3.0, 3, 1]))
+    d = L.CameraDesc(L.d3((0, 0, 0)), L.d3((0, 0, 3.0)), L.d3((0, 0, 0)), 30.0, 1.0, 64, 0)
+    g, _, _ = ctx.render(d, 64, seed=17, profile=L.PROFILE_CONFIG, max_depth=4)
+    cam = camera(d)
+    h, w = cam.height, cam.width
+    i, j = np.meshgrid(np.arange(w), np.arange(h))
+    pc, dh, dv, lf = (np.array(getattr(cam, f)[:]) for f in ("pixel_center", "delta_h", "delta_v", "look_from"))
+    dirs = pc[None, None] + i[..., None] * dh + j[..., None] * dv - lf
+    sec = np.linalg.norm(dirs, axis=-1) / np.abs(dirs[..., 2])
+    expect = 2.0 * np.exp(-2.0 * 0.6 * 0.5 * sec)
+    p = math.exp(-0.6)
+    se = math.sqrt(p * (1 - p) / (h * w * 64)) / p
+    assert abs((g[..., 0] / expect).mean() - 1) < 5 * se
+    ctx.set_medium(None)
