@@ -7,8 +7,8 @@ U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
 1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
 Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
 the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
-per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 64 per node visited + 48 per
-primitive tested, node/primitive counts from a counting run of the same
+per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 112 per four-wide node record
+visited (7 of its 8 float4 are read) + 48 per primitive tested, node/primitive counts from a counting run of the same
 frame, divided by the summed HIP-event durations of the extension launches.
 
 Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpix/s at equal relMSE (denoised vs plain PT); Mrays/s; LBVH build ms"
 HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 112.0  # bytes read per visit of a four-wide node record (the default traversal)
 
 
 def log(msg):
@@ -62,7 +63,7 @@ def bench_render(cpt, L, scenes, args):
     rays = float(np.mean([f.rays for f in frames]))
     ext_ms = float(np.mean([f.extend_ms for f in frames]))
     ext_rays = float(np.mean([f.extend_rays for f in frames]))
-    ext_bytes = 44.0 * stc.extend_rays + 64.0 * stc.extend_nodes + 48.0 * stc.extend_prims
+    ext_bytes = 44.0 * stc.extend_rays + NODE_BYTES * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
@@ -182,7 +183,7 @@ def bench_lbvh(cpt, scenes, args):
     n_nodes = float(nodes.double().sum().item())
     n_prims = float(prims.double().sum().item())
     hits = float((id_out >= 0).double().mean().item())
-    alg = 40.0 * args.rays + 64.0 * n_nodes + 48.0 * n_prims
+    alg = 40.0 * args.rays + NODE_BYTES * n_nodes + 48.0 * n_prims
     tr_avg, bd_avg = float(np.mean(trace_ms)), float(np.mean(build_ms))
     sa = np.mean(np.array(stage), axis=0).tolist()
     achieved = alg / (tr_avg * 1e-3) / 1e9
@@ -193,7 +194,8 @@ def bench_lbvh(cpt, scenes, args):
            "hit_fraction": round(hits, 4), "nodes_per_ray": round(n_nodes / args.rays, 2),
            "prims_per_ray": round(n_prims / args.rays, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_trace"}}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_trace"),
+                        "kernel": "k_trace"}}
     if not args.no_cpu:
         from oracle import oracle as O
         o = O.OracleScene(scene)
@@ -213,6 +215,32 @@ def bench_lbvh(cpt, scenes, args):
     return out
 
 
+def copy_probe(nbytes):
+    """A plain device copy of known size: its FETCH_SIZE / WRITE_SIZE per
+    dispatch calibrate the counters (bytes read and written = nbytes)."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by a copy of known
+    size), or None when no summary for it exists."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,14 +257,26 @@ def main():
     ap.add_argument("--eq-width", type=int, default=960)
     ap.add_argument("--eq-height", type=int, default=540)
     ap.add_argument("--eq-ref-spp", type=int, default=2048)
+    ap.add_argument("--only", choices=["render", "lbvh"], default=None,
+                    help="run one workload and print its record (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="copy this many bytes device-to-device 5 times and exit (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import cudapathtracing_amd as cpt
     from cudapathtracing_amd import _lib as L
     from cudapathtracing_amd import scenes
+    if args.only == "lbvh":
+        print(json.dumps({"lbvh": bench_lbvh(cpt, scenes, args)}), flush=True)
+        return
+    if args.only == "render":
+        args.no_lbvh = args.no_eq = True
 
     r, cpu, ntri = bench_render(cpt, L, scenes, args)
     lb = None if args.no_lbvh else bench_lbvh(cpt, scenes, args)
@@ -265,7 +305,8 @@ def main():
         "extend_prims_per_ray": round(r["prims_per_ray"], 2),
         "scene_lbvh_build_ms": round(r["build_ms"], 3),
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_extend",
+                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_extend"),
+                     "kernel": "k_extend",
                      "avg_launch_ms": round(r["ext_ms"] / max(1, r["bounces"]), 4)},
         "cpu_baseline": cpu,
         "lbvh": lb,

@@ -177,13 +177,14 @@ struct TraceCounters {
   int nodes = 0, prims = 0;
 };
 
+// kAny: return at the first primitive hit in [tmin, tbest] (occlusion).
 // Closest hit over the LBVH.  Starts at the root's children (the root box
 // is not tested, as in traverseIterative, BVH.cuh:293-301), tests both child
 // boxes against [tmin, best t], tests leaf children at once, descends into
 // the nearer internal child and pushes the farther.  Returns the leaf index
 // of the closest hit (-1 for none); tbest holds its FP32 distance.
 // stk points at this lane's LDS column; entries are stk[i * stride].
-template <bool kCount>
+template <bool kCount, bool kAny = false>
 __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
                                             const float4 *__restrict__ leaves, int n, f3 o, f3 d,
                                             float tmin, float &tbest, int *stk, int stride,
@@ -235,6 +236,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
       if (hitLeaf(leaves, ~ca, o, d, tmin, tbest, t)) {
         tbest = t;
         hit = ~ca;
+        if (kAny) return hit;
       }
       hA = false;
     }
@@ -244,6 +246,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
       if (hitLeaf(leaves, ~cb, o, d, tmin, tbest, t)) {
         tbest = t;
         hit = ~cb;
+        if (kAny) return hit;
       }
       hB = false;
     }
@@ -277,7 +280,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
 constexpr int kEmpty4 = (int)0x80000000;
 constexpr int kStack4Total = 96;  // <= 3 pushes per record, <= 31 record levels
 
-template <bool kCount>
+template <bool kCount, bool kAny = false>
 __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
                                              const float4 *__restrict__ leaves, int n, f3 o, f3 d,
                                              float tmin, float &tbest, int *stk, int stride,
@@ -330,6 +333,7 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
         if (hitLeaf(leaves, ~ref[k], o, d, tmin, tbest, t)) {
           tbest = t;
           hit = ~ref[k];
+          if (kAny) return hit;
         }
         h[k] = false;
       }

@@ -484,11 +484,15 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
 
 // Shadow rays: the light's contribution counts when the light itself is the
 // closest hit (Render.cu:180-185), times the medium's transmittance from a
-// ratio-tracking estimate when a medium is present.
+// ratio-tracking estimate when a medium is present.  The closest hit is the
+// light exactly when the ray meets the light at some tL and nothing else
+// lies in [tmin, tL): an occlusion query (first hit ends the walk) over
+// [tmin, prev(tL)] answers that (an exact tie at tL counts as unoccluded).
 __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow, int64_t shadowCap,
                                                             const uint32_t *__restrict__ counters,
                                                             const float4 *__restrict__ nodes,
-                                                            const float4 *__restrict__ leaves, int nPrims,
+                                                            const float4 *__restrict__ leaves,
+                                                            const float4 *__restrict__ rec, int nPrims,
                                                             float *__restrict__ fb, int wide, MediumDev med,
                                                             uint64_t seed) {
   __shared__ int stack[kStackLds * kRenderThreads];
@@ -496,17 +500,19 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
   if ((int64_t)k >= shadowCap || k >= counters[1]) return;
   const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
-  float tb = r1.w;
+  const f3 o = xyz(r0), d = xyz(r1);
+  float tL;
+  if (!hitLeaf(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
+  float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
-  const int leaf = traceAny<false>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
-                                   stack + threadIdx.x, kRenderThreads, cnt);
-  if (leaf < 0) return;
-  const int id = __float_as_int(leaves[3 * (int64_t)leaf].w);
-  if (id != __float_as_int(l.x)) return;
+  const int leaf = wide ? traceClosest4<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
+                                                     kRenderThreads, cnt)
+                        : traceClosest<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
+                                                    kRenderThreads, cnt);
+  if (leaf >= 0) return;
   const uint32_t pixel = (uint32_t)__float_as_int(c.w);
   float tr = 1.0f;
-  if (med.on) tr = ratioTrack(med, seed, pixel, __float_as_uint(l.y), __float_as_uint(l.z), xyz(r0), xyz(r1),
-                              r0.w, tb);
+  if (med.on) tr = ratioTrack(med, seed, pixel, __float_as_uint(l.y), __float_as_uint(l.z), o, d, r0.w, tL);
   float *pix = fb + 3 * (int64_t)pixel;
   atomicAdd(pix + 0, c.x * tr);
   atomicAdd(pix + 1, c.y * tr);
@@ -646,7 +652,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
                          (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, a.nodes, a.leaves,
-                         a.nPrims, R.fb.as<float>(), a.wide, a.med, a.seed);
+                         a.rec, a.nPrims, R.fb.as<float>(), a.wide, a.med, a.seed);
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
       uint32_t hc[2] = {0, 0};
