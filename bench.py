@@ -256,8 +256,8 @@ def main():
     ap.add_argument("--no-eq", action="store_true")
     ap.add_argument("--eq-width", type=int, default=960)
     ap.add_argument("--eq-height", type=int, default=540)
-    ap.add_argument("--eq-ref-spp", type=int, default=2048)
-    ap.add_argument("--only", choices=["render", "lbvh"], default=None,
+    ap.add_argument("--eq-ref-spp", type=int, default=4096)
+    ap.add_argument("--only", choices=["render", "lbvh", "eq"], default=None,
                     help="run one workload and print its record (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="copy this many bytes device-to-device 5 times and exit (counter calibration)")
@@ -274,6 +274,9 @@ def main():
     from cudapathtracing_amd import scenes
     if args.only == "lbvh":
         print(json.dumps({"lbvh": bench_lbvh(cpt, scenes, args)}), flush=True)
+        return
+    if args.only == "eq":
+        print(json.dumps({"equal_relmse": bench_equal_quality(cpt, L, scenes, args)}), flush=True)
         return
     if args.only == "render":
         args.no_lbvh = args.no_eq = True
