@@ -102,6 +102,7 @@ SIGNATURES = [
     ("cpt_trace_closest", C.c_int, [P, P, C.c_int64, P, P, C.c_int]),
     ("cpt_trace_count", C.c_int, [P, P, C.c_int64, P, P, C.c_int]),
     ("cpt_last_trace_ms", C.c_float, [P]),
+    ("cpt_last_trace_stats", C.c_int, [P, P, P, P]),
     ("cpt_render", C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), P, P,
                              C.POINTER(RenderStats), C.c_int]),
     ("cpt_camera_from_desc", C.c_int, [C.POINTER(CameraDesc), C.POINTER(Camera)]),

@@ -74,6 +74,10 @@ class Context:
         """4 (default): four-wide records; 2: binary LBVH nodes."""
         self._check(self._lib.cpt_set_option(self._h, 1, width))
 
+    def set_ray_sort_min(self, n: int):
+        """Trace batches of at least n rays in ray-key order (0: never)."""
+        self._check(self._lib.cpt_set_option(self._h, 2, n))
+
     def synchronize(self):
         self._check(self._lib.cpt_synchronize(self._h))
 
@@ -208,6 +212,12 @@ class Context:
 
     def last_trace_ms(self) -> float:
         return float(self._lib.cpt_last_trace_ms(self._h))
+
+    def last_trace_stats(self):
+        """(total, ray ordering, traversal kernel) ms of the last trace call."""
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self._check(self._lib.cpt_last_trace_stats(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     # -- path tracing -----------------------------------------------------------
     def render(self, camera_desc: L.CameraDesc, spp: int, seed: int = 3317,

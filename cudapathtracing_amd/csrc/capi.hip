@@ -57,7 +57,8 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->valsA,    &ctx->valsB,  &ctx->hist,      &ctx->rowTotal,   &ctx->keys64,
                    &ctx->order,    &ctx->nodes,  &ctx->parentInt, &ctx->parentLeaf, &ctx->flags,
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
-                   &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc, &ctx->medDensity};
+                   &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc,        &ctx->medDensity,
+                   &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -153,6 +154,10 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
     ctx->traversal = value;
     return CPT_OK;
   }
+  if (option == CPT_OPT_RAY_SORT_MIN && value >= 0) {
+    ctx->raySortMin = value;
+    return CPT_OK;
+  }
   return fail(ctx, CPT_ERR_ARG, "cpt_set_option", "unknown option or value");
 }
 
@@ -175,6 +180,9 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   ctx->built = false;
   ctx->hasMedium = false;
+  ctx->hasVolumeMat = false;
+  for (int i = 0; i < nm; i++)
+    if (materials[i].type == CPT_M_VOLUME) ctx->hasVolumeMat = true;
   int rc;
   if ((rc = ensure(ctx, ctx->center64, n * 24)) || (rc = ensure(ctx, ctx->box32, n * 24)) ||
       (rc = ensure(ctx, ctx->rec, n * 48)) || (rc = ensure(ctx, ctx->tri64, nt * 72)) ||
@@ -492,6 +500,23 @@ float cpt_last_trace_ms(const cpt_ctx *ctx) {
   if (hipEventSynchronize(ctx->ev[9]) != hipSuccess) return -1.f;
   if (hipEventElapsedTime(&ms, ctx->ev[8], ctx->ev[9]) != hipSuccess) return -1.f;
   return ms;
+}
+
+int cpt_last_trace_stats(const cpt_ctx *ctx, float *totalMs, float *sortMs, float *kernelMs) {
+  if (!ctx) return CPT_ERR_ARG;
+  if (hipEventSynchronize(ctx->ev[9]) != hipSuccess) return CPT_ERR_HIP;
+  float tot = 0.f, srt = 0.f, ker = 0.f;
+  if (hipEventElapsedTime(&tot, ctx->ev[8], ctx->ev[9]) != hipSuccess) return CPT_ERR_HIP;
+  if (ctx->lastTraceSorted) {
+    if (hipEventElapsedTime(&srt, ctx->ev[8], ctx->ev[5]) != hipSuccess) return CPT_ERR_HIP;
+    if (hipEventElapsedTime(&ker, ctx->ev[5], ctx->ev[9]) != hipSuccess) return CPT_ERR_HIP;
+  } else {
+    ker = tot;
+  }
+  if (totalMs) *totalMs = tot;
+  if (sortMs) *sortMs = srt;
+  if (kernelMs) *kernelMs = ker;
+  return CPT_OK;
 }
 
 }  // extern "C"

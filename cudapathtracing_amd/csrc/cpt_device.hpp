@@ -353,11 +353,35 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
       node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
       continue;
     }
+    // push the others farthest first, so the nearest is popped next
+    // (5-comparator network on (entry distance, ref), descending)
+    float pt[4];
+    int pr[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      if (h[k] && k != best) {
-        if (sp < kStackLds) stk[sp * stride] = ref[k];
-        else if (sp < kStack4Total) ovf[sp - kStackLds] = ref[k];
+      pt[k] = (h[k] && k != best) ? tn[k] : -INFINITY;
+      pr[k] = ref[k];
+    }
+#define CPT_CSWAP(a, b)                                        \
+  if (pt[a] < pt[b]) {                                         \
+    const float tt_ = pt[a];                                   \
+    pt[a] = pt[b];                                             \
+    pt[b] = tt_;                                               \
+    const int tr_ = pr[a];                                     \
+    pr[a] = pr[b];                                             \
+    pr[b] = tr_;                                               \
+  }
+    CPT_CSWAP(0, 1)
+    CPT_CSWAP(2, 3)
+    CPT_CSWAP(0, 2)
+    CPT_CSWAP(1, 3)
+    CPT_CSWAP(1, 2)
+#undef CPT_CSWAP
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (pt[k] != -INFINITY) {
+        if (sp < kStackLds) stk[sp * stride] = pr[k];
+        else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
         sp = sp < kStack4Total ? sp + 1 : sp;
       }
     }

@@ -64,10 +64,14 @@ struct cpt_ctx {
   float medBox[6] = {0, 0, 0, 0, 0, 0};
   float medSigma = 0.f, medAlbedo[3] = {0, 0, 0};
   int medN[3] = {0, 0, 0};
+  bool hasVolumeMat = false;  // some material is CPT_M_VOLUME (k_volume runs)
   int traversal = 4;    // 2: binary node walk, 4: four-wide records
+  int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered
+  bool lastTraceSorted = false;
 
   // traversal I/O staging
   cpt::DevBuf rayBuf, tBuf, idBuf, cntA, cntB;
+  cpt::DevBuf rkA, rkB, rvA, rvB;  // ray-order keys / permutation (ping-pong)
   float lastTraceMs = 0.f;
   hipEvent_t ev[16] = {};
 
@@ -83,6 +87,8 @@ void releaseAll(cpt_ctx *ctx);
 
 // bvh.hip
 int launchMorton(cpt_ctx *ctx, const double smin[3], const double smax[3], uint64_t *keysDev);
+// Stable LSD sort of (keys, vals); an even number of 8-bit passes leaves the
+// result in (keys, vals).
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
                    uint32_t *valsTmp, int64_t n, int bits, bool identityVals);
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);

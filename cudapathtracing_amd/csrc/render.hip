@@ -4,10 +4,12 @@
 // one path per (pixel, sample), state in HBM (SoA), index queues between
 //   k_gen     camera ray (Camera::getRandomSampleRay, Camera.cuh:37-43) in FP64
 //   k_extend  closest hit (traverseIterative) for the active queue
-//   k_shade   miss/sky, emitter, homogeneous volume (Render.cu:64-135), NEE to
-//             every quad light (Render.cu:161-187), Russian roulette
+//   k_shade   miss/sky, emitter, heterogeneous medium (delta tracking), NEE
+//             to every quad light (Render.cu:161-187), Russian roulette
 //             (Render.cu:189-204), BSDF sampling (Render.cu:206-218); pushes
-//             survivors to the next queue and shadow rays to the shadow queue
+//             survivors to the next queue, shadow rays to the shadow queue and
+//             volume-object hits to the volume queue
+//   k_volume  the homogeneous volume object walk (Render.cu:64-135)
 //   k_shadow  closest hit of each shadow ray; adds its contribution when the
 //             light itself is the closest hit (Render.cu:180-185)
 // Radiance is summed per pixel with FP32 atomics and divided by spp at the
@@ -36,7 +38,7 @@ constexpr int kMaxBounces = 4096;
 struct RenderBufs {
   int64_t cap = 0, shadowCap = 0;
   int64_t npix = 0;
-  DevBuf ray, thr, meta, hit, qA, qB, shadow, counters, fb, aov;
+  DevBuf ray, thr, meta, hit, qA, qB, qVol, shadow, counters, fb, aov;
 };
 
 struct CamDev {
@@ -253,8 +255,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, const uint32_t *__restrict__ q, uint32_t count, float4 *__restrict__ ray,
     float4 *__restrict__ thr, uint4 *__restrict__ meta, const float2 *__restrict__ hit,
     uint32_t *__restrict__ qNext, float4 *__restrict__ shadow, int64_t shadowCap,
-    uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov) {
-  __shared__ int stack[kStackLds * kRenderThreads];
+    uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov,
+    uint32_t *__restrict__ qVol) {
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   const bool active = k < count;
   bool alive = false;
@@ -319,38 +321,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       front = dot3(d, outw) < 0.0f;
       n = front ? outw : -outw;
       if (m.type == CPT_M_VOLUME) {
-        // homogeneous medium, sigma_maj = 25 (Render.cu:64-135)
-        const float sig = 25.0f;
-        TraceCounters cnt;
-        float tb = kRayTMax;
-        int l2 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb,
-                                 stack + threadIdx.x, kRenderThreads, cnt);
-        o = hp;
-        if (l2 >= 0) {
-          float dMax = tb;
-          float t = -__logf(1.0f - rng.next()) / sig;
-          f3 pos = hp + d * t;
-          o = pos;
-          for (int guard = 0; t < dMax && guard < kMaxCollisions; guard++) {
-            if (rng.next() > 0.8f) break;
-            T = T * (__expf(-t * sig) / 0.8f);
-            float z = 1.0f - 2.0f * rng.next();
-            float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-            float ph = 2.0f * kPi * rng.next();
-            float s, c;
-            __sincosf(ph, &s, &c);
-            d = unit3(mkf3(rr * c, rr * s, z));
-            o = pos;
-            pos = hp + d * t;
-            float tb2 = kRayTMax;
-            int l3 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
-                                     stack + threadIdx.x, kRenderThreads, cnt);
-            if (l3 >= 0) dMax = tb2;
-            t = -__logf(1.0f - rng.next()) / sig;
-            if (t > dMax || l3 < 0) break;
-          }
-        }
-        alive = true;
+        // homogeneous volume object: walked by k_volume (it traces inside)
+        const uint32_t vs = waveAppend(counters + 2, true);
+        qVol[vs] = p;
       } else if (m.type == CPT_M_LIGHT) {
         // emitter (Render.cu:144-149)
         if (!(flags & kFlagNee)) {
@@ -482,6 +455,67 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   }
 }
 
+// Paths that hit a volume-material object (Render.cu:64-135): homogeneous
+// medium with sigma_maj = 25, roulette 0.8 per collision, isotropic
+// scattering, ending at the far side of the object; the path then continues
+// from the last position with the last direction.  The queue length is
+// counters[2]; the grid covers an upper bound.
+__global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const uint32_t *__restrict__ qVol,
+                                                           uint32_t bound, float4 *__restrict__ ray,
+                                                           float4 *__restrict__ thr, uint4 *__restrict__ meta,
+                                                           const float2 *__restrict__ hit,
+                                                           uint32_t *__restrict__ qNext,
+                                                           uint32_t *__restrict__ counters) {
+  __shared__ int stack[kStackLds * kRenderThreads];
+  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  if (k >= bound || k >= counters[2]) return;
+  const uint32_t p = qVol[k];
+  const float4 o4 = ray[2 * (int64_t)p], d4 = ray[2 * (int64_t)p + 1], t4 = thr[p];
+  const uint4 mt = meta[p];
+  const float th = hit[p].y;
+  Rng rng;
+  rng.init(a.seed, mt.x, mt.y, mt.z);
+  f3 T = mkf3(t4.x, t4.y, t4.z);
+  f3 d = xyz(d4);
+  const f3 hp = xyz(o4) + d * th;
+  const float sig = 25.0f;
+  TraceCounters cnt;
+  float tb = kRayTMax;
+  const int l2 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb, stack + threadIdx.x,
+                                 kRenderThreads, cnt);
+  f3 o = hp;
+  if (l2 >= 0) {
+    float dMax = tb;
+    float t = -__logf(1.0f - rng.next()) / sig;
+    f3 pos = hp + d * t;
+    o = pos;
+    for (int guard = 0; t < dMax && guard < kMaxCollisions; guard++) {
+      if (rng.next() > 0.8f) break;
+      T = T * (__expf(-t * sig) / 0.8f);
+      const float z = 1.0f - 2.0f * rng.next();
+      const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+      const float ph = 2.0f * kPi * rng.next();
+      float sn, cs;
+      __sincosf(ph, &sn, &cs);
+      d = unit3(mkf3(rr * cs, rr * sn, z));
+      o = pos;
+      pos = hp + d * t;
+      float tb2 = kRayTMax;
+      const int l3 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
+                                     stack + threadIdx.x, kRenderThreads, cnt);
+      if (l3 >= 0) dMax = tb2;
+      t = -__logf(1.0f - rng.next()) / sig;
+      if (t > dMax || l3 < 0) break;
+    }
+  }
+  const uint32_t slot = waveAppend(counters + 0, true);
+  ray[2 * (int64_t)p] = make_float4(o.x, o.y, o.z, kRayTMin);
+  ray[2 * (int64_t)p + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
+  thr[p] = make_float4(T.x, T.y, T.z, 0.f);
+  meta[p] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
+  qNext[slot] = p;
+}
+
 // Shadow rays: the light's contribution counts when the light itself is the
 // closest hit (Render.cu:180-185), times the medium's transmittance from a
 // ratio-tracking estimate when a medium is present.  The closest hit is the
@@ -527,7 +561,8 @@ __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ ou
 void renderRelease(cpt_ctx *ctx) {
   if (!ctx->render) return;
   RenderBufs *r = ctx->render;
-  DevBuf *all[] = {&r->ray, &r->thr, &r->meta, &r->hit, &r->qA, &r->qB, &r->shadow, &r->counters, &r->fb, &r->aov};
+  DevBuf *all[] = {&r->ray,    &r->thr,      &r->meta, &r->hit, &r->qA, &r->qB,
+                   &r->qVol,   &r->shadow,   &r->counters, &r->fb,  &r->aov};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   delete r;
@@ -563,6 +598,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   if ((rc = ensure(ctx, R.ray, cap * 32)) || (rc = ensure(ctx, R.thr, cap * 16)) ||
       (rc = ensure(ctx, R.meta, cap * 16)) || (rc = ensure(ctx, R.hit, cap * 8)) ||
       (rc = ensure(ctx, R.qA, cap * 4)) || (rc = ensure(ctx, R.qB, cap * 4)) ||
+      (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
@@ -647,7 +683,12 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
       hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qa, qn, R.ray.as<float4>(),
                          R.thr.as<float4>(), R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb,
-                         R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(), R.aov.as<float>());
+                         R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(), R.aov.as<float>(),
+                         R.qVol.as<uint32_t>());
+      if (ctx->hasVolumeMat)
+        hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
+                           (const uint32_t *)R.qVol.as<uint32_t>(), qn, R.ray.as<float4>(), R.thr.as<float4>(),
+                           R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb, cnt);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,

@@ -1,7 +1,8 @@
 // trace.hip - closest-hit traversal kernels (traverseIterative,
 // CudaPathTracing/BVH.cuh:282-332, one ray per lane, FP32 box and primitive
-// tests, per-lane stack in LDS with a private overflow tail) and the FP64
-// refinement of the winning hit distance.
+// tests, per-lane stack in LDS with a private overflow tail), the FP64
+// refinement of the winning hit distance, and the ray ordering of large
+// batches (key = direction octant + origin Morton cell, LSD radix sort).
 #include <hip/hip_runtime.h>
 
 #include "cpt_device.hpp"
@@ -11,15 +12,48 @@ namespace cpt {
 
 constexpr int kTraceThreads = 256;
 
+// Ray-order key: direction octant (3 bits) above the 27-bit Morton code of
+// the origin on a 512^3 grid over the root box (origins outside clamp to
+// the boundary cells).  Rays with equal keys start in the same cell and head
+// into the same octant, so a wave of sorted rays walks similar subtrees.
+__device__ __forceinline__ uint32_t spread9(uint32_t v) {
+  v &= 0x1ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__restrict__ rays, int64_t n,
+                                                           const float *__restrict__ rootBox,
+                                                           uint32_t *__restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
+  if (i >= n) return;
+  const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+  const float o[3] = {r0.x, r0.y, r0.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float lo = rootBox[k], ext = rootBox[3 + k] - lo;
+    const float u = ext > 0.0f ? (o[k] - lo) / ext : 0.0f;
+    q[k] = (uint32_t)fminf(fmaxf(u * 512.0f, 0.0f), 511.0f);  // NaN -> 0
+  }
+  const uint32_t oct = (r1.x > 0.0f ? 4u : 0u) | (r1.y > 0.0f ? 2u : 0u) | (r1.z > 0.0f ? 1u : 0u);
+  keys[i] = (oct << 27) | (spread9(q[0]) << 2) | (spread9(q[1]) << 1) | spread9(q[2]);
+}
+
+// perm (optional): lane i traces ray perm[i] and writes its outputs at perm[i].
 template <bool kCount, bool kWide>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(
-    const float4 *__restrict__ rays, int64_t nRays, const float4 *__restrict__ nodes,
-    const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, int nPrims, Scene64 s64,
-    float *__restrict__ tOut, int32_t *__restrict__ idOut, int32_t *__restrict__ nodesOut,
-    int32_t *__restrict__ primsOut) {
+    const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
+    const float4 *__restrict__ nodes, const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves,
+    int nPrims, Scene64 s64, float *__restrict__ tOut, int32_t *__restrict__ idOut,
+    int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut) {
   __shared__ int stack[kStackLds * kTraceThreads];
-  const int64_t i = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
-  if (i >= nRays) return;
+  const int64_t li = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
+  if (li >= nRays) return;
+  const int64_t i = perm ? (int64_t)perm[li] : li;
   const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
   const f3 o = xyz(r0), d = xyz(r1);
   float tbest = r1.w;
@@ -52,21 +86,38 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   const float4 *nd = ctx->nodes.as<float4>(), *n4 = ctx->nodes4.as<float4>(), *lv = ctx->leaves.as<float4>();
   const int np = (int)ctx->nPrims;
   const bool wide = ctx->traversal == 4;
+  const bool sorted = ctx->raySortMin > 0 && n >= ctx->raySortMin && np > 1 && n <= 0x7fffffff;
+  int rc;
+  if (sorted && ((rc = ensure(ctx, ctx->rkA, n * 4)) || (rc = ensure(ctx, ctx->rkB, n * 4)) ||
+                 (rc = ensure(ctx, ctx->rvA, n * 4)) || (rc = ensure(ctx, ctx->rvB, n * 4))))
+    return rc;
+  ctx->lastTraceSorted = sorted;
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
+  const uint32_t *perm = nullptr;
+  if (sorted) {
+    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n,
+                       (const float *)ctx->rootBox.as<float>(), ctx->rkA.as<uint32_t>());
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    if ((rc = radixSortPairs(ctx, ctx->rkA.as<uint32_t>(), ctx->rvA.as<uint32_t>(), ctx->rkB.as<uint32_t>(),
+                             ctx->rvB.as<uint32_t>(), n, 30, true)))
+      return rc;
+    perm = ctx->rvA.as<uint32_t>();
+    CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+  }
   if (nodesDev) {
     if (wide)
-      hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv, np,
-                         s64, tDev, idDev, nodesDev, primsDev);
+      hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, n4,
+                         lv, np, s64, tDev, idDev, nodesDev, primsDev);
     else
-      hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
-                         np, s64, tDev, idDev, nodesDev, primsDev);
+      hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, n4,
+                         lv, np, s64, tDev, idDev, nodesDev, primsDev);
   } else {
     if (wide)
-      hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
-                         np, s64, tDev, idDev, nodesDev, primsDev);
+      hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd,
+                         n4, lv, np, s64, tDev, idDev, nodesDev, primsDev);
     else
-      hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, nd, n4, lv,
-                         np, s64, tDev, idDev, nodesDev, primsDev);
+      hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd,
+                         n4, lv, np, s64, tDev, idDev, nodesDev, primsDev);
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[9], ctx->stream));

@@ -131,8 +131,12 @@ int cpt_synchronize(cpt_ctx *ctx);
 const char *cpt_version(void);
 /* Options.  CPT_OPT_TRAVERSAL_WIDTH: 4 (default) walks four-wide records
  * built from the binary LBVH (each spans two binary levels); 2 walks the
- * binary nodes.  Results are identical up to exact-t ties. */
+ * binary nodes.  Results are identical up to exact-t ties.
+ * CPT_OPT_RAY_SORT_MIN: cpt_trace_closest batches of at least this many rays
+ * are traced in the order of a key (direction octant, origin Morton cell);
+ * results are written in the caller's order.  0 disables it; default 2^20. */
 #define CPT_OPT_TRAVERSAL_WIDTH 1
+#define CPT_OPT_RAY_SORT_MIN 2
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */
@@ -226,9 +230,14 @@ int cpt_trace_closest(cpt_ctx *ctx, const float *rays, int64_t n,
 int cpt_trace_count(cpt_ctx *ctx, const float *rays, int64_t n,
                     int32_t *nodes_out, int32_t *prims_out, int flags);
 
-/* Average duration of the last cpt_trace_closest kernel (ms), measured with
- * HIP events on the launching stream. */
+/* Duration of the last cpt_trace_closest / cpt_trace_count call on the
+ * device (ms: ray ordering + traversal), measured with HIP events on the
+ * launching stream. */
 float cpt_last_trace_ms(const cpt_ctx *ctx);
+/* The same split: total, ray ordering (0 when not sorted) and the traversal
+ * kernel alone (ms).  Any pointer may be NULL. */
+int cpt_last_trace_stats(const cpt_ctx *ctx, float *total_ms, float *sort_ms,
+                         float *kernel_ms);
 
 /* ---- path tracing ------------------------------------------------------- */
 #define CPT_PROFILE_REFERENCE 0 /* Render.cu STATIC: RR 0.7 every bounce,   */

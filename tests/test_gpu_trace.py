@@ -172,3 +172,27 @@ def test_device_pointer_trace(ctx):
     assert np.array_equal(i1.cpu().numpy(), i0) and np.array_equal(t1.cpu().numpy(), t0)
     nodes, prims = ctx.trace_count(rays)
     assert nodes.min() >= 1 and prims.max() >= 1
+
+
+def test_ray_ordering_keeps_results(ctx):
+    """Traced in ray-key order, every ray's (t, id) lands at its own index and
+    equals the unsorted trace."""
+    print("[gpu] trace ray ordering", flush=True)
+    s = scenes.SceneData()
+    m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    s.add_mesh(scenes.triangle_soup(200_000, seed=3, mu=-4.5), m)
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    rays = scenes.random_rays(300_001, seed=4)
+    ctx.set_ray_sort_min(0)
+    t0, i0 = ctx.trace(rays)
+    ctx.set_ray_sort_min(1)
+    t1, i1 = ctx.trace(rays)
+    tot, srt, ker = ctx.last_trace_stats()
+    assert srt > 0 and tot >= ker
+    n0, p0 = ctx.trace_count(rays)
+    ctx.set_ray_sort_min(1 << 20)
+    n1, p1 = ctx.trace_count(rays)
+    assert np.array_equal(i0, i1) and np.array_equal(t0, t1)
+    assert np.array_equal(n0, n1) and np.array_equal(p0, p1)
+    assert np.mean(i0 >= 0) > 0.1
