@@ -74,6 +74,10 @@ class Context:
         """4 (default): four-wide records; 2: binary LBVH nodes."""
         self._check(self._lib.cpt_set_option(self._h, 1, width))
 
+    def set_persistent(self, on: bool):
+        """Persistent lane-refilling traversal kernels (default off)."""
+        self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))
+
     def set_ray_sort_min(self, n: int):
         """Trace batches of at least n rays in ray-key order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 2, n))

@@ -117,6 +117,9 @@ int cpt_create(cpt_ctx **out, int dev) {
     return CPT_ERR_HIP;
   }
   ctx->stream = ctx->ownStream;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+    ctx->numCU = cus;
   *out = ctx;
   return CPT_OK;
 }
@@ -156,6 +159,10 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
   }
   if (option == CPT_OPT_RAY_SORT_MIN && value >= 0) {
     ctx->raySortMin = value;
+    return CPT_OK;
+  }
+  if (option == CPT_OPT_PERSISTENT && (value == 0 || value == 1)) {
+    ctx->persistent = value != 0;
     return CPT_OK;
   }
   return fail(ctx, CPT_ERR_ARG, "cpt_set_option", "unknown option or value");

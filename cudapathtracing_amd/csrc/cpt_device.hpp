@@ -390,6 +390,131 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
   return hit;
 }
 
+// ---- resumable four-wide walk (persistent kernels) ----------------------------
+// The same walk as traceClosest4, one record visit per call, so that a lane
+// whose ray is finished can take a new ray while the rest of its wave is
+// still walking.  n >= 2 (a tree with at least one record).
+struct Walk4 {
+  f3 o, d, inv, ood;
+  float tmin, tbest;
+  int node, sp, hit;
+};
+
+__device__ __forceinline__ void walk4Init(Walk4 &w, f3 o, f3 d, float tmin, float tmax) {
+  const float eps = 1e-20f;
+  const f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
+                     fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
+  w.o = o;
+  w.d = d;
+  w.inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  w.ood = mkf3(o.x * w.inv.x, o.y * w.inv.y, o.z * w.inv.z);
+  w.tmin = tmin;
+  w.tbest = tmax;
+  w.node = 0;
+  w.sp = 0;
+  w.hit = -1;
+}
+
+// Visits record w.node.  Returns false when the walk is over (w.hit, w.tbest
+// hold the result; kAny: the first hit ends it).
+template <bool kCount, bool kAny>
+__device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ nodes4,
+                                          const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
+                                          TraceCounters &cnt) {
+  if (kCount) cnt.nodes++;
+  const float4 *np = nodes4 + 8 * (int64_t)w.node;
+  const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+  const float4 rf = np[6];
+  const float bx0[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bx1[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
+  const float by0[4] = {mny.x, mny.y, mny.z, mny.w}, by1[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
+  const float bz0[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bz1[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
+  const int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+  float tn[4];
+  bool h[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float x0 = __builtin_fmaf(bx0[k], w.inv.x, -w.ood.x), x1 = __builtin_fmaf(bx1[k], w.inv.x, -w.ood.x);
+    const float y0 = __builtin_fmaf(by0[k], w.inv.y, -w.ood.y), y1 = __builtin_fmaf(by1[k], w.inv.y, -w.ood.y);
+    const float z0 = __builtin_fmaf(bz0[k], w.inv.z, -w.ood.z), z1 = __builtin_fmaf(bz1[k], w.inv.z, -w.ood.z);
+    tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), w.tmin));
+    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), w.tbest));
+    h[k] = tn[k] <= tf && ref[k] != kEmpty4;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (h[k] && ref[k] < 0) {
+      float t;
+      if (kCount) cnt.prims++;
+      if (hitLeaf(leaves, ~ref[k], w.o, w.d, w.tmin, w.tbest, t)) {
+        w.tbest = t;
+        w.hit = ~ref[k];
+        if (kAny) return false;
+      }
+      h[k] = false;
+    }
+  }
+  int best = -1;
+  float bt = INFINITY;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (h[k] && tn[k] < bt) {
+      bt = tn[k];
+      best = k;
+    }
+  if (best < 0) {
+    if (w.sp == 0) return false;
+    w.sp--;
+    w.node = w.sp < kStackLds ? stk[w.sp * stride] : ovf[w.sp - kStackLds];
+    return true;
+  }
+  float pt[4];
+  int pr[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    pt[k] = (h[k] && k != best) ? tn[k] : -INFINITY;
+    pr[k] = ref[k];
+  }
+#define CPT_CSWAP(a, b)        \
+  if (pt[a] < pt[b]) {         \
+    const float tt_ = pt[a];   \
+    pt[a] = pt[b];             \
+    pt[b] = tt_;               \
+    const int tr_ = pr[a];     \
+    pr[a] = pr[b];             \
+    pr[b] = tr_;               \
+  }
+  CPT_CSWAP(0, 1)
+  CPT_CSWAP(2, 3)
+  CPT_CSWAP(0, 2)
+  CPT_CSWAP(1, 3)
+  CPT_CSWAP(1, 2)
+#undef CPT_CSWAP
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (pt[k] != -INFINITY) {
+      if (w.sp < kStackLds) stk[w.sp * stride] = pr[k];
+      else if (w.sp < kStack4Total) ovf[w.sp - kStackLds] = pr[k];
+      w.sp = w.sp < kStack4Total ? w.sp + 1 : w.sp;
+    }
+  }
+  w.node = ref[best];
+  return true;
+}
+
+// Wave-aggregated fetch of work items for the lanes with `want`: returns the
+// item index of each such lane (>= total when the work is exhausted) and
+// sets `exhausted` (wave-uniform) once the counter has passed total.
+__device__ __forceinline__ uint64_t waveFetch(uint32_t *counter, bool want, uint64_t total, bool &exhausted) {
+  const uint64_t m = __ballot(want);
+  const int lane = (int)__lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if ((uint64_t)base + (uint64_t)__popcll(m) >= total) exhausted = true;
+  return (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // ---- FP64 refinement of the winning hit ------------------------------------
 // Recomputes the distance to primitive `id` with the reference's FP64
 // formulas (Hittable.cuh:27-54, 85-140, 367-398) on the FP64 scene copy.

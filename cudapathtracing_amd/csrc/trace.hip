@@ -5,6 +5,8 @@
 // batches (key = direction octant + origin Morton cell, LSD radix sort).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cpt_device.hpp"
 #include "internal.hpp"
 
@@ -76,6 +78,66 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   if (idOut) idOut[i] = id;
 }
 
+// Persistent variant of k_trace<kCount, true>: a fixed grid of waves takes
+// rays from a global counter, and a lane whose walk is over takes the next
+// ray as soon as kRefill lanes of its wave are idle, so lanes are not left
+// waiting for the longest walk of their wave.  Every wave leaves once the
+// counter has passed nRays and its lanes are idle.
+constexpr int kRefill = 16;
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
+    const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
+    const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, Scene64 s64, float *__restrict__ tOut,
+    int32_t *__restrict__ idOut, int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut,
+    uint32_t *__restrict__ work) {
+  __shared__ int stack[kStackLds * kTraceThreads];
+  int *stk = stack + threadIdx.x;
+  int ovf[kStack4Total - kStackLds];
+  Walk4 w;
+  TraceCounters cnt;
+  float4 r0 = make_float4(0, 0, 0, 0), r1 = r0;
+  int64_t ri = 0;
+  bool active = false, exhausted = false;
+  while (true) {
+    if (!exhausted && __popcll(__ballot(!active)) >= kRefill) {
+      const uint64_t item = waveFetch(work, !active, (uint64_t)nRays, exhausted);
+      if (!active && item < (uint64_t)nRays) {
+        ri = perm ? (int64_t)perm[item] : (int64_t)item;
+        r0 = rays[2 * ri];
+        r1 = rays[2 * ri + 1];
+        walk4Init(w, xyz(r0), xyz(r1), r0.w, r1.w);
+        cnt = TraceCounters();
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (active && !walk4Step<kCount, false>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
+      active = false;
+      if (kCount) {
+        nodesOut[ri] = cnt.nodes;
+        primsOut[ri] = cnt.prims;
+      } else {
+        int id = -1;
+        float t = r1.w;
+        if (w.hit >= 0) {
+          id = __float_as_int(leaves[3 * (int64_t)w.hit].w);
+          t = (float)refineT(s64, id, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, r1.w);
+        }
+        if (tOut) tOut[ri] = t;
+        if (idOut) idOut[ri] = id;
+      }
+    }
+  }
+}
+
+__global__ void k_zero_work(uint32_t *p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev) {
   if (n <= 0) return CPT_OK;
@@ -104,7 +166,24 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
     perm = ctx->rvA.as<uint32_t>();
     CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
   }
-  if (nodesDev) {
+  if (wide && ctx->persistent && np > 1) {
+    if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
+    uint32_t *work = ctx->cntA.as<uint32_t>();
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
+    // one resident wave set: as many workgroups as fit on every CU at once
+    int perCU = 0;
+    if (nodesDev)
+      CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_trace_persist<true>, kTraceThreads, 0));
+    else
+      CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_trace_persist<false>, kTraceThreads, 0));
+    const unsigned pgrid = std::min<unsigned>(grid, (unsigned)ctx->numCU * (unsigned)std::max(1, perCU));
+    if (nodesDev)
+      hipLaunchKernelGGL((k_trace_persist<true>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, n4,
+                         lv, s64, tDev, idDev, nodesDev, primsDev, work);
+    else
+      hipLaunchKernelGGL((k_trace_persist<false>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm,
+                         n4, lv, s64, tDev, idDev, nodesDev, primsDev, work);
+  } else if (nodesDev) {
     if (wide)
       hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, n4,
                          lv, np, s64, tDev, idDev, nodesDev, primsDev);

@@ -137,6 +137,10 @@ const char *cpt_version(void);
  * results are written in the caller's order.  0 disables it; default 2^20. */
 #define CPT_OPT_TRAVERSAL_WIDTH 1
 #define CPT_OPT_RAY_SORT_MIN 2
+/* CPT_OPT_PERSISTENT: 1 runs four-wide cpt_trace_closest traversals as
+ * persistent kernels whose lanes take a new ray when theirs is finished; 0
+ * (default) launches one lane per ray.  Results are identical. */
+#define CPT_OPT_PERSISTENT 3
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */

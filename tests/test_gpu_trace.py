@@ -49,17 +49,21 @@ def check_small(ctx, s, rays):
     tb, idb, cert = o.trace_brute(rays, eb=1e-4, et=1e-4)
     print(f"  certified {cert.mean():.5f} of {len(rays)} rays, hits {np.mean(idb >= 0):.3f}")
     assert cert.mean() > 0.95
-    for width in (2, 4):  # binary nodes and four-wide records
+    # binary nodes, four-wide records one lane per ray, four-wide persistent
+    for width, persistent in ((2, False), (4, False), (4, True)):
         ctx.set_traversal_width(width)
+        ctx.set_persistent(persistent)
         t, ids = ctx.trace(rays)
         bad = np.where(cert & (ids != idb))[0]
-        assert bad.size == 0, (width, bad[:10], ids[bad[:10]], idb[bad[:10]])
+        assert bad.size == 0, (width, persistent, bad[:10], ids[bad[:10]], idb[bad[:10]])
         h = cert & (idb >= 0)
-        rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
-        assert rel.max() <= T_REL, (width, rel.max())
+        if h.any():
+            rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
+            assert rel.max() <= T_REL, (width, persistent, rel.max())
         miss = cert & (idb < 0)
         assert np.all(t[miss] == rays[miss, 7])
     ctx.set_traversal_width(4)
+    ctx.set_persistent(False)
     return cert.mean()
 
 
@@ -185,14 +189,18 @@ def test_ray_ordering_keeps_results(ctx):
     ctx.build(s.bounds_min, s.bounds_max)
     rays = scenes.random_rays(300_001, seed=4)
     ctx.set_ray_sort_min(0)
+    ctx.set_persistent(False)
     t0, i0 = ctx.trace(rays)
-    ctx.set_ray_sort_min(1)
-    t1, i1 = ctx.trace(rays)
-    tot, srt, ker = ctx.last_trace_stats()
-    assert srt > 0 and tot >= ker
     n0, p0 = ctx.trace_count(rays)
+    ctx.set_persistent(True)
+    for smin in (0, 1):
+        ctx.set_ray_sort_min(smin)
+        t1, i1 = ctx.trace(rays)
+        tot, srt, ker = ctx.last_trace_stats()
+        assert (srt > 0) == (smin == 1) and tot >= ker
+        n1, p1 = ctx.trace_count(rays)
+        assert np.array_equal(i0, i1) and np.array_equal(t0, t1), smin
+        assert np.array_equal(n0, n1) and np.array_equal(p0, p1), smin
     ctx.set_ray_sort_min(1 << 20)
-    n1, p1 = ctx.trace_count(rays)
-    assert np.array_equal(i0, i1) and np.array_equal(t0, t1)
-    assert np.array_equal(n0, n1) and np.array_equal(p0, p1)
+    ctx.set_persistent(False)
     assert np.mean(i0 >= 0) > 0.1
