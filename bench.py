@@ -7,8 +7,8 @@ U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
 1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
 Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
 the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
-per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 112 per four-wide node record
-visited (7 of its 8 float4 are read) + 48 per primitive tested, node/primitive counts from a counting run of the same
+per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 64 per four-wide node record
+visited (quantized records; 112 with --records fp32) + 48 per primitive tested, node/primitive counts from a counting run of the same
 frame, divided by the summed HIP-event durations of the extension launches.
 
 Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
@@ -33,7 +33,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpix/s at equal relMSE (denoised vs plain PT); Mrays/s; LBVH build ms"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 112.0  # bytes read per visit of a four-wide node record (the default traversal)
+# bytes read per visit of a four-wide node record: 64 B quantized records
+# (the default) or 7 of the 8 float4 of an FP32 record
+NODE_BYTES = {"quant": 64.0, "fp32": 112.0}
 
 
 def log(msg):
@@ -47,6 +49,7 @@ def cores():
 def bench_render(cpt, L, scenes, args):
     s = scenes.displaced_icosphere(1920, 1080)
     ctx = cpt.Context(0)
+    ctx.set_quantized(args.records == "quant")
     ctx.upload(s)
     st_b = ctx.build(s.bounds_min, s.bounds_max)
     kw = dict(seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
@@ -63,7 +66,7 @@ def bench_render(cpt, L, scenes, args):
     rays = float(np.mean([f.rays for f in frames]))
     ext_ms = float(np.mean([f.extend_ms for f in frames]))
     ext_rays = float(np.mean([f.extend_rays for f in frames]))
-    ext_bytes = 44.0 * stc.extend_rays + NODE_BYTES * stc.extend_nodes + 48.0 * stc.extend_prims
+    ext_bytes = 44.0 * stc.extend_rays + NODE_BYTES[args.records] * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
@@ -160,6 +163,7 @@ def bench_lbvh(cpt, scenes, args):
     scene = scenes.triangle_soup_scene(args.tris, seed=1)
     rays_np = scenes.random_rays(args.rays, seed=2)
     ctx = cpt.Context(0)
+    ctx.set_quantized(args.records == "quant")
     ctx.upload(scene)
     rays = torch.from_numpy(rays_np).cuda()
     t_out = torch.empty(args.rays, dtype=torch.float32, device="cuda")
@@ -186,7 +190,7 @@ def bench_lbvh(cpt, scenes, args):
     n_nodes = float(nodes.double().sum().item())
     n_prims = float(prims.double().sum().item())
     hits = float((id_out >= 0).double().mean().item())
-    alg = 40.0 * args.rays + NODE_BYTES * n_nodes + 48.0 * n_prims
+    alg = 40.0 * args.rays + NODE_BYTES[args.records] * n_nodes + 48.0 * n_prims
     tr_avg, bd_avg = float(np.mean(trace_ms)), float(np.mean(build_ms))
     k_avg, s_avg = float(np.mean(kern_ms)), float(np.mean(sort_ms))
     sa = np.mean(np.array(stage), axis=0).tolist()
@@ -264,6 +268,8 @@ def main():
     ap.add_argument("--eq-width", type=int, default=960)
     ap.add_argument("--eq-height", type=int, default=540)
     ap.add_argument("--eq-ref-spp", type=int, default=4096)
+    ap.add_argument("--records", choices=["quant", "fp32"], default="quant",
+                    help="four-wide node records read by the traversal")
     ap.add_argument("--only", choices=["render", "lbvh", "eq"], default=None,
                     help="run one workload and print its record (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -304,7 +310,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain_path_tracer_displaced_icosphere", "triangles": ntri, "width": 1920,
+        "config": {"workload": "plain_path_tracer_displaced_icosphere", "records": args.records,
+                   "triangles": ntri, "width": 1920,
                    "height": 1080, "spp": args.spp, "max_depth": 8, "rr_depth": 3},
         "mpix_per_s": round(1920 * 1080 / (r["ms"] * 1e-3) / 1e6, 3),
         "rays_per_frame": r["rays"],

@@ -74,6 +74,10 @@ class Context:
         """4 (default): four-wide records; 2: binary LBVH nodes."""
         self._check(self._lib.cpt_set_option(self._h, 1, width))
 
+    def set_quantized(self, on: bool):
+        """Four-wide walks on 64 B quantized records (default) or 128 B FP32 records."""
+        self._check(self._lib.cpt_set_option(self._h, 4, 1 if on else 0))
+
     def set_persistent(self, on: bool):
         """Persistent lane-refilling traversal kernels (default off)."""
         self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))

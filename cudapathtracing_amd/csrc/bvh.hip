@@ -461,7 +461,58 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_cross(const int32_t *__
 
 // Four-wide record of every internal node (layout in cpt_device.hpp): the
 // node's children with each internal child replaced by its two children.
-__global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes4) {
+// One axis of a quantized record: origin = the lowest child plane, grid
+// step 2^e >= extent / 255, planes rounded outward to the grid and checked in
+// FP32 with the decode the traversal uses (fmaf(q, 2^e, origin)); e grows if
+// rounding leaves a top plane uncovered.
+__device__ __forceinline__ void quantAxis(const float lo[4], const float hi[4], const bool valid[4], float &origin,
+                                          uint32_t &ebyte, uint32_t &qlo, uint32_t &qhi) {
+  float mn = INFINITY, mx = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (valid[k]) {
+      mn = fminf(mn, lo[k]);
+      mx = fmaxf(mx, hi[k]);
+    }
+  if (!(mn <= mx)) mn = mx = 0.0f;
+  origin = mn;
+  const float step = (mx - mn) / 255.0f;
+  int e = -126;
+  if (step > 0.0f) {
+    int k2;
+    const float m = frexpf(step, &k2);  // step = m 2^k2, m in [0.5, 1)
+    e = (m == 0.5f) ? k2 - 1 : k2;
+  }
+  for (int attempt = 0; attempt < 8; attempt++) {
+    const int ec = e < -126 ? -126 : (e > 127 ? 127 : e);
+    const float sc = __uint_as_float((uint32_t)(ec + 127) << 23);
+    bool ok = true;
+    uint32_t l = 0, h = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint32_t ql = 255u, qh = 0u;  // empty slot: lo > hi
+      if (valid[k]) {
+        int a = (int)fminf(fmaxf(floorf((lo[k] - origin) / sc), 0.0f), 255.0f);
+        while (a > 0 && __builtin_fmaf((float)a, sc, origin) > lo[k]) a--;
+        int b = (int)fminf(fmaxf(ceilf((hi[k] - origin) / sc), 0.0f), 255.0f);
+        while (b < 255 && __builtin_fmaf((float)b, sc, origin) < hi[k]) b++;
+        if (__builtin_fmaf((float)b, sc, origin) < hi[k] || __builtin_fmaf((float)a, sc, origin) > lo[k]) ok = false;
+        ql = (uint32_t)a;
+        qh = (uint32_t)b;
+      }
+      l |= ql << (8 * k);
+      h |= qh << (8 * k);
+    }
+    ebyte = (uint32_t)(ec + 127);
+    qlo = l;
+    qhi = h;
+    if (ok || ec == 127) return;
+    e = ec + 1;
+  }
+}
+
+__global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes4,
+                            float4 *__restrict__ nodes4q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   float b[6][4];
@@ -506,6 +557,18 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
   o[5] = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
   o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
   o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  // quantized record
+  const bool valid[4] = {ref[0] != (int)0x80000000, ref[1] != (int)0x80000000, ref[2] != (int)0x80000000,
+                         ref[3] != (int)0x80000000};
+  float org[3];
+  uint32_t eb[3], ql[3], qh[3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++) quantAxis(b[ax], b[ax + 3], valid, org[ax], eb[ax], ql[ax], qh[ax]);
+  float4 *oq = nodes4q + 4 * (int64_t)i;
+  oq[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16)));
+  oq[1] = o[6];
+  oq[2] = make_float4(__uint_as_float(ql[0]), __uint_as_float(qh[0]), __uint_as_float(ql[1]), __uint_as_float(qh[1]));
+  oq[3] = make_float4(__uint_as_float(ql[2]), __uint_as_float(qh[2]), 0.f, 0.f);
 }
 
 __global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
@@ -630,7 +693,7 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
                        ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->flags.as<int32_t>(),
                        ctx->rootBox.as<float>());
     hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>());
+                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>(), ctx->nodes4q.as<float4>());
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
   return CPT_OK;
@@ -680,7 +743,8 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
       (rc = ensure(ctx, ctx->parentLeaf, (size_t)n * 4)) ||
       (rc = ensure(ctx, ctx->flags, (size_t)n * 4)) || (rc = ensure(ctx, ctx->leaves, (size_t)n * 48)) ||
       (rc = ensure(ctx, ctx->rootBox, 64)) ||
-      (rc = ensure(ctx, ctx->nodes4, (size_t)(n > 1 ? n - 1 : 1) * 128)))
+      (rc = ensure(ctx, ctx->nodes4, (size_t)(n > 1 ? n - 1 : 1) * 128)) ||
+      (rc = ensure(ctx, ctx->nodes4q, (size_t)(n > 1 ? n - 1 : 1) * 64)))
     return rc;
   double smin[3], smax[3];
   expandBounds(bmin, bmax, smin, smax);

@@ -58,7 +58,7 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->order,    &ctx->nodes,  &ctx->parentInt, &ctx->parentLeaf, &ctx->flags,
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
                    &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc,        &ctx->medDensity,
-                   &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB};
+                   &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB,        &ctx->nodes4q};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -159,6 +159,10 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
   }
   if (option == CPT_OPT_RAY_SORT_MIN && value >= 0) {
     ctx->raySortMin = value;
+    return CPT_OK;
+  }
+  if (option == CPT_OPT_QUANTIZED && (value == 0 || value == 1)) {
+    ctx->quantized = value != 0;
     return CPT_OK;
   }
   if (option == CPT_OPT_PERSISTENT && (value == 0 || value == 1)) {

@@ -280,7 +280,52 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
 constexpr int kEmpty4 = (int)0x80000000;
 constexpr int kStack4Total = 96;  // <= 3 pushes per record, <= 31 record levels
 
-template <bool kCount, bool kAny = false>
+// Quantized four-wide record (64 B, 4 x float4), same children and refs:
+// {origin x, y, z, exponent bytes} {ref[4]} {lo_x, hi_x, lo_y, hi_y}
+// {lo_z, hi_z, 0, 0}, where lo_x etc. hold one byte per child (byte k =
+// child k) and a plane decodes as fmaf(byte, 2^e_axis, origin_axis) with
+// e_axis + 127 in byte axis of the exponent word.  The build rounds every
+// plane outward on that grid and checks the decoded FP32 value, so each
+// decoded box contains the child's FP32 box.
+template <bool kQuant>
+__device__ __forceinline__ void loadRecord4(const float4 *__restrict__ recs, int node, float bx0[4], float bx1[4],
+                                            float by0[4], float by1[4], float bz0[4], float bz1[4], int ref[4]) {
+  if (!kQuant) {
+    const float4 *np = recs + 8 * (int64_t)node;
+    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
+    const float4 rf = np[6];
+    bx0[0] = mnx.x; bx0[1] = mnx.y; bx0[2] = mnx.z; bx0[3] = mnx.w;
+    bx1[0] = mxx.x; bx1[1] = mxx.y; bx1[2] = mxx.z; bx1[3] = mxx.w;
+    by0[0] = mny.x; by0[1] = mny.y; by0[2] = mny.z; by0[3] = mny.w;
+    by1[0] = mxy.x; by1[1] = mxy.y; by1[2] = mxy.z; by1[3] = mxy.w;
+    bz0[0] = mnz.x; bz0[1] = mnz.y; bz0[2] = mnz.z; bz0[3] = mnz.w;
+    bz1[0] = mxz.x; bz1[1] = mxz.y; bz1[2] = mxz.z; bz1[3] = mxz.w;
+    ref[0] = __float_as_int(rf.x); ref[1] = __float_as_int(rf.y);
+    ref[2] = __float_as_int(rf.z); ref[3] = __float_as_int(rf.w);
+  } else {
+    const float4 *np = recs + 4 * (int64_t)node;
+    const float4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+    const uint32_t eb = __float_as_uint(w0.w);
+    const float sx = __uint_as_float((eb & 0xffu) << 23);
+    const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+    const uint32_t lx = __float_as_uint(w2.x), hx = __float_as_uint(w2.y), ly = __float_as_uint(w2.z),
+                   hy = __float_as_uint(w2.w), lz = __float_as_uint(w3.x), hz = __float_as_uint(w3.y);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      bx0[k] = __builtin_fmaf((float)((lx >> (8 * k)) & 255u), sx, w0.x);
+      bx1[k] = __builtin_fmaf((float)((hx >> (8 * k)) & 255u), sx, w0.x);
+      by0[k] = __builtin_fmaf((float)((ly >> (8 * k)) & 255u), sy, w0.y);
+      by1[k] = __builtin_fmaf((float)((hy >> (8 * k)) & 255u), sy, w0.y);
+      bz0[k] = __builtin_fmaf((float)((lz >> (8 * k)) & 255u), sz, w0.z);
+      bz1[k] = __builtin_fmaf((float)((hz >> (8 * k)) & 255u), sz, w0.z);
+    }
+    ref[0] = __float_as_int(w1.x); ref[1] = __float_as_int(w1.y);
+    ref[2] = __float_as_int(w1.z); ref[3] = __float_as_int(w1.w);
+  }
+}
+
+template <bool kCount, bool kAny = false, bool kQuant = false>
 __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
                                              const float4 *__restrict__ leaves, int n, f3 o, f3 d,
                                              float tmin, float &tbest, int *stk, int stride,
@@ -306,13 +351,9 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
   int node = 0;
   while (true) {
     if (kCount) cnt.nodes++;
-    const float4 *np = nodes4 + 8 * (int64_t)node;
-    const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-    const float4 rf = np[6];
-    const float bx0[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bx1[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-    const float by0[4] = {mny.x, mny.y, mny.z, mny.w}, by1[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
-    const float bz0[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bz1[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-    const int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+    float bx0[4], bx1[4], by0[4], by1[4], bz0[4], bz1[4];
+    int ref[4];
+    loadRecord4<kQuant>(nodes4, node, bx0, bx1, by0, by1, bz0, bz1, ref);
     float tn[4];
     bool h[4];
 #pragma unroll
@@ -417,18 +458,14 @@ __device__ __forceinline__ void walk4Init(Walk4 &w, f3 o, f3 d, float tmin, floa
 
 // Visits record w.node.  Returns false when the walk is over (w.hit, w.tbest
 // hold the result; kAny: the first hit ends it).
-template <bool kCount, bool kAny>
+template <bool kCount, bool kAny, bool kQuant = false>
 __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ nodes4,
                                           const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
                                           TraceCounters &cnt) {
   if (kCount) cnt.nodes++;
-  const float4 *np = nodes4 + 8 * (int64_t)w.node;
-  const float4 mnx = np[0], mxx = np[1], mny = np[2], mxy = np[3], mnz = np[4], mxz = np[5];
-  const float4 rf = np[6];
-  const float bx0[4] = {mnx.x, mnx.y, mnx.z, mnx.w}, bx1[4] = {mxx.x, mxx.y, mxx.z, mxx.w};
-  const float by0[4] = {mny.x, mny.y, mny.z, mny.w}, by1[4] = {mxy.x, mxy.y, mxy.z, mxy.w};
-  const float bz0[4] = {mnz.x, mnz.y, mnz.z, mnz.w}, bz1[4] = {mxz.x, mxz.y, mxz.z, mxz.w};
-  const int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+  float bx0[4], bx1[4], by0[4], by1[4], bz0[4], bz1[4];
+  int ref[4];
+  loadRecord4<kQuant>(nodes4, w.node, bx0, bx1, by0, by1, bz0, bz1, ref);
   float tn[4];
   bool h[4];
 #pragma unroll

@@ -57,6 +57,7 @@ struct cpt_ctx {
   cpt::DevBuf codesA, codesB, valsA, valsB, hist, rowTotal;
   cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
   cpt::DevBuf nodes4;   // (N-1) x 128 B four-wide records (grandchildren of each node)
+  cpt::DevBuf nodes4q;  // (N-1) x 64 B quantized four-wide records
   cpt::DevBuf esc;      // refit escape list + its counter
   // heterogeneous medium (density grid), cleared by cpt_scene_upload
   bool hasMedium = false;
@@ -66,6 +67,7 @@ struct cpt_ctx {
   int medN[3] = {0, 0, 0};
   bool hasVolumeMat = false;  // some material is CPT_M_VOLUME (k_volume runs)
   int traversal = 4;    // 2: binary node walk, 4: four-wide records
+  bool quantized = true;  // four-wide walks read the 64 B quantized records
   bool persistent = false;  // four-wide traces as persistent, lane-refilling kernels
   int numCU = 256;
   int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered

@@ -46,7 +46,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__rest
 }
 
 // perm (optional): lane i traces ray perm[i] and writes its outputs at perm[i].
-template <bool kCount, bool kWide>
+// kFmt: 0 binary nodes, 1 four-wide FP32 records, 2 four-wide quantized
+template <bool kCount, int kFmt>
 __global__ __launch_bounds__(kTraceThreads) void k_trace(
     const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
     const float4 *__restrict__ nodes, const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves,
@@ -61,8 +62,11 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   float tbest = r1.w;
   TraceCounters cnt;
   const int leaf =
-      kWide ? traceClosest4<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt)
-            : traceClosest<kCount>(nodes, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt);
+      kFmt == 2 ? traceClosest4<kCount, false, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x,
+                                                      kTraceThreads, cnt)
+      : kFmt == 1
+          ? traceClosest4<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt)
+          : traceClosest<kCount>(nodes, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt);
   if (kCount) {
     nodesOut[i] = cnt.nodes;
     primsOut[i] = cnt.prims;
@@ -85,7 +89,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
 // counter has passed nRays and its lanes are idle.
 constexpr int kRefill = 16;
 
-template <bool kCount>
+template <bool kCount, bool kQuant>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
     const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, Scene64 s64, float *__restrict__ tOut,
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
       if (exhausted) break;
       continue;
     }
-    if (active && !walk4Step<kCount, false>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
+    if (active && !walk4Step<kCount, false, kQuant>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
       active = false;
       if (kCount) {
         nodesOut[ri] = cnt.nodes;
@@ -166,37 +170,37 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
     perm = ctx->rvA.as<uint32_t>();
     CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
   }
-  if (wide && ctx->persistent && np > 1) {
+  const int fmt = wide ? (ctx->quantized ? 2 : 1) : 0;
+  const float4 *rec4 = fmt == 2 ? ctx->nodes4q.as<float4>() : n4;
+  if (fmt != 0 && ctx->persistent && np > 1) {
     if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
     uint32_t *work = ctx->cntA.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
     // one resident wave set: as many workgroups as fit on every CU at once
     int perCU = 0;
-    if (nodesDev)
-      CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_trace_persist<true>, kTraceThreads, 0));
-    else
-      CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_trace_persist<false>, kTraceThreads, 0));
+    const void *kp = nodesDev ? (fmt == 2 ? (const void *)k_trace_persist<true, true> : (const void *)k_trace_persist<true, false>)
+                              : (fmt == 2 ? (const void *)k_trace_persist<false, true> : (const void *)k_trace_persist<false, false>);
+    CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kp, kTraceThreads, 0));
     const unsigned pgrid = std::min<unsigned>(grid, (unsigned)ctx->numCU * (unsigned)std::max(1, perCU));
-    if (nodesDev)
-      hipLaunchKernelGGL((k_trace_persist<true>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, n4,
-                         lv, s64, tDev, idDev, nodesDev, primsDev, work);
-    else
-      hipLaunchKernelGGL((k_trace_persist<false>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm,
-                         n4, lv, s64, tDev, idDev, nodesDev, primsDev, work);
-  } else if (nodesDev) {
-    if (wide)
-      hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, n4,
-                         lv, np, s64, tDev, idDev, nodesDev, primsDev);
-    else
-      hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, n4,
-                         lv, np, s64, tDev, idDev, nodesDev, primsDev);
+#define CPT_LAUNCH_P(C, Q)                                                                                         \
+  hipLaunchKernelGGL((k_trace_persist<C, Q>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, rec4, lv, \
+                     s64, tDev, idDev, nodesDev, primsDev, work)
+    if (nodesDev) {
+      if (fmt == 2) CPT_LAUNCH_P(true, true); else CPT_LAUNCH_P(true, false);
+    } else {
+      if (fmt == 2) CPT_LAUNCH_P(false, true); else CPT_LAUNCH_P(false, false);
+    }
+#undef CPT_LAUNCH_P
   } else {
-    if (wide)
-      hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd,
-                         n4, lv, np, s64, tDev, idDev, nodesDev, primsDev);
-    else
-      hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd,
-                         n4, lv, np, s64, tDev, idDev, nodesDev, primsDev);
+#define CPT_LAUNCH_T(C, F)                                                                                        \
+  hipLaunchKernelGGL((k_trace<C, F>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, rec4, lv, np, \
+                     s64, tDev, idDev, nodesDev, primsDev)
+    if (nodesDev) {
+      if (fmt == 2) CPT_LAUNCH_T(true, 2); else if (fmt == 1) CPT_LAUNCH_T(true, 1); else CPT_LAUNCH_T(true, 0);
+    } else {
+      if (fmt == 2) CPT_LAUNCH_T(false, 2); else if (fmt == 1) CPT_LAUNCH_T(false, 1); else CPT_LAUNCH_T(false, 0);
+    }
+#undef CPT_LAUNCH_T
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[9], ctx->stream));

@@ -49,20 +49,25 @@ def check_small(ctx, s, rays):
     tb, idb, cert = o.trace_brute(rays, eb=1e-4, et=1e-4)
     print(f"  certified {cert.mean():.5f} of {len(rays)} rays, hits {np.mean(idb >= 0):.3f}")
     assert cert.mean() > 0.95
-    # binary nodes, four-wide records one lane per ray, four-wide persistent
-    for width, persistent in ((2, False), (4, False), (4, True)):
+    # binary nodes; four-wide FP32 and quantized records, one lane per ray
+    # and persistent
+    for width, quant, persistent in ((2, False, False), (4, False, False), (4, True, False), (4, True, True),
+                                     (4, False, True)):
         ctx.set_traversal_width(width)
+        ctx.set_quantized(quant)
         ctx.set_persistent(persistent)
+        cfg = (width, quant, persistent)
         t, ids = ctx.trace(rays)
         bad = np.where(cert & (ids != idb))[0]
-        assert bad.size == 0, (width, persistent, bad[:10], ids[bad[:10]], idb[bad[:10]])
+        assert bad.size == 0, (cfg, bad[:10], ids[bad[:10]], idb[bad[:10]])
         h = cert & (idb >= 0)
         if h.any():
             rel = np.abs(t[h].astype(np.float64) - tb[h]) / np.abs(tb[h])
-            assert rel.max() <= T_REL, (width, persistent, rel.max())
+            assert rel.max() <= T_REL, (cfg, rel.max())
         miss = cert & (idb < 0)
         assert np.all(t[miss] == rays[miss, 7])
     ctx.set_traversal_width(4)
+    ctx.set_quantized(True)
     ctx.set_persistent(False)
     return cert.mean()
 
