@@ -7,8 +7,8 @@ U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
 1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
 Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
 the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
-per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 64 per four-wide node record
-visited (quantized records; 112 with --records fp32) + 48 per primitive tested, node/primitive counts from a counting run of the same
+per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 112 per four-wide node record
+visited (7 of its 8 float4; 64 with --records quant) + 48 per primitive tested, node/primitive counts from a counting run of the same
 frame, divided by the summed HIP-event durations of the extension launches.
 
 Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
@@ -33,8 +33,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpix/s at equal relMSE (denoised vs plain PT); Mrays/s; LBVH build ms"
 HBM_PEAK_GBS = 8000.0
-# bytes read per visit of a four-wide node record: 64 B quantized records
-# (the default) or 7 of the 8 float4 of an FP32 record
+# bytes read per visit of a four-wide node record: 7 of the 8 float4 of an
+# FP32 record (the default) or a 64 B quantized record
 NODE_BYTES = {"quant": 64.0, "fp32": 112.0}
 
 
@@ -268,7 +268,7 @@ def main():
     ap.add_argument("--eq-width", type=int, default=960)
     ap.add_argument("--eq-height", type=int, default=540)
     ap.add_argument("--eq-ref-spp", type=int, default=4096)
-    ap.add_argument("--records", choices=["quant", "fp32"], default="quant",
+    ap.add_argument("--records", choices=["quant", "fp32"], default="fp32",
                     help="four-wide node records read by the traversal")
     ap.add_argument("--only", choices=["render", "lbvh", "eq"], default=None,
                     help="run one workload and print its record (profiling runs)")

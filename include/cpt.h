@@ -141,10 +141,10 @@ const char *cpt_version(void);
  * persistent kernels whose lanes take a new ray when theirs is finished; 0
  * (default) launches one lane per ray.  Results are identical. */
 #define CPT_OPT_PERSISTENT 3
-/* CPT_OPT_QUANTIZED: 1 (default) makes four-wide walks read 64 B records
- * whose child boxes are 8-bit planes on a power-of-two grid, rounded
- * outward (each contains the FP32 box); 0 reads the 128 B FP32 records.
- * Hits are identical up to exact-t ties. */
+/* CPT_OPT_QUANTIZED: 1 makes four-wide walks read 64 B records whose child
+ * boxes are 8-bit planes on a power-of-two grid, rounded outward (each
+ * contains the FP32 box); 0 (default) reads the 128 B FP32 records.  Hits
+ * are identical up to exact-t ties. */
 #define CPT_OPT_QUANTIZED 4
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 

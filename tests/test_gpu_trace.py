@@ -67,7 +67,7 @@ def check_small(ctx, s, rays):
         miss = cert & (idb < 0)
         assert np.all(t[miss] == rays[miss, 7])
     ctx.set_traversal_width(4)
-    ctx.set_quantized(True)
+    ctx.set_quantized(False)
     ctx.set_persistent(False)
     return cert.mean()
 
