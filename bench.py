@@ -33,9 +33,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpix/s at equal relMSE (denoised vs plain PT); Mrays/s; LBVH build ms"
 HBM_PEAK_GBS = 8000.0
-# bytes read per visit of a four-wide node record: 7 of the 8 float4 of an
-# FP32 record (the default) or a 64 B quantized record
-NODE_BYTES = {"quant": 64.0, "fp32": 112.0}
+def node_bytes(args):
+    """Bytes read per visit of a node record: 7 of the 8 float4 of a four-wide
+    FP32 record (the default), a 64 B four-wide quantized record, 6 float4 of
+    an eight-wide quantized record, or a 64 B binary node."""
+    if args.width == 8:
+        return 96.0
+    if args.width == 2:
+        return 64.0
+    return 64.0 if args.records == "quant" else 112.0
 
 
 def log(msg):
@@ -50,6 +56,7 @@ def bench_render(cpt, L, scenes, args):
     s = scenes.displaced_icosphere(1920, 1080)
     ctx = cpt.Context(0)
     ctx.set_quantized(args.records == "quant")
+    ctx.set_traversal_width(args.width)
     ctx.upload(s)
     st_b = ctx.build(s.bounds_min, s.bounds_max)
     kw = dict(seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
@@ -66,7 +73,7 @@ def bench_render(cpt, L, scenes, args):
     rays = float(np.mean([f.rays for f in frames]))
     ext_ms = float(np.mean([f.extend_ms for f in frames]))
     ext_rays = float(np.mean([f.extend_rays for f in frames]))
-    ext_bytes = 44.0 * stc.extend_rays + NODE_BYTES[args.records] * stc.extend_nodes + 48.0 * stc.extend_prims
+    ext_bytes = 44.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
@@ -164,6 +171,7 @@ def bench_lbvh(cpt, scenes, args):
     rays_np = scenes.random_rays(args.rays, seed=2)
     ctx = cpt.Context(0)
     ctx.set_quantized(args.records == "quant")
+    ctx.set_traversal_width(args.width)
     ctx.upload(scene)
     rays = torch.from_numpy(rays_np).cuda()
     t_out = torch.empty(args.rays, dtype=torch.float32, device="cuda")
@@ -190,7 +198,7 @@ def bench_lbvh(cpt, scenes, args):
     n_nodes = float(nodes.double().sum().item())
     n_prims = float(prims.double().sum().item())
     hits = float((id_out >= 0).double().mean().item())
-    alg = 40.0 * args.rays + NODE_BYTES[args.records] * n_nodes + 48.0 * n_prims
+    alg = 40.0 * args.rays + node_bytes(args) * n_nodes + 48.0 * n_prims
     tr_avg, bd_avg = float(np.mean(trace_ms)), float(np.mean(build_ms))
     k_avg, s_avg = float(np.mean(kern_ms)), float(np.mean(sort_ms))
     sa = np.mean(np.array(stage), axis=0).tolist()
@@ -268,6 +276,7 @@ def main():
     ap.add_argument("--eq-width", type=int, default=960)
     ap.add_argument("--eq-height", type=int, default=540)
     ap.add_argument("--eq-ref-spp", type=int, default=4096)
+    ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="traversal width")
     ap.add_argument("--records", choices=["quant", "fp32"], default="fp32",
                     help="four-wide node records read by the traversal")
     ap.add_argument("--only", choices=["render", "lbvh", "eq"], default=None,
@@ -310,7 +319,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain_path_tracer_displaced_icosphere", "records": args.records,
+        "config": {"workload": "plain_path_tracer_displaced_icosphere", "width_bvh": args.width,
+                   "records": args.records,
                    "triangles": ntri, "width": 1920,
                    "height": 1080, "spp": args.spp, "max_depth": 8, "rr_depth": 3},
         "mpix_per_s": round(1920 * 1080 / (r["ms"] * 1e-3) / 1e6, 3),

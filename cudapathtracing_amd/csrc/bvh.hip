@@ -465,11 +465,12 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_cross(const int32_t *__
 // step 2^e >= extent / 255, planes rounded outward to the grid and checked in
 // FP32 with the decode the traversal uses (fmaf(q, 2^e, origin)); e grows if
 // rounding leaves a top plane uncovered.
-__device__ __forceinline__ void quantAxis(const float lo[4], const float hi[4], const bool valid[4], float &origin,
-                                          uint32_t &ebyte, uint32_t &qlo, uint32_t &qhi) {
+template <int N>
+__device__ __forceinline__ void quantAxis(const float lo[N], const float hi[N], const bool valid[N], float &origin,
+                                          uint32_t &ebyte, uint32_t qlo[N / 4], uint32_t qhi[N / 4]) {
   float mn = INFINITY, mx = -INFINITY;
 #pragma unroll
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < N; k++)
     if (valid[k]) {
       mn = fminf(mn, lo[k]);
       mx = fmaxf(mx, hi[k]);
@@ -487,9 +488,11 @@ __device__ __forceinline__ void quantAxis(const float lo[4], const float hi[4], 
     const int ec = e < -126 ? -126 : (e > 127 ? 127 : e);
     const float sc = __uint_as_float((uint32_t)(ec + 127) << 23);
     bool ok = true;
-    uint32_t l = 0, h = 0;
+    uint32_t l[N / 4], h[N / 4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
+    for (int w = 0; w < N / 4; w++) l[w] = h[w] = 0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
       uint32_t ql = 255u, qh = 0u;  // empty slot: lo > hi
       if (valid[k]) {
         int a = (int)fminf(fmaxf(floorf((lo[k] - origin) / sc), 0.0f), 255.0f);
@@ -500,12 +503,15 @@ __device__ __forceinline__ void quantAxis(const float lo[4], const float hi[4], 
         ql = (uint32_t)a;
         qh = (uint32_t)b;
       }
-      l |= ql << (8 * k);
-      h |= qh << (8 * k);
+      l[k / 4] |= ql << (8 * (k % 4));
+      h[k / 4] |= qh << (8 * (k % 4));
     }
     ebyte = (uint32_t)(ec + 127);
-    qlo = l;
-    qhi = h;
+#pragma unroll
+    for (int w = 0; w < N / 4; w++) {
+      qlo[w] = l[w];
+      qhi[w] = h[w];
+    }
     if (ok || ec == 127) return;
     e = ec + 1;
   }
@@ -547,28 +553,115 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
     for (int q = 0; q < 6; q++) b[q][k] = 0.f;
     ref[k] = (int)0x80000000;
   }
-  float4 *o = nodes4 + 8 * (int64_t)i;
-  // SoA: minx maxx miny maxy minz maxz refs
-  o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
-  o[1] = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
-  o[2] = make_float4(b[1][0], b[1][1], b[1][2], b[1][3]);
-  o[3] = make_float4(b[4][0], b[4][1], b[4][2], b[4][3]);
-  o[4] = make_float4(b[2][0], b[2][1], b[2][2], b[2][3]);
-  o[5] = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
-  o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
-  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (nodes4) {
+    float4 *o = nodes4 + 8 * (int64_t)i;
+    // SoA: minx maxx miny maxy minz maxz refs
+    o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+    o[1] = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
+    o[2] = make_float4(b[1][0], b[1][1], b[1][2], b[1][3]);
+    o[3] = make_float4(b[4][0], b[4][1], b[4][2], b[4][3]);
+    o[4] = make_float4(b[2][0], b[2][1], b[2][2], b[2][3]);
+    o[5] = make_float4(b[5][0], b[5][1], b[5][2], b[5][3]);
+    o[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
+                       __int_as_float(ref[3]));
+    o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  if (!nodes4q) return;
   // quantized record
   const bool valid[4] = {ref[0] != (int)0x80000000, ref[1] != (int)0x80000000, ref[2] != (int)0x80000000,
                          ref[3] != (int)0x80000000};
   float org[3];
-  uint32_t eb[3], ql[3], qh[3];
+  uint32_t eb[3], ql[3][1], qh[3][1];
 #pragma unroll
-  for (int ax = 0; ax < 3; ax++) quantAxis(b[ax], b[ax + 3], valid, org[ax], eb[ax], ql[ax], qh[ax]);
+  for (int ax = 0; ax < 3; ax++) quantAxis<4>(b[ax], b[ax + 3], valid, org[ax], eb[ax], ql[ax], qh[ax]);
   float4 *oq = nodes4q + 4 * (int64_t)i;
   oq[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16)));
-  oq[1] = o[6];
-  oq[2] = make_float4(__uint_as_float(ql[0]), __uint_as_float(qh[0]), __uint_as_float(ql[1]), __uint_as_float(qh[1]));
-  oq[3] = make_float4(__uint_as_float(ql[2]), __uint_as_float(qh[2]), 0.f, 0.f);
+  oq[1] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  oq[2] = make_float4(__uint_as_float(ql[0][0]), __uint_as_float(qh[0][0]), __uint_as_float(ql[1][0]),
+                      __uint_as_float(qh[1][0]));
+  oq[3] = make_float4(__uint_as_float(ql[2][0]), __uint_as_float(qh[2][0]), 0.f, 0.f);
+}
+
+// Eight-wide quantized record of every internal node (layout in
+// cpt_device.hpp): node i's subtree expanded three binary levels deep, so
+// every internal entry is a great-grandchild and every leaf entry lies at
+// depth 1-3.
+__global__ void k_collapse8(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes8) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  float b[6][8];
+  int ref[8];
+  int k = 0;
+  // depth-first expansion; entries are written in slot order
+  auto childBoxes = [&](int64_t g, float cb[2][6], int cr[2]) {
+    const float4 g0 = nodes[4 * g], g1 = nodes[4 * g + 1], g2 = nodes[4 * g + 2], g3 = nodes[4 * g + 3];
+    const float t[2][6] = {{g0.x, g0.z, g2.x, g0.y, g0.w, g2.y}, {g1.x, g1.z, g2.z, g1.y, g1.w, g2.w}};
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) cb[c][q] = t[c][q];
+    cr[0] = __float_as_int(g3.x);
+    cr[1] = __float_as_int(g3.y);
+  };
+  auto put = [&](const float bb[6], int r) {
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s == k) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) b[q][s] = bb[q];
+        ref[s] = r;
+      }
+    k++;
+  };
+  float c1[2][6];
+  int r1[2];
+  childBoxes(i, c1, r1);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    if (r1[a] < 0) {
+      put(c1[a], r1[a]);
+      continue;
+    }
+    float c2[2][6];
+    int r2[2];
+    childBoxes(r1[a], c2, r2);
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      if (r2[e] < 0) {
+        put(c2[e], r2[e]);
+        continue;
+      }
+      float c3[2][6];
+      int r3[2];
+      childBoxes(r2[e], c3, r3);
+      put(c3[0], r3[0]);
+      put(c3[1], r3[1]);
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 8; s++)
+    if (s >= k) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) b[q][s] = 0.f;
+      ref[s] = (int)0x80000000;
+    }
+  bool valid[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) valid[s] = ref[s] != (int)0x80000000;
+  float org[3];
+  uint32_t eb[3], ql[3][2], qh[3][2];
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++) quantAxis<8>(b[ax], b[ax + 3], valid, org[ax], eb[ax], ql[ax], qh[ax]);
+  float4 *o = nodes8 + 8 * (int64_t)i;
+  o[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16)));
+  o[1] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  o[2] = make_float4(__int_as_float(ref[4]), __int_as_float(ref[5]), __int_as_float(ref[6]), __int_as_float(ref[7]));
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++)
+    o[3 + ax] = make_float4(__uint_as_float(ql[ax][0]), __uint_as_float(ql[ax][1]), __uint_as_float(qh[ax][0]),
+                            __uint_as_float(qh[ax][1]));
+  o[6] = make_float4(0.f, 0.f, 0.f, 0.f);
+  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ void k_zero_i32(int32_t *__restrict__ p, int64_t n) {
@@ -672,14 +765,45 @@ int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysT
                     nullptr);
 }
 
-// Node boxes (two refit passes), sorted leaf records and four-wide records.
+int recordFormat(const cpt_ctx *ctx) {
+  if (ctx->traversal == 8) return 3;
+  if (ctx->traversal == 4) return ctx->quantized ? 2 : 1;
+  return 0;
+}
+
+// Wide records of the current traversal format, collapsed from the binary
+// nodes after a build or refit, or on first use after the format changed.
+int ensureRecords(cpt_ctx *ctx) {
+  const int fmt = recordFormat(ctx);
+  const int64_t n = ctx->nPrims;
+  if (fmt == 0 || n < 2 || (ctx->recValid & (1 << fmt))) return CPT_OK;
+  const size_t nn = (size_t)(n - 1);
+  int rc;
+  const unsigned grid = (unsigned)((n - 1 + 255) / 256);
+  if (fmt == 1 || fmt == 2) {
+    DevBuf &dst = fmt == 1 ? ctx->nodes4 : ctx->nodes4q;
+    if ((rc = ensure(ctx, dst, nn * (fmt == 1 ? 128 : 64)))) return rc;
+    hipLaunchKernelGGL(k_collapse4, dim3(grid), dim3(256), 0, ctx->stream, (int)n,
+                       (const float4 *)ctx->nodes.as<float4>(), fmt == 1 ? dst.as<float4>() : (float4 *)nullptr,
+                       fmt == 2 ? dst.as<float4>() : (float4 *)nullptr);
+  } else {
+    if ((rc = ensure(ctx, ctx->nodes8, nn * 128))) return rc;
+    hipLaunchKernelGGL(k_collapse8, dim3(grid), dim3(256), 0, ctx->stream, (int)n,
+                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes8.as<float4>());
+  }
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  ctx->recValid |= 1 << fmt;
+  return CPT_OK;
+}
+
+// Node boxes (two refit passes), sorted leaf records and the wide records of
+// the current traversal format.
 static int launchRefit(cpt_ctx *ctx, int64_t n) {
   const int64_t cap = refitEscapeBound(n);
   int rc = ensure(ctx, ctx->esc, (size_t)(cap + 64) * 4);
   if (rc) return rc;
   int32_t *esc = ctx->esc.as<int32_t>();
   uint32_t *escCount = (uint32_t *)(esc + cap);
-  const int bs = 256;
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)escCount, (int64_t)1);
   const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
   hipLaunchKernelGGL(k_refit_local, dim3(chunks), dim3(kRefitThreads), 0, ctx->stream, (int)n,
@@ -692,11 +816,10 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
                        dim3(kRefitThreads), 0, ctx->stream, (const int32_t *)esc, (const uint32_t *)escCount, cap,
                        ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->flags.as<int32_t>(),
                        ctx->rootBox.as<float>());
-    hipLaunchKernelGGL(k_collapse4, dim3((unsigned)((n - 1 + bs - 1) / bs)), dim3(bs), 0, ctx->stream, (int)n,
-                       (const float4 *)ctx->nodes.as<float4>(), ctx->nodes4.as<float4>(), ctx->nodes4q.as<float4>());
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
-  return CPT_OK;
+  ctx->recValid = 0;
+  return ensureRecords(ctx);
 }
 
 // Node boxes from the current primitives with the topology, parents and
@@ -742,9 +865,7 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
       (rc = ensure(ctx, ctx->parentInt, (size_t)n * 4)) ||
       (rc = ensure(ctx, ctx->parentLeaf, (size_t)n * 4)) ||
       (rc = ensure(ctx, ctx->flags, (size_t)n * 4)) || (rc = ensure(ctx, ctx->leaves, (size_t)n * 48)) ||
-      (rc = ensure(ctx, ctx->rootBox, 64)) ||
-      (rc = ensure(ctx, ctx->nodes4, (size_t)(n > 1 ? n - 1 : 1) * 128)) ||
-      (rc = ensure(ctx, ctx->nodes4q, (size_t)(n > 1 ? n - 1 : 1) * 64)))
+      (rc = ensure(ctx, ctx->rootBox, 64)))
     return rc;
   double smin[3], smax[3];
   expandBounds(bmin, bmax, smin, smax);

@@ -58,7 +58,8 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->order,    &ctx->nodes,  &ctx->parentInt, &ctx->parentLeaf, &ctx->flags,
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
                    &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc,        &ctx->medDensity,
-                   &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB,        &ctx->nodes4q};
+                   &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB,        &ctx->nodes4q,
+                   &ctx->nodes8};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -153,7 +154,7 @@ int64_t cpt_scene_count(const cpt_ctx *ctx) { return ctx ? ctx->nPrims : -1; }
 
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
   if (!ctx) return CPT_ERR_ARG;
-  if (option == CPT_OPT_TRAVERSAL_WIDTH && (value == 2 || value == 4)) {
+  if (option == CPT_OPT_TRAVERSAL_WIDTH && (value == 2 || value == 4 || value == 8)) {
     ctx->traversal = value;
     return CPT_OK;
   }

@@ -431,6 +431,135 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
   return hit;
 }
 
+// ---- eight-wide quantized traversal ------------------------------------------
+// Record of internal node i (128 B, 6 of 8 float4 used): node i's subtree
+// expanded three binary levels deep (internal entries are great-grandchildren,
+// leaf entries lie at depth 1-3), with 8-bit planes as in the quantized
+// four-wide record: {origin x, y, z, exponent bytes} {ref[0..3]} {ref[4..7]}
+// {lo_x[0..3], lo_x[4..7], hi_x[0..3], hi_x[4..7]} {… y} {… z}.  A walk
+// descends three binary levels per record (<= 22 records for depth <= 64),
+// pushing <= 7 entries each.
+constexpr int kStack8Total = 160;
+
+template <bool kCount, bool kAny = false>
+__device__ __forceinline__ int traceClosest8(const float4 *__restrict__ nodes8,
+                                             const float4 *__restrict__ leaves, int n, f3 o, f3 d,
+                                             float tmin, float &tbest, int *stk, int stride,
+                                             TraceCounters &cnt) {
+  int hit = -1;
+  if (n <= 0) return -1;
+  if (n == 1) {
+    float t;
+    if (kCount) cnt.prims++;
+    if (hitLeaf(leaves, 0, o, d, tmin, tbest, t)) {
+      tbest = t;
+      hit = 0;
+    }
+    return hit;
+  }
+  const float eps = 1e-20f;
+  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
+               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
+  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
+  int ovf[kStack8Total - kStackLds];
+  int sp = 0;
+  int node = 0;
+  while (true) {
+    if (kCount) cnt.nodes++;
+    const float4 *np = nodes8 + 8 * (int64_t)node;
+    const float4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+    const uint32_t eb = __float_as_uint(w0.w);
+    const float sx = __uint_as_float((eb & 0xffu) << 23);
+    const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+    const int ref[8] = {__float_as_int(w1.x), __float_as_int(w1.y), __float_as_int(w1.z), __float_as_int(w1.w),
+                        __float_as_int(w2.x), __float_as_int(w2.y), __float_as_int(w2.z), __float_as_int(w2.w)};
+    const uint32_t q[3][4] = {{__float_as_uint(w3.x), __float_as_uint(w3.y), __float_as_uint(w3.z), __float_as_uint(w3.w)},
+                              {__float_as_uint(w4.x), __float_as_uint(w4.y), __float_as_uint(w4.z), __float_as_uint(w4.w)},
+                              {__float_as_uint(w5.x), __float_as_uint(w5.y), __float_as_uint(w5.z), __float_as_uint(w5.w)}};
+    const float org[3] = {w0.x, w0.y, w0.z}, sc[3] = {sx, sy, sz};
+    const float iv[3] = {inv.x, inv.y, inv.z}, od[3] = {ood.x, ood.y, ood.z};
+    float tn[8];
+    bool h[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      float t0 = tmin, t1 = tbest;
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        const float lo = __builtin_fmaf((float)((q[ax][k / 4] >> (8 * (k % 4))) & 255u), sc[ax], org[ax]);
+        const float hi = __builtin_fmaf((float)((q[ax][2 + k / 4] >> (8 * (k % 4))) & 255u), sc[ax], org[ax]);
+        const float a = __builtin_fmaf(lo, iv[ax], -od[ax]), b = __builtin_fmaf(hi, iv[ax], -od[ax]);
+        t0 = fmaxf(t0, fminf(a, b));
+        t1 = fminf(t1, fmaxf(a, b));
+      }
+      tn[k] = t0;
+      h[k] = t0 <= t1 && ref[k] != kEmpty4;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (h[k] && ref[k] < 0) {
+        float t;
+        if (kCount) cnt.prims++;
+        if (hitLeaf(leaves, ~ref[k], o, d, tmin, tbest, t)) {
+          tbest = t;
+          hit = ~ref[k];
+          if (kAny) return hit;
+        }
+        h[k] = false;
+      }
+    }
+    int best = -1;
+    float bt = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (h[k] && tn[k] < bt) {
+        bt = tn[k];
+        best = k;
+      }
+    if (best < 0) {
+      if (sp == 0) break;
+      sp--;
+      node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
+      continue;
+    }
+    // the others farthest first (19-comparator network, descending)
+    float pt[8];
+    int pr[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      pt[k] = (h[k] && k != best) ? tn[k] : -INFINITY;
+      pr[k] = ref[k];
+    }
+#define CPT_CSWAP(a, b)        \
+  if (pt[a] < pt[b]) {         \
+    const float tt_ = pt[a];   \
+    pt[a] = pt[b];             \
+    pt[b] = tt_;               \
+    const int tr_ = pr[a];     \
+    pr[a] = pr[b];             \
+    pr[b] = tr_;               \
+  }
+    CPT_CSWAP(0, 2) CPT_CSWAP(1, 3) CPT_CSWAP(4, 6) CPT_CSWAP(5, 7)
+    CPT_CSWAP(0, 4) CPT_CSWAP(1, 5) CPT_CSWAP(2, 6) CPT_CSWAP(3, 7)
+    CPT_CSWAP(0, 1) CPT_CSWAP(2, 3) CPT_CSWAP(4, 5) CPT_CSWAP(6, 7)
+    CPT_CSWAP(2, 4) CPT_CSWAP(3, 5)
+    CPT_CSWAP(1, 4) CPT_CSWAP(3, 6)
+    CPT_CSWAP(1, 2) CPT_CSWAP(3, 4) CPT_CSWAP(5, 6)
+#undef CPT_CSWAP
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (pt[k] != -INFINITY) {
+        if (sp < kStackLds) stk[sp * stride] = pr[k];
+        else if (sp < kStack8Total) ovf[sp - kStackLds] = pr[k];
+        sp = sp < kStack8Total ? sp + 1 : sp;
+      }
+    }
+    node = ref[best];
+  }
+  return hit;
+}
+
 // ---- resumable four-wide walk (persistent kernels) ----------------------------
 // The same walk as traceClosest4, one record visit per call, so that a lane
 // whose ray is finished can take a new ray while the rest of its wave is

@@ -58,6 +58,8 @@ struct cpt_ctx {
   cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
   cpt::DevBuf nodes4;   // (N-1) x 128 B four-wide records (grandchildren of each node)
   cpt::DevBuf nodes4q;  // (N-1) x 64 B quantized four-wide records
+  cpt::DevBuf nodes8;   // (N-1) x 128 B quantized eight-wide records
+  int recValid = 0;     // bit f: wide records of format f are current
   cpt::DevBuf esc;      // refit escape list + its counter
   // heterogeneous medium (density grid), cleared by cpt_scene_upload
   bool hasMedium = false;
@@ -97,6 +99,10 @@ int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysT
                    uint32_t *valsTmp, int64_t n, int bits, bool identityVals);
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
 int refitBvh(cpt_ctx *ctx, cpt_build_stats *st);
+// Traversal record format: 0 binary nodes, 1 four-wide FP32, 2 four-wide
+// quantized, 3 eight-wide quantized; ensureRecords builds it if stale.
+int recordFormat(const cpt_ctx *ctx);
+int ensureRecords(cpt_ctx *ctx);
 int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, int64_t nt,
                   int64_t base);
 void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]);

@@ -110,7 +110,7 @@ __device__ __forceinline__ float ratioTrack(const MediumDev &md, uint64_t seed, 
 
 struct RenderArgs {
   const float4 *nodes, *leaves, *rec;  // nodes: binary nodes or four-wide records
-  int wide;                            // 0 binary, 1 four-wide FP32, 2 four-wide quantized
+  int wide;  // 0 binary, 1 four-wide FP32, 2 four-wide quantized, 3 eight-wide quantized
   const MatDev *mats;
   int nPrims, nLights;
   CamDev cam;
@@ -128,7 +128,8 @@ template <bool kCount>
 __device__ __forceinline__ int traceAny(int wide, const float4 *__restrict__ nodes, const float4 *__restrict__ leaves,
                                         int n, f3 o, f3 d, float tmin, float &tbest, int *stk, int stride,
                                         TraceCounters &cnt) {
-  return wide == 2   ? traceClosest4<kCount, false, true>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
+  return wide == 3   ? traceClosest8<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
+         : wide == 2 ? traceClosest4<kCount, false, true>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
          : wide == 1 ? traceClosest4<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
                      : traceClosest<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
 }
@@ -540,7 +541,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
   if (!hitLeaf(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
   float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
-  const int leaf = wide == 2 ? traceClosest4<false, true, true>(nodes, leaves, nPrims, o, d, r0.w, tb,
+  const int leaf = wide == 3 ? traceClosest8<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
+                                                          kRenderThreads, cnt)
+                   : wide == 2 ? traceClosest4<false, true, true>(nodes, leaves, nPrims, o, d, r0.w, tb,
                                                                  stack + threadIdx.x, kRenderThreads, cnt)
                    : wide == 1 ? traceClosest4<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
                                                             kRenderThreads, cnt)
@@ -610,9 +613,14 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   hipStream_t s = ctx->stream;
 
   RenderArgs a;
-  // 0: binary nodes, 1: four-wide FP32 records, 2: four-wide quantized records
-  a.wide = ctx->traversal == 4 ? (ctx->quantized ? 2 : 1) : 0;
-  a.nodes = a.wide == 2 ? ctx->nodes4q.as<float4>() : a.wide == 1 ? ctx->nodes4.as<float4>() : ctx->nodes.as<float4>();
+  // 0: binary nodes, 1: four-wide FP32 records, 2: four-wide quantized
+  // records, 3: eight-wide quantized records
+  if ((rc = ensureRecords(ctx))) return rc;
+  a.wide = recordFormat(ctx);
+  a.nodes = a.wide == 3   ? ctx->nodes8.as<float4>()
+            : a.wide == 2 ? ctx->nodes4q.as<float4>()
+            : a.wide == 1 ? ctx->nodes4.as<float4>()
+                          : ctx->nodes.as<float4>();
   a.leaves = ctx->leaves.as<float4>();
   a.rec = ctx->rec.as<float4>();
   a.mats = ctx->mats.as<MatDev>();

@@ -62,7 +62,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   float tbest = r1.w;
   TraceCounters cnt;
   const int leaf =
-      kFmt == 2 ? traceClosest4<kCount, false, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x,
+      kFmt == 3 ? traceClosest8<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads,
+                                        cnt)
+      : kFmt == 2 ? traceClosest4<kCount, false, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x,
                                                       kTraceThreads, cnt)
       : kFmt == 1
           ? traceClosest4<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt)
@@ -151,7 +153,6 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   const float4 *r = (const float4 *)raysDev;
   const float4 *nd = ctx->nodes.as<float4>(), *n4 = ctx->nodes4.as<float4>(), *lv = ctx->leaves.as<float4>();
   const int np = (int)ctx->nPrims;
-  const bool wide = ctx->traversal == 4;
   const bool sorted = ctx->raySortMin > 0 && n >= ctx->raySortMin && np > 1 && n <= 0x7fffffff;
   int rc;
   if (sorted && ((rc = ensure(ctx, ctx->rkA, n * 4)) || (rc = ensure(ctx, ctx->rkB, n * 4)) ||
@@ -170,9 +171,10 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
     perm = ctx->rvA.as<uint32_t>();
     CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
   }
-  const int fmt = wide ? (ctx->quantized ? 2 : 1) : 0;
-  const float4 *rec4 = fmt == 2 ? ctx->nodes4q.as<float4>() : n4;
-  if (fmt != 0 && ctx->persistent && np > 1) {
+  if ((rc = ensureRecords(ctx))) return rc;
+  const int fmt = recordFormat(ctx);
+  const float4 *rec4 = fmt == 3 ? ctx->nodes8.as<float4>() : fmt == 2 ? ctx->nodes4q.as<float4>() : n4;
+  if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1) {
     if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
     uint32_t *work = ctx->cntA.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
@@ -196,9 +198,15 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   hipLaunchKernelGGL((k_trace<C, F>), dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, nd, rec4, lv, np, \
                      s64, tDev, idDev, nodesDev, primsDev)
     if (nodesDev) {
-      if (fmt == 2) CPT_LAUNCH_T(true, 2); else if (fmt == 1) CPT_LAUNCH_T(true, 1); else CPT_LAUNCH_T(true, 0);
+      if (fmt == 3) CPT_LAUNCH_T(true, 3);
+      else if (fmt == 2) CPT_LAUNCH_T(true, 2);
+      else if (fmt == 1) CPT_LAUNCH_T(true, 1);
+      else CPT_LAUNCH_T(true, 0);
     } else {
-      if (fmt == 2) CPT_LAUNCH_T(false, 2); else if (fmt == 1) CPT_LAUNCH_T(false, 1); else CPT_LAUNCH_T(false, 0);
+      if (fmt == 3) CPT_LAUNCH_T(false, 3);
+      else if (fmt == 2) CPT_LAUNCH_T(false, 2);
+      else if (fmt == 1) CPT_LAUNCH_T(false, 1);
+      else CPT_LAUNCH_T(false, 0);
     }
 #undef CPT_LAUNCH_T
   }

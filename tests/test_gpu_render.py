@@ -53,11 +53,18 @@ def test_sky_over_floor_exact(ctx):
     assert st.rays > 0 and st.bounces >= 2
 
 
-def test_cornell_lambert_paths(ctx):
-    print("[gpu] render cornell paths", flush=True)
+@pytest.mark.parametrize("width,quant", [(4, False), (2, False), (4, True), (8, False)])
+def test_cornell_lambert_paths(ctx, width, quant):
+    print(f"[gpu] render cornell paths (width {width}, quantized {quant})", flush=True)
     s = scenes.cornell_icosphere()
     d = scenes.default_camera(48, 48)
-    f = path_match(ctx, s, d, L.PROFILE_CONFIG, samples=(0, 1, 2))
+    ctx.set_traversal_width(width)
+    ctx.set_quantized(quant)
+    try:
+        f = path_match(ctx, s, d, L.PROFILE_CONFIG, samples=(0, 1, 2))
+    finally:
+        ctx.set_traversal_width(4)
+        ctx.set_quantized(False)
     print(f"  cornell path agreement {f:.5f}")
     assert f >= 0.99
 

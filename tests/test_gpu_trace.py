@@ -52,7 +52,7 @@ def check_small(ctx, s, rays):
     # binary nodes; four-wide FP32 and quantized records, one lane per ray
     # and persistent
     for width, quant, persistent in ((2, False, False), (4, False, False), (4, True, False), (4, True, True),
-                                     (4, False, True)):
+                                     (4, False, True), (8, False, False)):
         ctx.set_traversal_width(width)
         ctx.set_quantized(quant)
         ctx.set_persistent(persistent)
