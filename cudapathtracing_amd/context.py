@@ -82,6 +82,10 @@ class Context:
         """Persistent lane-refilling traversal kernels (default off)."""
         self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))
 
+    def set_queue_sort_min(self, n: int):
+        """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never)."""
+        self._check(self._lib.cpt_set_option(self._h, 5, n))
+
     def set_ray_sort_min(self, n: int):
         """Trace batches of at least n rays in ray-key order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 2, n))

@@ -162,6 +162,10 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
     ctx->raySortMin = value;
     return CPT_OK;
   }
+  if (option == CPT_OPT_QUEUE_SORT_MIN && value >= 0) {
+    ctx->queueSortMin = value;
+    return CPT_OK;
+  }
   if (option == CPT_OPT_QUANTIZED && (value == 0 || value == 1)) {
     ctx->quantized = value != 0;
     return CPT_OK;

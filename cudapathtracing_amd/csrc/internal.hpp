@@ -73,6 +73,7 @@ struct cpt_ctx {
   bool persistent = false;  // four-wide traces as persistent, lane-refilling kernels
   int numCU = 256;
   int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered
+  int64_t queueSortMin = (int64_t)1 << 20;  // path-tracer queues this long are reordered
   bool lastTraceSorted = false;
 
   // traversal I/O staging
@@ -110,6 +111,11 @@ void expandBounds(const double bmin[3], const double bmax[3], double smin[3], do
 // trace.hip
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev);
+
+// Stable reorder of a path queue by ray key (direction octant, origin Morton
+// cell); the 4-pass sort leaves the result in q.
+int sortQueueByRayKey(cpt_ctx *ctx, const float4 *rays, uint32_t *q, uint32_t *qTmp, uint32_t *keysA,
+                      uint32_t *keysB, int64_t n);
 
 // render.hip
 void renderRelease(cpt_ctx *ctx);

@@ -38,7 +38,7 @@ constexpr int kMaxBounces = 4096;
 struct RenderBufs {
   int64_t cap = 0, shadowCap = 0;
   int64_t npix = 0;
-  DevBuf ray, thr, meta, hit, qA, qB, qVol, shadow, counters, fb, aov;
+  DevBuf ray, thr, meta, hit, qA, qB, qVol, keysA, keysB, shadow, counters, fb, aov;
 };
 
 struct CamDev {
@@ -567,8 +567,8 @@ __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ ou
 void renderRelease(cpt_ctx *ctx) {
   if (!ctx->render) return;
   RenderBufs *r = ctx->render;
-  DevBuf *all[] = {&r->ray,    &r->thr,      &r->meta, &r->hit, &r->qA, &r->qB,
-                   &r->qVol,   &r->shadow,   &r->counters, &r->fb,  &r->aov};
+  DevBuf *all[] = {&r->ray,  &r->thr,   &r->meta,   &r->hit,      &r->qA, &r->qB, &r->qVol,
+                   &r->keysA, &r->keysB, &r->shadow, &r->counters, &r->fb, &r->aov};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   delete r;
@@ -596,15 +596,16 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   RenderBufs &R = *ctx->render;
   const int64_t total = npix * prm->spp;
   const int nl = std::max(1, (int)ctx->nLights);
-  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 22);
-  cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 23) / nl));
+  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 26);
+  cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 27) / nl));
   cap = std::min(cap, total);
   const int64_t shadowCap = cap * nl;
   int rc;
   if ((rc = ensure(ctx, R.ray, cap * 32)) || (rc = ensure(ctx, R.thr, cap * 16)) ||
       (rc = ensure(ctx, R.meta, cap * 16)) || (rc = ensure(ctx, R.hit, cap * 8)) ||
       (rc = ensure(ctx, R.qA, cap * 4)) || (rc = ensure(ctx, R.qB, cap * 4)) ||
-      (rc = ensure(ctx, R.qVol, cap * 4)) ||
+      (rc = ensure(ctx, R.qVol, cap * 4)) || (rc = ensure(ctx, R.keysA, cap * 4)) ||
+      (rc = ensure(ctx, R.keysB, cap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
@@ -722,6 +723,11 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       bounces++;
       qn = hc[0];
       std::swap(qa, qb);
+      // incoherent bounce rays: trace the next queue in ray-key order
+      if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin &&
+          (rc = sortQueueByRayKey(ctx, (const float4 *)R.ray.as<float4>(), qa, qb, R.keysA.as<uint32_t>(),
+                                  R.keysB.as<uint32_t>(), qn)))
+        return rc;
     }
   }
   // mean over samples

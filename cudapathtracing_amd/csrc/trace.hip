@@ -27,22 +27,25 @@ __device__ __forceinline__ uint32_t spread9(uint32_t v) {
   return v;
 }
 
-__global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__restrict__ rays, int64_t n,
+// q (optional): key i belongs to ray q[i] (a path queue).
+__global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__restrict__ rays,
+                                                           const uint32_t *__restrict__ q, int64_t n,
                                                            const float *__restrict__ rootBox,
                                                            uint32_t *__restrict__ keys) {
-  const int64_t i = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
-  if (i >= n) return;
+  const int64_t k = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = q ? (int64_t)q[k] : k;
   const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
   const float o[3] = {r0.x, r0.y, r0.z};
-  uint32_t q[3];
+  uint32_t c[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const float lo = rootBox[k], ext = rootBox[3 + k] - lo;
-    const float u = ext > 0.0f ? (o[k] - lo) / ext : 0.0f;
-    q[k] = (uint32_t)fminf(fmaxf(u * 512.0f, 0.0f), 511.0f);  // NaN -> 0
+  for (int a = 0; a < 3; a++) {
+    const float lo = rootBox[a], ext = rootBox[3 + a] - lo;
+    const float u = ext > 0.0f ? (o[a] - lo) / ext : 0.0f;
+    c[a] = (uint32_t)fminf(fmaxf(u * 512.0f, 0.0f), 511.0f);  // NaN -> 0
   }
   const uint32_t oct = (r1.x > 0.0f ? 4u : 0u) | (r1.y > 0.0f ? 2u : 0u) | (r1.z > 0.0f ? 1u : 0u);
-  keys[i] = (oct << 27) | (spread9(q[0]) << 2) | (spread9(q[1]) << 1) | spread9(q[2]);
+  keys[k] = (oct << 27) | (spread9(c[0]) << 2) | (spread9(c[1]) << 1) | spread9(c[2]);
 }
 
 // perm (optional): lane i traces ray perm[i] and writes its outputs at perm[i].
@@ -144,6 +147,18 @@ __global__ void k_zero_work(uint32_t *p) {
   if (threadIdx.x == 0) p[0] = 0;
 }
 
+// Reorders the path queue q[0..n) (rays as 2 float4 per path) by ray key;
+// keysA/keysB hold n keys each, qTmp n indices.
+int sortQueueByRayKey(cpt_ctx *ctx, const float4 *rays, uint32_t *q, uint32_t *qTmp, uint32_t *keysA,
+                      uint32_t *keysB, int64_t n) {
+  if (n <= 1) return CPT_OK;
+  const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
+  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, (const uint32_t *)q, n,
+                     (const float *)ctx->rootBox.as<float>(), keysA);
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  return radixSortPairs(ctx, keysA, q, keysB, qTmp, n, 30, false);
+}
+
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev) {
   if (n <= 0) return CPT_OK;
@@ -162,7 +177,7 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
   const uint32_t *perm = nullptr;
   if (sorted) {
-    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n,
+    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, (const uint32_t *)nullptr, n,
                        (const float *)ctx->rootBox.as<float>(), ctx->rkA.as<uint32_t>());
     CPT_TRY_HIP(ctx, hipGetLastError());
     if ((rc = radixSortPairs(ctx, ctx->rkA.as<uint32_t>(), ctx->rvA.as<uint32_t>(), ctx->rkB.as<uint32_t>(),

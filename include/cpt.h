@@ -146,6 +146,10 @@ const char *cpt_version(void);
  * contains the FP32 box); 0 (default) reads the 128 B FP32 records.  Hits
  * are identical up to exact-t ties. */
 #define CPT_OPT_QUANTIZED 4
+/* CPT_OPT_QUEUE_SORT_MIN: cpt_render traces a bounce queue of at least this
+ * many paths in ray-key order (the same key as CPT_OPT_RAY_SORT_MIN); 0
+ * disables it; default 2^20.  Radiance is unchanged. */
+#define CPT_OPT_QUEUE_SORT_MIN 5
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */

@@ -53,18 +53,22 @@ def test_sky_over_floor_exact(ctx):
     assert st.rays > 0 and st.bounces >= 2
 
 
-@pytest.mark.parametrize("width,quant", [(4, False), (2, False), (4, True), (8, False)])
-def test_cornell_lambert_paths(ctx, width, quant):
-    print(f"[gpu] render cornell paths (width {width}, quantized {quant})", flush=True)
+@pytest.mark.parametrize("width,quant,qsort", [(4, False, 0), (4, False, 1), (2, False, 0), (4, True, 0),
+                                               (8, False, 0)])
+def test_cornell_lambert_paths(ctx, width, quant, qsort):
+    print(f"[gpu] render cornell paths (width {width}, quantized {quant}, queue sort {qsort})", flush=True)
     s = scenes.cornell_icosphere()
     d = scenes.default_camera(48, 48)
     ctx.set_traversal_width(width)
     ctx.set_quantized(quant)
+    if qsort:
+        ctx.set_queue_sort_min(1)
     try:
         f = path_match(ctx, s, d, L.PROFILE_CONFIG, samples=(0, 1, 2))
     finally:
         ctx.set_traversal_width(4)
         ctx.set_quantized(False)
+        ctx.set_queue_sort_min(1 << 20)
     print(f"  cornell path agreement {f:.5f}")
     assert f >= 0.99
 
