@@ -112,10 +112,10 @@ void expandBounds(const double bmin[3], const double bmax[3], double smin[3], do
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev);
 
-// Stable reorder of a path queue by ray key (direction octant, origin Morton
-// cell); the 4-pass sort leaves the result in q.
-int sortQueueByRayKey(cpt_ctx *ctx, const float4 *rays, uint32_t *q, uint32_t *qTmp, uint32_t *keysA,
-                      uint32_t *keysB, int64_t n);
+// Permutation of n rays into ray-key order (direction octant, origin Morton
+// cell), stable; the 4-pass sort leaves it in perm (keysB, permB scratch).
+int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
+                uint32_t *permB);
 
 // render.hip
 void renderRelease(cpt_ctx *ctx);

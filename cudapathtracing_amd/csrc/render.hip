@@ -38,7 +38,10 @@ constexpr int kMaxBounces = 4096;
 struct RenderBufs {
   int64_t cap = 0, shadowCap = 0;
   int64_t npix = 0;
-  DevBuf ray, thr, meta, hit, qA, qB, qVol, keysA, keysB, shadow, counters, fb, aov;
+  // path state, double-buffered and indexed by queue slot: k_shade writes
+  // the survivors compacted into the other set
+  DevBuf ray[2], thr[2], meta[2];
+  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov;
 };
 
 struct CamDev {
@@ -157,7 +160,7 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 
 __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t base, int64_t count,
                                                          float4 *__restrict__ ray, float4 *__restrict__ thr,
-                                                         uint4 *__restrict__ meta, uint32_t *__restrict__ q) {
+                                                         uint4 *__restrict__ meta) {
 #pragma clang fp contract(off)
   const int64_t p = (int64_t)blockIdx.x * kRenderThreads + threadIdx.x;
   if (p >= count) return;
@@ -179,12 +182,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   ray[2 * p + 1] = make_float4((float)d[0], (float)d[1], (float)d[2], kRayTMax);
   thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
-  q[p] = (uint32_t)p;
 }
 
 // kCount: also sums node visits and primitive tests into work[0], work[1]
 template <bool kCount>
-__global__ __launch_bounds__(kRenderThreads) void k_extend(const uint32_t *__restrict__ q, uint32_t count,
+__global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const float4 *__restrict__ ray,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(const uint32_t *__res
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   TraceCounters cnt;
   if (k < count) {
-    const uint32_t p = q[k];
+    const uint32_t p = k;
     const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
     float tb = r1.w;
     const int leaf = traceAny<kCount>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
@@ -253,17 +255,18 @@ __device__ __forceinline__ bool deltaTrack(const MediumDev &md, Rng &rng, f3 o, 
   return false;
 }
 
+// Reads the path state of slot k (< count) from the *In arrays; survivors go
+// compacted to the *Out arrays at the slot given by the next-queue counter.
 __global__ __launch_bounds__(kRenderThreads) void k_shade(
-    RenderArgs a, const uint32_t *__restrict__ q, uint32_t count, float4 *__restrict__ ray,
-    float4 *__restrict__ thr, uint4 *__restrict__ meta, const float2 *__restrict__ hit,
-    uint32_t *__restrict__ qNext, float4 *__restrict__ shadow, int64_t shadowCap,
+    RenderArgs a, uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ thr,
+    const uint4 *__restrict__ meta, const float2 *__restrict__ hit, float4 *__restrict__ rayOut,
+    float4 *__restrict__ thrOut, uint4 *__restrict__ metaOut, float4 *__restrict__ shadow, int64_t shadowCap,
     uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov,
     uint32_t *__restrict__ qVol) {
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   const bool active = k < count;
   bool alive = false;
-  uint32_t p = 0;
-  if (active) p = q[k];
+  const uint32_t p = k;
   float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, t4 = o4;
   uint4 mt = make_uint4(0, 0, 0, 0);
   int leaf = -1;
@@ -449,11 +452,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   }
   const uint32_t slot = waveAppend(counters + 0, alive);
   if (alive) {
-    ray[2 * (int64_t)p] = make_float4(o.x, o.y, o.z, kRayTMin);
-    ray[2 * (int64_t)p + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
-    thr[p] = make_float4(T.x, T.y, T.z, 0.f);
-    meta[p] = make_uint4(mt.x, mt.y, rng.cnt, flags);
-    qNext[slot] = p;
+    rayOut[2 * (int64_t)slot] = make_float4(o.x, o.y, o.z, kRayTMin);
+    rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
+    thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
+    metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, flags);
   }
 }
 
@@ -463,10 +465,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
 // from the last position with the last direction.  The queue length is
 // counters[2]; the grid covers an upper bound.
 __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const uint32_t *__restrict__ qVol,
-                                                           uint32_t bound, float4 *__restrict__ ray,
-                                                           float4 *__restrict__ thr, uint4 *__restrict__ meta,
+                                                           uint32_t bound, const float4 *__restrict__ ray,
+                                                           const float4 *__restrict__ thr,
+                                                           const uint4 *__restrict__ meta,
                                                            const float2 *__restrict__ hit,
-                                                           uint32_t *__restrict__ qNext,
+                                                           float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
+                                                           uint4 *__restrict__ metaOut,
                                                            uint32_t *__restrict__ counters) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
@@ -511,11 +515,26 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
     }
   }
   const uint32_t slot = waveAppend(counters + 0, true);
-  ray[2 * (int64_t)p] = make_float4(o.x, o.y, o.z, kRayTMin);
-  ray[2 * (int64_t)p + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
-  thr[p] = make_float4(T.x, T.y, T.z, 0.f);
-  meta[p] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
-  qNext[slot] = p;
+  rayOut[2 * (int64_t)slot] = make_float4(o.x, o.y, o.z, kRayTMin);
+  rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
+  thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
+  metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
+}
+
+// Moves the path state of slot perm[j] to slot j (ray-key order).
+__global__ __launch_bounds__(kRenderThreads) void k_permute(uint32_t n, const uint32_t *__restrict__ perm,
+                                                            const float4 *__restrict__ ray,
+                                                            const float4 *__restrict__ thr,
+                                                            const uint4 *__restrict__ meta,
+                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
+                                                            uint4 *__restrict__ metaOut) {
+  const uint32_t j = blockIdx.x * kRenderThreads + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = perm[j];
+  rayOut[2 * (int64_t)j] = ray[2 * (int64_t)p];
+  rayOut[2 * (int64_t)j + 1] = ray[2 * (int64_t)p + 1];
+  thrOut[j] = thr[p];
+  metaOut[j] = meta[p];
 }
 
 // Shadow rays: the light's contribution counts when the light itself is the
@@ -567,8 +586,9 @@ __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ ou
 void renderRelease(cpt_ctx *ctx) {
   if (!ctx->render) return;
   RenderBufs *r = ctx->render;
-  DevBuf *all[] = {&r->ray,  &r->thr,   &r->meta,   &r->hit,      &r->qA, &r->qB, &r->qVol,
-                   &r->keysA, &r->keysB, &r->shadow, &r->counters, &r->fb, &r->aov};
+  DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
+                   &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,
+                   &r->shadow, &r->counters, &r->fb,    &r->aov};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   delete r;
@@ -601,11 +621,13 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   cap = std::min(cap, total);
   const int64_t shadowCap = cap * nl;
   int rc;
-  if ((rc = ensure(ctx, R.ray, cap * 32)) || (rc = ensure(ctx, R.thr, cap * 16)) ||
-      (rc = ensure(ctx, R.meta, cap * 16)) || (rc = ensure(ctx, R.hit, cap * 8)) ||
-      (rc = ensure(ctx, R.qA, cap * 4)) || (rc = ensure(ctx, R.qB, cap * 4)) ||
-      (rc = ensure(ctx, R.qVol, cap * 4)) || (rc = ensure(ctx, R.keysA, cap * 4)) ||
-      (rc = ensure(ctx, R.keysB, cap * 4)) ||
+  for (int b = 0; b < 2; b++)
+    if ((rc = ensure(ctx, R.ray[b], cap * 32)) || (rc = ensure(ctx, R.thr[b], cap * 16)) ||
+        (rc = ensure(ctx, R.meta[b], cap * 16)))
+      return rc;
+  if ((rc = ensure(ctx, R.hit, cap * 8)) || (rc = ensure(ctx, R.permA, cap * 4)) ||
+      (rc = ensure(ctx, R.permB, cap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
+      (rc = ensure(ctx, R.keysA, cap * 4)) || (rc = ensure(ctx, R.keysB, cap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
@@ -674,34 +696,36 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   CPT_TRY_HIP(ctx, hipEventRecord(ev[12], s));
   for (int64_t base = 0; base < total; base += cap) {
     const int64_t count = std::min(cap, total - base);
+    int cur = 0;  // state set holding the current queue
     hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
-                       R.ray.as<float4>(), R.thr.as<float4>(), R.meta.as<uint4>(), R.qA.as<uint32_t>());
+                       R.ray[cur].as<float4>(), R.thr[cur].as<float4>(), R.meta[cur].as<uint4>());
     CPT_TRY_HIP(ctx, hipGetLastError());
     uint32_t qn = (uint32_t)count;
-    uint32_t *qa = R.qA.as<uint32_t>(), *qb = R.qB.as<uint32_t>();
     // paths end by roulette, depth cap, miss or absorption; the cap only
     // bounds pathological inputs (NaN geometry) so the loop always ends
     for (int pass = 0; qn > 0 && pass < kMaxBounces; pass++) {
+      const int nxt = cur ^ 1;
+      const float4 *rayC = R.ray[cur].as<float4>(), *thrC = R.thr[cur].as<float4>();
+      const uint4 *metaC = R.meta[cur].as<uint4>();
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
       if (countWork)
-        hipLaunchKernelGGL(k_extend<true>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
-                           (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>(),
-                           work, a.wide);
+        hipLaunchKernelGGL(k_extend<true>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes,
+                           a.leaves, a.nPrims, R.hit.as<float2>(), work, a.wide);
       else
-        hipLaunchKernelGGL(k_extend<false>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qa, qn,
-                           (const float4 *)R.ray.as<float4>(), a.nodes, a.leaves, a.nPrims, R.hit.as<float2>(),
-                           work, a.wide);
+        hipLaunchKernelGGL(k_extend<false>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes,
+                           a.leaves, a.nPrims, R.hit.as<float2>(), work, a.wide);
       extRays += qn;
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
-      hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qa, qn, R.ray.as<float4>(),
-                         R.thr.as<float4>(), R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb,
-                         R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(), R.aov.as<float>(),
-                         R.qVol.as<uint32_t>());
+      hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC,
+                         (const float2 *)R.hit.as<float2>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
+                         R.meta[nxt].as<uint4>(), R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(),
+                         R.aov.as<float>(), R.qVol.as<uint32_t>());
       if (ctx->hasVolumeMat)
         hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
-                           (const uint32_t *)R.qVol.as<uint32_t>(), qn, R.ray.as<float4>(), R.thr.as<float4>(),
-                           R.meta.as<uint4>(), (const float2 *)R.hit.as<float2>(), qb, cnt);
+                           (const uint32_t *)R.qVol.as<uint32_t>(), qn, rayC, thrC, metaC,
+                           (const float2 *)R.hit.as<float2>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
+                           R.meta[nxt].as<uint4>(), cnt);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
       hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
@@ -722,12 +746,20 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       rays += (double)qn + (double)std::min<int64_t>(hc[1], shadowCap);
       bounces++;
       qn = hc[0];
-      std::swap(qa, qb);
-      // incoherent bounce rays: trace the next queue in ray-key order
-      if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin &&
-          (rc = sortQueueByRayKey(ctx, (const float4 *)R.ray.as<float4>(), qa, qb, R.keysA.as<uint32_t>(),
-                                  R.keysB.as<uint32_t>(), qn)))
-        return rc;
+      cur = nxt;
+      // incoherent bounce rays: move the next queue's state into ray-key order
+      if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin) {
+        if ((rc = rayKeyOrder(ctx, (const float4 *)R.ray[cur].as<float4>(), qn, R.keysA.as<uint32_t>(),
+                              R.keysB.as<uint32_t>(), R.permA.as<uint32_t>(), R.permB.as<uint32_t>())))
+          return rc;
+        const int o2 = cur ^ 1;
+        hipLaunchKernelGGL(k_permute, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn,
+                           (const uint32_t *)R.permA.as<uint32_t>(), (const float4 *)R.ray[cur].as<float4>(),
+                           (const float4 *)R.thr[cur].as<float4>(), (const uint4 *)R.meta[cur].as<uint4>(),
+                           R.ray[o2].as<float4>(), R.thr[o2].as<float4>(), R.meta[o2].as<uint4>());
+        CPT_TRY_HIP(ctx, hipGetLastError());
+        cur = o2;
+      }
     }
   }
   // mean over samples

@@ -147,16 +147,16 @@ __global__ void k_zero_work(uint32_t *p) {
   if (threadIdx.x == 0) p[0] = 0;
 }
 
-// Reorders the path queue q[0..n) (rays as 2 float4 per path) by ray key;
-// keysA/keysB hold n keys each, qTmp n indices.
-int sortQueueByRayKey(cpt_ctx *ctx, const float4 *rays, uint32_t *q, uint32_t *qTmp, uint32_t *keysA,
-                      uint32_t *keysB, int64_t n) {
-  if (n <= 1) return CPT_OK;
+// Permutation perm[0..n) of rays[0..n) (2 float4 per ray) into ray-key order
+// (stable); keysB and permB are scratch.
+int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
+                uint32_t *permB) {
+  if (n <= 0) return CPT_OK;
   const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
-  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, (const uint32_t *)q, n,
+  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, (const uint32_t *)nullptr, n,
                      (const float *)ctx->rootBox.as<float>(), keysA);
   CPT_TRY_HIP(ctx, hipGetLastError());
-  return radixSortPairs(ctx, keysA, q, keysB, qTmp, n, 30, false);
+  return radixSortPairs(ctx, keysA, perm, keysB, permB, n, 30, true);
 }
 
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
