@@ -165,6 +165,87 @@ def bench_equal_quality(cpt, L, scenes, args):
             "plain_points": [(p[0], round(p[1], 6), round(p[2], 2)) for p in eq["points"]]}
 
 
+def bench_config5(cpt, L, scenes, args):
+    """Denoised inference vs plain tracer at equal relMSE on the displaced
+    icosphere plus a grid^3 fBm medium (sigma_max 8), many views at once.
+
+    Denoised: all views rendered at 4 spp with AOVs in one cpt_render_views
+    call, then the U-Net over the views (device buffers end to end; time =
+    wall time of both, synchronised).  Error: relMSE of the first
+    `c5_ref_views` views against their `c5_ref_spp` renders.  Plain: spp
+    doubled from 4 on those views until relMSE falls below the denoised
+    one (log-log interpolation); its time for all views = the measured time
+    of an all-view render at 16 spp scaled linearly to that spp."""
+    if not os.path.exists(WEIGHTS):
+        return None
+    import torch
+    from cudapathtracing_amd.denoiser import Denoiser
+    from cudapathtracing_amd.training import equal_quality, orbit_cameras, relmse
+    w, h, nv = args.c5_width, args.c5_height, args.c5_views
+    s = scenes.displaced_icosphere_medium(w, h, grid=args.c5_grid)
+    cams = orbit_cameras(nv, w, h, seed=7)
+    kw = dict(profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
+    chunk = min(nv, 8)
+    with cpt.Context(0) as ctx:
+        ctx.upload(s)
+        ctx.build(s.bounds_min, s.bounds_max)
+        t0 = time.time()
+        refs = [ctx.render(cams[v], args.c5_ref_spp, seed=99991 + v, **kw)[0] for v in range(args.c5_ref_views)]
+        log(f"config-5 references ({args.c5_ref_views} views, {args.c5_ref_spp} spp): {time.time() - t0:.1f}s")
+        dn = Denoiser(ctx, chunk, h, w)
+        dn.set_params(np.fromfile(WEIGHTS, dtype="<f4"))
+        rgb = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
+        aov = torch.empty((nv, h, w, 7), dtype=torch.float32, device="cuda")
+        den = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
+
+        def denoised_pipeline(seed):
+            ctx.render_views(cams, 4, seed=seed, aov=True, out=rgb, aov_out=aov, **kw)
+            x = torch.cat([rgb, aov], dim=-1).contiguous()
+            for c0 in range(0, nv, chunk):
+                dn(x[c0:c0 + chunk], out=den[c0:c0 + chunk])
+
+        denoised_pipeline(5)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        denoised_pipeline(7)
+        torch.cuda.synchronize()
+        t_den = (time.perf_counter() - t0) * 1e3
+        d_np = den[:args.c5_ref_views].cpu().numpy()
+        e_den = float(np.mean([relmse(d_np[v], refs[v]) for v in range(args.c5_ref_views)]))
+        e_noisy = float(np.mean([relmse(rgb[v].cpu().numpy(), refs[v]) for v in range(args.c5_ref_views)]))
+        eq = equal_quality(ctx, cams[0], refs[0], e_den, seed=11, spp_start=4, spp_max=args.c5_ref_spp // 2)
+        plain16 = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
+        ctx.render_views(cams, 16, seed=13, out=plain16, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.render_views(cams, 16, seed=17, out=plain16, **kw)
+        torch.cuda.synchronize()
+        t16 = (time.perf_counter() - t0) * 1e3
+        t_plain = t16 * eq["spp"] / 16.0
+        dn.close()
+    npix = nv * w * h
+    fit = {}
+    if not eq["reached"]:
+        # measured plain spp is only a lower bound; the a/n + b fit of relMSE
+        # (b = noise floor of the finite-spp reference) gives an estimate
+        fit = {"plain_fit_a": eq.get("fit_a"), "plain_fit_b": eq.get("fit_b")}
+        if "spp_fit" in eq:
+            t_fit = t16 * eq["spp_fit"] / 16.0
+            fit.update({"plain_spp_fit": round(eq["spp_fit"], 1), "plain_ms_fit": round(t_fit, 1),
+                        "mpix_per_s_plain_fit": round(nv * w * h / (t_fit * 1e-3) / 1e6, 4),
+                        "speedup_fit": round(t_fit / t_den, 1)})
+    return {**fit, "views": nv, "width": w, "height": h, "medium_grid": args.c5_grid, "medium_sigma_max": 8.0,
+            "reference_spp": args.c5_ref_spp, "reference_views": args.c5_ref_views, "denoised_spp": 4,
+            "relmse_noisy_4spp": e_noisy, "relmse_denoised_4spp": e_den, "denoised_ms": round(t_den, 2),
+            "plain_spp_at_equal_relmse": round(eq["spp"], 1), "plain_ms_at_equal_relmse": round(t_plain, 2),
+            "plain_ms_16spp_all_views": round(t16, 2), "reached": eq["reached"],
+            "mpix_per_s_denoised": round(npix / (t_den * 1e-3) / 1e6, 3),
+            "mpix_per_s_plain": round(npix / (t_plain * 1e-3) / 1e6, 3),
+            "speedup": round(t_plain / t_den, 2),
+            "speedup_is_lower_bound": not eq["reached"],
+            "plain_points": [(p[0], round(p[1], 6), round(p[2], 2)) for p in eq["points"]]}
+
+
 def bench_lbvh(cpt, scenes, args):
     import torch
     scene = scenes.triangle_soup_scene(args.tris, seed=1)
@@ -279,7 +360,14 @@ def main():
     ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="traversal width")
     ap.add_argument("--records", choices=["quant", "fp32"], default="fp32",
                     help="four-wide node records read by the traversal")
-    ap.add_argument("--only", choices=["render", "lbvh", "eq"], default=None,
+    ap.add_argument("--no-c5", action="store_true", help="skip the many-view medium equal-relMSE leg")
+    ap.add_argument("--c5-views", type=int, default=32)
+    ap.add_argument("--c5-width", type=int, default=1920)
+    ap.add_argument("--c5-height", type=int, default=1080)
+    ap.add_argument("--c5-grid", type=int, default=256)
+    ap.add_argument("--c5-ref-spp", type=int, default=4096)
+    ap.add_argument("--c5-ref-views", type=int, default=1)
+    ap.add_argument("--only", choices=["render", "lbvh", "eq", "c5"], default=None,
                     help="run one workload and print its record (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="copy this many bytes device-to-device 5 times and exit (counter calibration)")
@@ -300,12 +388,16 @@ def main():
     if args.only == "eq":
         print(json.dumps({"equal_relmse": bench_equal_quality(cpt, L, scenes, args)}), flush=True)
         return
+    if args.only == "c5":
+        print(json.dumps({"config5": bench_config5(cpt, L, scenes, args)}), flush=True)
+        return
     if args.only == "render":
-        args.no_lbvh = args.no_eq = True
+        args.no_lbvh = args.no_eq = args.no_c5 = True
 
     r, cpu, ntri = bench_render(cpt, L, scenes, args)
     lb = None if args.no_lbvh else bench_lbvh(cpt, scenes, args)
     eq = None if args.no_eq else bench_equal_quality(cpt, L, scenes, args)
+    c5 = None if args.no_c5 else bench_config5(cpt, L, scenes, args)
     out = {
         "metric": METRIC,
         "value": round(r["mrays"], 1),
@@ -338,6 +430,7 @@ def main():
         "cpu_baseline": cpu,
         "lbvh": lb,
         "equal_relmse": eq,
+        "config5": c5,
     }
     print(json.dumps(out), flush=True)
 

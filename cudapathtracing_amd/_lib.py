@@ -105,6 +105,8 @@ SIGNATURES = [
     ("cpt_last_trace_stats", C.c_int, [P, P, P, P]),
     ("cpt_render", C.c_int, [P, C.POINTER(Camera), C.POINTER(RenderParams), P, P,
                              C.POINTER(RenderStats), C.c_int]),
+    ("cpt_render_views", C.c_int, [P, C.POINTER(Camera), C.c_int32, C.POINTER(RenderParams), P, P,
+                                   C.POINTER(RenderStats), C.c_int]),
     ("cpt_camera_from_desc", C.c_int, [C.POINTER(CameraDesc), C.POINTER(Camera)]),
     ("cpt_tonemap_rgba8", C.c_int, [P, C.c_int32, C.c_int32, P]),
     ("cpt_bilateral", C.c_int, [P, P, P, P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "bsdf.hpp"
 #include "cpt_device.hpp"
@@ -41,7 +42,7 @@ struct RenderBufs {
   // path state, double-buffered and indexed by queue slot: k_shade writes
   // the survivors compacted into the other set
   DevBuf ray[2], thr[2], meta[2];
-  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov;
+  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov, cams;
 };
 
 struct CamDev {
@@ -116,7 +117,8 @@ struct RenderArgs {
   int wide;  // 0 binary, 1 four-wide FP32, 2 four-wide quantized, 3 eight-wide quantized
   const MatDev *mats;
   int nPrims, nLights;
-  CamDev cam;
+  const CamDev *cams;  // nviews cameras (device), all of the same size
+  int nviews;
   uint64_t seed;
   int spp, sampleBase, profile, maxDepth, rrDepth, wantAov;
   float rrProb;
@@ -165,20 +167,25 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   const int64_t p = (int64_t)blockIdx.x * kRenderThreads + threadIdx.x;
   if (p >= count) return;
   const int64_t gi = base + p;
-  const uint32_t pixel = (uint32_t)(gi % a.npix);
-  const uint32_t sample = (uint32_t)(a.sampleBase + gi / a.npix);
+  // path gi: sample-major over all views' pixels; the stream of a pixel is
+  // keyed by its index over all views (view * npix + pixel)
+  const int64_t npixAll = a.npix * a.nviews;
+  const uint32_t pixel = (uint32_t)(gi % npixAll);
+  const uint32_t sample = (uint32_t)(a.sampleBase + gi / npixAll);
+  const CamDev &cam = a.cams[pixel / (uint32_t)a.npix];
+  const uint32_t pv = pixel % (uint32_t)a.npix;
   Rng rng;
   rng.init(a.seed, pixel, sample, 0);
   const double rx = rng.next(), ry = rng.next();
-  const int i = (int)(pixel % (uint32_t)a.cam.w), j = (int)(pixel / (uint32_t)a.cam.w);
+  const int i = (int)(pv % (uint32_t)cam.w), j = (int)(pv / (uint32_t)cam.w);
   const double fx = rx - 0.5 + i, fy = ry - 0.5 + j;
   double d[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const double pix = (a.cam.pc[k] + fx * a.cam.dh[k]) + fy * a.cam.dv[k];
-    d[k] = pix - a.cam.lf[k];
+    const double pix = (cam.pc[k] + fx * cam.dh[k]) + fy * cam.dv[k];
+    d[k] = pix - cam.lf[k];
   }
-  ray[2 * p] = make_float4((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2], kRayTMin);
+  ray[2 * p] = make_float4((float)cam.lf[0], (float)cam.lf[1], (float)cam.lf[2], kRayTMin);
   ray[2 * p + 1] = make_float4((float)d[0], (float)d[1], (float)d[2], kRayTMax);
   thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
@@ -306,7 +313,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     if (inMedium) {
     } else if (leaf < 0) {
       // miss: sky colour (Render.cu:138-142)
-      const f3 bg = mkf3(a.cam.bg[0], a.cam.bg[1], a.cam.bg[2]);
+      const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
+      const f3 bg = mkf3(cv.bg[0], cv.bg[1], cv.bg[2]);
       atomicAdd(pix + 0, T.x * bg.x);
       atomicAdd(pix + 1, T.y * bg.y);
       atomicAdd(pix + 2, T.z * bg.z);
@@ -359,7 +367,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       atomicAdd(pa + 3, n.x);
       atomicAdd(pa + 4, n.y);
       atomicAdd(pa + 5, n.z);
-      atomicAdd(pa + 6, len3(mkf3((float)a.cam.lf[0], (float)a.cam.lf[1], (float)a.cam.lf[2]) - hp));
+      const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
+      atomicAdd(pa + 6, len3(mkf3((float)cv.lf[0], (float)cv.lf[1], (float)cv.lf[2]) - hp));
     }
     flags |= kFlagAov;
   }
@@ -588,7 +597,7 @@ void renderRelease(cpt_ctx *ctx) {
   RenderBufs *r = ctx->render;
   DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
                    &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,
-                   &r->shadow, &r->counters, &r->fb,    &r->aov};
+                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   delete r;
@@ -601,16 +610,21 @@ static unsigned blocksFor(int64_t n) { return (unsigned)((n + kRenderThreads - 1
 
 using namespace cpt;
 
-extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_params *prm, float *rgb,
-                          float *aovOut, cpt_render_stats *stats, int flags) {
-  if (!ctx || !cam || !prm || !rgb) return CPT_ERR_ARG;
+extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nviews, const cpt_render_params *prm,
+                                float *rgb, float *aovOut, cpt_render_stats *stats, int flags) {
+  if (!ctx || !cams || !prm || !rgb || nviews <= 0) return CPT_ERR_ARG;
   if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_render", "no BVH built");
+  const cpt_camera *cam = cams;
   if (cam->width <= 0 || cam->height <= 0 || prm->spp <= 0 || prm->sample_base < 0)
     return fail(ctx, CPT_ERR_ARG, "cpt_render", "bad image size or spp");
+  for (int v = 1; v < nviews; v++)
+    if (cams[v].width != cam->width || cams[v].height != cam->height)
+      return fail(ctx, CPT_ERR_ARG, "cpt_render_views", "views differ in size");
   if (prm->profile == CPT_PROFILE_CONFIG && (prm->max_depth <= 0 || !(prm->rr_prob > 0.0)))
     return fail(ctx, CPT_ERR_ARG, "cpt_render", "max_depth and rr_prob must be positive");
-  const int64_t npix = (int64_t)cam->width * cam->height;
-  if (npix > (int64_t)1 << 30) return fail(ctx, CPT_ERR_ARG, "cpt_render", "image too large");
+  const int64_t npixView = (int64_t)cam->width * cam->height;
+  const int64_t npix = npixView * nviews;  // pixels over all views
+  if (npix > (int64_t)1 << 31) return fail(ctx, CPT_ERR_ARG, "cpt_render", "image too large");
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   if (!ctx->render) ctx->render = new RenderBufs();
   RenderBufs &R = *ctx->render;
@@ -629,6 +643,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
       (rc = ensure(ctx, R.permB, cap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.keysA, cap * 4)) || (rc = ensure(ctx, R.keysB, cap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
+      (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
   R.cap = cap;
@@ -649,15 +664,24 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   a.mats = ctx->mats.as<MatDev>();
   a.nPrims = (int)ctx->nPrims;
   a.nLights = ctx->nLights;
-  for (int k = 0; k < 3; k++) {
-    a.cam.lf[k] = cam->look_from[k];
-    a.cam.pc[k] = cam->pixel_center[k];
-    a.cam.dh[k] = cam->delta_h[k];
-    a.cam.dv[k] = cam->delta_v[k];
-    a.cam.bg[k] = (float)cam->background[k];
+  {
+    std::vector<CamDev> cd(nviews);
+    for (int v = 0; v < nviews; v++) {
+      for (int k = 0; k < 3; k++) {
+        cd[v].lf[k] = cams[v].look_from[k];
+        cd[v].pc[k] = cams[v].pixel_center[k];
+        cd[v].dh[k] = cams[v].delta_h[k];
+        cd[v].dv[k] = cams[v].delta_v[k];
+        cd[v].bg[k] = (float)cams[v].background[k];
+      }
+      cd[v].w = cams[v].width;
+      cd[v].h = cams[v].height;
+    }
+    CPT_TRY_HIP(ctx, hipMemcpyAsync(R.cams.p, cd.data(), nviews * sizeof(CamDev), hipMemcpyHostToDevice, s));
+    CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
   }
-  a.cam.w = cam->width;
-  a.cam.h = cam->height;
+  a.cams = R.cams.as<CamDev>();
+  a.nviews = nviews;
   a.seed = prm->seed;
   a.spp = prm->spp;
   a.sampleBase = prm->sample_base;
@@ -666,7 +690,7 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
   a.rrDepth = prm->rr_depth;
   a.rrProb = (float)prm->rr_prob;
   a.wantAov = aovOut ? 1 : 0;
-  a.npix = npix;
+  a.npix = npixView;
   a.med.on = ctx->hasMedium ? 1 : 0;
   a.med.density = ctx->medDensity.as<float>();
   a.med.bmin = mkf3(ctx->medBox[0], ctx->medBox[1], ctx->medBox[2]);
@@ -801,4 +825,9 @@ extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_
     stats->extend_prims = (double)wk[1];
   }
   return CPT_OK;
+}
+
+extern "C" int cpt_render(cpt_ctx *ctx, const cpt_camera *cam, const cpt_render_params *prm, float *rgb,
+                          float *aovOut, cpt_render_stats *stats, int flags) {
+  return cpt_render_views(ctx, cam, 1, prm, rgb, aovOut, stats, flags);
 }

@@ -2,8 +2,10 @@
 
 * :func:`random_scene` draws a Cornell-box scene like the 1291-primitive
   workload: displaced icosphere (level 3-5, amplitude U[0, 0.15]), Lambert or
-  GGX material with random roughness/F0, random wall albedos and a random
-  camera pose - all from one integer seed.
+  GGX material with random roughness/F0, random wall albedos, a random
+  camera pose and, with probability 1/2, a 64^3 fBm medium (sigma_max
+  U[1, 8], grey albedo U[0.6, 0.95]) - all from one integer seed.
+* :func:`orbit_cameras` draws seeded views of the same kind.
 * :func:`render_pair` renders the network input (low spp + albedo, normal,
   depth) and the target (high spp) of one scene with the path tracer.
 * :func:`relmse` is mean((x - r)^2 / (r^2 + 1e-2)) over pixels and channels.
@@ -32,7 +34,7 @@ def relmse(x: np.ndarray, ref: np.ndarray) -> float:
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
 
 
-def random_scene(seed: int, width: int = 512, height: int = 512) -> scenes.SceneData:
+def random_scene(seed: int, width: int = 512, height: int = 512, medium_prob: float = 0.5) -> scenes.SceneData:
     rng = np.random.default_rng(1_000_003 * seed + 17)
     s = scenes.SceneData()
     scenes.cornell_box(s, rng)
@@ -56,7 +58,28 @@ def random_scene(seed: int, width: int = 512, height: int = 512) -> scenes.Scene
     at = tuple(rng.uniform(-0.08, 0.08, 3).tolist())
     fov = float(rng.uniform(34.0, 46.0))
     s.camera = L.CameraDesc(L.d3((0, 0, 0)), L.d3(frm), L.d3(at), fov, width / height, width, 0)
+    # heterogeneous fBm medium filling the box in some scenes (the
+    # config-5 workload has one)
+    if rng.random() < medium_prob:
+        a = float(rng.uniform(0.6, 0.95))
+        s.medium = scenes.Medium(scenes.fbm_grid(64, int(rng.integers(1, 1 << 30))), (-0.5, -0.5, -0.5),
+                                 (0.5, 0.5, 0.5), float(rng.uniform(1.0, 8.0)), (a, a, a))
     return s
+
+
+def orbit_cameras(n: int, width: int, height: int, seed: int = 7) -> List[L.CameraDesc]:
+    """n seeded views of the Cornell-box scenes from in front of the open face."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        th = rng.uniform(-0.35, 0.35)
+        ph = rng.uniform(-0.25, 0.25)
+        dist = rng.uniform(1.6, 2.2)
+        frm = (dist * math.sin(th) * math.cos(ph), dist * math.sin(ph), dist * math.cos(th) * math.cos(ph))
+        at = tuple(rng.uniform(-0.08, 0.08, 3).tolist())
+        fov = float(rng.uniform(34.0, 46.0))
+        out.append(L.CameraDesc(L.d3((0, 0, 0)), L.d3(frm), L.d3(at), fov, width / height, width, 0))
+    return out
 
 
 @dataclass
@@ -134,7 +157,19 @@ def equal_quality(ctx: Context, cam: L.CameraDesc, ref: np.ndarray, target_relms
         spp *= 2
     if len(pts) == 1 or pts[-1][1] > target_relmse:
         s, e, ms = pts[-1]
-        return dict(spp=float(s), ms=float(ms), points=pts, reached=pts[-1][1] <= target_relmse)
+        res = dict(spp=float(s), ms=float(ms), points=pts, reached=pts[-1][1] <= target_relmse)
+        # not reached: relMSE against a finite-spp reference levels off at
+        # the reference's own noise, relMSE(n) ~ a / n + b; fit the last four
+        # points and extrapolate where the target lies above the floor b
+        if len(pts) >= 4:
+            n = np.array([p[0] for p in pts[-4:]], np.float64)
+            e = np.array([p[1] for p in pts[-4:]], np.float64)
+            A = np.stack([1.0 / n, np.ones_like(n)], 1)
+            (a, b), *_ = np.linalg.lstsq(A, e, rcond=None)
+            res["fit_a"], res["fit_b"] = float(a), float(b)
+            if a > 0 and target_relmse > b:
+                res["spp_fit"] = float(a / (target_relmse - b))
+        return res
     (s0, e0, m0), (s1, e1, m1) = pts[-2], pts[-1]
     f = (math.log(target_relmse) - math.log(e0)) / (math.log(e1) - math.log(e0))
     spp_eq = math.exp(math.log(s0) + f * (math.log(s1) - math.log(s0)))

@@ -295,6 +295,15 @@ int cpt_render(cpt_ctx *ctx, const cpt_camera *camera,
                const cpt_render_params *params, float *rgb, float *aov,
                cpt_render_stats *stats, int flags);
 
+/* Several views of one scene in one wavefront pipeline (no reference
+ * counterpart; the config-5 workload renders 32 views at once).  All
+ * cameras have one size; rgb: nviews*h*w*3, aov (may be NULL):
+ * nviews*h*w*7.  The random stream of a pixel is keyed by its index over all
+ * views (view*h*w + pixel), so view 0 equals cpt_render. */
+int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cameras, int32_t nviews,
+                     const cpt_render_params *params, float *rgb, float *aov,
+                     cpt_render_stats *stats, int flags);
+
 /* Camera::Camera (Camera.cu:5-31) in FP64 on the host. */
 int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
 

@@ -64,6 +64,7 @@ def _register_render(L):
 RENDER_SIGS = {
     "orc_philox4x32_10": (None, [P, C.c_uint32, C.c_uint32, P]),
     "orc_render": (C.c_int, [P, P, P, P, P, C.c_int]),
+    "orc_render_view": (C.c_int, [P, P, P, P, P, C.c_int, C.c_int64]),
     "orc_scene_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_double, P]),
     "orc_medium_transmittance": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
     "orc_medium_collide": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
@@ -239,13 +240,14 @@ class OracleScene:
                               cert.ctypes.data)
         return t, ids, cert.astype(bool)
 
-    def render(self, cam, params, per_sample=False, nthreads=0):
-        """Mean radiance per pixel (h, w, 3) float64; per-sample radiance too."""
+    def render(self, cam, params, per_sample=False, nthreads=0, pixel_offset=0):
+        """Mean radiance per pixel (h, w, 3) float64; per-sample radiance too.
+        pixel_offset: the view's first pixel index in a multi-view stream."""
         h, w = cam.height, cam.width
         rgb = np.empty((h, w, 3), np.float64)
         ps = np.empty((h * w, params.spp, 3), np.float64) if per_sample else None
-        rc = lib().orc_render(self._h, C.byref(cam), C.byref(params), rgb.ctypes.data,
-                              ps.ctypes.data if per_sample else None, nthreads)
+        rc = lib().orc_render_view(self._h, C.byref(cam), C.byref(params), rgb.ctypes.data,
+                                   ps.ctypes.data if per_sample else None, nthreads, pixel_offset)
         if rc:
             raise RuntimeError("orc_render: build the oracle BVH first")
         return (rgb, ps) if per_sample else rgb

@@ -410,8 +410,11 @@ static double ratioTrack(const Medium &m, Stream rng, const d3 &o, const d3 &d, 
 }
 
 // One sample of the bounce loop, Render.cu:42-219.  Returns the radiance.
-d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, uint32_t sample) {
-  Stream rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)(j * cam.w + i), sample, 0};
+// streamPixel: the pixel's index in the random stream (view * h * w + pixel
+// for a multi-view render).
+d3 renderSample(const Scene &s, const Cam &cam, const Params &P, int i, int j, uint32_t sample,
+                uint32_t streamPixel) {
+  Stream rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), streamPixel, sample, 0};
   double rx = rng.next(), ry = rng.next();
   d3 pixel = cam.pc + (rx - 0.5 + i) * cam.dh + (ry - 0.5 + j) * cam.dv;
   d3 ro = cam.lf, rd = pixel - cam.lf;
@@ -634,8 +637,18 @@ void orc_philox4x32_10(const uint32_t ctr[4], uint32_t k0, uint32_t k1, uint32_t
 // Renders the pixels [px0, px1) of the camera image (all when px1 <= px0):
 // rgb gets the mean radiance per pixel (h*w*3 doubles, row 0 = bottom);
 // per_sample (optional) gets each sample's radiance (npix*spp*3).
+int orc_render_view(void *sp, const cpt_camera *cam, const cpt_render_params *prm, double *rgb,
+                    double *perSample, int nthreads, int64_t pixelOffset);
+
 int orc_render(void *sp, const cpt_camera *cam, const cpt_render_params *prm, double *rgb, double *perSample,
                int nthreads) {
+  return orc_render_view(sp, cam, prm, rgb, perSample, nthreads, 0);
+}
+
+// One view of a multi-view render: pixel (i, j) uses stream pixel
+// pixelOffset + j * w + i.
+int orc_render_view(void *sp, const cpt_camera *cam, const cpt_render_params *prm, double *rgb,
+                    double *perSample, int nthreads, int64_t pixelOffset) {
   Scene *s = (Scene *)sp;
   if (!s->built) return -1;
   Cam c;
@@ -654,7 +667,7 @@ int orc_render(void *sp, const cpt_camera *cam, const cpt_render_params *prm, do
     int i = (int)(px % c.w), j = (int)(px / c.w);
     d3 acc = mk(0, 0, 0);
     for (int k = 0; k < P.spp; k++) {
-      d3 r = renderSample(*s, c, P, i, j, (uint32_t)(P.sampleBase + k));
+      d3 r = renderSample(*s, c, P, i, j, (uint32_t)(P.sampleBase + k), (uint32_t)(pixelOffset + px));
       if (perSample) {
         double *o = perSample + 3 * (px * P.spp + k);
         o[0] = r.x; o[1] = r.y; o[2] = r.z;
