@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcpt.so")
+# CPT_LIB_PATH selects another build of the library (e.g. a debug build)
+LIB_PATH = os.environ.get("CPT_LIB_PATH") or os.path.join(_HERE, "libcpt.so")
 
 CPT_OK = 0
 CPT_ERR_ARG = -1

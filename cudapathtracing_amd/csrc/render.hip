@@ -42,7 +42,7 @@ struct RenderBufs {
   // path state, double-buffered and indexed by queue slot: k_shade writes
   // the survivors compacted into the other set
   DevBuf ray[2], thr[2], meta[2];
-  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov, cams;
+  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov, cams, lbuf;
 };
 
 struct CamDev {
@@ -124,19 +124,50 @@ struct RenderArgs {
   float rrProb;
   int64_t npix;
   MediumDev med;
+  int64_t base;  // first path of the batch (path g = (sample - sampleBase) * npix * nviews + pixel)
+  float4 *lbuf;  // radiance per path of the batch, index g - base
 };
+
+// Batch-local index of path (pixel over all views, sample).
+__device__ __forceinline__ int64_t pathSlot(const RenderArgs &a, uint32_t pixel, uint32_t sample) {
+  return (int64_t)(sample - (uint32_t)a.sampleBase) * (a.npix * a.nviews) + pixel - a.base;
+}
+
+// Adds to a path's radiance.  One lane owns a path within a kernel, so a
+// plain read-modify-write needs no atomic.
+__device__ __forceinline__ void addRadiance(float4 *lbuf, int64_t slot, f3 v) {
+  float4 l = lbuf[slot];
+  l.x += v.x;
+  l.y += v.y;
+  l.z += v.z;
+  lbuf[slot] = l;
+}
 
 __device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
 
-// Closest hit over the binary nodes or the four-wide records (`wide`).
-template <bool kCount>
+// Closest hit (kAny: any hit) over the record format kFmt: 0 binary nodes,
+// 1 four-wide FP32, 2 four-wide quantized, 3 eight-wide quantized.  Kernels
+// are instantiated per format so each carries only its walk's registers.
+template <bool kCount, int kFmt, bool kAny = false>
+__device__ __forceinline__ int traceFmt(const float4 *__restrict__ nodes, const float4 *__restrict__ leaves, int n,
+                                        f3 o, f3 d, float tmin, float &tbest, int *stk, int stride,
+                                        TraceCounters &cnt) {
+  if (kFmt == 3) return traceClosest8<kCount, kAny>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+  if (kFmt == 2) return traceClosest4<kCount, kAny, true>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+  if (kFmt == 1) return traceClosest4<kCount, kAny>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+  return traceClosest<kCount, kAny>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+}
+
+// Runtime-format closest hit, for the rare volume-object walk.
 __device__ __forceinline__ int traceAny(int wide, const float4 *__restrict__ nodes, const float4 *__restrict__ leaves,
                                         int n, f3 o, f3 d, float tmin, float &tbest, int *stk, int stride,
                                         TraceCounters &cnt) {
-  return wide == 3   ? traceClosest8<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
-         : wide == 2 ? traceClosest4<kCount, false, true>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
-         : wide == 1 ? traceClosest4<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt)
-                     : traceClosest<kCount>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+  switch (wide) {
+    case 3: return traceFmt<false, 3>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+    case 2: return traceFmt<false, 2>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+    case 1: return traceFmt<false, 1>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+    default: return traceFmt<false, 0>(nodes, leaves, n, o, d, tmin, tbest, stk, stride, cnt);
+  }
 }
 
 // Wave-aggregated append: returns the slot for lanes with pred, ~0u otherwise.
@@ -189,15 +220,26 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   ray[2 * p + 1] = make_float4((float)d[0], (float)d[1], (float)d[2], kRayTMax);
   thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
+  a.lbuf[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Hit geometry from a leaf record: outward normal (Hittable.cuh:51,128,393).
+__device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
+  if (type == CPT_OBJ_TRIANGLE) {
+    f3 v0 = xyz(r0);
+    return unit3(cross3(xyz(r1) - v0, xyz(r2) - v0));
+  }
+  if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
+  return xyz(r1);
 }
 
 // kCount: also sums node visits and primitive tests into work[0], work[1]
-template <bool kCount>
+template <bool kCount, int kFmt>
 __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const float4 *__restrict__ ray,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
-                                                            float2 *__restrict__ hit,
+                                                            float4 *__restrict__ hit,
                                                             unsigned long long *__restrict__ work, int wide) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
@@ -206,9 +248,23 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
     const uint32_t p = k;
     const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
     float tb = r1.w;
-    const int leaf = traceAny<kCount>(wide, nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb,
-                                          stack + threadIdx.x, kRenderThreads, cnt);
-    hit[p] = make_float2(__int_as_float(leaf), tb);
+    const int leaf = traceFmt<kCount, kFmt>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb, stack + threadIdx.x,
+                                            kRenderThreads, cnt);
+    // the winner's record was just read by the walk (still cached): hand the
+    // shading kernel its type, material and outward normal (Hittable.cuh:51,
+    // 128, 393) so that it gathers no leaf record
+    float4 h1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int type = 0, mat = 0;
+    if (leaf >= 0) {
+      const float4 l0 = leaves[3 * (int64_t)leaf], l1 = leaves[3 * (int64_t)leaf + 1],
+                   l2 = leaves[3 * (int64_t)leaf + 2];
+      type = __float_as_int(l1.w);
+      mat = __float_as_int(l2.w);
+      const f3 nrm = outwardNormal(l0, l1, l2, type, xyz(r0) + xyz(r1) * tb);
+      h1 = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+    }
+    hit[2 * (int64_t)p] = make_float4(__int_as_float(leaf), tb, __int_as_float(type), __int_as_float(mat));
+    hit[2 * (int64_t)p + 1] = h1;
   }
   if (kCount) {
     unsigned long long a = (unsigned long long)cnt.nodes, b = (unsigned long long)cnt.prims;
@@ -234,15 +290,6 @@ __device__ __forceinline__ ShadeMat loadMat(const MatDev *mats, int m) {
   return s;
 }
 
-// Hit geometry from a leaf record: outward normal (Hittable.cuh:51,128,393).
-__device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
-  if (type == CPT_OBJ_TRIANGLE) {
-    f3 v0 = xyz(r0);
-    return unit3(cross3(xyz(r1) - v0, xyz(r2) - v0));
-  }
-  if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
-  return xyz(r1);
-}
 
 // Delta tracking of the heterogeneous medium along [t0, t1] (Woodcock):
 // tentative collisions at exponential steps of the majorant, each real with
@@ -266,7 +313,7 @@ __device__ __forceinline__ bool deltaTrack(const MediumDev &md, Rng &rng, f3 o, 
 // compacted to the *Out arrays at the slot given by the next-queue counter.
 __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ thr,
-    const uint4 *__restrict__ meta, const float2 *__restrict__ hit, float4 *__restrict__ rayOut,
+    const uint4 *__restrict__ meta, const float4 *__restrict__ hit, float4 *__restrict__ rayOut,
     float4 *__restrict__ thrOut, uint4 *__restrict__ metaOut, float4 *__restrict__ shadow, int64_t shadowCap,
     uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov,
     uint32_t *__restrict__ qVol) {
@@ -278,21 +325,23 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   uint4 mt = make_uint4(0, 0, 0, 0);
   int leaf = -1;
   float th = 0.f;
+  float4 h0 = make_float4(0, 0, 0, 0), h1 = h0;
   if (active) {
     o4 = ray[2 * (int64_t)p];
     d4 = ray[2 * (int64_t)p + 1];
     t4 = thr[p];
     mt = meta[p];
-    const float2 h = hit[p];
-    leaf = __float_as_int(h.x);
-    th = h.y;
+    h0 = hit[2 * (int64_t)p];
+    h1 = hit[2 * (int64_t)p + 1];
+    leaf = __float_as_int(h0.x);
+    th = h0.y;
   }
   Rng rng;
   rng.init(a.seed, mt.x, mt.y, mt.z);
   uint32_t flags = mt.w;
   f3 T = mkf3(t4.x, t4.y, t4.z);
   f3 o = xyz(o4), d = xyz(d4);
-  float *pix = fb + 3 * (int64_t)mt.x;
+  const int64_t ls = pathSlot(a, mt.x, mt.y);
   float *pa = a.wantAov ? aov + 7 * (int64_t)mt.x : nullptr;
 
   // scattering vertex: a surface with a BSDF, or a real collision in the medium
@@ -315,22 +364,17 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       // miss: sky colour (Render.cu:138-142)
       const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
       const f3 bg = mkf3(cv.bg[0], cv.bg[1], cv.bg[2]);
-      atomicAdd(pix + 0, T.x * bg.x);
-      atomicAdd(pix + 1, T.y * bg.y);
-      atomicAdd(pix + 2, T.z * bg.z);
+      addRadiance(a.lbuf, ls, T * bg);
       if (pa && !(flags & kFlagAov)) {
         atomicAdd(pa + 0, bg.x);
         atomicAdd(pa + 1, bg.y);
         atomicAdd(pa + 2, bg.z);
       }
     } else {
-      const float4 r0 = a.leaves[3 * (int64_t)leaf], r1 = a.leaves[3 * (int64_t)leaf + 1],
-                   r2 = a.leaves[3 * (int64_t)leaf + 2];
-      const int type = __float_as_int(r1.w);
-      const int mi = __float_as_int(r2.w);
+      const int mi = __float_as_int(h0.w);
       m = loadMat(a.mats, mi);
       hp = o + d * th;
-      f3 outw = outwardNormal(r0, r1, r2, type, hp);
+      const f3 outw = xyz(h1);
       front = dot3(d, outw) < 0.0f;
       n = front ? outw : -outw;
       if (m.type == CPT_M_VOLUME) {
@@ -340,9 +384,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       } else if (m.type == CPT_M_LIGHT) {
         // emitter (Render.cu:144-149)
         if (!(flags & kFlagNee)) {
-          atomicAdd(pix + 0, T.x * m.color.x);
-          atomicAdd(pix + 1, T.y * m.color.y);
-          atomicAdd(pix + 2, T.z * m.color.z);
+          addRadiance(a.lbuf, ls, T * m.color);
         }
         if (pa && !(flags & kFlagAov)) {
           atomicAdd(pa + 0, 1.0f);
@@ -477,7 +519,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
                                                            uint32_t bound, const float4 *__restrict__ ray,
                                                            const float4 *__restrict__ thr,
                                                            const uint4 *__restrict__ meta,
-                                                           const float2 *__restrict__ hit,
+                                                           const float4 *__restrict__ hit,
                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
                                                            uint4 *__restrict__ metaOut,
                                                            uint32_t *__restrict__ counters) {
@@ -487,7 +529,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   const uint32_t p = qVol[k];
   const float4 o4 = ray[2 * (int64_t)p], d4 = ray[2 * (int64_t)p + 1], t4 = thr[p];
   const uint4 mt = meta[p];
-  const float th = hit[p].y;
+  const float th = hit[2 * (int64_t)p].y;
   Rng rng;
   rng.init(a.seed, mt.x, mt.y, mt.z);
   f3 T = mkf3(t4.x, t4.y, t4.z);
@@ -496,7 +538,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   const float sig = 25.0f;
   TraceCounters cnt;
   float tb = kRayTMax;
-  const int l2 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb, stack + threadIdx.x,
+  const int l2 = traceAny(a.wide, a.nodes, a.leaves, a.nPrims, hp, d, kRayTMin, tb, stack + threadIdx.x,
                                  kRenderThreads, cnt);
   f3 o = hp;
   if (l2 >= 0) {
@@ -516,7 +558,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
       o = pos;
       pos = hp + d * t;
       float tb2 = kRayTMax;
-      const int l3 = traceAny<false>(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
+      const int l3 = traceAny(a.wide, a.nodes, a.leaves, a.nPrims, o, d, kRayTMin, tb2,
                                      stack + threadIdx.x, kRenderThreads, cnt);
       if (l3 >= 0) dMax = tb2;
       t = -__logf(1.0f - rng.next()) / sig;
@@ -552,13 +594,17 @@ __global__ __launch_bounds__(kRenderThreads) void k_permute(uint32_t n, const ui
 // light exactly when the ray meets the light at some tL and nothing else
 // lies in [tmin, tL): an occlusion query (first hit ends the walk) over
 // [tmin, prev(tL)] answers that (an exact tie at tL counts as unoccluded).
-__global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restrict__ shadow, int64_t shadowCap,
-                                                            const uint32_t *__restrict__ counters,
-                                                            const float4 *__restrict__ nodes,
-                                                            const float4 *__restrict__ leaves,
-                                                            const float4 *__restrict__ rec, int nPrims,
-                                                            float *__restrict__ fb, int wide, MediumDev med,
-                                                            uint64_t seed) {
+// kAtomic: several lights, so several shadow rays of one path may add to
+// its radiance in one launch.
+template <bool kAtomic, int kFmt>
+__global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const float4 *__restrict__ shadow,
+                                                            int64_t shadowCap, const uint32_t *__restrict__ counters) {
+  const float4 *__restrict__ nodes = a.nodes;
+  const float4 *__restrict__ leaves = a.leaves;
+  const float4 *__restrict__ rec = a.rec;
+  const int nPrims = a.nPrims;
+  const MediumDev &med = a.med;
+  const uint64_t seed = a.seed;
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   if ((int64_t)k >= shadowCap || k >= counters[1]) return;
@@ -569,22 +615,41 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(const float4 *__restr
   if (!hitLeaf(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
   float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
-  const int leaf = wide == 3 ? traceClosest8<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
-                                                          kRenderThreads, cnt)
-                   : wide == 2 ? traceClosest4<false, true, true>(nodes, leaves, nPrims, o, d, r0.w, tb,
-                                                                 stack + threadIdx.x, kRenderThreads, cnt)
-                   : wide == 1 ? traceClosest4<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
-                                                            kRenderThreads, cnt)
-                               : traceClosest<false, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x,
-                                                           kRenderThreads, cnt);
+  const int leaf =
+      traceFmt<false, kFmt, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x, kRenderThreads, cnt);
   if (leaf >= 0) return;
   const uint32_t pixel = (uint32_t)__float_as_int(c.w);
   float tr = 1.0f;
   if (med.on) tr = ratioTrack(med, seed, pixel, __float_as_uint(l.y), __float_as_uint(l.z), o, d, r0.w, tL);
-  float *pix = fb + 3 * (int64_t)pixel;
-  atomicAdd(pix + 0, c.x * tr);
-  atomicAdd(pix + 1, c.y * tr);
-  atomicAdd(pix + 2, c.z * tr);
+  const int64_t ls = pathSlot(a, pixel, __float_as_uint(l.y));
+  if (kAtomic) {
+    float *pl = (float *)(a.lbuf + ls);
+    atomicAdd(pl + 0, c.x * tr);
+    atomicAdd(pl + 1, c.y * tr);
+    atomicAdd(pl + 2, c.z * tr);
+  } else {
+    addRadiance(a.lbuf, ls, mkf3(c.x * tr, c.y * tr, c.z * tr));
+  }
+}
+
+// Adds the batch's path radiance to the frame: thread = pixel (over all
+// views), summing its samples g = s * npixAll + pixel that lie in the batch.
+__global__ void k_gather(const float4 *__restrict__ lbuf, int64_t base, int64_t count, int64_t npixAll,
+                         float *__restrict__ fb) {
+  const int64_t px = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npixAll) return;
+  int64_t s = base / npixAll;
+  if (s * npixAll + px < base) s++;
+  float r = 0.f, g = 0.f, b = 0.f;
+  for (int64_t gi = s * npixAll + px; gi < base + count; gi += npixAll) {
+    const float4 l = lbuf[gi - base];
+    r += l.x;
+    g += l.y;
+    b += l.z;
+  }
+  fb[3 * px + 0] += r;
+  fb[3 * px + 1] += g;
+  fb[3 * px + 2] += b;
 }
 
 __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ out, int64_t n, float scale) {
@@ -597,7 +662,7 @@ void renderRelease(cpt_ctx *ctx) {
   RenderBufs *r = ctx->render;
   DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
                    &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,
-                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams};
+                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   delete r;
@@ -639,11 +704,11 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     if ((rc = ensure(ctx, R.ray[b], cap * 32)) || (rc = ensure(ctx, R.thr[b], cap * 16)) ||
         (rc = ensure(ctx, R.meta[b], cap * 16)))
       return rc;
-  if ((rc = ensure(ctx, R.hit, cap * 8)) || (rc = ensure(ctx, R.permA, cap * 4)) ||
+  if ((rc = ensure(ctx, R.hit, cap * 32)) || (rc = ensure(ctx, R.permA, cap * 4)) ||
       (rc = ensure(ctx, R.permB, cap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.keysA, cap * 4)) || (rc = ensure(ctx, R.keysB, cap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
-      (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) ||
+      (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) || (rc = ensure(ctx, R.lbuf, cap * 16)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
   R.cap = cap;
@@ -691,6 +756,8 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.rrProb = (float)prm->rr_prob;
   a.wantAov = aovOut ? 1 : 0;
   a.npix = npixView;
+  a.lbuf = R.lbuf.as<float4>();
+  a.base = 0;
   a.med.on = ctx->hasMedium ? 1 : 0;
   a.med.density = ctx->medDensity.as<float>();
   a.med.bmin = mkf3(ctx->medBox[0], ctx->medBox[1], ctx->medBox[2]);
@@ -721,6 +788,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   for (int64_t base = 0; base < total; base += cap) {
     const int64_t count = std::min(cap, total - base);
     int cur = 0;  // state set holding the current queue
+    a.base = base;
     hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
                        R.ray[cur].as<float4>(), R.thr[cur].as<float4>(), R.meta[cur].as<uint4>());
     CPT_TRY_HIP(ctx, hipGetLastError());
@@ -733,28 +801,47 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       const uint4 *metaC = R.meta[cur].as<uint4>();
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
-      if (countWork)
-        hipLaunchKernelGGL(k_extend<true>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes,
-                           a.leaves, a.nPrims, R.hit.as<float2>(), work, a.wide);
-      else
-        hipLaunchKernelGGL(k_extend<false>, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes,
-                           a.leaves, a.nPrims, R.hit.as<float2>(), work, a.wide);
+#define CPT_EXT(C, F)                                                                                        \
+  hipLaunchKernelGGL((k_extend<C, F>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes, a.leaves, \
+                     a.nPrims, R.hit.as<float4>(), work, a.wide)
+      switch (a.wide * 2 + (countWork ? 1 : 0)) {
+        case 0: CPT_EXT(false, 0); break;
+        case 1: CPT_EXT(true, 0); break;
+        case 2: CPT_EXT(false, 1); break;
+        case 3: CPT_EXT(true, 1); break;
+        case 4: CPT_EXT(false, 2); break;
+        case 5: CPT_EXT(true, 2); break;
+        case 6: CPT_EXT(false, 3); break;
+        default: CPT_EXT(true, 3); break;
+      }
+#undef CPT_EXT
       extRays += qn;
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
       hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC,
-                         (const float2 *)R.hit.as<float2>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
+                         (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
                          R.meta[nxt].as<uint4>(), R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(),
                          R.aov.as<float>(), R.qVol.as<uint32_t>());
       if (ctx->hasVolumeMat)
         hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
                            (const uint32_t *)R.qVol.as<uint32_t>(), qn, rayC, thrC, metaC,
-                           (const float2 *)R.hit.as<float2>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
+                           (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
                            R.meta[nxt].as<uint4>(), cnt);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
       const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
-      hipLaunchKernelGGL(k_shadow, dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s,
-                         (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, a.nodes, a.leaves,
-                         a.rec, a.nPrims, R.fb.as<float>(), a.wide, a.med, a.seed);
+#define CPT_SHD(A, F)                                                                                        \
+  hipLaunchKernelGGL((k_shadow<A, F>), dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s, a,          \
+                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt)
+      switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {
+        case 0: CPT_SHD(false, 0); break;
+        case 1: CPT_SHD(true, 0); break;
+        case 2: CPT_SHD(false, 1); break;
+        case 3: CPT_SHD(true, 1); break;
+        case 4: CPT_SHD(false, 2); break;
+        case 5: CPT_SHD(true, 2); break;
+        case 6: CPT_SHD(false, 3); break;
+        default: CPT_SHD(true, 3); break;
+      }
+#undef CPT_SHD
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
       uint32_t hc[2] = {0, 0};
@@ -785,6 +872,9 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
         cur = o2;
       }
     }
+    hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
+                       (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());
+    CPT_TRY_HIP(ctx, hipGetLastError());
   }
   // mean over samples
   const float inv = 1.0f / (float)prm->spp;
