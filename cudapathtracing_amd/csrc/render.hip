@@ -311,6 +311,9 @@ __device__ __forceinline__ bool deltaTrack(const MediumDev &md, Rng &rng, f3 o, 
 
 // Reads the path state of slot k (< count) from the *In arrays; survivors go
 // compacted to the *Out arrays at the slot given by the next-queue counter.
+// kMedium / kAov: the scene has a medium / AOVs are written (instantiated
+// separately so the common case carries neither in registers).
+template <bool kMedium, bool kAov>
 __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ thr,
     const uint4 *__restrict__ meta, const float4 *__restrict__ hit, float4 *__restrict__ rayOut,
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   f3 T = mkf3(t4.x, t4.y, t4.z);
   f3 o = xyz(o4), d = xyz(d4);
   const int64_t ls = pathSlot(a, mt.x, mt.y);
-  float *pa = a.wantAov ? aov + 7 * (int64_t)mt.x : nullptr;
+  float *pa = kAov ? aov + 7 * (int64_t)mt.x : nullptr;
 
   // scattering vertex: a surface with a BSDF, or a real collision in the medium
   bool vtx = false, inMedium = false;
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   m.type = CPT_M_NONE;
   bool front = true;
   if (active) {
-    if (a.med.on) {
+    if (kMedium) {
       float t0, t1;
       if (clipBox(a.med, o, d, o4.w, leaf >= 0 ? th : d4.w, t0, t1) &&
           deltaTrack(a.med, rng, o, d, t0, t1, hp)) {
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
       const f3 bg = mkf3(cv.bg[0], cv.bg[1], cv.bg[2]);
       addRadiance(a.lbuf, ls, T * bg);
-      if (pa && !(flags & kFlagAov)) {
+      if (kAov && !(flags & kFlagAov)) {
         atomicAdd(pa + 0, bg.x);
         atomicAdd(pa + 1, bg.y);
         atomicAdd(pa + 2, bg.z);
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         if (!(flags & kFlagNee)) {
           addRadiance(a.lbuf, ls, T * m.color);
         }
-        if (pa && !(flags & kFlagAov)) {
+        if (kAov && !(flags & kFlagAov)) {
           atomicAdd(pa + 0, 1.0f);
           atomicAdd(pa + 1, 1.0f);
           atomicAdd(pa + 2, 1.0f);
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   }
   const f3 alb = inMedium ? a.med.albedo : m.color;
   if (vtx) {
-    if (pa && !(flags & kFlagAov)) {
+    if (kAov && !(flags & kFlagAov)) {
       atomicAdd(pa + 0, alb.x);
       atomicAdd(pa + 1, alb.y);
       atomicAdd(pa + 2, alb.z);
@@ -817,10 +820,18 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
 #undef CPT_EXT
       extRays += qn;
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
-      hipLaunchKernelGGL(k_shade, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC,
-                         (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
-                         R.meta[nxt].as<uint4>(), R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(),
-                         R.aov.as<float>(), R.qVol.as<uint32_t>());
+#define CPT_SHADE(M, V)                                                                                         \
+  hipLaunchKernelGGL((k_shade<M, V>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC,   \
+                     (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),         \
+                     R.meta[nxt].as<uint4>(), R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(),             \
+                     R.aov.as<float>(), R.qVol.as<uint32_t>())
+      switch ((a.med.on ? 2 : 0) + (a.wantAov ? 1 : 0)) {
+        case 0: CPT_SHADE(false, false); break;
+        case 1: CPT_SHADE(false, true); break;
+        case 2: CPT_SHADE(true, false); break;
+        default: CPT_SHADE(true, true); break;
+      }
+#undef CPT_SHADE
       if (ctx->hasVolumeMat)
         hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
                            (const uint32_t *)R.qVol.as<uint32_t>(), qn, rayC, thrC, metaC,
