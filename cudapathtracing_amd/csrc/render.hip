@@ -182,6 +182,32 @@ __device__ __forceinline__ uint32_t waveAppend(uint32_t *counter, bool pred) {
   return pred ? base + (uint32_t)__popcll(m & lt) : ~0u;
 }
 
+// Workgroup-aggregated append: one global atomic per workgroup instead of
+// one per wave (every wave of a kernel's workgroup appends to the same
+// counter, so per-wave atomics on that one address serialise).  Every
+// thread of the workgroup must call it; sh holds kRenderThreads / 64 + 1
+// words of LDS.
+__device__ __forceinline__ uint32_t blockAppend(uint32_t *counter, bool pred, uint32_t *sh) {
+  constexpr int kWaves = kRenderThreads / 64;
+  const uint64_t m = __ballot(pred);
+  const int wave = (int)(threadIdx.x >> 6), lane = (int)laneId();
+  if (lane == 0) sh[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) tot += sh[w];
+    sh[kWaves] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = sh[kWaves];
+#pragma unroll
+  for (int w = 0; w < kWaves; w++)
+    if (w < wave) base += sh[w];
+  __syncthreads();  // sh is reused by the next call
+  return pred ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : ~0u;
+}
+
 __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -320,6 +346,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     float4 *__restrict__ thrOut, uint4 *__restrict__ metaOut, float4 *__restrict__ shadow, int64_t shadowCap,
     uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov,
     uint32_t *__restrict__ qVol) {
+  __shared__ uint32_t appendSh[kRenderThreads / 64 + 1];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   const bool active = k < count;
   bool alive = false;
@@ -420,12 +447,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   const f3 ud = unit3(d);
   const f3 V = -ud;
   // NEE to every light (Render.cu:161-187): opaque-type surfaces (Render.cu:166)
-  // and medium collisions.  The ballots in waveAppend need every lane of the
-  // wave, so the loop runs on all lanes with a per-lane predicate.
+  // and medium collisions.  blockAppend needs every thread of the workgroup,
+  // so the loop runs on all threads with a per-lane predicate.
   const bool neeHere = vtx && (inMedium || m.type == CPT_M_OPAQUE || m.type == CPT_M_LAMBERT ||
                                m.type == CPT_M_GGX);
   const int nLoop = __builtin_amdgcn_readfirstlane(a.nLights);
-  if (__ballot(neeHere) != 0) {
+  {
     for (int li = 0; li < nLoop; li++) {
       bool want = false;
       f3 dir = mkf3(0, 0, 0), c = dir;
@@ -451,7 +478,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         }
         flags |= kFlagNee;
       }
-      const uint32_t slot = waveAppend(counters + 1, want);
+      const uint32_t slot = blockAppend(counters + 1, want, appendSh);
       if (want && slot < shadowCap) {
         shadow[4 * (int64_t)slot + 0] = make_float4(hp.x, hp.y, hp.z, kRayTMin);
         shadow[4 * (int64_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, kRayTMax);
@@ -504,7 +531,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       }
     }
   }
-  const uint32_t slot = waveAppend(counters + 0, alive);
+  const uint32_t slot = blockAppend(counters + 0, alive, appendSh);
   if (alive) {
     rayOut[2 * (int64_t)slot] = make_float4(o.x, o.y, o.z, kRayTMin);
     rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
