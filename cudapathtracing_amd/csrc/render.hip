@@ -260,19 +260,34 @@ __device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int
 }
 
 // kCount: also sums node visits and primitive tests into work[0], work[1]
-template <bool kCount, int kFmt>
+// kPerm: the queue is in ray-key order perm; lane k traces the path in state
+// slot perm[k] and moves its state to slot k of the *Out arrays (the
+// shading kernel then reads slot k there).
+template <bool kCount, int kFmt, bool kPerm>
 __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const float4 *__restrict__ ray,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
                                                             float4 *__restrict__ hit,
-                                                            unsigned long long *__restrict__ work, int wide) {
+                                                            unsigned long long *__restrict__ work, int wide,
+                                                            const uint32_t *__restrict__ perm,
+                                                            const float4 *__restrict__ thr,
+                                                            const uint4 *__restrict__ meta,
+                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
+                                                            uint4 *__restrict__ metaOut) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   TraceCounters cnt;
   if (k < count) {
+    const uint32_t src = kPerm ? perm[k] : k;
     const uint32_t p = k;
-    const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
+    const float4 r0 = ray[2 * (int64_t)src], r1 = ray[2 * (int64_t)src + 1];
+    if (kPerm) {
+      rayOut[2 * (int64_t)p] = r0;
+      rayOut[2 * (int64_t)p + 1] = r1;
+      thrOut[p] = thr[src];
+      metaOut[p] = meta[src];
+    }
     float tb = r1.w;
     const int leaf = traceFmt<kCount, kFmt>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb, stack + threadIdx.x,
                                             kRenderThreads, cnt);
@@ -602,21 +617,6 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
 }
 
-// Moves the path state of slot perm[j] to slot j (ray-key order).
-__global__ __launch_bounds__(kRenderThreads) void k_permute(uint32_t n, const uint32_t *__restrict__ perm,
-                                                            const float4 *__restrict__ ray,
-                                                            const float4 *__restrict__ thr,
-                                                            const uint4 *__restrict__ meta,
-                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
-                                                            uint4 *__restrict__ metaOut) {
-  const uint32_t j = blockIdx.x * kRenderThreads + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t p = perm[j];
-  rayOut[2 * (int64_t)j] = ray[2 * (int64_t)p];
-  rayOut[2 * (int64_t)j + 1] = ray[2 * (int64_t)p + 1];
-  thrOut[j] = thr[p];
-  metaOut[j] = meta[p];
-}
 
 // Shadow rays: the light's contribution counts when the light itself is the
 // closest hit (Render.cu:180-185), times the medium's transmittance from a
@@ -818,6 +818,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   for (int64_t base = 0; base < total; base += cap) {
     const int64_t count = std::min(cap, total - base);
     int cur = 0;  // state set holding the current queue
+    bool permPending = false;
     a.base = base;
     hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
                        R.ray[cur].as<float4>(), R.thr[cur].as<float4>(), R.meta[cur].as<uint4>());
@@ -826,25 +827,40 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     // paths end by roulette, depth cap, miss or absorption; the cap only
     // bounds pathological inputs (NaN geometry) so the loop always ends
     for (int pass = 0; qn > 0 && pass < kMaxBounces; pass++) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
+      {
+        // with a pending ray-key order the extension kernel moves the state
+        // into that order (set cur ^ 1), which then becomes the current set
+        const int dst = cur ^ 1;
+        const float4 *rayIn = R.ray[cur].as<float4>();
+        const float4 *thrIn = R.thr[cur].as<float4>();
+        const uint4 *metaIn = R.meta[cur].as<uint4>();
+        const uint32_t *pm = permPending ? R.permA.as<uint32_t>() : nullptr;
+#define CPT_EXT(C, F, P)                                                                                      \
+  hipLaunchKernelGGL((k_extend<C, F, P>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes,     \
+                     a.leaves, a.nPrims, R.hit.as<float4>(), work, a.wide, pm, thrIn, metaIn, R.ray[dst].as<float4>(), \
+                     R.thr[dst].as<float4>(), R.meta[dst].as<uint4>())
+#define CPT_EXT_F(C, P)                  \
+  switch (a.wide) {                      \
+    case 0: CPT_EXT(C, 0, P); break;     \
+    case 1: CPT_EXT(C, 1, P); break;     \
+    case 2: CPT_EXT(C, 2, P); break;     \
+    default: CPT_EXT(C, 3, P); break;    \
+  }
+        if (countWork) {
+          if (permPending) CPT_EXT_F(true, true) else CPT_EXT_F(true, false)
+        } else {
+          if (permPending) CPT_EXT_F(false, true) else CPT_EXT_F(false, false)
+        }
+#undef CPT_EXT_F
+#undef CPT_EXT
+        if (permPending) cur = dst;
+        permPending = false;
+      }
       const int nxt = cur ^ 1;
       const float4 *rayC = R.ray[cur].as<float4>(), *thrC = R.thr[cur].as<float4>();
       const uint4 *metaC = R.meta[cur].as<uint4>();
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
-#define CPT_EXT(C, F)                                                                                        \
-  hipLaunchKernelGGL((k_extend<C, F>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayC, a.nodes, a.leaves, \
-                     a.nPrims, R.hit.as<float4>(), work, a.wide)
-      switch (a.wide * 2 + (countWork ? 1 : 0)) {
-        case 0: CPT_EXT(false, 0); break;
-        case 1: CPT_EXT(true, 0); break;
-        case 2: CPT_EXT(false, 1); break;
-        case 3: CPT_EXT(true, 1); break;
-        case 4: CPT_EXT(false, 2); break;
-        case 5: CPT_EXT(true, 2); break;
-        case 6: CPT_EXT(false, 3); break;
-        default: CPT_EXT(true, 3); break;
-      }
-#undef CPT_EXT
       extRays += qn;
       CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
 #define CPT_SHADE(M, V)                                                                                         \
@@ -896,18 +912,13 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       bounces++;
       qn = hc[0];
       cur = nxt;
-      // incoherent bounce rays: move the next queue's state into ray-key order
+      // incoherent bounce rays: trace the next queue in ray-key order (the
+      // next extension kernel moves the state into that order)
       if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin) {
         if ((rc = rayKeyOrder(ctx, (const float4 *)R.ray[cur].as<float4>(), qn, R.keysA.as<uint32_t>(),
                               R.keysB.as<uint32_t>(), R.permA.as<uint32_t>(), R.permB.as<uint32_t>())))
           return rc;
-        const int o2 = cur ^ 1;
-        hipLaunchKernelGGL(k_permute, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn,
-                           (const uint32_t *)R.permA.as<uint32_t>(), (const float4 *)R.ray[cur].as<float4>(),
-                           (const float4 *)R.thr[cur].as<float4>(), (const uint4 *)R.meta[cur].as<uint4>(),
-                           R.ray[o2].as<float4>(), R.thr[o2].as<float4>(), R.meta[o2].as<uint4>());
-        CPT_TRY_HIP(ctx, hipGetLastError());
-        cur = o2;
+        permPending = true;
       }
     }
     hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
