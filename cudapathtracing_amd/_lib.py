@@ -112,6 +112,8 @@ SIGNATURES = [
     ("cpt_tonemap_rgba8", C.c_int, [P, C.c_int32, C.c_int32, P]),
     ("cpt_bilateral", C.c_int, [P, P, P, P, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                 C.c_float, C.c_int]),
+    ("cpt_temporal_blend", C.c_int, [P, P, P, P, P, C.POINTER(Camera), C.POINTER(CameraDesc), C.c_int32, P,
+                                     C.c_int]),
     ("cpt_scene_update_triangles", C.c_int, [P, C.c_int64, P, C.c_int64]),
     ("cpt_bvh_refit", C.c_int, [P, C.POINTER(BuildStats)]),
     ("cpt_denoiser_create", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]),

@@ -156,6 +156,32 @@ class Context:
                                             sigma_g, sigma_r, sigma_n, sigma_d, 0))
         return out
 
+    def temporal_blend(self, rgb, prev_rgb, camera: L.CameraDesc, prev_camera: L.CameraDesc, position=None,
+                       aov=None, moving=False):
+        """addPrevious (Render.cu:400-421): blends ``rgb`` with ``prev_rgb``
+        reprojected through ``prev_camera`` (host arrays, h x w x 3).
+        ``position`` (h x w x 3 first-hit points, x + y > 999999 = miss) or
+        ``aov`` (its depth channel) locates each pixel's surface."""
+        if position is None and aov is None:
+            raise ValueError("temporal_blend needs position or aov")
+        h, w = rgb.shape[:2]
+        cam = L.Camera()
+        self._check(self._lib.cpt_camera_from_desc(C.byref(camera), C.byref(cam)))
+        if (cam.width, cam.height) != (w, h):
+            raise ValueError(f"image is {w}x{h}, camera is {cam.width}x{cam.height}")
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        prev_rgb = np.ascontiguousarray(prev_rgb, np.float32)
+        if prev_rgb.shape != rgb.shape:
+            raise ValueError("prev_rgb and rgb differ in shape")
+        pos = None if position is None else np.ascontiguousarray(position, np.float32).reshape(h, w, 3)
+        av = None if aov is None else np.ascontiguousarray(aov, np.float32).reshape(h, w, 7)
+        out = np.empty((h, w, 3), np.float32)
+        self._check(self._lib.cpt_temporal_blend(self._h, rgb.ctypes.data, prev_rgb.ctypes.data,
+                                                 None if pos is None else pos.ctypes.data,
+                                                 None if av is None else av.ctypes.data, C.byref(cam),
+                                                 C.byref(prev_camera), 1 if moving else 0, out.ctypes.data, 0))
+        return out
+
     def export(self) -> dict:
         n = self.n_prims
         keys = np.empty(n, np.uint64)

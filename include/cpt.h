@@ -349,6 +349,18 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri_xy
  * order of the last cpt_bvh_build. */
 int cpt_bvh_refit(cpt_ctx *ctx, cpt_build_stats *stats);
 
+/* Temporal accumulation for a moving camera: addPrevious (Render.cu:400-421)
+ * with Camera::getPrePositionInImg (Camera.cuh:104-132), FP64.  rgb,
+ * prev_rgb, out: w*h*3 (w, h from camera).  position: w*h*3 first-hit
+ * positions (x + y > 999999 marks a miss), or NULL to derive them from the
+ * mean first-hit distance in aov (w*h*7, channel 6; 0 = miss) along each
+ * pixel-centre ray.  prev: the previous frame's camera.  alpha = 0.4 when
+ * moving, else 0.1; out = rgb*alpha + prev_rgb[reprojected]*(1-alpha). */
+int cpt_temporal_blend(cpt_ctx *ctx, const float *rgb, const float *prev_rgb,
+                       const float *position, const float *aov,
+                       const cpt_camera *camera, const cpt_camera_desc *prev,
+                       int32_t moving, float *out, int flags);
+
 /* ---- learned denoiser (no reference counterpart; replaces the role of the
  * joint bilateral filter of Render.cu:353-460) ---------------------------- */
 /* 5-level U-Net (channels 32,64,64,96,128; 3x3 conv + ReLU, 2x2 average
