@@ -86,6 +86,10 @@ class Context:
         """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 5, n))
 
+    def set_shadow_sort_min(self, n: int):
+        """cpt_render shadow queues of at least n rays are traced in origin-cell order (0: never)."""
+        self._check(self._lib.cpt_set_option(self._h, 6, n))
+
     def set_ray_sort_min(self, n: int):
         """Trace batches of at least n rays in ray-key order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 2, n))

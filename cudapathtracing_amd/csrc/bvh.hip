@@ -460,7 +460,8 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_cross(const int32_t *__
 }
 
 // Four-wide record of every internal node (layout in cpt_device.hpp): the
-// node's children with each internal child replaced by its two children.
+// node's children with each internal child replaced by its two children,
+// and a free slot filled from the larger-area grandchild.
 // One axis of a quantized record: origin = the lowest child plane, grid
 // step 2^e >= extent / 255, planes rounded outward to the grid and checked in
 // FP32 with the decode the traversal uses (fmaf(q, 2^e, origin)); e grows if
@@ -546,6 +547,40 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
         for (int q = 0; q < 6; q++) b[q][k] = gb[e][q];
         ref[k++] = gr[e];
       }
+    }
+  }
+  // One leaf child and one internal child leave a slot free: open the
+  // grandchild with the larger surface area into it.  Every internal entry
+  // then still lies at least two binary levels below node i, so a walk pushes
+  // at most three entries per two levels.
+  if (k == 3) {
+    int pick = -1;
+    float best = -1.f;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      if (ref[s] < 0) continue;
+      const float ex = b[3][s] - b[0][s], ey = b[4][s] - b[1][s], ez = b[5][s] - b[2][s];
+      const float area = ex * ey + ey * ez + ez * ex;
+      if (area > best) {
+        best = area;
+        pick = s;
+      }
+    }
+    if (pick >= 0) {
+      const int64_t g = ref[pick];
+      const float4 g0 = nodes[4 * g], g1 = nodes[4 * g + 1], g2 = nodes[4 * g + 2], g3 = nodes[4 * g + 3];
+      const float gb[2][6] = {{g0.x, g0.z, g2.x, g0.y, g0.w, g2.y}, {g1.x, g1.z, g2.z, g1.y, g1.w, g2.w}};
+#pragma unroll
+      for (int s = 0; s < 3; s++)
+        if (s == pick) {
+#pragma unroll
+          for (int q = 0; q < 6; q++) b[q][s] = gb[0][q];
+          ref[s] = __float_as_int(g3.x);
+        }
+#pragma unroll
+      for (int q = 0; q < 6; q++) b[q][3] = gb[1][q];
+      ref[3] = __float_as_int(g3.y);
+      k = 4;
     }
   }
   for (; k < 4; k++) {

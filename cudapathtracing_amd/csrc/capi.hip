@@ -111,7 +111,7 @@ int cpt_create(cpt_ctx **out, int dev) {
   ctx->device = dev;
   hipError_t e = hipSetDevice(dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->ownStream, hipStreamNonBlocking);
-  for (int i = 0; i < 16 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+  for (int i = 0; i < 18 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     delete ctx;
@@ -130,7 +130,7 @@ void cpt_destroy(cpt_ctx *ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   releaseAll(ctx);
-  for (int i = 0; i < 16; i++)
+  for (int i = 0; i < 18; i++)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
   if (ctx->ownStream) hipStreamDestroy(ctx->ownStream);
   delete ctx;
@@ -164,6 +164,10 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
   }
   if (option == CPT_OPT_QUEUE_SORT_MIN && value >= 0) {
     ctx->queueSortMin = value;
+    return CPT_OK;
+  }
+  if (option == CPT_OPT_SHADOW_SORT_MIN && value >= 0) {
+    ctx->shadowSortMin = value;
     return CPT_OK;
   }
   if (option == CPT_OPT_QUANTIZED && (value == 0 || value == 1)) {

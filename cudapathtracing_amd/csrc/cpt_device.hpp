@@ -272,13 +272,14 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
 
 // ---- four-wide traversal ------------------------------------------------------
 // Record of internal node i (128 B, 8 x float4): the boxes of i's children,
-// each internal child replaced by its own two children (so a record spans
-// two binary levels), as SoA {minx[4]} {maxx[4]} {miny[4]} {maxy[4]}
-// {minz[4]} {maxz[4]} {ref[4]} {unused}; ref >= 0 is a binary node index
-// (whose record continues the walk), ~k is leaf k, kEmpty4 an unused slot.
-// Walking from the root touches only even-depth records.
+// each internal child replaced by its own two children; when one child is a
+// leaf, the larger-area internal grandchild is opened into the fourth slot.
+// Every internal entry lies two or three binary levels below i.  SoA
+// {minx[4]} {maxx[4]} {miny[4]} {maxy[4]} {minz[4]} {maxz[4]} {ref[4]}
+// {unused}; ref >= 0 is a binary node index (whose record continues the
+// walk), ~k is leaf k, kEmpty4 an unused slot.
 constexpr int kEmpty4 = (int)0x80000000;
-constexpr int kStack4Total = 96;  // <= 3 pushes per record, <= 31 record levels
+constexpr int kStack4Total = 96;  // <= 3 pushes per >= 2 binary levels, depth <= 64
 
 // Quantized four-wide record (64 B, 4 x float4), same children and refs:
 // {origin x, y, z, exponent bytes} {ref[4]} {lo_x, hi_x, lo_y, hi_y}

@@ -74,13 +74,14 @@ struct cpt_ctx {
   int numCU = 256;
   int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered
   int64_t queueSortMin = (int64_t)1 << 20;  // path-tracer queues this long are reordered
+  int64_t shadowSortMin = (int64_t)1 << 20;  // shadow queues this long are reordered
   bool lastTraceSorted = false;
 
   // traversal I/O staging
   cpt::DevBuf rayBuf, tBuf, idBuf, cntA, cntB;
   cpt::DevBuf rkA, rkB, rvA, rvB;  // ray-order keys / permutation (ping-pong)
   float lastTraceMs = 0.f;
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[18] = {};
 
   cpt::RenderBufs *render = nullptr;
 };
@@ -114,8 +115,9 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
 
 // Permutation of n rays into ray-key order (direction octant, origin Morton
 // cell), stable; the 4-pass sort leaves it in perm (keysB, permB scratch).
+// Ray k is {origin, tmin} {direction, tmax} at rays[stride * k].
 int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
-                uint32_t *permB);
+                uint32_t *permB, int stride = 2);
 
 // render.hip
 void renderRelease(cpt_ctx *ctx);

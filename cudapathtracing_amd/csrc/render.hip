@@ -625,10 +625,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
 // lies in [tmin, tL): an occlusion query (first hit ends the walk) over
 // [tmin, prev(tL)] answers that (an exact tie at tL counts as unoccluded).
 // kAtomic: several lights, so several shadow rays of one path may add to
-// its radiance in one launch.
+// its radiance in one launch.  perm (optional): lane i takes record perm[i].
 template <bool kAtomic, int kFmt>
 __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const float4 *__restrict__ shadow,
-                                                            int64_t shadowCap, const uint32_t *__restrict__ counters) {
+                                                            int64_t shadowCap, const uint32_t *__restrict__ counters,
+                                                            const uint32_t *__restrict__ perm) {
   const float4 *__restrict__ nodes = a.nodes;
   const float4 *__restrict__ leaves = a.leaves;
   const float4 *__restrict__ rec = a.rec;
@@ -636,8 +637,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   const MediumDev &med = a.med;
   const uint64_t seed = a.seed;
   __shared__ int stack[kStackLds * kRenderThreads];
-  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
-  if ((int64_t)k >= shadowCap || k >= counters[1]) return;
+  const uint32_t li = blockIdx.x * kRenderThreads + threadIdx.x;
+  if ((int64_t)li >= shadowCap || li >= counters[1]) return;
+  const uint32_t k = perm ? perm[li] : li;
   const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
   const f3 o = xyz(r0), d = xyz(r1);
@@ -734,9 +736,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     if ((rc = ensure(ctx, R.ray[b], cap * 32)) || (rc = ensure(ctx, R.thr[b], cap * 16)) ||
         (rc = ensure(ctx, R.meta[b], cap * 16)))
       return rc;
-  if ((rc = ensure(ctx, R.hit, cap * 32)) || (rc = ensure(ctx, R.permA, cap * 4)) ||
-      (rc = ensure(ctx, R.permB, cap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
-      (rc = ensure(ctx, R.keysA, cap * 4)) || (rc = ensure(ctx, R.keysB, cap * 4)) ||
+  // sort keys and permutations serve the bounce queue and the shadow queue
+  if ((rc = ensure(ctx, R.hit, cap * 32)) || (rc = ensure(ctx, R.permA, shadowCap * 4)) ||
+      (rc = ensure(ctx, R.permB, shadowCap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
+      (rc = ensure(ctx, R.keysA, shadowCap * 4)) || (rc = ensure(ctx, R.keysB, shadowCap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
       (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) || (rc = ensure(ctx, R.lbuf, cap * 16)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
@@ -808,6 +811,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
 
   double rays = 0, extMs = 0, shMs = 0, sdMs = 0, extRays = 0;
   int bounces = 0;
+  bool shadowTimed = false;  // events 16/17 bracket a shadow pass not yet added
   uint32_t *cnt = R.counters.as<uint32_t>();
   // counters: [0] next queue, [1] shadow queue; [8..11] two u64 work sums
   unsigned long long *work = (unsigned long long *)(cnt + 8);
@@ -881,11 +885,37 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
                            (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
                            R.meta[nxt].as<uint4>(), cnt);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
-      const int64_t shadowBound = std::min<int64_t>(shadowCap, (int64_t)qn * nl);
+      uint32_t hc[2] = {0, 0};
+      CPT_TRY_HIP(ctx, hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, s));
+      CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+      float e1 = 0, e2 = 0;
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e1, ev[13], ev[14]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e2, ev[14], ev[15]));
+      extMs += e1;
+      shMs += e2;
+      if (shadowTimed) {  // the previous bounce's shadow pass has finished
+        float e3 = 0;
+        CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[16], ev[17]));
+        sdMs += e3;
+        shadowTimed = false;
+      }
+      const int64_t ns = std::min<int64_t>(hc[1], shadowCap);
+      rays += (double)qn + (double)ns;
+      bounces++;
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[16], s));
+      // shadow rays in the order of their origin cell (all rays of one
+      // origin cell then walk the same records towards the lights)
+      const uint32_t *shPerm = nullptr;
+      if (ctx->shadowSortMin > 0 && ns >= ctx->shadowSortMin) {
+        if ((rc = rayKeyOrder(ctx, (const float4 *)R.shadow.as<float4>(), ns, R.keysA.as<uint32_t>(),
+                              R.keysB.as<uint32_t>(), R.permA.as<uint32_t>(), R.permB.as<uint32_t>(), 4)))
+          return rc;
+        shPerm = R.permA.as<uint32_t>();
+      }
 #define CPT_SHD(A, F)                                                                                        \
-  hipLaunchKernelGGL((k_shadow<A, F>), dim3(blocksFor(shadowBound)), dim3(kRenderThreads), 0, s, a,          \
-                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt)
-      switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {
+  hipLaunchKernelGGL((k_shadow<A, F>), dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,                   \
+                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, shPerm)
+      if (ns > 0) switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {
         case 0: CPT_SHD(false, 0); break;
         case 1: CPT_SHD(true, 0); break;
         case 2: CPT_SHD(false, 1); break;
@@ -897,19 +927,8 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       }
 #undef CPT_SHD
       CPT_TRY_HIP(ctx, hipGetLastError());
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[11], s));
-      uint32_t hc[2] = {0, 0};
-      CPT_TRY_HIP(ctx, hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, s));
-      CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
-      float e1 = 0, e2 = 0, e3 = 0;
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e1, ev[13], ev[14]));
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e2, ev[14], ev[15]));
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[15], ev[11]));
-      extMs += e1;
-      shMs += e2;
-      sdMs += e3;
-      rays += (double)qn + (double)std::min<int64_t>(hc[1], shadowCap);
-      bounces++;
+      CPT_TRY_HIP(ctx, hipEventRecord(ev[17], s));
+      shadowTimed = true;
       qn = hc[0];
       cur = nxt;
       // incoherent bounce rays: trace the next queue in ray-key order (the
@@ -920,6 +939,13 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
           return rc;
         permPending = true;
       }
+    }
+    if (shadowTimed) {
+      float e3 = 0;
+      CPT_TRY_HIP(ctx, hipEventSynchronize(ev[17]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[16], ev[17]));
+      sdMs += e3;
+      shadowTimed = false;
     }
     hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
                        (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());

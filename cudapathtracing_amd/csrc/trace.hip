@@ -27,15 +27,16 @@ __device__ __forceinline__ uint32_t spread9(uint32_t v) {
   return v;
 }
 
-// q (optional): key i belongs to ray q[i] (a path queue).
-__global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__restrict__ rays,
+// q (optional): key i belongs to ray q[i] (a path queue).  Ray i is
+// {origin, tmin} {direction, tmax} at rays[stride * i].
+__global__ __launch_bounds__(kTraceThreads) void k_ray_keys(const float4 *__restrict__ rays, int stride,
                                                            const uint32_t *__restrict__ q, int64_t n,
                                                            const float *__restrict__ rootBox,
                                                            uint32_t *__restrict__ keys) {
   const int64_t k = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
   if (k >= n) return;
   const int64_t i = q ? (int64_t)q[k] : k;
-  const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+  const float4 r0 = rays[stride * i], r1 = rays[stride * i + 1];
   const float o[3] = {r0.x, r0.y, r0.z};
   uint32_t c[3];
 #pragma unroll
@@ -150,10 +151,11 @@ __global__ void k_zero_work(uint32_t *p) {
 // Permutation perm[0..n) of rays[0..n) (2 float4 per ray) into ray-key order
 // (stable); keysB and permB are scratch.
 int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
-                uint32_t *permB) {
+                uint32_t *permB, int stride) {
   if (n <= 0) return CPT_OK;
   const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
-  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, (const uint32_t *)nullptr, n,
+  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, stride,
+                     (const uint32_t *)nullptr, n,
                      (const float *)ctx->rootBox.as<float>(), keysA);
   CPT_TRY_HIP(ctx, hipGetLastError());
   return radixSortPairs(ctx, keysA, perm, keysB, permB, n, 30, true);
@@ -177,7 +179,7 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
   const uint32_t *perm = nullptr;
   if (sorted) {
-    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, (const uint32_t *)nullptr, n,
+    hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, 2, (const uint32_t *)nullptr, n,
                        (const float *)ctx->rootBox.as<float>(), ctx->rkA.as<uint32_t>());
     CPT_TRY_HIP(ctx, hipGetLastError());
     if ((rc = radixSortPairs(ctx, ctx->rkA.as<uint32_t>(), ctx->rvA.as<uint32_t>(), ctx->rkB.as<uint32_t>(),

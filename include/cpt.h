@@ -150,6 +150,10 @@ const char *cpt_version(void);
  * many paths in ray-key order (the same key as CPT_OPT_RAY_SORT_MIN); 0
  * disables it; default 2^20.  Radiance is unchanged. */
 #define CPT_OPT_QUEUE_SORT_MIN 5
+/* CPT_OPT_SHADOW_SORT_MIN: cpt_render traces a bounce's shadow rays of at
+ * least this many in the order of their origin Morton cell; 0 disables it;
+ * default 2^20.  Radiance is unchanged. */
+#define CPT_OPT_SHADOW_SORT_MIN 6
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */

@@ -61,15 +61,34 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort):
     d = scenes.default_camera(48, 48)
     ctx.set_traversal_width(width)
     ctx.set_quantized(quant)
-    if qsort:
+    if qsort:  # bounce and shadow queues both traced in key order
         ctx.set_queue_sort_min(1)
+        ctx.set_shadow_sort_min(1)
     try:
         f = path_match(ctx, s, d, L.PROFILE_CONFIG, samples=(0, 1, 2))
     finally:
         ctx.set_traversal_width(4)
         ctx.set_quantized(False)
         ctx.set_queue_sort_min(1 << 20)
+        ctx.set_shadow_sort_min(1 << 20)
     print(f"  cornell path agreement {f:.5f}")
+    assert f >= 0.99
+
+
+@pytest.mark.parametrize("ssort", [0, 1])
+def test_cornell_two_lights(ctx, ssort):
+    # two shadow rays per vertex: both add to the path radiance in one launch
+    print(f"[gpu] render cornell two lights (shadow sort {ssort})", flush=True)
+    s = scenes.cornell_icosphere()
+    s.lights.append(L.Light(L.d3((0.3, 0.5 - 1e-3, -0.2)), 0.1, 0.15, L.d3((0.0, -1.0, 0.0)),
+                            L.d3((6.0, 4.0, 2.0)), 1, 0))
+    d = scenes.default_camera(40, 40)
+    ctx.set_shadow_sort_min(1 if ssort else 0)
+    try:
+        f = path_match(ctx, s, d, L.PROFILE_CONFIG)
+    finally:
+        ctx.set_shadow_sort_min(1 << 20)
+    print(f"  two-light path agreement {f:.5f}")
     assert f >= 0.99
 
 
