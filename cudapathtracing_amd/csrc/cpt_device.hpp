@@ -366,18 +366,24 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
       const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbest));
       h[k] = tn[k] <= tf && ref[k] != kEmpty4;
     }
-    // leaf children first, in slot order
+    // leaf children first, in slot order: one test per loop trip, so a wave
+    // runs as many trips as its lane with the most leaf hits
+    uint32_t lm = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      if (h[k] && ref[k] < 0) {
-        float t;
-        if (kCount) cnt.prims++;
-        if (hitLeaf(leaves, ~ref[k], o, d, tmin, tbest, t)) {
-          tbest = t;
-          hit = ~ref[k];
-          if (kAny) return hit;
-        }
-        h[k] = false;
+      if (h[k] && ref[k] < 0) lm |= 1u << k;
+      h[k] = h[k] && ref[k] >= 0;
+    }
+    while (lm) {
+      const int k = __builtin_ctz(lm);
+      lm &= lm - 1;
+      const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
+      float t;
+      if (kCount) cnt.prims++;
+      if (hitLeaf(leaves, li, o, d, tmin, tbest, t)) {
+        tbest = t;
+        hit = li;
+        if (kAny) return hit;
       }
     }
     // internal children: continue with the nearest, push the others
@@ -419,12 +425,23 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
     CPT_CSWAP(1, 3)
     CPT_CSWAP(1, 2)
 #undef CPT_CSWAP
+    // the pushed entries are pr[0 .. c-1] (the -inf ones sort last; at most
+    // three, the nearest is not pushed)
+    const int c = (pt[0] != -INFINITY) + (pt[1] != -INFINITY) + (pt[2] != -INFINITY);
+    if (sp + 3 <= kStackLds) {
+      // room in LDS for three: store all three, keep c of them
+      stk[sp * stride] = pr[0];
+      stk[(sp + 1) * stride] = pr[1];
+      stk[(sp + 2) * stride] = pr[2];
+      sp += c;
+    } else {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      if (pt[k] != -INFINITY) {
-        if (sp < kStackLds) stk[sp * stride] = pr[k];
-        else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
-        sp = sp < kStack4Total ? sp + 1 : sp;
+      for (int k = 0; k < 3; k++) {
+        if (k < c) {
+          if (sp < kStackLds) stk[sp * stride] = pr[k];
+          else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
+          sp = sp < kStack4Total ? sp + 1 : sp;
+        }
       }
     }
     node = ref[best];
@@ -497,17 +514,25 @@ __device__ __forceinline__ int traceClosest8(const float4 *__restrict__ nodes8,
       tn[k] = t0;
       h[k] = t0 <= t1 && ref[k] != kEmpty4;
     }
+    // leaf children in slot order, one test per loop trip
+    uint32_t lm = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      if (h[k] && ref[k] < 0) {
-        float t;
-        if (kCount) cnt.prims++;
-        if (hitLeaf(leaves, ~ref[k], o, d, tmin, tbest, t)) {
-          tbest = t;
-          hit = ~ref[k];
-          if (kAny) return hit;
-        }
-        h[k] = false;
+      if (h[k] && ref[k] < 0) lm |= 1u << k;
+      h[k] = h[k] && ref[k] >= 0;
+    }
+    while (lm) {
+      const int k = __builtin_ctz(lm);
+      lm &= lm - 1;
+      int r = ref[0];
+#pragma unroll
+      for (int j = 1; j < 8; j++) r = k == j ? ref[j] : r;
+      float t;
+      if (kCount) cnt.prims++;
+      if (hitLeaf(leaves, ~r, o, d, tmin, tbest, t)) {
+        tbest = t;
+        hit = ~r;
+        if (kAny) return hit;
       }
     }
     int best = -1;
@@ -607,17 +632,22 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
     const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), w.tbest));
     h[k] = tn[k] <= tf && ref[k] != kEmpty4;
   }
+  uint32_t lm = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    if (h[k] && ref[k] < 0) {
-      float t;
-      if (kCount) cnt.prims++;
-      if (hitLeaf(leaves, ~ref[k], w.o, w.d, w.tmin, w.tbest, t)) {
-        w.tbest = t;
-        w.hit = ~ref[k];
-        if (kAny) return false;
-      }
-      h[k] = false;
+    if (h[k] && ref[k] < 0) lm |= 1u << k;
+    h[k] = h[k] && ref[k] >= 0;
+  }
+  while (lm) {
+    const int k = __builtin_ctz(lm);
+    lm &= lm - 1;
+    const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
+    float t;
+    if (kCount) cnt.prims++;
+    if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
+      w.tbest = t;
+      w.hit = li;
+      if (kAny) return false;
     }
   }
   int best = -1;
@@ -656,12 +686,20 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
   CPT_CSWAP(1, 3)
   CPT_CSWAP(1, 2)
 #undef CPT_CSWAP
+  const int c = (pt[0] != -INFINITY) + (pt[1] != -INFINITY) + (pt[2] != -INFINITY);
+  if (w.sp + 3 <= kStackLds) {
+    stk[w.sp * stride] = pr[0];
+    stk[(w.sp + 1) * stride] = pr[1];
+    stk[(w.sp + 2) * stride] = pr[2];
+    w.sp += c;
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    if (pt[k] != -INFINITY) {
-      if (w.sp < kStackLds) stk[w.sp * stride] = pr[k];
-      else if (w.sp < kStack4Total) ovf[w.sp - kStackLds] = pr[k];
-      w.sp = w.sp < kStack4Total ? w.sp + 1 : w.sp;
+    for (int k = 0; k < 3; k++) {
+      if (k < c) {
+        if (w.sp < kStackLds) stk[w.sp * stride] = pr[k];
+        else if (w.sp < kStack4Total) ovf[w.sp - kStackLds] = pr[k];
+        w.sp = w.sp < kStack4Total ? w.sp + 1 : w.sp;
+      }
     }
   }
   w.node = ref[best];
