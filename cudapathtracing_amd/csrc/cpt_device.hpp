@@ -326,6 +326,74 @@ __device__ __forceinline__ void loadRecord4(const float4 *__restrict__ recs, int
   }
 }
 
+// Next record of a four-wide walk from the internal children hit (h, their
+// entry distances tn).  They are ordered farthest first by entry distance
+// with a 5-comparator network (misses carry -inf and sort last).  Closest
+// hit: all of them are sorted, the m hit children are pr[0 .. m-1] and the
+// walk continues at pr[m-1], the nearest.  Occlusion: the walk continues at
+// the first nearest child in slot order (ties at tmin are common there, and
+// slot order finds occluders sooner) and only the others are sorted.  The
+// others are pushed farthest first, so pops come nearest first.  Returns
+// false when no internal child was hit.
+template <bool kAny>
+__device__ __forceinline__ bool descend4(const bool h[4], const float tn[4], const int ref[4], int *stk, int stride,
+                                         int *ovf, int &sp, int &node) {
+  const int m = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
+  if (m == 0) return false;
+  int best = 0;
+  if (kAny) {
+    float bt = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (h[k] && tn[k] < bt) {
+        bt = tn[k];
+        best = k;
+      }
+  }
+  float pt[4];
+  int pr[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    pt[k] = (h[k] && !(kAny && k == best)) ? tn[k] : -INFINITY;
+    pr[k] = ref[k];
+  }
+#define CPT_CSWAP(a, b)      \
+  if (pt[a] < pt[b]) {       \
+    const float tt_ = pt[a]; \
+    pt[a] = pt[b];           \
+    pt[b] = tt_;             \
+    const int tr_ = pr[a];   \
+    pr[a] = pr[b];           \
+    pr[b] = tr_;             \
+  }
+  CPT_CSWAP(0, 1)
+  CPT_CSWAP(2, 3)
+  CPT_CSWAP(0, 2)
+  CPT_CSWAP(1, 3)
+  CPT_CSWAP(1, 2)
+#undef CPT_CSWAP
+  const int c = m - 1;  // entries pushed (at most three)
+  if (sp + 3 <= kStackLds) {
+    // room in LDS for three: store all three, keep c of them
+    stk[sp * stride] = pr[0];
+    stk[(sp + 1) * stride] = pr[1];
+    stk[(sp + 2) * stride] = pr[2];
+    sp += c;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      if (k < c) {
+        if (sp < kStackLds) stk[sp * stride] = pr[k];
+        else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
+        sp = sp < kStack4Total ? sp + 1 : sp;
+      }
+    }
+  }
+  if (kAny) node = best == 0 ? ref[0] : best == 1 ? ref[1] : best == 2 ? ref[2] : ref[3];
+  else node = c == 0 ? pr[0] : c == 1 ? pr[1] : c == 2 ? pr[2] : pr[3];
+  return true;
+}
+
 template <bool kCount, bool kAny = false, bool kQuant = false>
 __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
                                              const float4 *__restrict__ leaves, int n, f3 o, f3 d,
@@ -387,64 +455,11 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
       }
     }
     // internal children: continue with the nearest, push the others
-    int best = -1;
-    float bt = INFINITY;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (h[k] && tn[k] < bt) {
-        bt = tn[k];
-        best = k;
-      }
-    if (best < 0) {
+    if (!descend4<kAny>(h, tn, ref, stk, stride, ovf, sp, node)) {
       if (sp == 0) break;
       sp--;
       node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
-      continue;
     }
-    // push the others farthest first, so the nearest is popped next
-    // (5-comparator network on (entry distance, ref), descending)
-    float pt[4];
-    int pr[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      pt[k] = (h[k] && k != best) ? tn[k] : -INFINITY;
-      pr[k] = ref[k];
-    }
-#define CPT_CSWAP(a, b)                                        \
-  if (pt[a] < pt[b]) {                                         \
-    const float tt_ = pt[a];                                   \
-    pt[a] = pt[b];                                             \
-    pt[b] = tt_;                                               \
-    const int tr_ = pr[a];                                     \
-    pr[a] = pr[b];                                             \
-    pr[b] = tr_;                                               \
-  }
-    CPT_CSWAP(0, 1)
-    CPT_CSWAP(2, 3)
-    CPT_CSWAP(0, 2)
-    CPT_CSWAP(1, 3)
-    CPT_CSWAP(1, 2)
-#undef CPT_CSWAP
-    // the pushed entries are pr[0 .. c-1] (the -inf ones sort last; at most
-    // three, the nearest is not pushed)
-    const int c = (pt[0] != -INFINITY) + (pt[1] != -INFINITY) + (pt[2] != -INFINITY);
-    if (sp + 3 <= kStackLds) {
-      // room in LDS for three: store all three, keep c of them
-      stk[sp * stride] = pr[0];
-      stk[(sp + 1) * stride] = pr[1];
-      stk[(sp + 2) * stride] = pr[2];
-      sp += c;
-    } else {
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        if (k < c) {
-          if (sp < kStackLds) stk[sp * stride] = pr[k];
-          else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
-          sp = sp < kStack4Total ? sp + 1 : sp;
-        }
-      }
-    }
-    node = ref[best];
   }
   return hit;
 }
@@ -650,59 +665,11 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
       if (kAny) return false;
     }
   }
-  int best = -1;
-  float bt = INFINITY;
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-    if (h[k] && tn[k] < bt) {
-      bt = tn[k];
-      best = k;
-    }
-  if (best < 0) {
+  if (!descend4<kAny>(h, tn, ref, stk, stride, ovf, w.sp, w.node)) {
     if (w.sp == 0) return false;
     w.sp--;
     w.node = w.sp < kStackLds ? stk[w.sp * stride] : ovf[w.sp - kStackLds];
-    return true;
   }
-  float pt[4];
-  int pr[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    pt[k] = (h[k] && k != best) ? tn[k] : -INFINITY;
-    pr[k] = ref[k];
-  }
-#define CPT_CSWAP(a, b)        \
-  if (pt[a] < pt[b]) {         \
-    const float tt_ = pt[a];   \
-    pt[a] = pt[b];             \
-    pt[b] = tt_;               \
-    const int tr_ = pr[a];     \
-    pr[a] = pr[b];             \
-    pr[b] = tr_;               \
-  }
-  CPT_CSWAP(0, 1)
-  CPT_CSWAP(2, 3)
-  CPT_CSWAP(0, 2)
-  CPT_CSWAP(1, 3)
-  CPT_CSWAP(1, 2)
-#undef CPT_CSWAP
-  const int c = (pt[0] != -INFINITY) + (pt[1] != -INFINITY) + (pt[2] != -INFINITY);
-  if (w.sp + 3 <= kStackLds) {
-    stk[w.sp * stride] = pr[0];
-    stk[(w.sp + 1) * stride] = pr[1];
-    stk[(w.sp + 2) * stride] = pr[2];
-    w.sp += c;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      if (k < c) {
-        if (w.sp < kStackLds) stk[w.sp * stride] = pr[k];
-        else if (w.sp < kStack4Total) ovf[w.sp - kStackLds] = pr[k];
-        w.sp = w.sp < kStack4Total ? w.sp + 1 : w.sp;
-      }
-    }
-  }
-  w.node = ref[best];
   return true;
 }
 
