@@ -82,6 +82,7 @@ def bench_render(cpt, L, scenes, args):
                bounces=int(frames[0].bounces), launches=int(frames[0].bounces) * 4,
                nodes_per_ray=stc.extend_nodes / max(stc.extend_rays, 1),
                prims_per_ray=stc.extend_prims / max(stc.extend_rays, 1),
+               lane_utilisation=stc.extend_nodes / max(stc.extend_wave_nodes, 1),
                achieved=achieved, build_ms=st_b.total_ms, mean=float(img.mean()))
     cpu = None
     if not args.no_cpu:
@@ -294,8 +295,8 @@ def bench_lbvh(cpt, scenes, args):
            "hit_fraction": round(hits, 4), "nodes_per_ray": round(n_nodes / args.rays, 2),
            "prims_per_ray": round(n_prims / args.rays, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_trace"),
-                        "kernel": "k_trace"}}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_trace_persist"),
+                        "kernel": "k_trace_persist"}}
     if not args.no_cpu:
         from oracle import oracle as O
         o = O.OracleScene(scene)
@@ -422,6 +423,7 @@ def main():
         "bounces": r["bounces"],
         "extend_nodes_per_ray": round(r["nodes_per_ray"], 2),
         "extend_prims_per_ray": round(r["prims_per_ray"], 2),
+        "extend_lane_utilisation": round(r["lane_utilisation"], 3),
         "scene_lbvh_build_ms": round(r["build_ms"], 3),
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_extend"),

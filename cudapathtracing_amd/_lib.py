@@ -75,7 +75,8 @@ class RenderStats(C.Structure):
     _fields_ = [("ms", C.c_double), ("rays", C.c_double), ("paths", C.c_double),
                 ("extend_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("bounces", C.c_int32), ("pad_", C.c_int32), ("extend_rays", C.c_double),
-                ("extend_nodes", C.c_double), ("extend_prims", C.c_double)]
+                ("extend_nodes", C.c_double), ("extend_prims", C.c_double),
+                ("extend_wave_nodes", C.c_double)]
 
 
 P = C.c_void_p

@@ -385,9 +385,18 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
     const int i = c0 + r * kRefitThreads + (int)threadIdx.x;
     if (i > c1) break;
     const uint32_t p = order[i];
-    leaves[3 * (int64_t)i + 0] = rec[3 * (int64_t)p + 0];
-    leaves[3 * (int64_t)i + 1] = rec[3 * (int64_t)p + 1];
-    leaves[3 * (int64_t)i + 2] = rec[3 * (int64_t)p + 2];
+    {
+      // triangles: store the edges the FP32 test forms from the vertices
+      const float4 q0 = rec[3 * (int64_t)p + 0];
+      float4 q1 = rec[3 * (int64_t)p + 1], q2 = rec[3 * (int64_t)p + 2];
+      if (__float_as_int(q1.w) == CPT_OBJ_TRIANGLE) {
+        q1 = make_float4(q1.x - q0.x, q1.y - q0.y, q1.z - q0.z, q1.w);
+        q2 = make_float4(q2.x - q0.x, q2.y - q0.y, q2.z - q0.z, q2.w);
+      }
+      leaves[3 * (int64_t)i + 0] = q0;
+      leaves[3 * (int64_t)i + 1] = q1;
+      leaves[3 * (int64_t)i + 2] = q2;
+    }
     float bx[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) bx[k] = box32[6 * (int64_t)p + k];

@@ -7,9 +7,11 @@
 //             [2] = A.min.z A.max.z B.min.z B.max.z
 //             [3] = refA refB 0 0 (int bits; ref >= 0 internal, ~k = leaf k)
 //   leaves  : N x 48 B in leaf (sorted Morton) order.  float4 x3:
-//             triangle: {v0, id} {v1, type} {v2, material}
+//             triangle: {v0, id} {v1 - v0, type} {v2 - v0, material} (the
+//                       edges are the FP32 differences the test would form)
 //             sphere  : {centre, id} {radius 0 0, type} {0 0 0, material}
 //             light   : {centre, id} {normal, type} {width height visible, mat}
+//   rec     : the same records in insertion order, triangles with v1 and v2
 // Child boxes are FP32, rounded outward from the reference's FP64 padded
 // boxes (Hittable.cuh:421,453,460-461), so every FP32 box contains the FP64
 // box the reference tests.
@@ -99,10 +101,9 @@ struct Rng {
 
 // Triangle::hit (Hittable.cuh:85-140) in FP32 with the reference's accept
 // rules: |det| < 1e-8 misses, u in [0,1], v >= 0, u+v <= 1, t in [tmin,tmax]
-// inclusive.
-__device__ __forceinline__ bool hitTri(f3 v0, f3 v1, f3 v2, f3 o, f3 d, float tmin, float tmax,
-                                       float &t) {
-  f3 e1 = v1 - v0, e2 = v2 - v0;
+// inclusive.  e1 = v1 - v0, e2 = v2 - v0.
+__device__ __forceinline__ bool hitTriE(f3 v0, f3 e1, f3 e2, f3 o, f3 d, float tmin, float tmax,
+                                        float &t) {
   f3 h = cross3(d, e2);
   float a = dot3(e1, h);
   if (fabsf(a) < 1e-8f) return false;
@@ -160,13 +161,31 @@ __device__ __forceinline__ bool hitQuad(f3 c, f3 n, float w, float hgt, bool vis
   return false;
 }
 
+// Primitive li of the leaf-order records (triangles carry edges).
 __device__ __forceinline__ bool hitLeaf(const float4 *__restrict__ leaves, int li, f3 o, f3 d,
                                         float tmin, float tmax, float &t) {
   const float4 r0 = leaves[3 * li + 0];
   const float4 r1 = leaves[3 * li + 1];
   const float4 r2 = leaves[3 * li + 2];
   const int type = __float_as_int(r1.w);
-  if (type == CPT_OBJ_TRIANGLE) return hitTri(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_TRIANGLE) return hitTriE(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_SPHERE) return hitSph(xyz(r0), r1.x, o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_LIGHT)
+    return hitQuad(xyz(r0), xyz(r1), r2.x, r2.y, r2.z != 0.0f, o, d, tmin, tmax, t);
+  return false;
+}
+
+// Primitive i of the insertion-order records (triangles carry vertices).
+__device__ __forceinline__ bool hitRec(const float4 *__restrict__ rec, int i, f3 o, f3 d, float tmin, float tmax,
+                                       float &t) {
+  const float4 r0 = rec[3 * i + 0];
+  const float4 r1 = rec[3 * i + 1];
+  const float4 r2 = rec[3 * i + 2];
+  const int type = __float_as_int(r1.w);
+  if (type == CPT_OBJ_TRIANGLE) {
+    const f3 v0 = xyz(r0);
+    return hitTriE(v0, xyz(r1) - v0, xyz(r2) - v0, o, d, tmin, tmax, t);
+  }
   if (type == CPT_OBJ_SPHERE) return hitSph(xyz(r0), r1.x, o, d, tmin, tmax, t);
   if (type == CPT_OBJ_LIGHT)
     return hitQuad(xyz(r0), xyz(r1), r2.x, r2.y, r2.z != 0.0f, o, d, tmin, tmax, t);

@@ -249,17 +249,16 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   a.lbuf[p] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Hit geometry from a leaf record: outward normal (Hittable.cuh:51,128,393).
+// Hit geometry from a leaf-order record (triangles carry their edges):
+// outward normal (Hittable.cuh:51,128,393).
 __device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
-  if (type == CPT_OBJ_TRIANGLE) {
-    f3 v0 = xyz(r0);
-    return unit3(cross3(xyz(r1) - v0, xyz(r2) - v0));
-  }
+  if (type == CPT_OBJ_TRIANGLE) return unit3(cross3(xyz(r1), xyz(r2)));
   if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
   return xyz(r1);
 }
 
-// kCount: also sums node visits and primitive tests into work[0], work[1]
+// kCount: also sums node visits and primitive tests into work[0], work[1],
+// and each wave's largest node-visit count into work[2]
 // kPerm: the queue is in ray-key order perm; lane k traces the path in state
 // slot perm[k] and moves its state to slot k of the *Out arrays (the
 // shading kernel then reads slot k there).
@@ -308,14 +307,19 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
     hit[2 * (int64_t)p + 1] = h1;
   }
   if (kCount) {
+    // sums over lanes, and the wave's longest walk (the wave runs that many
+    // record visits, so sum / (64 * max) is the SIMD lane utilisation)
     unsigned long long a = (unsigned long long)cnt.nodes, b = (unsigned long long)cnt.prims;
+    int mx = cnt.nodes;
     for (int off = 32; off > 0; off >>= 1) {
       a += __shfl_xor(a, off, 64);
       b += __shfl_xor(b, off, 64);
+      mx = max(mx, __shfl_xor(mx, off, 64));
     }
     if (__lane_id() == 0) {
       atomicAdd(work + 0, a);
       atomicAdd(work + 1, b);
+      atomicAdd(work + 2, (unsigned long long)mx);
     }
   }
 }
@@ -644,7 +648,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
   const f3 o = xyz(r0), d = xyz(r1);
   float tL;
-  if (!hitLeaf(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
+  if (!hitRec(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
   float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
   const int leaf =
@@ -813,7 +817,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   int bounces = 0;
   bool shadowTimed = false;  // events 16/17 bracket a shadow pass not yet added
   uint32_t *cnt = R.counters.as<uint32_t>();
-  // counters: [0] next queue, [1] shadow queue; [8..11] two u64 work sums
+  // counters: [0] next queue, [1] shadow queue; [8..13] three u64 work sums
   unsigned long long *work = (unsigned long long *)(cnt + 8);
   const bool countWork = (prm->flags & CPT_RENDER_COUNT_WORK) != 0;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
@@ -984,10 +988,11 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     stats->shadow_ms = sdMs;
     stats->bounces = bounces;
     stats->extend_rays = extRays;
-    unsigned long long wk[2] = {0, 0};
-    if (countWork) CPT_TRY_HIP(ctx, hipMemcpy(wk, work, 16, hipMemcpyDeviceToHost));
+    unsigned long long wk[3] = {0, 0, 0};
+    if (countWork) CPT_TRY_HIP(ctx, hipMemcpy(wk, work, 24, hipMemcpyDeviceToHost));
     stats->extend_nodes = (double)wk[0];
     stats->extend_prims = (double)wk[1];
+    stats->extend_wave_nodes = 64.0 * (double)wk[2];
   }
   return CPT_OK;
 }

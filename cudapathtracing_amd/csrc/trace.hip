@@ -88,60 +88,111 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   if (idOut) idOut[i] = id;
 }
 
-// Persistent variant of k_trace<kCount, true>: a fixed grid of waves takes
-// rays from a global counter, and a lane whose walk is over takes the next
-// ray as soon as kRefill lanes of its wave are idle, so lanes are not left
-// waiting for the longest walk of their wave.  Every wave leaves once the
-// counter has passed nRays and its lanes are idle.
-constexpr int kRefill = 16;
+// Persistent variant of k_trace over four-wide records: a fixed grid of
+// waves; a lane whose walk is over takes the next ray, so lanes are not left
+// waiting for the longest walk of their wave.
+//  * A wave takes kChunk rays per grab of the global counter and issues the
+//    grab of its next chunk as soon as it starts one, so no lane ever waits
+//    for the counter.
+//  * A lane takes its ray in stages, one loop trip each: claim (the
+//    permutation entry is loaded), load (the ray is loaded), start (the walk
+//    is set up).  The loads of one trip complete while the walk steps of the
+//    same trip wait for their records.
+//  * A finished lane stores the hit's primitive id (and, counting, its
+//    counts); k_refine_t computes t in FP64 afterwards with full waves.
+// Every wave leaves once a grab has passed nRays and its lanes are idle.
+constexpr int kChunk = 256;
+enum LaneState { kIdle = 0, kClaimed = 1, kLoading = 2, kActive = 3 };
 
 template <bool kCount, bool kQuant>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
-    const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, Scene64 s64, float *__restrict__ tOut,
-    int32_t *__restrict__ idOut, int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut,
-    uint32_t *__restrict__ work) {
+    const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, int32_t *__restrict__ idOut,
+    int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut, uint32_t *__restrict__ work) {
   __shared__ int stack[kStackLds * kTraceThreads];
   int *stk = stack + threadIdx.x;
   int ovf[kStack4Total - kStackLds];
   Walk4 w;
   TraceCounters cnt;
-  float4 r0 = make_float4(0, 0, 0, 0), r1 = r0;
+  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
   int64_t ri = 0;
-  bool active = false, exhausted = false;
+  uint32_t pv = 0;
+  int st = kIdle;
+  const uint32_t lane = __lane_id();
+  // the wave's chunk [cur, end) and its next grab (held by lane 0)
+  uint64_t cur = 0, end = 0;
+  bool more = true;
+  uint32_t ticket = 0;
+  if (lane == 0) ticket = atomicAdd(work, (uint32_t)kChunk);
   while (true) {
-    if (!exhausted && __popcll(__ballot(!active)) >= kRefill) {
-      const uint64_t item = waveFetch(work, !active, (uint64_t)nRays, exhausted);
-      if (!active && item < (uint64_t)nRays) {
-        ri = perm ? (int64_t)perm[item] : (int64_t)item;
-        r0 = rays[2 * ri];
-        r1 = rays[2 * ri + 1];
-        walk4Init(w, xyz(r0), xyz(r1), r0.w, r1.w);
-        cnt = TraceCounters();
-        active = true;
+    // rays loaded in the last trip start their walk
+    if (st == kLoading) {
+      walk4Init(w, xyz(r0), xyz(r1), r0.w, r1.w);
+      cnt = TraceCounters();
+      st = kActive;
+    }
+    // claimed rays whose permutation entry has arrived are loaded
+    if (st == kClaimed) {
+      ri = (int64_t)pv;
+      r0 = rays[2 * ri];
+      r1 = rays[2 * ri + 1];
+      st = kLoading;
+    }
+    // idle lanes claim the next rays of the wave's chunk
+    const uint64_t idle = __ballot(st == kIdle);
+    if (idle && more) {
+      if (cur >= end) {
+        const uint32_t base = __shfl(ticket, 0, 64);
+        if ((int64_t)base >= nRays) {
+          more = false;
+        } else {
+          cur = base;
+          end = (uint64_t)min((int64_t)base + kChunk, nRays);
+          if (lane == 0) ticket = atomicAdd(work, (uint32_t)kChunk);
+        }
+      }
+      if (more) {
+        const uint64_t avail = end - cur;
+        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+        if (st == kIdle && rank < avail) {
+          const uint64_t item = cur + rank;
+          if (perm) {
+            pv = perm[item];
+            st = kClaimed;
+          } else {
+            ri = (int64_t)item;
+            r0 = rays[2 * ri];
+            r1 = rays[2 * ri + 1];
+            st = kLoading;
+          }
+        }
+        cur += min((uint64_t)__popcll(idle), avail);
       }
     }
-    if (__ballot(active) == 0) {
-      if (exhausted) break;
-      continue;
-    }
-    if (active && !walk4Step<kCount, false, kQuant>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
-      active = false;
+    if (!more && __ballot(st != kIdle) == 0) break;
+    // one record visit of every walking lane
+    if (st == kActive && !walk4Step<kCount, false, kQuant>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
+      st = kIdle;
       if (kCount) {
         nodesOut[ri] = cnt.nodes;
         primsOut[ri] = cnt.prims;
       } else {
-        int id = -1;
-        float t = r1.w;
-        if (w.hit >= 0) {
-          id = __float_as_int(leaves[3 * (int64_t)w.hit].w);
-          t = (float)refineT(s64, id, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, r1.w);
-        }
-        if (tOut) tOut[ri] = t;
-        if (idOut) idOut[ri] = id;
+        idOut[ri] = w.hit >= 0 ? __float_as_int(leaves[3 * (int64_t)w.hit].w) : -1;
       }
     }
   }
+}
+
+// t of every ray from its primitive id (FP64, as k_trace does); misses keep
+// tmax.
+__global__ __launch_bounds__(kTraceThreads) void k_refine_t(const float4 *__restrict__ rays, int64_t nRays,
+                                                             Scene64 s64, const int32_t *__restrict__ id,
+                                                             float *__restrict__ tOut) {
+  const int64_t i = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
+  if (i >= nRays) return;
+  const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+  const int32_t h = id[i];
+  tOut[i] = h >= 0 ? (float)refineT(s64, h, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z, r0.w, r1.w) : r1.w;
 }
 
 __global__ void k_zero_work(uint32_t *p) {
@@ -191,7 +242,8 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   if ((rc = ensureRecords(ctx))) return rc;
   const int fmt = recordFormat(ctx);
   const float4 *rec4 = fmt == 3 ? ctx->nodes8.as<float4>() : fmt == 2 ? ctx->nodes4q.as<float4>() : n4;
-  if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1) {
+  if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1 && (nodesDev || (tDev && idDev)) &&
+      n < ((int64_t)1 << 32) - kChunk * 65536) {
     if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
     uint32_t *work = ctx->cntA.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
@@ -203,11 +255,14 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
     const unsigned pgrid = std::min<unsigned>(grid, (unsigned)ctx->numCU * (unsigned)std::max(1, perCU));
 #define CPT_LAUNCH_P(C, Q)                                                                                         \
   hipLaunchKernelGGL((k_trace_persist<C, Q>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, rec4, lv, \
-                     s64, tDev, idDev, nodesDev, primsDev, work)
+                     idDev, nodesDev, primsDev, work)
     if (nodesDev) {
       if (fmt == 2) CPT_LAUNCH_P(true, true); else CPT_LAUNCH_P(true, false);
     } else {
       if (fmt == 2) CPT_LAUNCH_P(false, true); else CPT_LAUNCH_P(false, false);
+      CPT_TRY_HIP(ctx, hipGetLastError());
+      hipLaunchKernelGGL(k_refine_t, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, n, s64,
+                         (const int32_t *)idDev, tDev);
     }
 #undef CPT_LAUNCH_P
   } else {

@@ -137,9 +137,9 @@ const char *cpt_version(void);
  * results are written in the caller's order.  0 disables it; default 2^20. */
 #define CPT_OPT_TRAVERSAL_WIDTH 1
 #define CPT_OPT_RAY_SORT_MIN 2
-/* CPT_OPT_PERSISTENT: 1 runs four-wide cpt_trace_closest traversals as
- * persistent kernels whose lanes take a new ray when theirs is finished; 0
- * (default) launches one lane per ray.  Results are identical. */
+/* CPT_OPT_PERSISTENT: 1 (default) runs four-wide cpt_trace_closest
+ * traversals as persistent kernels whose lanes take a new ray when theirs is
+ * finished; 0 launches one lane per ray.  Results are identical. */
 #define CPT_OPT_PERSISTENT 3
 /* CPT_OPT_QUANTIZED: 1 makes four-wide walks read 64 B records whose child
  * boxes are 8-bit planes on a power-of-two grid, rounded outward (each
@@ -289,6 +289,9 @@ typedef struct cpt_render_stats {
   double extend_rays;  /* extension rays traced                        */
   double extend_nodes; /* nodes visited by extension rays (COUNT_WORK) */
   double extend_prims; /* primitives tested by extension rays (ditto)  */
+  double extend_wave_nodes; /* 64 x each wave's largest node count, summed
+                               (ditto): extend_nodes / extend_wave_nodes is
+                               the lane utilisation of the walk */
 } cpt_render_stats;
 
 /* render<<<>>> (Render.cu:16-261) as a wavefront pipeline.  rgb: h*w*3
