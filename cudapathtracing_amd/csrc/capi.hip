@@ -190,6 +190,9 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
     return fail(ctx, CPT_ERR_ARG, "cpt_scene_upload", "negative count or missing array");
   const int64_t n = (int64_t)nl + ns + nt;
   if (n > (int64_t)0x7ffffffe) return fail(ctx, CPT_ERR_ARG, "cpt_scene_upload", "more than 2^31-2 primitives");
+  // the path tracer packs material << 3 | object type into one word
+  if ((int64_t)nm + nl + 1 > ((int64_t)1 << 28))
+    return fail(ctx, CPT_ERR_ARG, "cpt_scene_upload", "more than 2^28 materials and lights");
   for (int i = 0; i < ns; i++)
     if (spheres[i].material < 0 || spheres[i].material >= (nm > 0 ? nm : 1))
       return fail(ctx, CPT_ERR_ARG, "cpt_scene_upload", "sphere material index out of range");

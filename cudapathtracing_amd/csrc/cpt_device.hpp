@@ -161,6 +161,17 @@ __device__ __forceinline__ bool hitQuad(f3 c, f3 n, float w, float hgt, bool vis
   return false;
 }
 
+// A leaf-order record {r0, r1, r2} (triangles carry edges).
+__device__ __forceinline__ bool hitRecord(float4 r0, float4 r1, float4 r2, f3 o, f3 d, float tmin, float tmax,
+                                          float &t) {
+  const int type = __float_as_int(r1.w);
+  if (type == CPT_OBJ_TRIANGLE) return hitTriE(xyz(r0), xyz(r1), xyz(r2), o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_SPHERE) return hitSph(xyz(r0), r1.x, o, d, tmin, tmax, t);
+  if (type == CPT_OBJ_LIGHT)
+    return hitQuad(xyz(r0), xyz(r1), r2.x, r2.y, r2.z != 0.0f, o, d, tmin, tmax, t);
+  return false;
+}
+
 // Primitive li of the leaf-order records (triangles carry edges).
 __device__ __forceinline__ bool hitLeaf(const float4 *__restrict__ leaves, int li, f3 o, f3 d,
                                         float tmin, float tmax, float &t) {
@@ -190,6 +201,14 @@ __device__ __forceinline__ bool hitRec(const float4 *__restrict__ rec, int i, f3
   if (type == CPT_OBJ_LIGHT)
     return hitQuad(xyz(r0), xyz(r1), r2.x, r2.y, r2.z != 0.0f, o, d, tmin, tmax, t);
   return false;
+}
+
+// Hit geometry from a leaf-order record (triangles carry their edges):
+// outward normal (Hittable.cuh:51,128,393).
+__device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
+  if (type == CPT_OBJ_TRIANGLE) return unit3(cross3(xyz(r1), xyz(r2)));
+  if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
+  return xyz(r1);
 }
 
 struct TraceCounters {
@@ -628,26 +647,47 @@ struct Walk4 {
   f3 o, d, inv, ood;
   float tmin, tbest;
   int node, sp, hit;
+  // hit payload (walk4Step<..., kPayload>): the best hit's outward normal,
+  // and material << 3 | type, set from its record when the hit is found
+  f3 nrm;
+  int tm;
 };
 
-__device__ __forceinline__ void walk4Init(Walk4 &w, f3 o, f3 d, float tmin, float tmax) {
+// Starts the walk of the ray already in w.o, w.d, w.tmin, w.tbest (= tmax).
+__device__ __forceinline__ void walk4Start(Walk4 &w) {
   const float eps = 1e-20f;
+  const f3 d = w.d;
   const f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
                      fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
-  w.o = o;
-  w.d = d;
   w.inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
-  w.ood = mkf3(o.x * w.inv.x, o.y * w.inv.y, o.z * w.inv.z);
-  w.tmin = tmin;
-  w.tbest = tmax;
+  w.ood = mkf3(w.o.x * w.inv.x, w.o.y * w.inv.y, w.o.z * w.inv.z);
   w.node = 0;
   w.sp = 0;
   w.hit = -1;
+  w.nrm = mkf3(0.f, 0.f, 0.f);
+  w.tm = 0;
+}
+
+// The ray {o, tmin} {d, tmax} into w (its walk starts with walk4Start).
+__device__ __forceinline__ void walk4Ray(Walk4 &w, float4 r0, float4 r1) {
+  w.o = xyz(r0);
+  w.d = xyz(r1);
+  w.tmin = r0.w;
+  w.tbest = r1.w;
+}
+
+__device__ __forceinline__ void walk4Init(Walk4 &w, f3 o, f3 d, float tmin, float tmax) {
+  w.o = o;
+  w.d = d;
+  w.tmin = tmin;
+  w.tbest = tmax;
+  walk4Start(w);
 }
 
 // Visits record w.node.  Returns false when the walk is over (w.hit, w.tbest
-// hold the result; kAny: the first hit ends it).
-template <bool kCount, bool kAny, bool kQuant = false>
+// hold the result; kAny: the first hit ends it).  kPayload: a new best hit
+// also sets w.nrm, w.type, w.mat from its record.
+template <bool kCount, bool kAny, bool kQuant = false, bool kPayload = false>
 __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ nodes4,
                                           const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
                                           TraceCounters &cnt) {
@@ -678,7 +718,16 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
     const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
     float t;
     if (kCount) cnt.prims++;
-    if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
+    if (kPayload) {
+      const float4 q0 = leaves[3 * li + 0], q1 = leaves[3 * li + 1], q2 = leaves[3 * li + 2];
+      if (hitRecord(q0, q1, q2, w.o, w.d, w.tmin, w.tbest, t)) {
+        w.tbest = t;
+        w.hit = li;
+        const int type = __float_as_int(q1.w);
+        w.tm = (__float_as_int(q2.w) << 3) | type;
+        w.nrm = outwardNormal(q0, q1, q2, type, w.o + w.d * t);
+      }
+    } else if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
       w.tbest = t;
       w.hit = li;
       if (kAny) return false;
@@ -690,6 +739,52 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
     w.node = w.sp < kStackLds ? stk[w.sp * stride] : ovf[w.sp - kStackLds];
   }
   return true;
+}
+
+// Persistent walks: a wave takes kChunk items per grab of a global counter;
+// a lane moves through these states, one loop trip each, as it takes an item.
+constexpr int kChunk = 256;
+enum LaneState { kIdle = 0, kClaimed = 1, kLoading = 2, kActive = 3 };
+
+// Wave-level item claiming for persistent walks.  The wave holds a chunk
+// [cur, end) of items and, in lane 0's `ticket`, the result of the grab for
+// its next chunk, issued when the current one was taken so that no lane
+// waits for the counter.  Every idle lane gets the next item of the chunk
+// (returns true, item set) while the chunk lasts.  `more` turns false once a
+// grab has passed n.  Call with the whole wave active (the caller's loop is
+// wave-uniform).
+struct WaveChunk {
+  uint32_t cur = 0, end = 0;  // wave-uniform
+  uint32_t ticket = 0;
+  bool more = true;
+};
+
+__device__ __forceinline__ void waveChunkInit(WaveChunk &c, uint32_t *counter) {
+  if (__lane_id() == 0) c.ticket = atomicAdd(counter, (uint32_t)kChunk);
+}
+
+// n < 2^32 - 2^24 (the counter overshoots n by at most two grabs per wave).
+__device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint32_t n, bool idle, uint32_t &item) {
+  const uint64_t im = __ballot(idle);
+  if (!im || !c.more) return false;
+  const uint32_t lane = __lane_id();
+  if (c.cur >= c.end) {
+    const uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(c.ticket, 0, 64));
+    if (base >= n) {
+      c.more = false;
+      return false;
+    }
+    c.cur = base;
+    c.end = n - base > (uint32_t)kChunk ? base + (uint32_t)kChunk : n;
+    if (lane == 0) c.ticket = atomicAdd(counter, (uint32_t)kChunk);
+  }
+  const uint32_t avail = c.end - c.cur;
+  const uint32_t nIdle = (uint32_t)__popcll(im);
+  const uint32_t rank = (uint32_t)__popcll(im & ((1ull << lane) - 1ull));
+  const bool got = idle && rank < avail;
+  item = c.cur + rank;
+  c.cur += nIdle < avail ? nIdle : avail;
+  return got;
 }
 
 // Wave-aggregated fetch of work items for the lanes with `want`: returns the

@@ -249,19 +249,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   a.lbuf[p] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Hit geometry from a leaf-order record (triangles carry their edges):
-// outward normal (Hittable.cuh:51,128,393).
-__device__ __forceinline__ f3 outwardNormal(float4 r0, float4 r1, float4 r2, int type, f3 hp) {
-  if (type == CPT_OBJ_TRIANGLE) return unit3(cross3(xyz(r1), xyz(r2)));
-  if (type == CPT_OBJ_SPHERE) return (hp - xyz(r0)) * (1.0f / r1.x);
-  return xyz(r1);
-}
 
 // kCount: also sums node visits and primitive tests into work[0], work[1],
 // and each wave's largest node-visit count into work[2]
-// kPerm: the queue is in ray-key order perm; lane k traces the path in state
-// slot perm[k] and moves its state to slot k of the *Out arrays (the
-// shading kernel then reads slot k there).
+// kPerm: the queue is traced in ray-key order perm: lane k traces the path
+// in slot perm[k] and writes its hit record at that slot, so the shading
+// kernel reads every array at the same slot.
 template <bool kCount, int kFmt, bool kPerm>
 __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const float4 *__restrict__ ray,
@@ -269,24 +262,13 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const float4 *__restrict__ leaves, int nPrims,
                                                             float4 *__restrict__ hit,
                                                             unsigned long long *__restrict__ work, int wide,
-                                                            const uint32_t *__restrict__ perm,
-                                                            const float4 *__restrict__ thr,
-                                                            const uint4 *__restrict__ meta,
-                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
-                                                            uint4 *__restrict__ metaOut) {
+                                                            const uint32_t *__restrict__ perm) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   TraceCounters cnt;
   if (k < count) {
-    const uint32_t src = kPerm ? perm[k] : k;
-    const uint32_t p = k;
-    const float4 r0 = ray[2 * (int64_t)src], r1 = ray[2 * (int64_t)src + 1];
-    if (kPerm) {
-      rayOut[2 * (int64_t)p] = r0;
-      rayOut[2 * (int64_t)p + 1] = r1;
-      thrOut[p] = thr[src];
-      metaOut[p] = meta[src];
-    }
+    const uint32_t p = kPerm ? perm[k] : k;
+    const float4 r0 = ray[2 * (int64_t)p], r1 = ray[2 * (int64_t)p + 1];
     float tb = r1.w;
     const int leaf = traceFmt<kCount, kFmt>(nodes, leaves, nPrims, xyz(r0), xyz(r1), r0.w, tb, stack + threadIdx.x,
                                             kRenderThreads, cnt);
@@ -320,6 +302,77 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
       atomicAdd(work + 0, a);
       atomicAdd(work + 1, b);
       atomicAdd(work + 2, (unsigned long long)mx);
+    }
+  }
+}
+
+// Persistent k_extend for the four-wide formats (kQuant: 64 B records): a
+// fixed grid of waves whose lanes take the next path of the queue as soon as
+// their walk is over (claimed in chunks, loads staged one loop trip ahead of
+// use), so that lanes do not wait for the longest walk of their wave.  The
+// walk keeps the best hit's type, material and outward normal, so a finished
+// lane writes its hit record without a further load.  Same outputs as
+// k_extend<kCount, 1 or 2, kPerm>; counting sums the walks and, in work[2],
+// the loop trips in which the wave walked.  grab: a zeroed counter.
+template <bool kCount, bool kQuant, bool kPerm>
+__global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
+    uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ nodes4,
+    const float4 *__restrict__ leaves, float4 *__restrict__ hit, unsigned long long *__restrict__ work,
+    const uint32_t *__restrict__ perm, uint32_t *grab) {
+  __shared__ int stack[kStackLds * kRenderThreads];
+  int *stk = stack + threadIdx.x;
+  int ovf[kStack4Total - kStackLds];
+  // a lane's ray is loaded straight into its (idle) walk state
+  Walk4 w;
+  walk4Init(w, mkf3(0.f, 0.f, 0.f), mkf3(1.f, 1.f, 1.f), 0.f, 0.f);
+  TraceCounters cnt;
+  uint32_t p = 0;
+  int st = kIdle;
+  unsigned long long trips = 0;
+  WaveChunk ch;
+  waveChunkInit(ch, grab);
+  while (true) {
+    // paths loaded in the last trip start their walk
+    if (st == kLoading) {
+      walk4Start(w);
+      st = kActive;
+    }
+    // claimed paths whose queue slot has arrived are loaded
+    if (kPerm && st == kClaimed) {
+      walk4Ray(w, ray[2 * (int64_t)p], ray[2 * (int64_t)p + 1]);
+      st = kLoading;
+    }
+    uint32_t item = 0;
+    if (waveClaim(ch, grab, count, st == kIdle, item)) {
+      if (kPerm) {
+        p = perm[item];
+        st = kClaimed;
+      } else {
+        p = item;
+        walk4Ray(w, ray[2 * (int64_t)p], ray[2 * (int64_t)p + 1]);
+        st = kLoading;
+      }
+    }
+    if (!ch.more && __ballot(st != kIdle) == 0) break;
+    if (kCount && __ballot(st == kActive)) trips++;
+    if (st == kActive &&
+        !walk4Step<kCount, false, kQuant, true>(w, nodes4, leaves, stk, kRenderThreads, ovf, cnt)) {
+      hit[2 * (int64_t)p] =
+          make_float4(__int_as_float(w.hit), w.tbest, __int_as_float(w.tm & 7), __int_as_float(w.tm >> 3));
+      hit[2 * (int64_t)p + 1] = make_float4(w.nrm.x, w.nrm.y, w.nrm.z, 0.f);
+      st = kIdle;
+    }
+  }
+  if (kCount) {
+    unsigned long long a = (unsigned long long)cnt.nodes, b = (unsigned long long)cnt.prims;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+    }
+    if (__lane_id() == 0) {
+      atomicAdd(work + 0, a);
+      atomicAdd(work + 1, b);
+      atomicAdd(work + 2, trips);
     }
   }
 }
@@ -815,9 +868,20 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
 
   double rays = 0, extMs = 0, shMs = 0, sdMs = 0, extRays = 0;
   int bounces = 0;
+  // four-wide formats trace bounce queues with the persistent extension
+  // kernel, on one resident set of workgroups
+  const bool extPersist = ctx->persistent && (a.wide == 1 || a.wide == 2) && a.nPrims > 1;
+  unsigned extGrid = 0;
+  if (extPersist) {
+    int perCU = 0;
+    CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &perCU, (const void *)k_extend_persist<false, false, true>, kRenderThreads, 0));
+    extGrid = (unsigned)ctx->numCU * (unsigned)std::max(1, perCU);
+  }
   bool shadowTimed = false;  // events 16/17 bracket a shadow pass not yet added
   uint32_t *cnt = R.counters.as<uint32_t>();
-  // counters: [0] next queue, [1] shadow queue; [8..13] three u64 work sums
+  // counters: [0] next queue, [1] shadow queue, [2] volume queue, [3] grab
+  // counter of the persistent extension kernel; [8..13] three u64 work sums
   unsigned long long *work = (unsigned long long *)(cnt + 8);
   const bool countWork = (prm->flags & CPT_RENDER_COUNT_WORK) != 0;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
@@ -838,17 +902,13 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
       CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
       {
-        // with a pending ray-key order the extension kernel moves the state
-        // into that order (set cur ^ 1), which then becomes the current set
-        const int dst = cur ^ 1;
+        // with a pending ray-key order the extension kernel traces the queue
+        // in that order; hit records stay at the paths' own slots
         const float4 *rayIn = R.ray[cur].as<float4>();
-        const float4 *thrIn = R.thr[cur].as<float4>();
-        const uint4 *metaIn = R.meta[cur].as<uint4>();
         const uint32_t *pm = permPending ? R.permA.as<uint32_t>() : nullptr;
 #define CPT_EXT(C, F, P)                                                                                      \
   hipLaunchKernelGGL((k_extend<C, F, P>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes,     \
-                     a.leaves, a.nPrims, R.hit.as<float4>(), work, a.wide, pm, thrIn, metaIn, R.ray[dst].as<float4>(), \
-                     R.thr[dst].as<float4>(), R.meta[dst].as<uint4>())
+                     a.leaves, a.nPrims, R.hit.as<float4>(), work, a.wide, pm)
 #define CPT_EXT_F(C, P)                  \
   switch (a.wide) {                      \
     case 0: CPT_EXT(C, 0, P); break;     \
@@ -856,14 +916,30 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     case 2: CPT_EXT(C, 2, P); break;     \
     default: CPT_EXT(C, 3, P); break;    \
   }
-        if (countWork) {
+#define CPT_EXT_P(C, Q, P)                                                                                  \
+  hipLaunchKernelGGL((k_extend_persist<C, Q, P>), dim3(std::min<unsigned>(blocksFor(qn), extGrid)),             \
+                     dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes, a.leaves, R.hit.as<float4>(), work, pm, cnt + 3)
+        if (extPersist) {
+          // counters[3] (cleared with the queue counters) is the grab counter
+          const int q = a.wide == 2 ? 1 : 0;
+          switch ((countWork ? 4 : 0) + q * 2 + (permPending ? 1 : 0)) {
+            case 0: CPT_EXT_P(false, false, false); break;
+            case 1: CPT_EXT_P(false, false, true); break;
+            case 2: CPT_EXT_P(false, true, false); break;
+            case 3: CPT_EXT_P(false, true, true); break;
+            case 4: CPT_EXT_P(true, false, false); break;
+            case 5: CPT_EXT_P(true, false, true); break;
+            case 6: CPT_EXT_P(true, true, false); break;
+            default: CPT_EXT_P(true, true, true); break;
+          }
+        } else if (countWork) {
           if (permPending) CPT_EXT_F(true, true) else CPT_EXT_F(true, false)
         } else {
           if (permPending) CPT_EXT_F(false, true) else CPT_EXT_F(false, false)
         }
+#undef CPT_EXT_P
 #undef CPT_EXT_F
 #undef CPT_EXT
-        if (permPending) cur = dst;
         permPending = false;
       }
       const int nxt = cur ^ 1;

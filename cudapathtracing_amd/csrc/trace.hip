@@ -101,8 +101,6 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
 //  * A finished lane stores the hit's primitive id (and, counting, its
 //    counts); k_refine_t computes t in FP64 afterwards with full waves.
 // Every wave leaves once a grab has passed nRays and its lanes are idle.
-constexpr int kChunk = 256;
-enum LaneState { kIdle = 0, kClaimed = 1, kLoading = 2, kActive = 3 };
 
 template <bool kCount, bool kQuant>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
@@ -112,64 +110,39 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
   __shared__ int stack[kStackLds * kTraceThreads];
   int *stk = stack + threadIdx.x;
   int ovf[kStack4Total - kStackLds];
+  // a lane's ray is loaded straight into its (idle) walk state
   Walk4 w;
+  walk4Init(w, mkf3(0.f, 0.f, 0.f), mkf3(1.f, 1.f, 1.f), 0.f, 0.f);
   TraceCounters cnt;
-  float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
   int64_t ri = 0;
-  uint32_t pv = 0;
   int st = kIdle;
-  const uint32_t lane = __lane_id();
-  // the wave's chunk [cur, end) and its next grab (held by lane 0)
-  uint64_t cur = 0, end = 0;
-  bool more = true;
-  uint32_t ticket = 0;
-  if (lane == 0) ticket = atomicAdd(work, (uint32_t)kChunk);
+  WaveChunk ch;
+  waveChunkInit(ch, work);
   while (true) {
     // rays loaded in the last trip start their walk
     if (st == kLoading) {
-      walk4Init(w, xyz(r0), xyz(r1), r0.w, r1.w);
+      walk4Start(w);
       cnt = TraceCounters();
       st = kActive;
     }
     // claimed rays whose permutation entry has arrived are loaded
     if (st == kClaimed) {
-      ri = (int64_t)pv;
-      r0 = rays[2 * ri];
-      r1 = rays[2 * ri + 1];
+      walk4Ray(w, rays[2 * ri], rays[2 * ri + 1]);
       st = kLoading;
     }
     // idle lanes claim the next rays of the wave's chunk
-    const uint64_t idle = __ballot(st == kIdle);
-    if (idle && more) {
-      if (cur >= end) {
-        const uint32_t base = __shfl(ticket, 0, 64);
-        if ((int64_t)base >= nRays) {
-          more = false;
-        } else {
-          cur = base;
-          end = (uint64_t)min((int64_t)base + kChunk, nRays);
-          if (lane == 0) ticket = atomicAdd(work, (uint32_t)kChunk);
-        }
-      }
-      if (more) {
-        const uint64_t avail = end - cur;
-        const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
-        if (st == kIdle && rank < avail) {
-          const uint64_t item = cur + rank;
-          if (perm) {
-            pv = perm[item];
-            st = kClaimed;
-          } else {
-            ri = (int64_t)item;
-            r0 = rays[2 * ri];
-            r1 = rays[2 * ri + 1];
-            st = kLoading;
-          }
-        }
-        cur += min((uint64_t)__popcll(idle), avail);
+    uint32_t item = 0;
+    if (waveClaim(ch, work, (uint32_t)nRays, st == kIdle, item)) {
+      if (perm) {
+        ri = (int64_t)perm[item];
+        st = kClaimed;
+      } else {
+        ri = (int64_t)item;
+        walk4Ray(w, rays[2 * ri], rays[2 * ri + 1]);
+        st = kLoading;
       }
     }
-    if (!more && __ballot(st != kIdle) == 0) break;
+    if (!ch.more && __ballot(st != kIdle) == 0) break;
     // one record visit of every walking lane
     if (st == kActive && !walk4Step<kCount, false, kQuant>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
       st = kIdle;

@@ -53,14 +53,17 @@ def test_sky_over_floor_exact(ctx):
     assert st.rays > 0 and st.bounces >= 2
 
 
-@pytest.mark.parametrize("width,quant,qsort", [(4, False, 0), (4, False, 1), (2, False, 0), (4, True, 0),
-                                               (8, False, 0)])
-def test_cornell_lambert_paths(ctx, width, quant, qsort):
-    print(f"[gpu] render cornell paths (width {width}, quantized {quant}, queue sort {qsort})", flush=True)
+@pytest.mark.parametrize("width,quant,qsort,persist", [(4, False, 0, 1), (4, False, 1, 1), (4, True, 1, 1),
+                                                       (4, False, 1, 0), (2, False, 0, 1), (4, True, 0, 0),
+                                                       (8, False, 0, 1)])
+def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
+    print(f"[gpu] render cornell paths (width {width}, quantized {quant}, queue sort {qsort}, "
+          f"persistent {persist})", flush=True)
     s = scenes.cornell_icosphere()
     d = scenes.default_camera(48, 48)
     ctx.set_traversal_width(width)
     ctx.set_quantized(quant)
+    ctx.set_persistent(bool(persist))
     if qsort:  # bounce and shadow queues both traced in key order
         ctx.set_queue_sort_min(1)
         ctx.set_shadow_sort_min(1)
@@ -69,6 +72,7 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort):
     finally:
         ctx.set_traversal_width(4)
         ctx.set_quantized(False)
+        ctx.set_persistent(True)
         ctx.set_queue_sort_min(1 << 20)
         ctx.set_shadow_sort_min(1 << 20)
     print(f"  cornell path agreement {f:.5f}")
