@@ -528,6 +528,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     for (int li = 0; li < nLoop; li++) {
       bool want = false;
       f3 dir = mkf3(0, 0, 0), c = dir;
+      float tL = 0.f;
       if (neeHere) {
         const float4 l0 = a.rec[3 * li], l1 = a.rec[3 * li + 1], l2 = a.rec[3 * li + 2];
         const f3 lc = xyz(l0), ln = xyz(l1);
@@ -541,7 +542,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         const f3 toL = lp - hp;
         dir = unit3(toL);
         const float cl = dot3(-dir, ln);
-        if ((inMedium || dot3(dir, n) > 0.0f) && cl > 0.0f && l2.z != 0.0f) {
+        // the shadow ray's own distance to the light (Light::hit,
+        // Hittable.cuh:367-398): a ray that misses it in FP32 carries nothing
+        if ((inMedium || dot3(dir, n) > 0.0f) && cl > 0.0f && l2.z != 0.0f &&
+            hitQuad(lc, ln, l2.x, l2.y, true, hp, dir, kRayTMin, kRayTMax, tL)) {
           const ShadeMat lm = loadMat(a.mats, __float_as_int(l2.w));
           // surface: fr cos; medium: albedo times the isotropic phase 1/(4 pi)
           const f3 f = inMedium ? alb * (0.25f / kPi) : bsdfEval(m, V, dir, n, mkf3(0, 0, 0), front) * dot3(dir, n);
@@ -555,9 +559,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         shadow[4 * (int64_t)slot + 0] = make_float4(hp.x, hp.y, hp.z, kRayTMin);
         shadow[4 * (int64_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, kRayTMax);
         shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)mt.x));
-        // light id, and the stream of the transmittance estimate
+        // light id, the stream of the transmittance estimate, distance to the light
         shadow[4 * (int64_t)slot + 3] =
-            make_float4(__int_as_float(li), __uint_as_float(mt.y), __uint_as_float(rng.cnt), 0.f);
+            make_float4(__int_as_float(li), __uint_as_float(mt.y), __uint_as_float(rng.cnt), tL);
       }
     }
   }
@@ -681,6 +685,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
 // light exactly when the ray meets the light at some tL and nothing else
 // lies in [tmin, tL): an occlusion query (first hit ends the walk) over
 // [tmin, prev(tL)] answers that (an exact tie at tL counts as unoccluded).
+// k_shade computes tL (the FP32 Light::hit of the same ray) and emits only
+// rays that meet their light.
 // kAtomic: several lights, so several shadow rays of one path may add to
 // its radiance in one launch.  perm (optional): lane i takes record perm[i].
 template <bool kAtomic, int kFmt>
@@ -689,7 +695,6 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
                                                             const uint32_t *__restrict__ perm) {
   const float4 *__restrict__ nodes = a.nodes;
   const float4 *__restrict__ leaves = a.leaves;
-  const float4 *__restrict__ rec = a.rec;
   const int nPrims = a.nPrims;
   const MediumDev &med = a.med;
   const uint64_t seed = a.seed;
@@ -700,8 +705,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
   const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
   const f3 o = xyz(r0), d = xyz(r1);
-  float tL;
-  if (!hitRec(rec, __float_as_int(l.x), o, d, r0.w, r1.w, tL)) return;
+  const float tL = l.w;
   float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
   const int leaf =
