@@ -57,6 +57,7 @@ def bench_render(cpt, L, scenes, args):
     ctx = cpt.Context(0)
     ctx.set_quantized(args.records == "quant")
     ctx.set_traversal_width(args.width)
+    ctx.set_persistent(not args.no_persistent)
     ctx.upload(s)
     st_b = ctx.build(s.bounds_min, s.bounds_max)
     kw = dict(seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
@@ -73,7 +74,9 @@ def bench_render(cpt, L, scenes, args):
     rays = float(np.mean([f.rays for f in frames]))
     ext_ms = float(np.mean([f.extend_ms for f in frames]))
     ext_rays = float(np.mean([f.extend_rays for f in frames]))
-    ext_bytes = 44.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
+    # per ray: ray 32 B, queue permutation 4 B, hit record 32 B; per record
+    # visit and per primitive test their record bytes
+    ext_bytes = 68.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
@@ -254,6 +257,7 @@ def bench_lbvh(cpt, scenes, args):
     ctx = cpt.Context(0)
     ctx.set_quantized(args.records == "quant")
     ctx.set_traversal_width(args.width)
+    ctx.set_persistent(not args.no_persistent)
     ctx.upload(scene)
     rays = torch.from_numpy(rays_np).cuda()
     t_out = torch.empty(args.rays, dtype=torch.float32, device="cuda")
@@ -287,6 +291,8 @@ def bench_lbvh(cpt, scenes, args):
     # the kernel reads each ray through the ordering permutation (+4 B)
     alg += 4.0 * args.rays if s_avg > 0 else 0.0
     achieved = alg / (k_avg * 1e-3) / 1e9
+    # the persistent walk (four-wide formats) and its t pass, or one lane per ray
+    tr_kernel = "k_trace_persist" if (args.width == 4 and not args.no_persistent) else "k_trace"
     out = {"triangles": args.tris, "rays": args.rays, "build_ms": round(bd_avg, 4),
            "build_stage_ms": {"morton": round(sa[0], 4), "sort": round(sa[1], 4), "emit": round(sa[2], 4),
                               "refit": round(sa[3], 4)},
@@ -295,8 +301,8 @@ def bench_lbvh(cpt, scenes, args):
            "hit_fraction": round(hits, 4), "nodes_per_ray": round(n_nodes / args.rays, 2),
            "prims_per_ray": round(n_prims / args.rays, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_trace_persist"),
-                        "kernel": "k_trace_persist"}}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(tr_kernel),
+                        "kernel": tr_kernel}}
     if not args.no_cpu:
         from oracle import oracle as O
         o = O.OracleScene(scene)
@@ -362,6 +368,8 @@ def main():
     ap.add_argument("--records", choices=["quant", "fp32"], default="fp32",
                     help="four-wide node records read by the traversal")
     ap.add_argument("--no-c5", action="store_true", help="skip the many-view medium equal-relMSE leg")
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="one lane per ray instead of the persistent lane-refilling walks")
     ap.add_argument("--c5-views", type=int, default=32)
     ap.add_argument("--c5-width", type=int, default=1920)
     ap.add_argument("--c5-height", type=int, default=1080)
@@ -399,6 +407,7 @@ def main():
     lb = None if args.no_lbvh else bench_lbvh(cpt, scenes, args)
     eq = None if args.no_eq else bench_equal_quality(cpt, L, scenes, args)
     c5 = None if args.no_c5 else bench_config5(cpt, L, scenes, args)
+    ext_kernel = "k_extend_persist" if (args.width == 4 and not args.no_persistent) else "k_extend"
     out = {
         "metric": METRIC,
         "value": round(r["mrays"], 1),
@@ -426,8 +435,8 @@ def main():
         "extend_lane_utilisation": round(r["lane_utilisation"], 3),
         "scene_lbvh_build_ms": round(r["build_ms"], 3),
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic("k_extend"),
-                     "kernel": "k_extend",
+                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(ext_kernel),
+                     "kernel": ext_kernel,
                      "avg_launch_ms": round(r["ext_ms"] / max(1, r["bounces"]), 4)},
         "cpu_baseline": cpu,
         "lbvh": lb,
