@@ -76,7 +76,9 @@ class RenderStats(C.Structure):
                 ("extend_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("bounces", C.c_int32), ("pad_", C.c_int32), ("extend_rays", C.c_double),
                 ("extend_nodes", C.c_double), ("extend_prims", C.c_double),
-                ("extend_wave_nodes", C.c_double)]
+                ("extend_wave_nodes", C.c_double), ("shadow_rays", C.c_double),
+                ("shadow_nodes", C.c_double), ("shadow_prims", C.c_double),
+                ("shadow_wave_nodes", C.c_double)]
 
 
 P = C.c_void_p

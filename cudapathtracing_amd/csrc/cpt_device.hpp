@@ -744,6 +744,7 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
 // Persistent walks: a wave takes kChunk items per grab of a global counter;
 // a lane moves through these states, one loop trip each, as it takes an item.
 constexpr int kChunk = 256;
+constexpr int kClaimMin = 16;
 enum LaneState { kIdle = 0, kClaimed = 1, kLoading = 2, kActive = 3 };
 
 // Wave-level item claiming for persistent walks.  The wave holds a chunk
@@ -767,6 +768,10 @@ __device__ __forceinline__ void waveChunkInit(WaveChunk &c, uint32_t *counter) {
 __device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint32_t n, bool idle, uint32_t &item) {
   const uint64_t im = __ballot(idle);
   if (!im || !c.more) return false;
+  // a trip that claims waits for the claimed lanes' loads before its walk
+  // steps can issue theirs: claim only once kClaimMin lanes are idle (or
+  // the whole wave is), so that such trips are few
+  if (__popcll(im) < kClaimMin && __ballot(!idle) != 0) return false;
   const uint32_t lane = __lane_id();
   if (c.cur >= c.end) {
     const uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(c.ticket, 0, 64));

@@ -556,12 +556,14 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       }
       const uint32_t slot = blockAppend(counters + 1, want, appendSh);
       if (want && slot < shadowCap) {
+        // {origin, tmin} {direction, distance to the light} {contribution,
+        // radiance slot of the path} {pixel, sample and draw count of the
+        // transmittance estimate's stream, light id}
         shadow[4 * (int64_t)slot + 0] = make_float4(hp.x, hp.y, hp.z, kRayTMin);
-        shadow[4 * (int64_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, kRayTMax);
-        shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)mt.x));
-        // light id, the stream of the transmittance estimate, distance to the light
+        shadow[4 * (int64_t)slot + 1] = make_float4(dir.x, dir.y, dir.z, tL);
+        shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)ls));
         shadow[4 * (int64_t)slot + 3] =
-            make_float4(__int_as_float(li), __uint_as_float(mt.y), __uint_as_float(rng.cnt), tL);
+            make_float4(__uint_as_float(mt.x), __uint_as_float(mt.y), __uint_as_float(rng.cnt), __int_as_float(li));
       }
     }
   }
@@ -689,10 +691,13 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
 // rays that meet their light.
 // kAtomic: several lights, so several shadow rays of one path may add to
 // its radiance in one launch.  perm (optional): lane i takes record perm[i].
-template <bool kAtomic, int kFmt>
+// kCount: also sums the walks' record visits and primitive tests into
+// work[0], work[1] and each wave's largest visit count into work[2].
+template <bool kAtomic, int kFmt, bool kCount>
 __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const float4 *__restrict__ shadow,
                                                             int64_t shadowCap, const uint32_t *__restrict__ counters,
-                                                            const uint32_t *__restrict__ perm) {
+                                                            const uint32_t *__restrict__ perm,
+                                                            unsigned long long *__restrict__ work) {
   const float4 *__restrict__ nodes = a.nodes;
   const float4 *__restrict__ leaves = a.leaves;
   const int nPrims = a.nPrims;
@@ -700,28 +705,46 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   const uint64_t seed = a.seed;
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t li = blockIdx.x * kRenderThreads + threadIdx.x;
-  if ((int64_t)li >= shadowCap || li >= counters[1]) return;
-  const uint32_t k = perm ? perm[li] : li;
-  const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
-  const float4 c = shadow[4 * (int64_t)k + 2], l = shadow[4 * (int64_t)k + 3];
-  const f3 o = xyz(r0), d = xyz(r1);
-  const float tL = l.w;
-  float tb = nextafterf(tL, -INFINITY);
   TraceCounters cnt;
-  const int leaf =
-      traceFmt<false, kFmt, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x, kRenderThreads, cnt);
-  if (leaf >= 0) return;
-  const uint32_t pixel = (uint32_t)__float_as_int(c.w);
-  float tr = 1.0f;
-  if (med.on) tr = ratioTrack(med, seed, pixel, __float_as_uint(l.y), __float_as_uint(l.z), o, d, r0.w, tL);
-  const int64_t ls = pathSlot(a, pixel, __float_as_uint(l.y));
-  if (kAtomic) {
-    float *pl = (float *)(a.lbuf + ls);
-    atomicAdd(pl + 0, c.x * tr);
-    atomicAdd(pl + 1, c.y * tr);
-    atomicAdd(pl + 2, c.z * tr);
-  } else {
-    addRadiance(a.lbuf, ls, mkf3(c.x * tr, c.y * tr, c.z * tr));
+  if ((int64_t)li < shadowCap && li < counters[1]) {
+    const uint32_t k = perm ? perm[li] : li;
+    const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
+    const float4 c = shadow[4 * (int64_t)k + 2];
+    const f3 o = xyz(r0), d = xyz(r1);
+    const float tL = r1.w;
+    float tb = nextafterf(tL, -INFINITY);
+    const int leaf =
+        traceFmt<kCount, kFmt, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x, kRenderThreads, cnt);
+    if (leaf < 0) {
+      float tr = 1.0f;
+      if (med.on) {
+        const float4 l = shadow[4 * (int64_t)k + 3];
+        tr = ratioTrack(med, seed, __float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), o, d, r0.w, tL);
+      }
+      const int64_t ls = __float_as_int(c.w);
+      if (kAtomic) {
+        float *pl = (float *)(a.lbuf + ls);
+        atomicAdd(pl + 0, c.x * tr);
+        atomicAdd(pl + 1, c.y * tr);
+        atomicAdd(pl + 2, c.z * tr);
+      } else {
+        addRadiance(a.lbuf, ls, mkf3(c.x * tr, c.y * tr, c.z * tr));
+      }
+    }
+  }
+  if (kCount) {
+    unsigned long long s0 = (unsigned long long)cnt.nodes, s1 = (unsigned long long)cnt.prims;
+    int mx = cnt.nodes;
+    for (int off = 32; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+      mx = max(mx, __shfl_xor(mx, off, 64));
+    }
+    if (__lane_id() == 0) {
+      atomicAdd(work + 0, s0);
+      atomicAdd(work + 1, s1);
+      atomicAdd(work + 2, (unsigned long long)mx);
+    }
   }
 }
 
@@ -801,7 +824,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   if ((rc = ensure(ctx, R.hit, cap * 32)) || (rc = ensure(ctx, R.permA, shadowCap * 4)) ||
       (rc = ensure(ctx, R.permB, shadowCap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.keysA, shadowCap * 4)) || (rc = ensure(ctx, R.keysB, shadowCap * 4)) ||
-      (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 64)) ||
+      (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 128)) ||
       (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) || (rc = ensure(ctx, R.lbuf, cap * 16)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
@@ -886,9 +909,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   uint32_t *cnt = R.counters.as<uint32_t>();
   // counters: [0] next queue, [1] shadow queue, [2] volume queue, [3] grab
   // counter of the persistent extension kernel; [8..13] three u64 work sums
+  // of the extension walks, [14..19] of the shadow walks
   unsigned long long *work = (unsigned long long *)(cnt + 8);
   const bool countWork = (prm->flags & CPT_RENDER_COUNT_WORK) != 0;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 16);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 32);
   hipEvent_t *ev = ctx->ev;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[12], s));
   for (int64_t base = 0; base < total; base += cap) {
@@ -996,19 +1020,24 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
           return rc;
         shPerm = R.permA.as<uint32_t>();
       }
-#define CPT_SHD(A, F)                                                                                        \
-  hipLaunchKernelGGL((k_shadow<A, F>), dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,                   \
-                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, shPerm)
-      if (ns > 0) switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {
-        case 0: CPT_SHD(false, 0); break;
-        case 1: CPT_SHD(true, 0); break;
-        case 2: CPT_SHD(false, 1); break;
-        case 3: CPT_SHD(true, 1); break;
-        case 4: CPT_SHD(false, 2); break;
-        case 5: CPT_SHD(true, 2); break;
-        case 6: CPT_SHD(false, 3); break;
-        default: CPT_SHD(true, 3); break;
+#define CPT_SHD(A, F, C)                                                                                     \
+  hipLaunchKernelGGL((k_shadow<A, F, C>), dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,                \
+                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, shPerm, work + 3)
+#define CPT_SHD_F(C)                                   \
+  switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {             \
+    case 0: CPT_SHD(false, 0, C); break;               \
+    case 1: CPT_SHD(true, 0, C); break;                \
+    case 2: CPT_SHD(false, 1, C); break;               \
+    case 3: CPT_SHD(true, 1, C); break;                \
+    case 4: CPT_SHD(false, 2, C); break;               \
+    case 5: CPT_SHD(true, 2, C); break;                \
+    case 6: CPT_SHD(false, 3, C); break;               \
+    default: CPT_SHD(true, 3, C); break;               \
+  }
+      if (ns > 0) {
+        if (countWork) CPT_SHD_F(true) else CPT_SHD_F(false)
       }
+#undef CPT_SHD_F
 #undef CPT_SHD
       CPT_TRY_HIP(ctx, hipGetLastError());
       CPT_TRY_HIP(ctx, hipEventRecord(ev[17], s));
@@ -1068,11 +1097,15 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     stats->shadow_ms = sdMs;
     stats->bounces = bounces;
     stats->extend_rays = extRays;
-    unsigned long long wk[3] = {0, 0, 0};
-    if (countWork) CPT_TRY_HIP(ctx, hipMemcpy(wk, work, 24, hipMemcpyDeviceToHost));
+    unsigned long long wk[6] = {0, 0, 0, 0, 0, 0};
+    if (countWork) CPT_TRY_HIP(ctx, hipMemcpy(wk, work, 48, hipMemcpyDeviceToHost));
     stats->extend_nodes = (double)wk[0];
     stats->extend_prims = (double)wk[1];
     stats->extend_wave_nodes = 64.0 * (double)wk[2];
+    stats->shadow_rays = rays - extRays;
+    stats->shadow_nodes = (double)wk[3];
+    stats->shadow_prims = (double)wk[4];
+    stats->shadow_wave_nodes = 64.0 * (double)wk[5];
   }
   return CPT_OK;
 }

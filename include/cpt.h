@@ -292,6 +292,10 @@ typedef struct cpt_render_stats {
   double extend_wave_nodes; /* 64 x each wave's largest node count, summed
                                (ditto): extend_nodes / extend_wave_nodes is
                                the lane utilisation of the walk */
+  double shadow_rays;       /* shadow rays traced */
+  double shadow_nodes;      /* the same three for the shadow walks */
+  double shadow_prims;
+  double shadow_wave_nodes;
 } cpt_render_stats;
 
 /* render<<<>>> (Render.cu:16-261) as a wavefront pipeline.  rgb: h*w*3
