@@ -761,7 +761,8 @@ int launchMorton(cpt_ctx *ctx, const double bmin[3], const double bmax[3], uint6
 // If `keys64Out` is set (internal use) the last pass writes 64-bit keys.
 static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
                       uint32_t *valsTmp, int64_t n, int bits, bool identityVals,
-                      bool firstHistDone, uint64_t *keys64Out, uint32_t *orderOut) {
+                      bool firstHistDone, uint64_t *keys64Out, uint32_t *orderOut,
+                      uint32_t **valsOut = nullptr) {
   const int nTiles = (int)((n + kTile - 1) / kTile);
   const int passes = (bits + 7) / 8;
   uint32_t *hist = ctx->hist.as<uint32_t>();
@@ -791,7 +792,9 @@ static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *ke
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
-  if (!keys64Out && kin != keys) {
+  if (valsOut) {
+    *valsOut = vin;  // sorted values stay where the last pass put them
+  } else if (!keys64Out && kin != keys) {
     CPT_TRY_HIP(ctx, hipMemcpyAsync(keys, kin, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
     CPT_TRY_HIP(ctx, hipMemcpyAsync(vals, vin, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
   }
@@ -799,14 +802,15 @@ static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *ke
 }
 
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
-                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals) {
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut) {
+  if (valsOut) *valsOut = vals;
   if (n <= 0) return CPT_OK;
   const int nTiles = (int)((n + kTile - 1) / kTile);
   int rc;
   if ((rc = ensure(ctx, ctx->hist, (size_t)nTiles * 256 * 4))) return rc;
   if ((rc = ensure(ctx, ctx->rowTotal, 256 * 4))) return rc;
   return sortPasses(ctx, keys, vals, keysTmp, valsTmp, n, bits, identityVals, false, nullptr,
-                    nullptr);
+                    nullptr, valsOut);
 }
 
 int recordFormat(const cpt_ctx *ctx) {

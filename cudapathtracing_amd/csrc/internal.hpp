@@ -95,10 +95,11 @@ void releaseAll(cpt_ctx *ctx);
 
 // bvh.hip
 int launchMorton(cpt_ctx *ctx, const double smin[3], const double smax[3], uint64_t *keysDev);
-// Stable LSD sort of (keys, vals); an even number of 8-bit passes leaves the
-// result in (keys, vals).
+// Stable LSD sort of (keys, vals) in 8-bit passes; the result is in (keys,
+// vals), or, with valsOut, the sorted values are left where the last pass put
+// them (vals or valsTmp) and *valsOut points there.
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
-                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals);
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut = nullptr);
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
 int refitBvh(cpt_ctx *ctx, cpt_build_stats *st);
 // Traversal record format: 0 binary nodes, 1 four-wide FP32, 2 four-wide
@@ -112,12 +113,6 @@ void expandBounds(const double bmin[3], const double bmax[3], double smin[3], do
 // trace.hip
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev);
-
-// Permutation of n rays into ray-key order (direction octant, origin Morton
-// cell), stable; the 4-pass sort leaves it in perm (keysB, permB scratch).
-// Ray k is {origin, tmin} {direction, tmax} at rays[stride * k].
-int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
-                uint32_t *permB, int stride = 2);
 
 // render.hip
 void renderRelease(cpt_ctx *ctx);

@@ -42,7 +42,7 @@ struct RenderBufs {
   // path state, double-buffered and indexed by queue slot: k_shade writes
   // the survivors compacted into the other set
   DevBuf ray[2], thr[2], meta[2];
-  DevBuf hit, permA, permB, qVol, keysA, keysB, shadow, counters, fb, aov, cams, lbuf;
+  DevBuf hit, permA, permB, qVol, keysA, keysB, keysS, shadow, counters, fb, aov, cams, lbuf;
 };
 
 struct CamDev {
@@ -126,7 +126,34 @@ struct RenderArgs {
   MediumDev med;
   int64_t base;  // first path of the batch (path g = (sample - sampleBase) * npix * nviews + pixel)
   float4 *lbuf;  // radiance per path of the batch, index g - base
+  // sort keys written with each next-queue and shadow ray (renderKey)
+  const float *rootBox;
+  uint32_t *qkeys, *skeys;
 };
+
+// 24-bit sort key of a ray for the queue orderings: direction octant over
+// the Morton code of the origin's cell on a 128^3 grid over the root box.
+__device__ __forceinline__ uint32_t spread7(uint32_t v) {
+  v &= 0x7fu;
+  v = (v | (v << 8)) & 0x0000f00fu;
+  v = (v | (v << 4)) & 0x000c30c3u;
+  v = (v | (v << 2)) & 0x00249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t renderKey(const float *__restrict__ box, f3 o, f3 d) {
+  const float p[3] = {o.x, o.y, o.z};
+  uint32_t c[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float lo = box[k], ext = box[3 + k] - lo;
+    const float u = ext > 0.0f ? (p[k] - lo) / ext : 0.0f;
+    c[k] = (uint32_t)fminf(fmaxf(u * 128.0f, 0.0f), 127.0f);  // NaN -> 0
+  }
+  const uint32_t oct = (d.x > 0.0f ? 4u : 0u) | (d.y > 0.0f ? 2u : 0u) | (d.z > 0.0f ? 1u : 0u);
+  return (oct << 21) | (spread7(c[0]) << 2) | (spread7(c[1]) << 1) | spread7(c[2]);
+}
+constexpr int kRenderKeyBits = 24;
 
 // Batch-local index of path (pixel over all views, sample).
 __device__ __forceinline__ int64_t pathSlot(const RenderArgs &a, uint32_t pixel, uint32_t sample) {
@@ -564,6 +591,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)ls));
         shadow[4 * (int64_t)slot + 3] =
             make_float4(__uint_as_float(mt.x), __uint_as_float(mt.y), __uint_as_float(rng.cnt), __int_as_float(li));
+        a.skeys[slot] = renderKey(a.rootBox, hp, dir);
       }
     }
   }
@@ -615,6 +643,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
     thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
     metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, flags);
+    a.qkeys[slot] = renderKey(a.rootBox, o, d);
   }
 }
 
@@ -678,6 +707,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
   thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
   metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
+  a.qkeys[slot] = renderKey(a.rootBox, o, d);
 }
 
 
@@ -777,7 +807,7 @@ void renderRelease(cpt_ctx *ctx) {
   if (!ctx->render) return;
   RenderBufs *r = ctx->render;
   DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
-                   &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,
+                   &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,   &r->keysS,
                    &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
@@ -824,6 +854,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   if ((rc = ensure(ctx, R.hit, cap * 32)) || (rc = ensure(ctx, R.permA, shadowCap * 4)) ||
       (rc = ensure(ctx, R.permB, shadowCap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.keysA, shadowCap * 4)) || (rc = ensure(ctx, R.keysB, shadowCap * 4)) ||
+      (rc = ensure(ctx, R.keysS, shadowCap * 4)) ||
       (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 128)) ||
       (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) || (rc = ensure(ctx, R.lbuf, cap * 16)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
@@ -875,6 +906,9 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.npix = npixView;
   a.lbuf = R.lbuf.as<float4>();
   a.base = 0;
+  a.rootBox = ctx->rootBox.as<float>();
+  a.qkeys = R.keysA.as<uint32_t>();
+  a.skeys = R.keysS.as<uint32_t>();
   a.med.on = ctx->hasMedium ? 1 : 0;
   a.med.density = ctx->medDensity.as<float>();
   a.med.bmin = mkf3(ctx->medBox[0], ctx->medBox[1], ctx->medBox[2]);
@@ -919,6 +953,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     const int64_t count = std::min(cap, total - base);
     int cur = 0;  // state set holding the current queue
     bool permPending = false;
+    uint32_t *queuePerm = nullptr;  // the pending order (where the sort left it)
     a.base = base;
     hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
                        R.ray[cur].as<float4>(), R.thr[cur].as<float4>(), R.meta[cur].as<uint4>());
@@ -933,7 +968,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
         // with a pending ray-key order the extension kernel traces the queue
         // in that order; hit records stay at the paths' own slots
         const float4 *rayIn = R.ray[cur].as<float4>();
-        const uint32_t *pm = permPending ? R.permA.as<uint32_t>() : nullptr;
+        const uint32_t *pm = permPending ? queuePerm : nullptr;
 #define CPT_EXT(C, F, P)                                                                                      \
   hipLaunchKernelGGL((k_extend<C, F, P>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes,     \
                      a.leaves, a.nPrims, R.hit.as<float4>(), work, a.wide, pm)
@@ -1013,12 +1048,11 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       CPT_TRY_HIP(ctx, hipEventRecord(ev[16], s));
       // shadow rays in the order of their origin cell (all rays of one
       // origin cell then walk the same records towards the lights)
-      const uint32_t *shPerm = nullptr;
+      uint32_t *shPerm = nullptr;
       if (ctx->shadowSortMin > 0 && ns >= ctx->shadowSortMin) {
-        if ((rc = rayKeyOrder(ctx, (const float4 *)R.shadow.as<float4>(), ns, R.keysA.as<uint32_t>(),
-                              R.keysB.as<uint32_t>(), R.permA.as<uint32_t>(), R.permB.as<uint32_t>(), 4)))
+        if ((rc = radixSortPairs(ctx, R.keysS.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
+                                 R.permB.as<uint32_t>(), ns, kRenderKeyBits, true, &shPerm)))
           return rc;
-        shPerm = R.permA.as<uint32_t>();
       }
 #define CPT_SHD(A, F, C)                                                                                     \
   hipLaunchKernelGGL((k_shadow<A, F, C>), dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,                \
@@ -1044,11 +1078,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       shadowTimed = true;
       qn = hc[0];
       cur = nxt;
-      // incoherent bounce rays: trace the next queue in ray-key order (the
-      // next extension kernel moves the state into that order)
+      // incoherent bounce rays: trace the next queue in ray-key order
       if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin) {
-        if ((rc = rayKeyOrder(ctx, (const float4 *)R.ray[cur].as<float4>(), qn, R.keysA.as<uint32_t>(),
-                              R.keysB.as<uint32_t>(), R.permA.as<uint32_t>(), R.permB.as<uint32_t>())))
+        if ((rc = radixSortPairs(ctx, R.keysA.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
+                                 R.permB.as<uint32_t>(), qn, kRenderKeyBits, true, &queuePerm)))
           return rc;
         permPending = true;
       }

@@ -172,19 +172,6 @@ __global__ void k_zero_work(uint32_t *p) {
   if (threadIdx.x == 0) p[0] = 0;
 }
 
-// Permutation perm[0..n) of rays[0..n) (2 float4 per ray) into ray-key order
-// (stable); keysB and permB are scratch.
-int rayKeyOrder(cpt_ctx *ctx, const float4 *rays, int64_t n, uint32_t *keysA, uint32_t *keysB, uint32_t *perm,
-                uint32_t *permB, int stride) {
-  if (n <= 0) return CPT_OK;
-  const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
-  hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, rays, stride,
-                     (const uint32_t *)nullptr, n,
-                     (const float *)ctx->rootBox.as<float>(), keysA);
-  CPT_TRY_HIP(ctx, hipGetLastError());
-  return radixSortPairs(ctx, keysA, perm, keysB, permB, n, 30, true);
-}
-
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
                 int32_t *nodesDev, int32_t *primsDev) {
   if (n <= 0) return CPT_OK;
