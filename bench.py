@@ -7,9 +7,11 @@ U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
 1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
 Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
 the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
-per ray = 32 (ray) + 4 (queue index) + 8 (hit) + 112 per four-wide node record
-visited (7 of its 8 float4; 64 with --records quant) + 48 per primitive tested, node/primitive counts from a counting run of the same
-frame, divided by the summed HIP-event durations of the extension launches.
+per ray = 32 (ray) + 4 (queue permutation) + 32 (hit record) + 64 per
+four-wide quantized node record visited (112 = 7 of the 8 float4 of an FP32
+record with --records fp32) + 48 per primitive tested, node/primitive counts
+from a counting run of the same frame, divided by the summed HIP-event
+durations of the extension launches.
 
 Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
 U[0,1]^3, lognormal side mu=-6 sigma=0.5, random orientation, seed 1) and
@@ -34,9 +36,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpix/s at equal relMSE (denoised vs plain PT); Mrays/s; LBVH build ms"
 HBM_PEAK_GBS = 8000.0
 def node_bytes(args):
-    """Bytes read per visit of a node record: 7 of the 8 float4 of a four-wide
-    FP32 record (the default), a 64 B four-wide quantized record, 6 float4 of
-    an eight-wide quantized record, or a 64 B binary node."""
+    """Bytes read per visit of a node record: a 64 B four-wide quantized
+    record (the default), 7 of the 8 float4 of a four-wide FP32 record, 6
+    float4 of an eight-wide quantized record, or a 64 B binary node."""
     if args.width == 8:
         return 96.0
     if args.width == 2:
@@ -365,7 +367,7 @@ def main():
     ap.add_argument("--eq-height", type=int, default=540)
     ap.add_argument("--eq-ref-spp", type=int, default=4096)
     ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="traversal width")
-    ap.add_argument("--records", choices=["quant", "fp32"], default="fp32",
+    ap.add_argument("--records", choices=["quant", "fp32"], default="quant",
                     help="four-wide node records read by the traversal")
     ap.add_argument("--no-c5", action="store_true", help="skip the many-view medium equal-relMSE leg")
     ap.add_argument("--no-persistent", action="store_true",

@@ -69,7 +69,7 @@ struct cpt_ctx {
   int medN[3] = {0, 0, 0};
   bool hasVolumeMat = false;  // some material is CPT_M_VOLUME (k_volume runs)
   int traversal = 4;    // 2: binary node walk, 4: four-wide records
-  bool quantized = false;  // four-wide walks read the 64 B quantized records
+  bool quantized = true;  // four-wide walks read the 64 B quantized records
   bool persistent = true;  // four-wide traces as persistent, lane-refilling kernels
   int numCU = 256;
   int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered

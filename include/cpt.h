@@ -141,10 +141,10 @@ const char *cpt_version(void);
  * traversals as persistent kernels whose lanes take a new ray when theirs is
  * finished; 0 launches one lane per ray.  Results are identical. */
 #define CPT_OPT_PERSISTENT 3
-/* CPT_OPT_QUANTIZED: 1 makes four-wide walks read 64 B records whose child
- * boxes are 8-bit planes on a power-of-two grid, rounded outward (each
- * contains the FP32 box); 0 (default) reads the 128 B FP32 records.  Hits
- * are identical up to exact-t ties. */
+/* CPT_OPT_QUANTIZED: 1 (default) makes four-wide walks read 64 B records
+ * whose child boxes are 8-bit planes on a power-of-two grid, rounded outward
+ * (each contains the FP32 box); 0 reads the 128 B FP32 records.  Hits are
+ * identical up to exact-t ties. */
 #define CPT_OPT_QUANTIZED 4
 /* CPT_OPT_QUEUE_SORT_MIN: cpt_render traces a bounce queue of at least this
  * many paths in ray-key order (the same key as CPT_OPT_RAY_SORT_MIN); 0

@@ -71,7 +71,7 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
         f = path_match(ctx, s, d, L.PROFILE_CONFIG, samples=(0, 1, 2))
     finally:
         ctx.set_traversal_width(4)
-        ctx.set_quantized(False)
+        ctx.set_quantized(True)
         ctx.set_persistent(True)
         ctx.set_queue_sort_min(1 << 20)
         ctx.set_shadow_sort_min(1 << 20)

@@ -67,8 +67,8 @@ def check_small(ctx, s, rays):
         miss = cert & (idb < 0)
         assert np.all(t[miss] == rays[miss, 7])
     ctx.set_traversal_width(4)
-    ctx.set_quantized(False)
-    ctx.set_persistent(False)
+    ctx.set_quantized(True)
+    ctx.set_persistent(True)
     return cert.mean()
 
 
@@ -207,5 +207,5 @@ def test_ray_ordering_keeps_results(ctx):
         assert np.array_equal(i0, i1) and np.array_equal(t0, t1), smin
         assert np.array_equal(n0, n1) and np.array_equal(p0, p1), smin
     ctx.set_ray_sort_min(1 << 20)
-    ctx.set_persistent(False)
+    ctx.set_persistent(True)
     assert np.mean(i0 >= 0) > 0.1
