@@ -179,7 +179,7 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
               ctx->nLights, ctx->nSpheres};
   const unsigned grid = (unsigned)((n + kTraceThreads - 1) / kTraceThreads);
   const float4 *r = (const float4 *)raysDev;
-  const float4 *nd = ctx->nodes.as<float4>(), *n4 = ctx->nodes4.as<float4>(), *lv = ctx->leaves.as<float4>();
+  const float4 *nd = ctx->nodes.as<float4>(), *lv = ctx->leaves.as<float4>();
   const int np = (int)ctx->nPrims;
   const bool sorted = ctx->raySortMin > 0 && n >= ctx->raySortMin && np > 1 && n <= 0x7fffffff;
   int rc;
@@ -201,7 +201,10 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   }
   if ((rc = ensureRecords(ctx))) return rc;
   const int fmt = recordFormat(ctx);
-  const float4 *rec4 = fmt == 3 ? ctx->nodes8.as<float4>() : fmt == 2 ? ctx->nodes4q.as<float4>() : n4;
+  // read after ensureRecords, which may (re)allocate the records
+  const float4 *rec4 = fmt == 3   ? ctx->nodes8.as<float4>()
+                       : fmt == 2 ? ctx->nodes4q.as<float4>()
+                                  : ctx->nodes4.as<float4>();
   if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1 && (nodesDev || (tDev && idDev)) &&
       n < ((int64_t)1 << 32) - kChunk * 65536) {
     if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
