@@ -432,6 +432,52 @@ __device__ __forceinline__ bool descend4(const bool h[4], const float tn[4], con
   return true;
 }
 
+// Slab distances of the ray to the six planes of each child box of record
+// `node` (t = (plane - o) / d as fma(plane, inv, -o*inv)).  Quantized
+// records fold the plane decode into the product: with S = 2^e * inv and
+// O = fma(origin, inv, -o*inv) per axis, t = fma(byte, S, O), one FMA per
+// plane instead of two (the same FP32 slab arithmetic up to a few ulps).
+template <bool kQuant>
+__device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, int node, f3 inv, f3 ood, float x0[4],
+                                             float x1[4], float y0[4], float y1[4], float z0[4], float z1[4],
+                                             int ref[4]) {
+  if (!kQuant) {
+    float bx0[4], bx1[4], by0[4], by1[4], bz0[4], bz1[4];
+    loadRecord4<false>(recs, node, bx0, bx1, by0, by1, bz0, bz1, ref);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      x0[k] = __builtin_fmaf(bx0[k], inv.x, -ood.x);
+      x1[k] = __builtin_fmaf(bx1[k], inv.x, -ood.x);
+      y0[k] = __builtin_fmaf(by0[k], inv.y, -ood.y);
+      y1[k] = __builtin_fmaf(by1[k], inv.y, -ood.y);
+      z0[k] = __builtin_fmaf(bz0[k], inv.z, -ood.z);
+      z1[k] = __builtin_fmaf(bz1[k], inv.z, -ood.z);
+    }
+  } else {
+    const float4 *np = recs + 4 * (int64_t)node;
+    const float4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+    const uint32_t eb = __float_as_uint(w0.w);
+    const float sx = __uint_as_float((eb & 0xffu) << 23) * inv.x;
+    const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23) * inv.y;
+    const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23) * inv.z;
+    const float ox = __builtin_fmaf(w0.x, inv.x, -ood.x), oy = __builtin_fmaf(w0.y, inv.y, -ood.y),
+                oz = __builtin_fmaf(w0.z, inv.z, -ood.z);
+    const uint32_t lx = __float_as_uint(w2.x), hx = __float_as_uint(w2.y), ly = __float_as_uint(w2.z),
+                   hy = __float_as_uint(w2.w), lz = __float_as_uint(w3.x), hz = __float_as_uint(w3.y);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      x0[k] = __builtin_fmaf((float)((lx >> (8 * k)) & 255u), sx, ox);
+      x1[k] = __builtin_fmaf((float)((hx >> (8 * k)) & 255u), sx, ox);
+      y0[k] = __builtin_fmaf((float)((ly >> (8 * k)) & 255u), sy, oy);
+      y1[k] = __builtin_fmaf((float)((hy >> (8 * k)) & 255u), sy, oy);
+      z0[k] = __builtin_fmaf((float)((lz >> (8 * k)) & 255u), sz, oz);
+      z1[k] = __builtin_fmaf((float)((hz >> (8 * k)) & 255u), sz, oz);
+    }
+    ref[0] = __float_as_int(w1.x); ref[1] = __float_as_int(w1.y);
+    ref[2] = __float_as_int(w1.z); ref[3] = __float_as_int(w1.w);
+  }
+}
+
 template <bool kCount, bool kAny = false, bool kQuant = false>
 __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
                                              const float4 *__restrict__ leaves, int n, f3 o, f3 d,
@@ -458,16 +504,14 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
   int node = 0;
   while (true) {
     if (kCount) cnt.nodes++;
-    float bx0[4], bx1[4], by0[4], by1[4], bz0[4], bz1[4];
+    float sx0[4], sx1[4], sy0[4], sy1[4], sz0[4], sz1[4];
     int ref[4];
-    loadRecord4<kQuant>(nodes4, node, bx0, bx1, by0, by1, bz0, bz1, ref);
+    recordSlabs4<kQuant>(nodes4, node, inv, ood, sx0, sx1, sy0, sy1, sz0, sz1, ref);
     float tn[4];
     bool h[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const float x0 = __builtin_fmaf(bx0[k], inv.x, -ood.x), x1 = __builtin_fmaf(bx1[k], inv.x, -ood.x);
-      const float y0 = __builtin_fmaf(by0[k], inv.y, -ood.y), y1 = __builtin_fmaf(by1[k], inv.y, -ood.y);
-      const float z0 = __builtin_fmaf(bz0[k], inv.z, -ood.z), z1 = __builtin_fmaf(bz1[k], inv.z, -ood.z);
+      const float x0 = sx0[k], x1 = sx1[k], y0 = sy0[k], y1 = sy1[k], z0 = sz0[k], z1 = sz1[k];
       tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
       const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbest));
       h[k] = tn[k] <= tf && ref[k] != kEmpty4;
@@ -692,16 +736,14 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
                                           const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
                                           TraceCounters &cnt) {
   if (kCount) cnt.nodes++;
-  float bx0[4], bx1[4], by0[4], by1[4], bz0[4], bz1[4];
+  float sx0[4], sx1[4], sy0[4], sy1[4], sz0[4], sz1[4];
   int ref[4];
-  loadRecord4<kQuant>(nodes4, w.node, bx0, bx1, by0, by1, bz0, bz1, ref);
+  recordSlabs4<kQuant>(nodes4, w.node, w.inv, w.ood, sx0, sx1, sy0, sy1, sz0, sz1, ref);
   float tn[4];
   bool h[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const float x0 = __builtin_fmaf(bx0[k], w.inv.x, -w.ood.x), x1 = __builtin_fmaf(bx1[k], w.inv.x, -w.ood.x);
-    const float y0 = __builtin_fmaf(by0[k], w.inv.y, -w.ood.y), y1 = __builtin_fmaf(by1[k], w.inv.y, -w.ood.y);
-    const float z0 = __builtin_fmaf(bz0[k], w.inv.z, -w.ood.z), z1 = __builtin_fmaf(bz1[k], w.inv.z, -w.ood.z);
+    const float x0 = sx0[k], x1 = sx1[k], y0 = sy0[k], y1 = sy1[k], z0 = sz0[k], z1 = sz1[k];
     tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), w.tmin));
     const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), w.tbest));
     h[k] = tn[k] <= tf && ref[k] != kEmpty4;
