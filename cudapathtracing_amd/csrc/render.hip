@@ -133,6 +133,9 @@ struct RenderArgs {
 
 // 24-bit sort key of a ray for the queue orderings: direction octant over
 // the Morton code of the origin's cell on a 128^3 grid over the root box.
+// Queues sort on its top bits: bounce rays on 8 (octant and a 4^3 cell grid,
+// one radix pass), shadow rays on 16 (octant, 32^3 cells, two passes); finer
+// keys cost more sorting than they save in traversal.
 __device__ __forceinline__ uint32_t spread7(uint32_t v) {
   v &= 0x7fu;
   v = (v | (v << 8)) & 0x0000f00fu;
@@ -153,7 +156,7 @@ __device__ __forceinline__ uint32_t renderKey(const float *__restrict__ box, f3 
   const uint32_t oct = (d.x > 0.0f ? 4u : 0u) | (d.y > 0.0f ? 2u : 0u) | (d.z > 0.0f ? 1u : 0u);
   return (oct << 21) | (spread7(c[0]) << 2) | (spread7(c[1]) << 1) | spread7(c[2]);
 }
-constexpr int kRenderKeyBits = 24;
+constexpr int kQueueKeyBits = 8, kShadowKeyBits = 16;
 
 // Batch-local index of path (pixel over all views, sample).
 __device__ __forceinline__ int64_t pathSlot(const RenderArgs &a, uint32_t pixel, uint32_t sample) {
@@ -591,7 +594,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         shadow[4 * (int64_t)slot + 2] = make_float4(c.x, c.y, c.z, __int_as_float((int)ls));
         shadow[4 * (int64_t)slot + 3] =
             make_float4(__uint_as_float(mt.x), __uint_as_float(mt.y), __uint_as_float(rng.cnt), __int_as_float(li));
-        a.skeys[slot] = renderKey(a.rootBox, hp, dir);
+        a.skeys[slot] = renderKey(a.rootBox, hp, dir) >> (24 - kShadowKeyBits);
       }
     }
   }
@@ -643,7 +646,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
     thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
     metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, flags);
-    a.qkeys[slot] = renderKey(a.rootBox, o, d);
+    a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
   }
 }
 
@@ -707,7 +710,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
   thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
   metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
-  a.qkeys[slot] = renderKey(a.rootBox, o, d);
+  a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
 }
 
 
@@ -1051,7 +1054,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       uint32_t *shPerm = nullptr;
       if (ctx->shadowSortMin > 0 && ns >= ctx->shadowSortMin) {
         if ((rc = radixSortPairs(ctx, R.keysS.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
-                                 R.permB.as<uint32_t>(), ns, kRenderKeyBits, true, &shPerm)))
+                                 R.permB.as<uint32_t>(), ns, kShadowKeyBits, true, &shPerm)))
           return rc;
       }
 #define CPT_SHD(A, F, C)                                                                                     \
@@ -1081,7 +1084,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       // incoherent bounce rays: trace the next queue in ray-key order
       if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin) {
         if ((rc = radixSortPairs(ctx, R.keysA.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
-                                 R.permB.as<uint32_t>(), qn, kRenderKeyBits, true, &queuePerm)))
+                                 R.permB.as<uint32_t>(), qn, kQueueKeyBits, true, &queuePerm)))
           return rc;
         permPending = true;
       }
