@@ -462,8 +462,15 @@ __device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, in
     const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23) * inv.z;
     const float ox = __builtin_fmaf(w0.x, inv.x, -ood.x), oy = __builtin_fmaf(w0.y, inv.y, -ood.y),
                 oz = __builtin_fmaf(w0.z, inv.z, -ood.z);
-    const uint32_t lx = __float_as_uint(w2.x), hx = __float_as_uint(w2.y), ly = __float_as_uint(w2.z),
-                   hy = __float_as_uint(w2.w), lz = __float_as_uint(w3.x), hz = __float_as_uint(w3.y);
+    // the near plane of each axis is the low plane where the ray's direction
+    // is positive on it and the high one where it is negative: x0/y0/z0 get
+    // the near distances, x1/y1/z1 the far ones (a child's low byte is never
+    // above its high byte, and t is monotone in the byte)
+    const uint32_t w2x = __float_as_uint(w2.x), w2y = __float_as_uint(w2.y), w2z = __float_as_uint(w2.z),
+                   w2w = __float_as_uint(w2.w), w3x = __float_as_uint(w3.x), w3y = __float_as_uint(w3.y);
+    const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
+    const uint32_t lx = nx ? w2y : w2x, hx = nx ? w2x : w2y, ly = ny ? w2w : w2z, hy = ny ? w2z : w2w,
+                   lz = nz ? w3y : w3x, hz = nz ? w3x : w3y;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       x0[k] = __builtin_fmaf((float)((lx >> (8 * k)) & 255u), sx, ox);
@@ -512,8 +519,12 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const float x0 = sx0[k], x1 = sx1[k], y0 = sy0[k], y1 = sy1[k], z0 = sz0[k], z1 = sz1[k];
-      tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), tmin));
-      const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tbest));
+      // kQuant: the slabs come ordered (near, far)
+      const float nx = kQuant ? x0 : fminf(x0, x1), fx = kQuant ? x1 : fmaxf(x0, x1);
+      const float ny = kQuant ? y0 : fminf(y0, y1), fy = kQuant ? y1 : fmaxf(y0, y1);
+      const float nz = kQuant ? z0 : fminf(z0, z1), fz = kQuant ? z1 : fmaxf(z0, z1);
+      tn[k] = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+      const float tf = fminf(fminf(fx, fy), fminf(fz, tbest));
       h[k] = tn[k] <= tf && ref[k] != kEmpty4;
     }
     // leaf children first, in slot order: one test per loop trip, so a wave
@@ -744,8 +755,11 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const float x0 = sx0[k], x1 = sx1[k], y0 = sy0[k], y1 = sy1[k], z0 = sz0[k], z1 = sz1[k];
-    tn[k] = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), w.tmin));
-    const float tf = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), w.tbest));
+    const float nx = kQuant ? x0 : fminf(x0, x1), fx = kQuant ? x1 : fmaxf(x0, x1);
+    const float ny = kQuant ? y0 : fminf(y0, y1), fy = kQuant ? y1 : fmaxf(y0, y1);
+    const float nz = kQuant ? z0 : fminf(z0, z1), fz = kQuant ? z1 : fmaxf(z0, z1);
+    tn[k] = fmaxf(fmaxf(nx, ny), fmaxf(nz, w.tmin));
+    const float tf = fminf(fminf(fx, fy), fminf(fz, w.tbest));
     h[k] = tn[k] <= tf && ref[k] != kEmpty4;
   }
   uint32_t lm = 0;
