@@ -436,7 +436,26 @@ __device__ __forceinline__ bool descend4(const bool h[4], const float tn[4], con
 // `node` (t = (plane - o) / d as fma(plane, inv, -o*inv)).  Quantized
 // records fold the plane decode into the product: with S = 2^e * inv and
 // O = fma(origin, inv, -o*inv) per axis, t = fma(byte, S, O), one FMA per
-// plane instead of two (the same FP32 slab arithmetic up to a few ulps).
+// plane instead of two (the same FP32 slab arithmetic up to a few ulps of
+// max(|O|, 8 |255 S|), i.e. of the record's distance or extent along the
+// ray: far below the 1e-4 padding of primitive boxes at scene scales).
+// fp16 value of half `hi` of p, as float
+__device__ __forceinline__ float halfOf(uint32_t p, int hi) {
+  const uint16_t b = hi ? (uint16_t)(p >> 16) : (uint16_t)(p & 0xffffu);
+  return (float)__builtin_bit_cast(_Float16, b);
+}
+
+// t[k] = fma(1024 + byte k of w, s, o): v_perm_b32 puts bytes (0, 1) and
+// (2, 3) of w under the fp16 high byte 0x64 (the value 1024 + byte, exact).
+__device__ __forceinline__ void planes4(uint32_t w, float s, float o, float t[4]) {
+  const uint32_t p01 = __builtin_amdgcn_perm(0x64646464u, w, 0x04010400u);
+  const uint32_t p23 = __builtin_amdgcn_perm(0x64646464u, w, 0x04030402u);
+  t[0] = __builtin_fmaf(halfOf(p01, 0), s, o);
+  t[1] = __builtin_fmaf(halfOf(p01, 1), s, o);
+  t[2] = __builtin_fmaf(halfOf(p23, 0), s, o);
+  t[3] = __builtin_fmaf(halfOf(p23, 1), s, o);
+}
+
 template <bool kQuant>
 __device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, int node, f3 inv, f3 ood, float x0[4],
                                              float x1[4], float y0[4], float y1[4], float z0[4], float z1[4],
@@ -460,8 +479,12 @@ __device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, in
     const float sx = __uint_as_float((eb & 0xffu) << 23) * inv.x;
     const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23) * inv.y;
     const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23) * inv.z;
-    const float ox = __builtin_fmaf(w0.x, inv.x, -ood.x), oy = __builtin_fmaf(w0.y, inv.y, -ood.y),
-                oz = __builtin_fmaf(w0.z, inv.z, -ood.z);
+    // planes are decoded as fp16 values 1024 + byte (one v_perm_b32 makes two
+    // from a record word, and the FMA reads an fp16 half directly), so the
+    // offsets carry -1024 S: t = fma(1024 + byte, S, O - 1024 S)
+    const float ox = __builtin_fmaf(-1024.0f, sx, __builtin_fmaf(w0.x, inv.x, -ood.x)),
+                oy = __builtin_fmaf(-1024.0f, sy, __builtin_fmaf(w0.y, inv.y, -ood.y)),
+                oz = __builtin_fmaf(-1024.0f, sz, __builtin_fmaf(w0.z, inv.z, -ood.z));
     // the near plane of each axis is the low plane where the ray's direction
     // is positive on it and the high one where it is negative: x0/y0/z0 get
     // the near distances, x1/y1/z1 the far ones (a child's low byte is never
@@ -471,15 +494,12 @@ __device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, in
     const bool nx = inv.x < 0.f, ny = inv.y < 0.f, nz = inv.z < 0.f;
     const uint32_t lx = nx ? w2y : w2x, hx = nx ? w2x : w2y, ly = ny ? w2w : w2z, hy = ny ? w2z : w2w,
                    lz = nz ? w3y : w3x, hz = nz ? w3x : w3y;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      x0[k] = __builtin_fmaf((float)((lx >> (8 * k)) & 255u), sx, ox);
-      x1[k] = __builtin_fmaf((float)((hx >> (8 * k)) & 255u), sx, ox);
-      y0[k] = __builtin_fmaf((float)((ly >> (8 * k)) & 255u), sy, oy);
-      y1[k] = __builtin_fmaf((float)((hy >> (8 * k)) & 255u), sy, oy);
-      z0[k] = __builtin_fmaf((float)((lz >> (8 * k)) & 255u), sz, oz);
-      z1[k] = __builtin_fmaf((float)((hz >> (8 * k)) & 255u), sz, oz);
-    }
+    planes4(lx, sx, ox, x0);
+    planes4(hx, sx, ox, x1);
+    planes4(ly, sy, oy, y0);
+    planes4(hy, sy, oy, y1);
+    planes4(lz, sz, oz, z0);
+    planes4(hz, sz, oz, z1);
     ref[0] = __float_as_int(w1.x); ref[1] = __float_as_int(w1.y);
     ref[2] = __float_as_int(w1.z); ref[3] = __float_as_int(w1.w);
   }
