@@ -61,7 +61,7 @@ class Camera(C.Structure):
 class BuildStats(C.Structure):
     _fields_ = [("morton_ms", C.c_float), ("sort_ms", C.c_float), ("emit_ms", C.c_float),
                 ("refit_ms", C.c_float), ("total_ms", C.c_float), ("n_prims", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("cluster_ms", C.c_float)]
 
 
 class RenderParams(C.Structure):
@@ -119,6 +119,8 @@ SIGNATURES = [
                                      C.c_int]),
     ("cpt_scene_update_triangles", C.c_int, [P, C.c_int64, P, C.c_int64]),
     ("cpt_bvh_refit", C.c_int, [P, C.POINTER(BuildStats)]),
+    ("cpt_bvh_cluster_info", C.c_int, [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
     ("cpt_denoiser_create", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]),
     ("cpt_denoiser_destroy", None, [P]),
     ("cpt_denoiser_num_params", C.c_int64, [P]),

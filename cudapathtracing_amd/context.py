@@ -90,6 +90,13 @@ class Context:
         """cpt_render shadow queues of at least n rays are traced in origin-cell order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 6, n))
 
+    def set_cluster(self, mode: int):
+        """Four-wide records over a clustered (PLOC) tree of the same leaves
+        where its surface-area cost is clearly lower: 2 (default) built at the
+        first render after a build or refit, 1 after every build or refit,
+        0 never (True/False mean 1/0)."""
+        self._check(self._lib.cpt_set_option(self._h, 7, int(mode)))
+
     def set_ray_sort_min(self, n: int):
         """Trace batches of at least n rays in ray-key order (0: never)."""
         self._check(self._lib.cpt_set_option(self._h, 2, n))
@@ -149,6 +156,14 @@ class Context:
         st = L.BuildStats()
         self._check(self._lib.cpt_bvh_refit(self._h, C.byref(st)))
         return st
+
+    def cluster_info(self) -> dict:
+        """Which tree the four-wide records group after the last build or
+        refit, the clustered tree's depth and both trees' surface-area costs."""
+        u, d = C.c_int32(), C.c_int32()
+        a, b = C.c_double(), C.c_double()
+        self._check(self._lib.cpt_bvh_cluster_info(self._h, C.byref(u), C.byref(d), C.byref(a), C.byref(b)))
+        return {"in_use": bool(u.value), "depth": d.value, "lbvh_cost": a.value, "cluster_cost": b.value}
 
     def bilateral(self, rgb, aov, sigma_g=8.0, sigma_r=5.0, sigma_n=0.1, sigma_d=0.2):
         """Joint bilateral filter of Render.cu:353-398 (host arrays)."""

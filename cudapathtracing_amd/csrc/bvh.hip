@@ -821,18 +821,24 @@ int recordFormat(const cpt_ctx *ctx) {
 
 // Wide records of the current traversal format, collapsed from the binary
 // nodes after a build or refit, or on first use after the format changed.
-int ensureRecords(cpt_ctx *ctx) {
+int ensureRecords(cpt_ctx *ctx, bool lbvhOnly) {
   const int fmt = recordFormat(ctx);
   const int64_t n = ctx->nPrims;
   if (fmt == 0 || n < 2 || (ctx->recValid & (1 << fmt))) return CPT_OK;
   const size_t nn = (size_t)(n - 1);
   int rc;
+  // four-wide records from the clustered tree when it is on, current and
+  // chosen (mode 1 also builds it here after the option was switched on)
+  const bool four = fmt == 1 || fmt == 2;
+  if (!lbvhOnly && ctx->cluster == 1 && four && !ctx->clusterValid && (rc = buildCluster(ctx))) return rc;
+  const bool fromCluster = !lbvhOnly && ctx->cluster != 0 && four && ctx->clusterValid && ctx->clusterOk;
+  const float4 *bin = fromCluster ? ctx->nodesP.as<float4>() : ctx->nodes.as<float4>();
   const unsigned grid = (unsigned)((n - 1 + 255) / 256);
   if (fmt == 1 || fmt == 2) {
     DevBuf &dst = fmt == 1 ? ctx->nodes4 : ctx->nodes4q;
     if ((rc = ensure(ctx, dst, nn * (fmt == 1 ? 128 : 64)))) return rc;
-    hipLaunchKernelGGL(k_collapse4, dim3(grid), dim3(256), 0, ctx->stream, (int)n,
-                       (const float4 *)ctx->nodes.as<float4>(), fmt == 1 ? dst.as<float4>() : (float4 *)nullptr,
+    hipLaunchKernelGGL(k_collapse4, dim3(grid), dim3(256), 0, ctx->stream, (int)n, bin,
+                       fmt == 1 ? dst.as<float4>() : (float4 *)nullptr,
                        fmt == 2 ? dst.as<float4>() : (float4 *)nullptr);
   } else {
     if ((rc = ensure(ctx, ctx->nodes8, nn * 128))) return rc;
@@ -867,7 +873,28 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
   ctx->recValid = 0;
-  return ensureRecords(ctx);
+  ctx->clusterValid = false;
+  return ensureRecords(ctx, true);
+}
+
+// The clustered tree of the current leaves (CPT_OPT_CLUSTER), and the wide
+// records from it when it is chosen.  Returns its time in *ms.
+int clusterStage(cpt_ctx *ctx, float *ms) {
+  *ms = 0.f;
+  const int fmt = recordFormat(ctx);
+  if (!(fmt == 1 || fmt == 2) || ctx->nPrims < 2) return CPT_OK;
+  hipEvent_t *ev = ctx->ev;
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
+  int rc = buildCluster(ctx);
+  if (rc) return rc;
+  if (ctx->clusterOk) {
+    ctx->recValid = 0;
+    if ((rc = ensureRecords(ctx))) return rc;
+  }
+  CPT_TRY_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
+  CPT_TRY_HIP(ctx, hipEventSynchronize(ev[6]));
+  CPT_TRY_HIP(ctx, hipEventElapsedTime(ms, ev[5], ev[6]));
+  return CPT_OK;
 }
 
 // Node boxes from the current primitives with the topology, parents and
@@ -882,12 +909,14 @@ int refitBvh(cpt_ctx *ctx, cpt_build_stats *st) {
   if (rc) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
   CPT_TRY_HIP(ctx, hipEventSynchronize(ev[1]));
+  float ms = 0, cms = 0;
+  CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
+  if (ctx->cluster == 1 && (rc = clusterStage(ctx, &cms))) return rc;
   if (st) {
-    float ms = 0;
-    CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
     std::memset(st, 0, sizeof(*st));
     st->refit_ms = ms;
     st->total_ms = ms;
+    st->cluster_ms = cms;
     st->n_prims = (int32_t)n;
   }
   return CPT_OK;
@@ -940,8 +969,10 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
   CPT_TRY_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
   if ((rc = launchRefit(ctx, n))) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+  CPT_TRY_HIP(ctx, hipEventSynchronize(ev[4]));
+  float cms = 0;
+  if (ctx->cluster == 1 && (rc = clusterStage(ctx, &cms))) return rc;
   if (st) {
-    CPT_TRY_HIP(ctx, hipEventSynchronize(ev[4]));
     float a = 0, b = 0, c = 0, d = 0;
     CPT_TRY_HIP(ctx, hipEventElapsedTime(&a, ev[0], ev[1]));
     CPT_TRY_HIP(ctx, hipEventElapsedTime(&b, ev[1], ev[2]));
@@ -952,6 +983,7 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
     st->emit_ms = c;
     st->refit_ms = d;
     st->total_ms = a + b + c + d;
+    st->cluster_ms = cms;
     st->n_prims = (int32_t)n;
   }
   ctx->built = true;

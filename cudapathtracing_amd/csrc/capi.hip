@@ -59,7 +59,9 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->leaves,   &ctx->rootBox, &ctx->rayBuf,   &ctx->tBuf,       &ctx->idBuf,
                    &ctx->cntA,     &ctx->cntB,    &ctx->nodes4,     &ctx->esc,        &ctx->medDensity,
                    &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB,        &ctx->nodes4q,
-                   &ctx->nodes8};
+                   &ctx->nodes8,   &ctx->plC[0],  &ctx->plC[1],     &ctx->plB[0],     &ctx->plB[1],
+                   &ctx->plD[0],   &ctx->plD[1],  &ctx->plNN,       &ctx->plSum,      &ctx->nodesP,
+                   &ctx->plS[0],   &ctx->plS[1],  &ctx->plCtr,      &ctx->plTmp,      &ctx->plSah};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }
@@ -168,6 +170,11 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
   }
   if (option == CPT_OPT_SHADOW_SORT_MIN && value >= 0) {
     ctx->shadowSortMin = value;
+    return CPT_OK;
+  }
+  if (option == CPT_OPT_CLUSTER && value >= 0 && value <= 2) {
+    if (ctx->cluster != value) ctx->recValid = 0;
+    ctx->cluster = value;
     return CPT_OK;
   }
   if (option == CPT_OPT_QUANTIZED && (value == 0 || value == 1)) {
@@ -376,6 +383,17 @@ int cpt_bvh_refit(cpt_ctx *ctx, cpt_build_stats *st) {
   if (!ctx) return CPT_ERR_ARG;
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   return refitBvh(ctx, st);
+}
+
+int cpt_bvh_cluster_info(const cpt_ctx *ctx, int32_t *in_use, int32_t *depth, double *lbvh_cost,
+                         double *cluster_cost) {
+  if (!ctx) return CPT_ERR_ARG;
+  const int fmt = cpt::recordFormat(ctx);
+  if (in_use) *in_use = ctx->cluster != 0 && ctx->clusterValid && ctx->clusterOk && (fmt == 1 || fmt == 2) ? 1 : 0;
+  if (depth) *depth = ctx->clusterDepth;
+  if (lbvh_cost) *lbvh_cost = ctx->lbvhCost;
+  if (cluster_cost) *cluster_cost = ctx->clusterCost;
+  return CPT_OK;
 }
 
 int cpt_bvh_build(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st) {

@@ -60,6 +60,16 @@ struct cpt_ctx {
   cpt::DevBuf nodes4q;  // (N-1) x 64 B quantized four-wide records
   cpt::DevBuf nodes8;   // (N-1) x 128 B quantized eight-wide records
   int recValid = 0;     // bit f: wide records of format f are current
+  // clustered (PLOC) binary tree over the same leaves, for the wide records
+  cpt::DevBuf plC[2], plB[2], plD[2], plS[2], plNN, plSum, plCtr, plTmp, plSah, nodesP;
+  // clustered tree: 0 never, 1 after each build or refit, 2 at the first
+  // render after a build or refit; the four-wide records group it when it
+  // is the clearly cheaper tree
+  int cluster = 2;
+  bool clusterValid = false;  // nodesP matches the current leaves and boxes
+  bool clusterOk = false;     // nodesP is used: depth within the walks' stacks, lower cost
+  int clusterDepth = 0;
+  double lbvhCost = 0.0, clusterCost = 0.0;  // summed child-box areas of either tree
   cpt::DevBuf esc;      // refit escape list + its counter
   // heterogeneous medium (density grid), cleared by cpt_scene_upload
   bool hasMedium = false;
@@ -103,9 +113,16 @@ int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysT
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
 int refitBvh(cpt_ctx *ctx, cpt_build_stats *st);
 // Traversal record format: 0 binary nodes, 1 four-wide FP32, 2 four-wide
-// quantized, 3 eight-wide quantized; ensureRecords builds it if stale.
+// quantized, 3 eight-wide quantized; ensureRecords builds that of the
+// context's traversal options if stale (lbvhOnly: from the LBVH's boxes even
+// where the clustered tree is on).
 int recordFormat(const cpt_ctx *ctx);
-int ensureRecords(cpt_ctx *ctx);
+// cluster.hip: builds ctx->nodesP from the current leaf order and boxes.
+int buildCluster(cpt_ctx *ctx);
+// bvh.hip: builds the clustered tree and, when it is chosen, the wide
+// records from it; *ms gets the time.
+int clusterStage(cpt_ctx *ctx, float *ms);
+int ensureRecords(cpt_ctx *ctx, bool lbvhOnly = false);
 int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, int64_t nt,
                   int64_t base);
 void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]);

@@ -869,6 +869,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   RenderArgs a;
   // 0: binary nodes, 1: four-wide FP32 records, 2: four-wide quantized
   // records, 3: eight-wide quantized records
+  if (ctx->cluster == 2 && !ctx->clusterValid) {
+    float cms = 0.f;
+    if ((rc = clusterStage(ctx, &cms))) return rc;
+  }
   if ((rc = ensureRecords(ctx))) return rc;
   a.wide = recordFormat(ctx);
   a.nodes = a.wide == 3   ? ctx->nodes8.as<float4>()

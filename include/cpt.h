@@ -154,6 +154,15 @@ const char *cpt_version(void);
  * least this many in the order of their origin Morton cell; 0 disables it;
  * default 2^20.  Radiance is unchanged. */
 #define CPT_OPT_SHADOW_SORT_MIN 6
+/* CPT_OPT_CLUSTER: a second binary tree over the same leaves, built by
+ * locally-ordered clustering of the Morton-ordered leaf boxes; the four-wide
+ * records group its boxes when its surface-area cost (summed child-box
+ * areas) is at least 5% below the LBVH's.  2 (default): built by the first
+ * cpt_render / cpt_render_views after a build or refit (a render traces
+ * enough rays to repay it); 1: built by every cpt_bvh_build / cpt_bvh_refit
+ * (timed in cluster_ms); 0: never.  cpt_bvh_export always returns the LBVH.
+ * Hits are identical up to exact-t ties. */
+#define CPT_OPT_CLUSTER 7
 int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value);
 
 /* ---- scene upload ------------------------------------------------------- */
@@ -198,9 +207,9 @@ typedef struct cpt_build_stats {
   float sort_ms;
   float emit_ms;
   float refit_ms;
-  float total_ms;
+  float total_ms;   /* the four LBVH stages above */
   int32_t n_prims;
-  int32_t pad_;
+  float cluster_ms; /* after them: the clustered tree and its records (CPT_OPT_CLUSTER 1) */
 } cpt_build_stats;
 
 /* Device::buildBVH (Device.cuh:169-195).  scene_min/max are the reference's
@@ -359,6 +368,13 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri_xy
  * Scene.cuh:147-173, reduced to a refit).  The topology keeps the Morton
  * order of the last cpt_bvh_build. */
 int cpt_bvh_refit(cpt_ctx *ctx, cpt_build_stats *stats);
+
+/* No counterpart in the reference: which tree the four-wide records group
+ * (in_use 1: the clustered tree of CPT_OPT_CLUSTER, 0: the LBVH), the
+ * clustered tree's depth, and the two trees' surface-area costs (summed
+ * child-box areas) from the last build or refit.  Any pointer may be NULL. */
+int cpt_bvh_cluster_info(const cpt_ctx *ctx, int32_t *in_use, int32_t *depth, double *lbvh_cost,
+                         double *cluster_cost);
 
 /* Temporal accumulation for a moving camera: addPrevious (Render.cu:400-421)
  * with Camera::getPrePositionInImg (Camera.cuh:104-132), FP64.  rgb,

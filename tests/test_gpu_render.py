@@ -79,6 +79,24 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
     assert f >= 0.99
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_displaced_icosphere_cluster_paths(ctx, mode):
+    # GGX patches on a displaced icosphere, whose clustered tree is chosen:
+    # mode 1 builds it with the BVH, mode 2 at the first render
+    print(f"[gpu] render displaced icosphere paths (cluster mode {mode})", flush=True)
+    s = scenes.displaced_icosphere(32, 32, level=5)
+    d = scenes.default_camera(40, 40)
+    ctx.set_cluster(mode)
+    try:
+        f = path_match(ctx, s, d, L.PROFILE_CONFIG)
+        info = ctx.cluster_info()
+    finally:
+        ctx.set_cluster(2)
+    print(f"  {info}; path agreement {f:.5f}")
+    assert info["in_use"]
+    assert f >= 0.99
+
+
 @pytest.mark.parametrize("ssort", [0, 1])
 def test_cornell_two_lights(ctx, ssort):
     # two shadow rays per vertex: both add to the path radiance in one launch

@@ -50,14 +50,19 @@ def check_small(ctx, s, rays):
     print(f"  certified {cert.mean():.5f} of {len(rays)} rays, hits {np.mean(idb >= 0):.3f}")
     assert cert.mean() > 0.95
     # binary nodes; four-wide FP32 and quantized records, one lane per ray
-    # and persistent
-    for width, quant, persistent in ((2, False, False), (4, False, False), (4, True, False), (4, True, True),
-                                     (4, False, True), (8, False, False)):
+    # and persistent, over the LBVH's boxes and over the clustered tree's
+    for width, quant, persistent, cluster in ((2, False, False, False), (4, False, False, False),
+                                              (4, True, False, False), (4, True, True, False),
+                                              (4, False, True, True), (4, True, True, True),
+                                              (4, True, False, True), (8, False, False, False)):
         ctx.set_traversal_width(width)
         ctx.set_quantized(quant)
         ctx.set_persistent(persistent)
-        cfg = (width, quant, persistent)
+        ctx.set_cluster(cluster)
+        cfg = (width, quant, persistent, cluster)
         t, ids = ctx.trace(rays)
+        if cluster:
+            print(f"  cluster {ctx.cluster_info()}", flush=True)
         bad = np.where(cert & (ids != idb))[0]
         assert bad.size == 0, (cfg, bad[:10], ids[bad[:10]], idb[bad[:10]])
         h = cert & (idb >= 0)
@@ -69,6 +74,7 @@ def check_small(ctx, s, rays):
     ctx.set_traversal_width(4)
     ctx.set_quantized(True)
     ctx.set_persistent(True)
+    ctx.set_cluster(2)
     return cert.mean()
 
 
@@ -164,6 +170,49 @@ def test_soup_1m_reference_traversal(ctx):
         tg = o.prim_t(rays[mm], ids[mm])
         ok = (ids[mm] >= 0) & (idr[mm] >= 0) & (np.abs(tg - tr[mm]) <= 1e-6 * np.abs(tr[mm]))
         assert ok.all(), (mm[~ok][:5], ids[mm][~ok][:5], idr[mm][~ok][:5])
+
+
+def test_cluster_tree_icosphere(ctx):
+    """The clustered tree over a displaced icosphere (cheaper than the LBVH,
+    so the records group it): hits equal the oracle's reference-order LBVH
+    traversal except near-ties, with and without it, before and after a
+    refit."""
+    print("[gpu] trace displaced icosphere, clustered tree vs LBVH records", flush=True)
+    s = scenes.displaced_icosphere(64, 64, level=6)
+    ctx.upload(s)
+    ctx.set_cluster(1)
+    st = ctx.build(s.bounds_min, s.bounds_max)
+    info = ctx.cluster_info()
+    print(f"  {info}, cluster stage {st.cluster_ms:.3f} ms", flush=True)
+    assert info["in_use"] and 0 < info["depth"] <= 64
+    assert info["cluster_cost"] < info["lbvh_cost"]
+    rng = np.random.default_rng(5)
+    rays = scenes.random_rays(100_000, seed=7, lo=-0.45, hi=0.45)
+    cam = camera_rays(scenes.default_camera(96, 96), 96, 96, jitter_seed=3)
+    rays = np.concatenate([rays, cam[rng.permutation(len(cam))[:20_000]]])
+    o = O.OracleScene(s)
+    o.build(s.bounds_min, s.bounds_max)
+    tr, idr, _ = o.trace_ref(rays)
+    for stage in ("build", "refit"):
+        if stage == "refit":
+            st = ctx.refit()
+            assert ctx.cluster_info()["in_use"] and st.cluster_ms > 0
+        for cluster in (True, False):
+            ctx.set_cluster(cluster)
+            t, ids = ctx.trace(rays)
+            same = ids == idr
+            print(f"  {stage} cluster={cluster}: id agreement {same.mean():.6f}", flush=True)
+            assert same.mean() > 0.999
+            h = same & (idr >= 0)
+            rel = np.abs(t[h].astype(np.float64) - tr[h]) / np.abs(tr[h])
+            assert rel.max() <= T_REL
+            mm = np.where(~same)[0]
+            if mm.size:
+                tg = o.prim_t(rays[mm], ids[mm])
+                ok = (ids[mm] >= 0) & (idr[mm] >= 0) & (np.abs(tg - tr[mm]) <= 1e-6 * np.abs(tr[mm]))
+                assert ok.all(), (stage, cluster, mm[~ok][:5])
+        ctx.set_cluster(1)
+    ctx.set_cluster(2)
 
 
 def test_device_pointer_trace(ctx):
