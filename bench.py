@@ -11,7 +11,11 @@ per ray = 32 (ray) + 4 (queue permutation) + 32 (hit record) + 64 per
 four-wide quantized node record visited (112 = 7 of the 8 float4 of an FP32
 record with --records fp32) + 48 per primitive tested, node/primitive counts
 from a counting run of the same frame, divided by the summed HIP-event
-durations of the extension launches.
+durations of the extension launches.  The four-wide records group the
+clustered tree (CPT_OPT_CLUSTER) where its surface-area cost is clearly below
+the LBVH's, as on this scene; the render leg builds it with the BVH (mode 1,
+so its time is reported as `cluster_ms`) rather than at the first frame
+(mode 2, the default): the records are the same.
 
 Secondary workload (`lbvh`): LBVH build of a 10M-triangle soup (centroids
 U[0,1]^3, lognormal side mu=-6 sigma=0.5, random orientation, seed 1) and
@@ -60,8 +64,11 @@ def bench_render(cpt, L, scenes, args):
     ctx.set_quantized(args.records == "quant")
     ctx.set_traversal_width(args.width)
     ctx.set_persistent(not args.no_persistent)
+    ctx.set_cluster(0 if args.no_cluster else 1)
     ctx.upload(s)
     st_b = ctx.build(s.bounds_min, s.bounds_max)
+    tree = ctx.cluster_info()
+    log(f"scene build: LBVH {st_b.total_ms:.3f} ms, clustered tree {st_b.cluster_ms:.3f} ms, {tree}")
     kw = dict(seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
     for _ in range(args.warmup):
         ctx.render(s.camera, 1, **kw)
@@ -88,7 +95,8 @@ def bench_render(cpt, L, scenes, args):
                nodes_per_ray=stc.extend_nodes / max(stc.extend_rays, 1),
                prims_per_ray=stc.extend_prims / max(stc.extend_rays, 1),
                lane_utilisation=stc.extend_nodes / max(stc.extend_wave_nodes, 1),
-               achieved=achieved, build_ms=st_b.total_ms, mean=float(img.mean()))
+               achieved=achieved, build_ms=st_b.total_ms, cluster_ms=st_b.cluster_ms, tree=tree,
+               mean=float(img.mean()))
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -370,6 +378,8 @@ def main():
     ap.add_argument("--records", choices=["quant", "fp32"], default="quant",
                     help="four-wide node records read by the traversal")
     ap.add_argument("--no-c5", action="store_true", help="skip the many-view medium equal-relMSE leg")
+    ap.add_argument("--no-cluster", action="store_true",
+                    help="four-wide records from the LBVH's boxes even where the clustered tree is cheaper")
     ap.add_argument("--no-persistent", action="store_true",
                     help="one lane per ray instead of the persistent lane-refilling walks")
     ap.add_argument("--c5-views", type=int, default=32)
@@ -425,6 +435,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain_path_tracer_displaced_icosphere", "width_bvh": args.width,
                    "records": args.records,
+                   "traversal_tree": "clustered" if r["tree"]["in_use"] else "lbvh",
                    "triangles": ntri, "width": 1920,
                    "height": 1080, "spp": args.spp, "max_depth": 8, "rr_depth": 3},
         "mpix_per_s": round(1920 * 1080 / (r["ms"] * 1e-3) / 1e6, 3),
@@ -436,6 +447,9 @@ def main():
         "extend_prims_per_ray": round(r["prims_per_ray"], 2),
         "extend_lane_utilisation": round(r["lane_utilisation"], 3),
         "scene_lbvh_build_ms": round(r["build_ms"], 3),
+        "scene_cluster_ms": round(r["cluster_ms"], 3),
+        "tree_cost": {"lbvh": round(r["tree"]["lbvh_cost"], 2), "clustered": round(r["tree"]["cluster_cost"], 2),
+                      "clustered_depth": r["tree"]["depth"]},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(ext_kernel),
                      "kernel": ext_kernel,
