@@ -29,7 +29,7 @@ int ensure(cpt_ctx *ctx, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.bytes >= bytes) return CPT_OK;
   if (b.p) {
-    hipFree(b.p);
+    (void)hipFree(b.p);  // replaced by a larger allocation
     b.p = nullptr;
     b.bytes = 0;
   }
@@ -46,7 +46,7 @@ int ensure(cpt_ctx *ctx, DevBuf &b, size_t bytes) {
 }
 
 static void freeBuf(DevBuf &b) {
-  if (b.p) hipFree(b.p);
+  if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
 }
@@ -129,12 +129,13 @@ int cpt_create(cpt_ctx **out, int dev) {
 
 void cpt_destroy(cpt_ctx *ctx) {
   if (!ctx) return;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
+  // release path: errors have nothing to report to
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   releaseAll(ctx);
   for (int i = 0; i < 18; i++)
-    if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
-  if (ctx->ownStream) hipStreamDestroy(ctx->ownStream);
+    if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+  if (ctx->ownStream) (void)hipStreamDestroy(ctx->ownStream);
   delete ctx;
 }
 
@@ -307,7 +308,7 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
     }
     rc = packTriangles(ctx, ctx->tri64.as<double>(), matTmp.as<int32_t>(), nt, small);
     hipError_t e = hipStreamSynchronize(s);
-    if (matTmp.p) hipFree(matTmp.p);
+    if (matTmp.p) (void)hipFree(matTmp.p);
     if (rc) return rc;
     if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_upload", hipGetErrorString(e));
   } else {
@@ -339,7 +340,7 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri, i
   CPT_TRY_HIP(ctx, hipMemcpyAsync(mat.p, mids.data(), n * 4, hipMemcpyHostToDevice, s));
   rc = packTriangles(ctx, dst, mat.as<int32_t>(), n, base + first);
   hipError_t e = hipStreamSynchronize(s);
-  hipFree(mat.p);
+  (void)hipFree(mat.p);
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_update_triangles", hipGetErrorString(e));
   return CPT_OK;
@@ -473,16 +474,22 @@ int cpt_radix_sort_pairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n
     if ((rc = ensure(ctx, kA, n * 4)) || (rc = ensure(ctx, vA, n * 4))) { cleanup(); return rc; }
     dk = kA.as<uint32_t>();
     dv = vA.as<uint32_t>();
-    hipMemcpyAsync(dk, keys, n * 4, hipMemcpyHostToDevice, ctx->stream);
-    hipMemcpyAsync(dv, vals, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t ce = hipMemcpyAsync(dk, keys, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(dv, vals, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (ce != hipSuccess) {
+      cleanup();
+      return fail(ctx, CPT_ERR_HIP, "cpt_radix_sort_pairs", hipGetErrorString(ce));
+    }
   }
   if ((rc = ensure(ctx, kB, n * 4)) || (rc = ensure(ctx, vB, n * 4))) { cleanup(); return rc; }
   rc = radixSortPairs(ctx, dk, dv, kB.as<uint32_t>(), vB.as<uint32_t>(), n, bits, false);
+  hipError_t e = hipSuccess;
   if (!rc && !(flags & CPT_DEVICE_PTRS)) {
-    hipMemcpyAsync(keys, dk, n * 4, hipMemcpyDeviceToHost, ctx->stream);
-    hipMemcpyAsync(vals, dv, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    e = hipMemcpyAsync(keys, dk, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(vals, dv, n * 4, hipMemcpyDeviceToHost, ctx->stream);
   }
-  hipError_t e = hipStreamSynchronize(ctx->stream);
+  const hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = es;
   cleanup();
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_radix_sort_pairs", hipGetErrorString(e));

@@ -564,20 +564,21 @@ int cpt_denoiser_create(cpt_ctx *ctx, int32_t maxB, int32_t maxH, int32_t maxW, 
 void cpt_denoiser_destroy(cpt_denoiser *dn) {
   if (!dn) return;
   cpt_ctx *ctx = dn->ctx;
-  hipStreamSynchronize(ctx->stream);
+  // release path: errors have nothing to report to
+  (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
                     &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf};
   for (DevBuf *b : bufs)
-    if (b->p) hipFree(b->p);
+    if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kLevels; l++)
     for (int k = 0; k < 2; k++) {
-      if (dn->e[l][k].buf.p) hipFree(dn->e[l][k].buf.p);
-      if (dn->ge[l][k].buf.p) hipFree(dn->ge[l][k].buf.p);
+      if (dn->e[l][k].buf.p) (void)hipFree(dn->e[l][k].buf.p);
+      if (dn->ge[l][k].buf.p) (void)hipFree(dn->ge[l][k].buf.p);
     }
   for (int l = 0; l < kLevels - 1; l++) {
     Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->cat[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
     for (Tensor *t : ts)
-      if (t->buf.p) hipFree(t->buf.p);
+      if (t->buf.p) (void)hipFree(t->buf.p);
   }
   delete dn;
 }
@@ -710,7 +711,11 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
   auto cleanup = [&]() {
     DevBuf *all[] = {&dx, &dw, &dyb, &db, &dout, &wt};
     for (DevBuf *b : all)
-      if (b->p) hipFree(b->p);
+      if (b->p) (void)hipFree(b->p);
+  };
+  hipError_t ce = hipSuccess;  // first error of the calls below
+  auto chk = [&](hipError_t r) {
+    if (ce == hipSuccess) ce = r;
   };
   size_t nout = mode == 0 ? ny : (mode == 1 ? nw : nx);
   if ((rc = ensure(ctx, dx, nx * 2)) || (rc = ensure(ctx, dw, nw * 2)) || (rc = ensure(ctx, dyb, ny * 2)) ||
@@ -718,18 +723,18 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
     cleanup();
     return rc;
   }
-  if (x) hipMemcpyAsync(dx.p, x, nx * 2, hipMemcpyHostToDevice, s);
-  if (w) hipMemcpyAsync(dw.p, w, nw * 2, hipMemcpyHostToDevice, s);
-  if (dy) hipMemcpyAsync(dyb.p, dy, ny * 2, hipMemcpyHostToDevice, s);
-  if (bias) hipMemcpyAsync(db.p, bias, Cout * 4, hipMemcpyHostToDevice, s);
-  else hipMemsetAsync(db.p, 0, Cout * 4, s);
+  if (x) chk(hipMemcpyAsync(dx.p, x, nx * 2, hipMemcpyHostToDevice, s));
+  if (w) chk(hipMemcpyAsync(dw.p, w, nw * 2, hipMemcpyHostToDevice, s));
+  if (dy) chk(hipMemcpyAsync(dyb.p, dy, ny * 2, hipMemcpyHostToDevice, s));
+  if (bias) chk(hipMemcpyAsync(db.p, bias, Cout * 4, hipMemcpyHostToDevice, s));
+  else chk(hipMemsetAsync(db.p, 0, Cout * 4, s));
   const unsigned gx = (W + kConvPix - 1) / kConvPix;
   if (mode == 0) {
     hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cout / 32)), dim3(kConvThreads), 0, s, dx.as<uint16_t>(), Cin,
                        dw.as<uint16_t>(), db.as<float>(), Cout, H, W, (uint16_t *)nullptr, dout.as<float>(), Cout, 0,
                        (const float *)nullptr, 3);
   } else if (mode == 1) {
-    hipMemsetAsync(dout.p, 0, nout * 4, s);
+    chk(hipMemsetAsync(dout.p, 0, nout * 4, s));
     const int rows = 4, groups = (H + rows - 1) / rows;
     hipLaunchKernelGGL(k_conv_bwd_w, dim3((Cout + 63) / 64, Cin / 32, B * groups), dim3(kConvThreads), 0, s,
                        dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, H, W, rows, dout.as<float>());
@@ -748,16 +753,17 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
       uint32_t u = ((uint32_t)hw[i]) << 16;
       std::memcpy(&hf[i], &u, 4);
     }
-    hipMemcpyAsync(wf.p, hf.data(), nw * 4, hipMemcpyHostToDevice, s);
+    chk(hipMemcpyAsync(wf.p, hf.data(), nw * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack_weights, dim3(nb256(nw)), dim3(256), 0, s, wf.as<float>(), Cin, Cout, dw.as<uint16_t>(),
                        wt.as<uint16_t>());
     hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cin / 32)), dim3(kConvThreads), 0, s, dyb.as<uint16_t>(), Cout,
                        wt.as<uint16_t>(), (const float *)nullptr, Cin, H, W, (uint16_t *)nullptr, dout.as<float>(), Cin,
                        0, (const float *)nullptr, 3);
-    hipStreamSynchronize(s);
-    hipFree(wf.p);
+    chk(hipStreamSynchronize(s));
+    (void)hipFree(wf.p);
   }
-  hipError_t e = hipGetLastError();
+  chk(hipGetLastError());
+  hipError_t e = ce;
   if (e == hipSuccess) e = hipMemcpyAsync(out, dout.p, nout * 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   cleanup();

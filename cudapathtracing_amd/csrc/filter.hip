@@ -149,8 +149,12 @@ extern "C" int cpt_bilateral(cpt_ctx *ctx, const float *rgb, const float *aov, f
     dIn = base + 3 * n;
     dAov = base + 6 * n;
     dOut = base + 13 * n;
-    hipMemcpyAsync((void *)dIn, rgb, n * 12, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync((void *)dAov, aov, n * 28, hipMemcpyHostToDevice, s);
+    hipError_t ce = hipMemcpyAsync((void *)dIn, rgb, n * 12, hipMemcpyHostToDevice, s);
+    if (ce == hipSuccess) ce = hipMemcpyAsync((void *)dAov, aov, n * 28, hipMemcpyHostToDevice, s);
+    if (ce != hipSuccess) {
+      (void)hipFree(tmp.p);
+      return fail(ctx, CPT_ERR_HIP, "cpt_bilateral", hipGetErrorString(ce));
+    }
   }
   dim3 bs(16, 16), grid((w + 15) / 16, (h + 15) / 16);
   hipLaunchKernelGGL(k_bilateral_pass, grid, bs, 0, s, dIn, dAov, dMid, w, h, sigmaG, sigmaR, sigmaN, sigmaD, 1);
@@ -159,7 +163,7 @@ extern "C" int cpt_bilateral(cpt_ctx *ctx, const float *rgb, const float *aov, f
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && !(flags & CPT_DEVICE_PTRS)) e = hipMemcpyAsync(out, dOut, n * 12, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(tmp.p);
+  (void)hipFree(tmp.p);
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_bilateral", hipGetErrorString(e));
   return CPT_OK;
 }
@@ -184,15 +188,16 @@ extern "C" int cpt_temporal_blend(cpt_ctx *ctx, const float *rgb, const float *p
     dPrev = b + 3 * n;
     dOut = b + 6 * n;
     float *dIn = b + 9 * n;
-    hipMemcpyAsync((void *)dRgb, rgb, n * 12, hipMemcpyHostToDevice, s);
-    hipMemcpyAsync((void *)dPrev, prevRgb, n * 12, hipMemcpyHostToDevice, s);
-    if (position) {
-      hipMemcpyAsync(dIn, position, n * 12, hipMemcpyHostToDevice, s);
-      dPos = dIn;
-    } else {
-      hipMemcpyAsync(dIn, aov, n * 28, hipMemcpyHostToDevice, s);
-      dAov = dIn;
+    hipError_t ce = hipMemcpyAsync((void *)dRgb, rgb, n * 12, hipMemcpyHostToDevice, s);
+    if (ce == hipSuccess) ce = hipMemcpyAsync((void *)dPrev, prevRgb, n * 12, hipMemcpyHostToDevice, s);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(dIn, position ? (const void *)position : (const void *)aov,
+                                              n * (position ? 12 : 28), hipMemcpyHostToDevice, s);
+    if (ce != hipSuccess) {
+      (void)hipFree(tmp.p);
+      return fail(ctx, CPT_ERR_HIP, "cpt_temporal_blend", hipGetErrorString(ce));
     }
+    if (position) dPos = dIn;
+    else dAov = dIn;
   }
   PrevCam pc;
   for (int k = 0; k < 3; k++) {
@@ -207,7 +212,7 @@ extern "C" int cpt_temporal_blend(cpt_ctx *ctx, const float *rgb, const float *p
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && !(flags & CPT_DEVICE_PTRS)) e = hipMemcpyAsync(out, dOut, n * 12, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (tmp.p) hipFree(tmp.p);
+  if (tmp.p) (void)hipFree(tmp.p);
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_temporal_blend", hipGetErrorString(e));
   return CPT_OK;
 }

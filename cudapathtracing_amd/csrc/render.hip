@@ -813,7 +813,7 @@ void renderRelease(cpt_ctx *ctx) {
                    &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,   &r->keysS,
                    &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf};
   for (DevBuf *b : all)
-    if (b->p) hipFree(b->p);
+    if (b->p) (void)hipFree(b->p);  // release path: nothing to report to
   delete r;
   ctx->render = nullptr;
 }
@@ -1119,12 +1119,14 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(),
                        dstAov, npix * 7, inv);
   CPT_TRY_HIP(ctx, hipEventRecord(ev[10], s));
+  hipError_t e = hipSuccess;
   if (!(flags & CPT_DEVICE_PTRS)) {
-    hipMemcpyAsync(rgb, dstRgb, npix * 12, hipMemcpyDeviceToHost, s);
-    if (aovOut) hipMemcpyAsync(aovOut, dstAov, npix * 28, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(rgb, dstRgb, npix * 12, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && aovOut) e = hipMemcpyAsync(aovOut, dstAov, npix * 28, hipMemcpyDeviceToHost, s);
   }
-  hipError_t e = hipStreamSynchronize(s);
-  if (tmp.p) hipFree(tmp.p);
+  const hipError_t es = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = es;
+  if (tmp.p) (void)hipFree(tmp.p);
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_render", hipGetErrorString(e));
   if (stats) {
     float ms = 0;
