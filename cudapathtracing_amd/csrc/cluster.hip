@@ -285,7 +285,8 @@ int buildCluster(cpt_ctx *ctx) {
   const int64_t n = ctx->nPrims;
   ctx->clusterValid = true;
   ctx->clusterOk = false;
-  if (n < 2) return CPT_OK;
+  // preorder indices plus leaf counts stay below 2^31 up to 2^30 leaves
+  if (n < 2 || n > ((int64_t)1 << 30)) return CPT_OK;
   const int64_t nbMax = (n + kScanBlock - 1) / kScanBlock;
   int rc;
   if ((rc = ensure(ctx, ctx->plC[0], n * 4)) || (rc = ensure(ctx, ctx->plC[1], n * 4)) ||
