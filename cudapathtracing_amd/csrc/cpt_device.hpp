@@ -97,6 +97,18 @@ struct Rng {
   }
 };
 
+// Per-axis 1/d for the slab tests: zero direction components are replaced
+// by a tiny value so that the slab products stay finite (the primitive tests
+// use the true direction), and the reciprocal is the hardware one (1 ulp; a
+// slab distance only has to stay within the boxes' 1e-4 padding of the true
+// one, and the sign, which picks the near planes, is exact).
+__device__ __forceinline__ f3 slabInv(f3 d) {
+  const float eps = 1e-20f;
+  const f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
+                     fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
+  return mkf3(__builtin_amdgcn_rcpf(dd.x), __builtin_amdgcn_rcpf(dd.y), __builtin_amdgcn_rcpf(dd.z));
+}
+
 // ---- FP32 primitive tests --------------------------------------------------
 
 // Triangle::hit (Hittable.cuh:85-140) in FP32 with the reference's accept
@@ -238,12 +250,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
     }
     return hit;
   }
-  // Zero direction components are replaced by a tiny value so that the
-  // slab products stay finite (the triangle tests use the true direction).
-  const float eps = 1e-20f;
-  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
-               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
-  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  const f3 inv = slabInv(d);
   f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   int ovf[kStackTotal - kStackLds];
   int sp = 0;
@@ -521,10 +528,7 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
     }
     return hit;
   }
-  const float eps = 1e-20f;
-  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
-               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
-  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  const f3 inv = slabInv(d);
   f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   int ovf[kStack4Total - kStackLds];
   int sp = 0;
@@ -603,10 +607,7 @@ __device__ __forceinline__ int traceClosest8(const float4 *__restrict__ nodes8,
     }
     return hit;
   }
-  const float eps = 1e-20f;
-  f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
-               fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
-  f3 inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  const f3 inv = slabInv(d);
   f3 ood = mkf3(o.x * inv.x, o.y * inv.y, o.z * inv.z);
   int ovf[kStack8Total - kStackLds];
   int sp = 0;
@@ -730,11 +731,7 @@ struct Walk4 {
 
 // Starts the walk of the ray already in w.o, w.d, w.tmin, w.tbest (= tmax).
 __device__ __forceinline__ void walk4Start(Walk4 &w) {
-  const float eps = 1e-20f;
-  const f3 d = w.d;
-  const f3 dd = mkf3(fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x, fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y,
-                     fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z);
-  w.inv = mkf3(1.0f / dd.x, 1.0f / dd.y, 1.0f / dd.z);
+  w.inv = slabInv(w.d);
   w.ood = mkf3(w.o.x * w.inv.x, w.o.y * w.inv.y, w.o.z * w.inv.z);
   w.node = 0;
   w.sp = 0;
