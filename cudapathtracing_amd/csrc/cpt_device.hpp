@@ -723,6 +723,7 @@ struct Walk4 {
   f3 o, d, inv, ood;
   float tmin, tbest;
   int node, sp, hit;
+  int npend;  // queued leaf tests (walk4Step<..., kPostpone>); node < 0: none to visit
   // hit payload (walk4Step<..., kPayload>): the best hit's outward normal,
   // and material << 3 | type, set from its record when the hit is found
   f3 nrm;
@@ -736,6 +737,7 @@ __device__ __forceinline__ void walk4Start(Walk4 &w) {
   w.node = 0;
   w.sp = 0;
   w.hit = -1;
+  w.npend = 0;
   w.nrm = mkf3(0.f, 0.f, 0.f);
   w.tm = 0;
 }
@@ -756,13 +758,62 @@ __device__ __forceinline__ void walk4Init(Walk4 &w, f3 o, f3 d, float tmin, floa
   walk4Start(w);
 }
 
-// Visits record w.node.  Returns false when the walk is over (w.hit, w.tbest
-// hold the result; kAny: the first hit ends it).  kPayload: a new best hit
-// also sets w.nrm, w.type, w.mat from its record.
-template <bool kCount, bool kAny, bool kQuant = false, bool kPayload = false>
+// Leaf test of leaf li for walk4Step; returns true when a kAny walk is over.
+template <bool kCount, bool kAny, bool kPayload>
+__device__ __forceinline__ bool walk4Leaf(Walk4 &w, const float4 *__restrict__ leaves, int li, TraceCounters &cnt) {
+  float t;
+  if (kCount) cnt.prims++;
+  if (kPayload) {
+    const float4 q0 = leaves[3 * li + 0], q1 = leaves[3 * li + 1], q2 = leaves[3 * li + 2];
+    if (hitRecord(q0, q1, q2, w.o, w.d, w.tmin, w.tbest, t)) {
+      w.tbest = t;
+      w.hit = li;
+      const int type = __float_as_int(q1.w);
+      w.tm = (__float_as_int(q2.w) << 3) | type;
+      w.nrm = outwardNormal(q0, q1, q2, type, w.o + w.d * t);
+    }
+  } else if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
+    w.tbest = t;
+    w.hit = li;
+    if (kAny) return true;
+  }
+  return false;
+}
+
+constexpr int kPend = 6;  // queued leaf tests per lane (kPostpone)
+
+// One trip of a four-wide walk.  Returns false when the walk is over (w.hit,
+// w.tbest hold the result; kAny: the first hit ends it).  kPayload: a new
+// best hit also sets w.nrm, w.tm from its record.
+//
+// Without kPostpone a trip visits record w.node and tests its hit leaves in
+// a loop, one per loop round: a wave runs as many rounds as its lane with
+// the most leaf hits, while most lanes have none in a given record.  With
+// kPostpone (pnd: kPend LDS words per lane, stride apart) the leaves a visit
+// hits are queued, and the walking lanes of the wave vote: a leaf trip, in
+// which every lane with queued leaves tests one, once half of them have
+// some or none can visit (no record left, or no room for four more leaves);
+// otherwise a visit trip.  *visit says which (wave-uniform).
+template <bool kCount, bool kAny, bool kQuant = false, bool kPayload = false, bool kPostpone = false>
 __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ nodes4,
                                           const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
-                                          TraceCounters &cnt) {
+                                          TraceCounters &cnt, int *pnd = nullptr, bool *visit = nullptr) {
+  if (kPostpone) {
+    const bool canVisit = w.node >= 0 && w.npend <= kPend - 4;
+    const int nWalk = __popcll(__ballot(true));
+    const uint64_t vis = __ballot(canVisit);
+    const int nQueued = __popcll(__ballot(w.npend > 0));
+    const bool leafTrip = vis == 0 || 2 * nQueued >= nWalk;
+    *visit = !leafTrip;
+    if (leafTrip) {
+      if (w.npend > 0) {
+        w.npend--;
+        if (walk4Leaf<kCount, kAny, kPayload>(w, leaves, pnd[w.npend * stride], cnt)) return false;
+      }
+      return w.node >= 0 || w.npend > 0;
+    }
+    if (!canVisit) return true;
+  }
   if (kCount) cnt.nodes++;
   float sx0[4], sx1[4], sy0[4], sy1[4], sz0[4], sz1[4];
   int ref[4];
@@ -785,29 +836,27 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
     if (h[k] && ref[k] < 0) lm |= 1u << k;
     h[k] = h[k] && ref[k] >= 0;
   }
-  while (lm) {
-    const int k = __builtin_ctz(lm);
-    lm &= lm - 1;
-    const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
-    float t;
-    if (kCount) cnt.prims++;
-    if (kPayload) {
-      const float4 q0 = leaves[3 * li + 0], q1 = leaves[3 * li + 1], q2 = leaves[3 * li + 2];
-      if (hitRecord(q0, q1, q2, w.o, w.d, w.tmin, w.tbest, t)) {
-        w.tbest = t;
-        w.hit = li;
-        const int type = __float_as_int(q1.w);
-        w.tm = (__float_as_int(q2.w) << 3) | type;
-        w.nrm = outwardNormal(q0, q1, q2, type, w.o + w.d * t);
+  if (kPostpone) {
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (lm & (1u << k)) {
+        pnd[w.npend * stride] = ~ref[k];
+        w.npend++;
       }
-    } else if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
-      w.tbest = t;
-      w.hit = li;
-      if (kAny) return false;
+  } else {
+    while (lm) {
+      const int k = __builtin_ctz(lm);
+      lm &= lm - 1;
+      const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
+      if (walk4Leaf<kCount, kAny, kPayload>(w, leaves, li, cnt)) return false;
     }
   }
   if (!descend4<kAny>(h, tn, ref, stk, stride, ovf, w.sp, w.node)) {
-    if (w.sp == 0) return false;
+    if (w.sp == 0) {
+      if (!kPostpone) return false;
+      w.node = -1;
+      return w.npend > 0;
+    }
     w.sp--;
     w.node = w.sp < kStackLds ? stk[w.sp * stride] : ovf[w.sp - kStackLds];
   }

@@ -350,7 +350,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
     const float4 *__restrict__ leaves, float4 *__restrict__ hit, unsigned long long *__restrict__ work,
     const uint32_t *__restrict__ perm, uint32_t *grab) {
   __shared__ int stack[kStackLds * kRenderThreads];
+  __shared__ int pend[kPend * kRenderThreads];
   int *stk = stack + threadIdx.x;
+  int *pnd = pend + threadIdx.x;
   int ovf[kStack4Total - kStackLds];
   // a lane's ray is loaded straight into its (idle) walk state
   Walk4 w;
@@ -384,14 +386,16 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
       }
     }
     if (!ch.more && __ballot(st != kIdle) == 0) break;
-    if (kCount && __ballot(st == kActive)) trips++;
-    if (st == kActive &&
-        !walk4Step<kCount, false, kQuant, true>(w, nodes4, leaves, stk, kRenderThreads, ovf, cnt)) {
+    bool visit = false;
+    if (st == kActive && !walk4Step<kCount, false, kQuant, true, true>(w, nodes4, leaves, stk, kRenderThreads,
+                                                                          ovf, cnt, pnd, &visit)) {
       hit[2 * (int64_t)p] =
           make_float4(__int_as_float(w.hit), w.tbest, __int_as_float(w.tm & 7), __int_as_float(w.tm >> 3));
       hit[2 * (int64_t)p + 1] = make_float4(w.nrm.x, w.nrm.y, w.nrm.z, 0.f);
       st = kIdle;
     }
+    // record-visit trips only (the lane utilisation of the visits)
+    if (kCount && __ballot(visit)) trips++;
   }
   if (kCount) {
     unsigned long long a = (unsigned long long)cnt.nodes, b = (unsigned long long)cnt.prims;
