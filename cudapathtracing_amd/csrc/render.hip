@@ -741,6 +741,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   const MediumDev &med = a.med;
   const uint64_t seed = a.seed;
   __shared__ int stack[kStackLds * kRenderThreads];
+  __shared__ int pend[kPend * kRenderThreads];
   const uint32_t li = blockIdx.x * kRenderThreads + threadIdx.x;
   TraceCounters cnt;
   if ((int64_t)li < shadowCap && li < counters[1]) {
@@ -750,8 +751,21 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
     const f3 o = xyz(r0), d = xyz(r1);
     const float tL = r1.w;
     float tb = nextafterf(tL, -INFINITY);
-    const int leaf =
-        traceFmt<kCount, kFmt, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x, kRenderThreads, cnt);
+    int leaf;
+    if ((kFmt == 1 || kFmt == 2) && nPrims > 1) {
+      // four-wide walk with queued leaf tests (walk4Step)
+      int ovf[kStack4Total - kStackLds];
+      Walk4 w;
+      walk4Init(w, o, d, r0.w, tb);
+      bool visit = false;
+      while (walk4Step<kCount, true, kFmt == 2, false, true>(w, nodes, leaves, stack + threadIdx.x, kRenderThreads,
+                                                              ovf, cnt, pend + threadIdx.x, &visit)) {
+      }
+      leaf = w.hit;
+    } else {
+      leaf = traceFmt<kCount, kFmt, true>(nodes, leaves, nPrims, o, d, r0.w, tb, stack + threadIdx.x, kRenderThreads,
+                                          cnt);
+    }
     if (leaf < 0) {
       float tr = 1.0f;
       if (med.on) {
