@@ -23,7 +23,11 @@
 
 namespace cpt {
 
-constexpr int kStackLds = 16;    // traversal stack entries kept in LDS per lane
+// traversal stack entries kept in LDS per lane: 14, so that with the 6
+// queued-leaf words a 256-lane workgroup of the four-wide walks uses 20 KB of
+// LDS and 8 workgroups fit a CU's 160 KB (8 waves per SIMD where the
+// registers allow)
+constexpr int kStackLds = 14;
 constexpr int kStackTotal = 64;  // Karras depth <= 62 for unique 64-bit keys
 
 struct f3 {
