@@ -215,6 +215,37 @@ def test_cluster_tree_icosphere(ctx):
     ctx.set_cluster(2)
 
 
+def test_cluster_tree_coincident_primitives(ctx):
+    """20k copies of one triangle beside a random soup: every union area of
+    the copies ties, and the clustering pairs them off in about log2(n)
+    rounds (one pair per round without the buddy rule: seconds).  Hits over
+    either tree agree, up to which copy wins a tie."""
+    print("[gpu] clustered tree over coincident primitives", flush=True)
+    s = scenes.SceneData()
+    m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    one = np.array([[0.2, 0.2, 0.5, 0.8, 0.3, 0.5, 0.4, 0.9, 0.5]])
+    s.add_mesh(np.repeat(one, 20_000, axis=0), m)
+    s.add_mesh(scenes.triangle_soup(5_000, seed=6, mu=-3.5), m)
+    ctx.upload(s)
+    ctx.set_cluster(1)
+    try:
+        st = ctx.build(s.bounds_min, s.bounds_max)
+        info = ctx.cluster_info()
+        print(f"  cluster stage {st.cluster_ms:.2f} ms, {info}", flush=True)
+        assert st.cluster_ms < 250 and info["depth"] <= 64
+        rays = scenes.random_rays(50_000, seed=9, lo=-0.2, hi=1.2)
+        t1, i1 = ctx.trace(rays)
+        ctx.set_cluster(0)
+        t0, i0 = ctx.trace(rays)
+    finally:
+        ctx.set_cluster(2)
+    copies = lambda ids: (ids >= 0) & (ids < 20_000)
+    same = (i1 == i0) | (copies(i1) & copies(i0))
+    assert same.all(), np.where(~same)[0][:5]
+    assert np.array_equal(t1, t0)
+    assert copies(i0).mean() > 0.01
+
+
 def test_device_pointer_trace(ctx):
     import torch
     print("[gpu] trace with device pointers", flush=True)
