@@ -83,7 +83,7 @@ class Context:
         self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))
 
     def set_queue_sort_min(self, n: int):
-        """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never)."""
+        """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never, the default)."""
         self._check(self._lib.cpt_set_option(self._h, 5, n))
 
     def set_shadow_sort_min(self, n: int):

@@ -83,7 +83,10 @@ struct cpt_ctx {
   bool persistent = true;  // four-wide traces as persistent, lane-refilling kernels
   int numCU = 256;
   int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered
-  int64_t queueSortMin = (int64_t)1 << 20;  // path-tracer queues this long are reordered
+  // path-tracer bounce queues this long are reordered (0: never, the
+  // default: the walks are faster on the unsorted queue, whose append order
+  // keeps neighbouring pixels' paths together)
+  int64_t queueSortMin = 0;
   int64_t shadowSortMin = (int64_t)1 << 20;  // shadow queues this long are reordered
   bool lastTraceSorted = false;
 

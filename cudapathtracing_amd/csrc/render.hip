@@ -650,7 +650,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
     thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
     metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, flags);
-    a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
+    if (a.qkeys) a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
   }
 }
 
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
   rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
   thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
   metaOut[slot] = make_uint4(mt.x, mt.y, rng.cnt, mt.w);
-  a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
+  if (a.qkeys) a.qkeys[slot] = renderKey(a.rootBox, o, d) >> (24 - kQueueKeyBits);
 }
 
 
@@ -932,7 +932,8 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.lbuf = R.lbuf.as<float4>();
   a.base = 0;
   a.rootBox = ctx->rootBox.as<float>();
-  a.qkeys = R.keysA.as<uint32_t>();
+  // bounce-queue keys only where a queue may be reordered
+  a.qkeys = ctx->queueSortMin > 0 ? R.keysA.as<uint32_t>() : nullptr;
   a.skeys = R.keysS.as<uint32_t>();
   a.med.on = ctx->hasMedium ? 1 : 0;
   a.med.density = ctx->medDensity.as<float>();

@@ -147,8 +147,8 @@ const char *cpt_version(void);
  * identical up to exact-t ties. */
 #define CPT_OPT_QUANTIZED 4
 /* CPT_OPT_QUEUE_SORT_MIN: cpt_render traces a bounce queue of at least this
- * many paths in ray-key order (the same key as CPT_OPT_RAY_SORT_MIN); 0
- * disables it; default 2^20.  Radiance is unchanged. */
+ * many paths in the order of an 8-bit ray key (direction octant, origin
+ * cell); 0 (default) never.  Radiance is unchanged. */
 #define CPT_OPT_QUEUE_SORT_MIN 5
 /* CPT_OPT_SHADOW_SORT_MIN: cpt_render traces a bounce's shadow rays of at
  * least this many in the order of their origin Morton cell; 0 disables it;

@@ -73,7 +73,7 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
         ctx.set_traversal_width(4)
         ctx.set_quantized(True)
         ctx.set_persistent(True)
-        ctx.set_queue_sort_min(1 << 20)
+        ctx.set_queue_sort_min(0)
         ctx.set_shadow_sort_min(1 << 20)
     print(f"  cornell path agreement {f:.5f}")
     assert f >= 0.99
