@@ -7,7 +7,8 @@ U[0.05,0.8] and F0 U[0.02,0.9] per 64-triangle patch) inside the Cornell box,
 1920x1080, 64 spp, 8 path vertices, roulette from the 4th.  `value` is
 Mrays/s = (extension + shadow rays traced) / frame time.  `roofline` covers
 the dominant kernel, the extension (closest-hit) kernel: algorithmic bytes
-per ray = 32 (ray) + 4 (queue permutation) + 32 (hit record) + 64 per
+per ray = 32 (ray) + 32 (hit record) (+ 4 for the queue permutation where
+bounce queues are sorted, off by default) + 64 per
 four-wide quantized node record visited (112 = 7 of the 8 float4 of an FP32
 record with --records fp32) + 48 per primitive tested, node/primitive counts
 from a counting run of the same frame, divided by the summed HIP-event
@@ -83,9 +84,10 @@ def bench_render(cpt, L, scenes, args):
     rays = float(np.mean([f.rays for f in frames]))
     ext_ms = float(np.mean([f.extend_ms for f in frames]))
     ext_rays = float(np.mean([f.extend_rays for f in frames]))
-    # per ray: ray 32 B, queue permutation 4 B, hit record 32 B; per record
+    # per ray: ray 32 B, hit record 32 B (bounce queues are traced unsorted
+    # by default, so no permutation is read); per record
     # visit and per primitive test their record bytes
-    ext_bytes = 68.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
+    ext_bytes = 64.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
