@@ -121,6 +121,7 @@ struct RenderArgs {
   int nviews;
   uint64_t seed;
   int spp, sampleBase, profile, maxDepth, rrDepth, wantAov;
+  int tiled;  // batches hold whole samples: k_gen orders each sample's pixels in 8x8 tiles
   float rrProb;
   int64_t npix;
   MediumDev med;
@@ -247,6 +248,29 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// Pixel of position q (0 <= q < w*h) of a sample in tile order: 8-row bands
+// top to bottom, each in 8x8 tiles left to right (a narrower last tile, a
+// shorter last band), row-major inside a tile; a bijection on [0, w*h).  A
+// wave of 64 consecutive paths then covers an 8x8 tile rather than a strip
+// of one row, so its camera rays, and its first bounces, stay coherent.
+__device__ __forceinline__ uint32_t tilePixel(uint32_t q, uint32_t w, uint32_t h) {
+  const uint32_t j0 = (q / (8u * w)) * 8u;
+  const uint32_t hb = min(8u, h - j0);
+  const uint32_t r = q - j0 * w;
+  const uint32_t nT = w / 8u;
+  uint32_t i, jj;
+  if (r < nT * 8u * hb) {
+    const uint32_t t = r / (8u * hb), rr = r % (8u * hb);
+    jj = rr / 8u;
+    i = t * 8u + rr % 8u;
+  } else {
+    const uint32_t wr = w - nT * 8u, r2 = r - nT * 8u * hb;
+    jj = r2 / wr;
+    i = nT * 8u + r2 % wr;
+  }
+  return (j0 + jj) * w + i;
+}
+
 __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t base, int64_t count,
                                                          float4 *__restrict__ ray, float4 *__restrict__ thr,
                                                          uint4 *__restrict__ meta) {
@@ -257,10 +281,14 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   // path gi: sample-major over all views' pixels; the stream of a pixel is
   // keyed by its index over all views (view * npix + pixel)
   const int64_t npixAll = a.npix * a.nviews;
-  const uint32_t pixel = (uint32_t)(gi % npixAll);
+  const uint32_t q = (uint32_t)(gi % npixAll);
   const uint32_t sample = (uint32_t)(a.sampleBase + gi / npixAll);
-  const CamDev &cam = a.cams[pixel / (uint32_t)a.npix];
-  const uint32_t pv = pixel % (uint32_t)a.npix;
+  const CamDev &cam = a.cams[q / (uint32_t)a.npix];
+  // with whole samples per batch, the batch's paths are the same set in
+  // tile order (each radiance slot is the path's own, see pathSlot)
+  const uint32_t pv = a.tiled ? tilePixel(q % (uint32_t)a.npix, (uint32_t)cam.w, (uint32_t)cam.h)
+                              : q % (uint32_t)a.npix;
+  const uint32_t pixel = q - q % (uint32_t)a.npix + pv;
   Rng rng;
   rng.init(a.seed, pixel, sample, 0);
   const double rx = rng.next(), ry = rng.next();
@@ -865,6 +893,9 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 26);
   cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 27) / nl));
   cap = std::min(cap, total);
+  // whole samples per batch where a batch holds at least one (k_gen then
+  // orders each sample's pixels in tiles)
+  if (cap >= npix) cap -= cap % npix;
   const int64_t shadowCap = cap * nl;
   int rc;
   for (int b = 0; b < 2; b++)
@@ -929,6 +960,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.rrProb = (float)prm->rr_prob;
   a.wantAov = aovOut ? 1 : 0;
   a.npix = npixView;
+  a.tiled = cap % npix == 0 ? 1 : 0;
   a.lbuf = R.lbuf.as<float4>();
   a.base = 0;
   a.rootBox = ctx->rootBox.as<float>();
