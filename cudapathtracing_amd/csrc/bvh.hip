@@ -339,17 +339,13 @@ __global__ void k_emit(const uint64_t *__restrict__ keys, int n, float4 *__restr
 __device__ __forceinline__ void storeChildBox(float *__restrict__ nf, int64_t node, int slot,
                                               const float bx[6]) {
   float *p = nf + 16 * node;
-  const int o = slot * 4;
-  p[o + 0] = bx[0];
-  p[o + 1] = bx[3];
-  p[o + 2] = bx[1];
-  p[o + 3] = bx[4];
-  p[8 + slot * 2 + 0] = bx[2];
-  p[8 + slot * 2 + 1] = bx[5];
+  *(float4 *)(p + slot * 4) = make_float4(bx[0], bx[3], bx[1], bx[4]);
+  *(float2 *)(p + 8 + slot * 2) = make_float2(bx[2], bx[5]);
 }
 
 // Refit in two passes.  Pass 1 (k_refit_local): a workgroup owns a chunk of
-// kRefitChunk consecutive sorted leaves and finishes, bottom-up in LDS with
+// kRefitChunk consecutive sorted leaves, one thread per leaf so that every
+// climb of the chunk is in flight at once, and finishes, bottom-up in LDS with
 // workgroup-scope counters, every internal node whose leaf range lies inside
 // the chunk.  Each child of a node that crosses a chunk boundary is written
 // to global memory and its parent appended to an escape list.  Pass 2
@@ -358,15 +354,15 @@ __device__ __forceinline__ void storeChildBox(float *__restrict__ nf, int64_t no
 // common ancestor of leaves split(i), split(i)+1, so the crossing nodes are
 // the ancestors of the chunk-boundary splits: at most 64 per boundary
 // (depth <= 64 for unique 64-bit keys), two escapes each.
-constexpr int kRefitThreads = 256;
-constexpr int kRefitChunk = 1024;
+constexpr int kRefitThreads = 256;  // k_refit_cross
+constexpr int kRefitChunk = 1024;   // leaves (and threads) per k_refit_local workgroup
 
 int64_t refitEscapeBound(int64_t n) {
   const int64_t chunks = (n + kRefitChunk - 1) / kRefitChunk;
   return std::min<int64_t>(2 * n, 128 * chunks + 2);
 }
 
-__global__ __launch_bounds__(kRefitThreads) void k_refit_local(
+__global__ __launch_bounds__(kRefitChunk) void k_refit_local(
     int n, const uint32_t *__restrict__ order, const float4 *__restrict__ rec, const float *__restrict__ box32,
     float4 *__restrict__ nodes, const int32_t *__restrict__ parentInt, const int32_t *__restrict__ parentLeaf,
     int32_t *__restrict__ flags, float4 *__restrict__ leaves, float *__restrict__ rootBox,
@@ -375,63 +371,69 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_local(
   __shared__ int cnt[kRefitChunk];
   const int c0 = blockIdx.x * kRefitChunk;
   const int c1 = min(c0 + kRefitChunk, n) - 1;  // last leaf of the chunk
-  for (int k = threadIdx.x; k < kRefitChunk; k += kRefitThreads) {
-    cnt[k] = 0;
-    if (c0 + k < n - 1) flags[c0 + k] = 0;  // counters of pass 2
-  }
+  cnt[threadIdx.x] = 0;
+  if (c0 + (int)threadIdx.x < n - 1) flags[c0 + threadIdx.x] = 0;  // counters of pass 2
   __syncthreads();
+  const int i = c0 + (int)threadIdx.x;
+  if (i > c1) return;
   float *nf = (float *)nodes;
-  for (int r = 0; r < kRefitChunk / kRefitThreads; r++) {
-    const int i = c0 + r * kRefitThreads + (int)threadIdx.x;
-    if (i > c1) break;
-    const uint32_t p = order[i];
-    {
-      // triangles: store the edges the FP32 test forms from the vertices
-      const float4 q0 = rec[3 * (int64_t)p + 0];
-      float4 q1 = rec[3 * (int64_t)p + 1], q2 = rec[3 * (int64_t)p + 2];
-      if (__float_as_int(q1.w) == CPT_OBJ_TRIANGLE) {
-        q1 = make_float4(q1.x - q0.x, q1.y - q0.y, q1.z - q0.z, q1.w);
-        q2 = make_float4(q2.x - q0.x, q2.y - q0.y, q2.z - q0.z, q2.w);
-      }
-      leaves[3 * (int64_t)i + 0] = q0;
-      leaves[3 * (int64_t)i + 1] = q1;
-      leaves[3 * (int64_t)i + 2] = q2;
+  const uint32_t p = order[i];
+  {
+    // triangles: store the edges the FP32 test forms from the vertices
+    const float4 q0 = rec[3 * (int64_t)p + 0];
+    float4 q1 = rec[3 * (int64_t)p + 1], q2 = rec[3 * (int64_t)p + 2];
+    if (__float_as_int(q1.w) == CPT_OBJ_TRIANGLE) {
+      q1 = make_float4(q1.x - q0.x, q1.y - q0.y, q1.z - q0.z, q1.w);
+      q2 = make_float4(q2.x - q0.x, q2.y - q0.y, q2.z - q0.z, q2.w);
     }
-    float bx[6];
+    leaves[3 * (int64_t)i + 0] = q0;
+    leaves[3 * (int64_t)i + 1] = q1;
+    leaves[3 * (int64_t)i + 2] = q2;
+  }
+  float bx[6];
+  {
+    const float2 *b2 = (const float2 *)(box32 + 6 * (int64_t)p);
+    const float2 u = b2[0], v = b2[1], w = b2[2];
+    bx[0] = u.x;
+    bx[1] = u.y;
+    bx[2] = v.x;
+    bx[3] = v.y;
+    bx[4] = w.x;
+    bx[5] = w.y;
+  }
+  if (n == 1) {
 #pragma unroll
-    for (int k = 0; k < 6; k++) bx[k] = box32[6 * (int64_t)p + k];
-    if (n == 1) {
+    for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+    return;
+  }
+  int par = parentLeaf[i];
+  while (true) {
+    const int node = par >> 1, slot = par & 1;
+    storeChildBox(nf, node, slot, bx);
+    // the node's leaf range and its parent, loaded together (parentInt[0] = -1)
+    const float4 rg = nodes[4 * (int64_t)node + 3];  // z, w: first and last leaf
+    const int parNext = parentInt[node];
+    if (__float_as_int(rg.z) < c0 || __float_as_int(rg.w) > c1) {
+      const uint32_t e = atomicAdd(escCount, 1u);
+      if ((int64_t)e < escCap) esc[e] = node;
+      return;
+    }
+    float *s = cbox + (node - c0) * 12;
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[slot * 6 + k] = bx[k];
+    const int old = __hip_atomic_fetch_add(&cnt[node - c0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0) return;  // the sibling's thread continues
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bx[k] = fminf(bx[k], s[(slot ^ 1) * 6 + k]);
+      bx[k + 3] = fmaxf(bx[k + 3], s[(slot ^ 1) * 6 + k + 3]);
+    }
+    if (node == 0) {
 #pragma unroll
       for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
-      break;
+      return;
     }
-    int par = parentLeaf[i];
-    while (true) {
-      const int node = par >> 1, slot = par & 1;
-      storeChildBox(nf, node, slot, bx);
-      const float4 rg = nodes[4 * (int64_t)node + 3];  // z, w: first and last leaf
-      if (__float_as_int(rg.z) < c0 || __float_as_int(rg.w) > c1) {
-        const uint32_t e = atomicAdd(escCount, 1u);
-        if ((int64_t)e < escCap) esc[e] = node;
-        break;
-      }
-      float *s = cbox + (node - c0) * 12;
-#pragma unroll
-      for (int k = 0; k < 6; k++) s[slot * 6 + k] = bx[k];
-      const int old = __hip_atomic_fetch_add(&cnt[node - c0], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;  // the sibling's thread continues
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        bx[k] = fminf(bx[k], s[(slot ^ 1) * 6 + k]);
-        bx[k + 3] = fmaxf(bx[k + 3], s[(slot ^ 1) * 6 + k + 3]);
-      }
-      if (node == 0) {
-#pragma unroll
-        for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
-        break;
-      }
-      par = parentInt[node];
-    }
+    par = parNext;
   }
 }
 
@@ -860,7 +862,7 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
   uint32_t *escCount = (uint32_t *)(esc + cap);
   hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)escCount, (int64_t)1);
   const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
-  hipLaunchKernelGGL(k_refit_local, dim3(chunks), dim3(kRefitThreads), 0, ctx->stream, (int)n,
+  hipLaunchKernelGGL(k_refit_local, dim3(chunks), dim3(kRefitChunk), 0, ctx->stream, (int)n,
                      ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
                      ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>(),
                      ctx->flags.as<int32_t>(), ctx->leaves.as<float4>(), ctx->rootBox.as<float>(), esc, escCount,
