@@ -533,38 +533,61 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
                             float4 *__restrict__ nodes4q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  float b[6][4];
-  int ref[4];
-  int k = 0;
+  // Every slot index below is a compile-time constant (selects, no running
+  // counter), so the slot arrays stay in registers.
   const float4 a0 = nodes[4 * (int64_t)i], a1 = nodes[4 * (int64_t)i + 1], a2 = nodes[4 * (int64_t)i + 2],
                a3 = nodes[4 * (int64_t)i + 3];
-  const int cr[2] = {__float_as_int(a3.x), __float_as_int(a3.y)};
+  const int cr0 = __float_as_int(a3.x), cr1 = __float_as_int(a3.y);
+  const bool in0 = cr0 >= 0, in1 = cr1 >= 0;
   // child boxes of a node: slot 0 = A, 1 = B (see storeChildBox)
-  const float cb[2][6] = {{a0.x, a0.z, a2.x, a0.y, a0.w, a2.y}, {a1.x, a1.z, a2.z, a1.y, a1.w, a2.w}};
+  const float cb0[6] = {a0.x, a0.z, a2.x, a0.y, a0.w, a2.y}, cb1[6] = {a1.x, a1.z, a2.z, a1.y, a1.w, a2.w};
+  // the two children of each internal child
+  float g0[2][6] = {}, g1[2][6] = {};
+  int g0r[2] = {0, 0}, g1r[2] = {0, 0};
+  if (in0) {
+    const int64_t g = cr0;
+    const float4 q0 = nodes[4 * g], q1 = nodes[4 * g + 1], q2 = nodes[4 * g + 2], q3 = nodes[4 * g + 3];
+    const float gb[2][6] = {{q0.x, q0.z, q2.x, q0.y, q0.w, q2.y}, {q1.x, q1.z, q2.z, q1.y, q1.w, q2.w}};
 #pragma unroll
-  for (int c = 0; c < 2; c++) {
-    if (cr[c] < 0) {
-#pragma unroll
-      for (int q = 0; q < 6; q++) b[q][k] = cb[c][q];
-      ref[k++] = cr[c];
-    } else {
-      const int64_t g = cr[c];
-      const float4 g0 = nodes[4 * g], g1 = nodes[4 * g + 1], g2 = nodes[4 * g + 2], g3 = nodes[4 * g + 3];
-      const float gb[2][6] = {{g0.x, g0.z, g2.x, g0.y, g0.w, g2.y}, {g1.x, g1.z, g2.z, g1.y, g1.w, g2.w}};
-      const int gr[2] = {__float_as_int(g3.x), __float_as_int(g3.y)};
-#pragma unroll
-      for (int e = 0; e < 2; e++) {
-#pragma unroll
-        for (int q = 0; q < 6; q++) b[q][k] = gb[e][q];
-        ref[k++] = gr[e];
-      }
+    for (int q = 0; q < 6; q++) {
+      g0[0][q] = gb[0][q];
+      g0[1][q] = gb[1][q];
     }
+    g0r[0] = __float_as_int(q3.x);
+    g0r[1] = __float_as_int(q3.y);
   }
+  if (in1) {
+    const int64_t g = cr1;
+    const float4 q0 = nodes[4 * g], q1 = nodes[4 * g + 1], q2 = nodes[4 * g + 2], q3 = nodes[4 * g + 3];
+    const float gb[2][6] = {{q0.x, q0.z, q2.x, q0.y, q0.w, q2.y}, {q1.x, q1.z, q2.z, q1.y, q1.w, q2.w}};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      g1[0][q] = gb[0][q];
+      g1[1][q] = gb[1][q];
+    }
+    g1r[0] = __float_as_int(q3.x);
+    g1r[1] = __float_as_int(q3.y);
+  }
+  // Slots in order: child A's entries (itself if a leaf, else its two
+  // children), then child B's; unused slots are empty.
+  float b[6][4];
+  int ref[4];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    b[q][0] = in0 ? g0[0][q] : cb0[q];
+    b[q][1] = in0 ? g0[1][q] : (in1 ? g1[0][q] : cb1[q]);
+    b[q][2] = in0 ? (in1 ? g1[0][q] : cb1[q]) : (in1 ? g1[1][q] : 0.f);
+    b[q][3] = (in0 && in1) ? g1[1][q] : 0.f;
+  }
+  ref[0] = in0 ? g0r[0] : cr0;
+  ref[1] = in0 ? g0r[1] : (in1 ? g1r[0] : cr1);
+  ref[2] = in0 ? (in1 ? g1r[0] : cr1) : (in1 ? g1r[1] : kEmpty4);
+  ref[3] = (in0 && in1) ? g1r[1] : kEmpty4;
   // One leaf child and one internal child leave a slot free: open the
   // grandchild with the larger surface area into it.  Every internal entry
   // then still lies at least two binary levels below node i, so a walk pushes
   // at most three entries per two levels.
-  if (k == 3) {
+  if (in0 != in1) {
     int pick = -1;
     float best = -1.f;
 #pragma unroll
@@ -578,26 +601,21 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
       }
     }
     if (pick >= 0) {
-      const int64_t g = ref[pick];
-      const float4 g0 = nodes[4 * g], g1 = nodes[4 * g + 1], g2 = nodes[4 * g + 2], g3 = nodes[4 * g + 3];
-      const float gb[2][6] = {{g0.x, g0.z, g2.x, g0.y, g0.w, g2.y}, {g1.x, g1.z, g2.z, g1.y, g1.w, g2.w}};
+      const int64_t g = pick == 0 ? ref[0] : pick == 1 ? ref[1] : ref[2];
+      const float4 g0v = nodes[4 * g], g1v = nodes[4 * g + 1], g2v = nodes[4 * g + 2], g3v = nodes[4 * g + 3];
+      const float gb[2][6] = {{g0v.x, g0v.z, g2v.x, g0v.y, g0v.w, g2v.y},
+                              {g1v.x, g1v.z, g2v.z, g1v.y, g1v.w, g2v.w}};
 #pragma unroll
       for (int s = 0; s < 3; s++)
         if (s == pick) {
 #pragma unroll
           for (int q = 0; q < 6; q++) b[q][s] = gb[0][q];
-          ref[s] = __float_as_int(g3.x);
+          ref[s] = __float_as_int(g3v.x);
         }
 #pragma unroll
       for (int q = 0; q < 6; q++) b[q][3] = gb[1][q];
-      ref[3] = __float_as_int(g3.y);
-      k = 4;
+      ref[3] = __float_as_int(g3v.y);
     }
-  }
-  for (; k < 4; k++) {
-#pragma unroll
-    for (int q = 0; q < 6; q++) b[q][k] = 0.f;
-    ref[k] = (int)0x80000000;
   }
   if (nodes4) {
     float4 *o = nodes4 + 8 * (int64_t)i;
