@@ -650,7 +650,7 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
 // cpt_device.hpp): node i's subtree expanded three binary levels deep, so
 // every internal entry is a great-grandchild and every leaf entry lies at
 // depth 1-3.
-__global__ void k_collapse8(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes8) {
+__global__ __launch_bounds__(256) void k_collapse8(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes8) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   float b[6][8];
