@@ -5,7 +5,7 @@ in ``include/cpt.h``); every computation runs in the HIP library.
 """
 from . import _lib
 from ._lib import lib
-from .context import Context, CptError, camera, tonemap_rgba8
+from .context import Context, CptError, camera, tonemap_rgba8, write_pfm, write_png
 from .scenes import SceneData
 
-__all__ = ["Context", "CptError", "SceneData", "camera", "tonemap_rgba8", "lib", "_lib"]
+__all__ = ["Context", "CptError", "SceneData", "camera", "tonemap_rgba8", "write_pfm", "write_png", "lib", "_lib"]

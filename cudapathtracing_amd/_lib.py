@@ -29,6 +29,9 @@ OBJ_NONE, OBJ_SPHERE, OBJ_LIGHT, OBJ_TRIANGLE = range(4)
 
 PROFILE_REFERENCE = 0
 PROFILE_CONFIG = 1
+PROFILE_REAL_TIME = 2
+PROFILE_NORMAL = 3
+PROFILE_DEPTH = 4
 RENDER_COUNT_WORK = 1
 
 D3 = C.c_double * 3
@@ -139,6 +142,10 @@ SIGNATURES = [
     ("cpt_tri_bounds", C.c_int, [P, C.c_int64, dp, dp]),
     ("cpt_scene_set_medium", C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, dp, dp, C.c_double, dp]),
     ("cpt_gen_fbm_grid", C.c_int, [C.c_int32, C.c_uint64, P]),
+    ("cpt_request_abort", C.c_int, [P]),
+    ("cpt_clear_abort", C.c_int, [P]),
+    ("cpt_write_pfm", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32]),
+    ("cpt_write_png", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32, C.c_int32]),
 ]
 
 _lib = None

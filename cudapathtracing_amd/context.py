@@ -104,6 +104,14 @@ class Context:
     def synchronize(self):
         self._check(self._lib.cpt_synchronize(self._h))
 
+    def request_abort(self):
+        """Stops the running or next call (it raises CptError with code
+        CPT_ERR_ABORTED) until clear_abort; callable from another thread."""
+        self._check(self._lib.cpt_request_abort(self._h))
+
+    def clear_abort(self):
+        self._check(self._lib.cpt_clear_abort(self._h))
+
     def _torch_stream(self, *tensors):
         """Runs the library on torch's current stream when given device tensors."""
         import torch
@@ -341,3 +349,22 @@ def tonemap_rgba8(rgb: np.ndarray) -> np.ndarray:
     if rc:
         raise CptError(rc, "tonemap")
     return out
+
+
+def write_pfm(path: str, rgb: np.ndarray):
+    """Float framebuffer (h x w x 3, row 0 = bottom) as a PFM file."""
+    h, w = rgb.shape[:2]
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    rc = L.lib().cpt_write_pfm(path.encode(), rgb.ctypes.data, w, h)
+    if rc:
+        raise CptError(rc, f"cannot write {path}")
+
+
+def write_png(path: str, rgb: np.ndarray, gamma: bool = True):
+    """The reference's displayed bytes (pic2RGBW with gamma, the NORMAL /
+    DEPTH display without) as an 8-bit RGBA PNG."""
+    h, w = rgb.shape[:2]
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    rc = L.lib().cpt_write_png(path.encode(), rgb.ctypes.data, w, h, 1 if gamma else 0)
+    if rc:
+        raise CptError(rc, f"cannot write {path}")

@@ -120,6 +120,22 @@ int cpt_create(cpt_ctx **out, int dev) {
     return CPT_ERR_HIP;
   }
   ctx->stream = ctx->ownStream;
+  // the abort word: coherent pinned memory, so a host store reaches a
+  // running kernel's system-scope loads
+  void *ab = nullptr, *abDev = nullptr;
+  if (hipHostMalloc(&ab, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer(&abDev, ab, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    if (ab) (void)hipHostFree(ab);
+    for (int i = 0; i < 18; i++)
+      if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->ownStream);
+    delete ctx;
+    return CPT_ERR_HIP;
+  }
+  ctx->abortHost = (uint32_t *)ab;
+  ctx->abortDev = (const uint32_t *)abDev;
+  __atomic_store_n(ctx->abortHost, 0u, __ATOMIC_RELEASE);
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
     ctx->numCU = cus;
@@ -136,7 +152,20 @@ void cpt_destroy(cpt_ctx *ctx) {
   for (int i = 0; i < 18; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
   if (ctx->ownStream) (void)hipStreamDestroy(ctx->ownStream);
+  if (ctx->abortHost) (void)hipHostFree(ctx->abortHost);
   delete ctx;
+}
+
+int cpt_request_abort(cpt_ctx *ctx) {
+  if (!ctx || !ctx->abortHost) return CPT_ERR_ARG;
+  __atomic_store_n(ctx->abortHost, 1u, __ATOMIC_RELEASE);
+  return CPT_OK;
+}
+
+int cpt_clear_abort(cpt_ctx *ctx) {
+  if (!ctx || !ctx->abortHost) return CPT_ERR_ARG;
+  __atomic_store_n(ctx->abortHost, 0u, __ATOMIC_RELEASE);
+  return CPT_OK;
 }
 
 const char *cpt_last_error(const cpt_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -496,11 +525,16 @@ int cpt_radix_sort_pairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, int64_t n
   return CPT_OK;
 }
 
+static bool aborted(const cpt_ctx *ctx) {
+  return ctx->abortHost && __atomic_load_n(ctx->abortHost, __ATOMIC_ACQUIRE) != 0u;
+}
+
 static int traceCommon(cpt_ctx *ctx, const float *rays, int64_t n, float *tOut, int32_t *idOut,
                        int32_t *nodesOut, int32_t *primsOut, int flags) {
   if (!ctx || n < 0 || (n > 0 && !rays)) return CPT_ERR_ARG;
   if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_trace_closest", "no BVH built");
   if (n == 0) return CPT_OK;
+  if (aborted(ctx)) return fail(ctx, CPT_ERR_ABORTED, "cpt_trace_closest", "aborted by cpt_request_abort");
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   int rc;
   if (flags & CPT_DEVICE_PTRS) {
@@ -527,6 +561,9 @@ static int traceCommon(cpt_ctx *ctx, const float *rays, int64_t n, float *tOut, 
     }
     CPT_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
+  // the persistent walks stop taking rays once the abort word is set: the
+  // outputs are then incomplete
+  if (aborted(ctx)) return fail(ctx, CPT_ERR_ABORTED, "cpt_trace_closest", "aborted by cpt_request_abort");
   return CPT_OK;
 }
 

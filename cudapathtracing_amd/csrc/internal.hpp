@@ -96,6 +96,11 @@ struct cpt_ctx {
   float lastTraceMs = 0.f;
   hipEvent_t ev[18] = {};
 
+  // abort word in pinned host memory (cpt_request_abort): the persistent
+  // walks poll it through abortDev, the render loop between bounces
+  uint32_t *abortHost = nullptr;
+  const uint32_t *abortDev = nullptr;
+
   cpt::RenderBufs *render = nullptr;
 };
 

@@ -34,7 +34,20 @@ constexpr float kRayTMax = 999999.0f;     // INF, rtmath.cuh:8
 constexpr uint32_t kFlagNee = 1u << 16;   // hasSampleLight (Render.cu:56)
 constexpr uint32_t kFlagAov = 1u << 17;   // first-vertex AOV written
 constexpr uint32_t kDepthMask = 0xffffu;
+// Roulette draws u are multiples of 2^-24 (exact in FP32 and FP64), so the
+// reference's FP64 test u > p equals the FP32 test u > rdn(p), p rounded
+// down to FP32: no multiple of 2^-24 lies in (rdn(p), p].  Round-to-nearest
+// can round p up past such a multiple (0.8f > 0.8), flipping that draw.
+constexpr float kRoulette07 = 0.699999988079071044921875f;  // rdn(0.7) (= 0.7f), Render.cu:202
+constexpr float kRoulette08 = 0.7999999523162841796875f;    // rdn(0.8) (0.8f rounds up), Render.cu:89
 constexpr int kMaxBounces = 4096;
+// Error bits the kernels set (atomicOr) where an invariant the queue sizes
+// rely on is broken; the host turns any of them into CPT_ERR_STATE instead
+// of a silently wrong image.
+constexpr uint32_t kErrShadowCap = 1u;  // more shadow rays than the shadow queue holds
+constexpr uint32_t kErrQueueCap = 2u;   // a next-queue or volume-queue slot beyond the batch
+constexpr uint32_t kErrMaterial = 4u;   // a hit record's material index outside the table
+constexpr uint32_t kErrSlot = 8u;       // a radiance slot or queue entry outside the batch
 
 struct RenderBufs {
   int64_t cap = 0, shadowCap = 0;
@@ -43,6 +56,9 @@ struct RenderBufs {
   // the survivors compacted into the other set
   DevBuf ray[2], thr[2], meta[2];
   DevBuf hit, permA, permB, qVol, keysA, keysB, keysS, shadow, counters, fb, aov, cams, lbuf;
+  DevBuf stage;                   // device copy of the outputs for host-pointer calls
+  uint32_t *hostCnt = nullptr;    // pinned: per bounce parity, the counters read back
+  hipEvent_t ev[2][7] = {};       // per bounce parity: extend, shade, shadow pairs, readback
 };
 
 struct CamDev {
@@ -122,7 +138,8 @@ struct RenderArgs {
   uint64_t seed;
   int spp, sampleBase, profile, maxDepth, rrDepth, wantAov;
   int tiled;  // batches hold whole samples: k_gen orders each sample's pixels in 8x8 tiles
-  float rrProb;
+  float rrProb;  // survival probability (the weight 1 / rrProb)
+  float rrCmp;   // rdn(rr_prob): the roulette compares u > rrCmp
   int64_t npix;
   MediumDev med;
   int64_t base;  // first path of the batch (path g = (sample - sampleBase) * npix * nviews + pixel)
@@ -130,6 +147,9 @@ struct RenderArgs {
   // sort keys written with each next-queue and shadow ray (renderKey)
   const float *rootBox;
   uint32_t *qkeys, *skeys;
+  uint32_t *err;   // error bits (kErr*)
+  int64_t cap;     // paths per batch: bound of every queue slot and radiance slot
+  int nMats;       // entries of the material table (materials + one per light)
 };
 
 // 24-bit sort key of a ray for the queue orderings: direction octant over
@@ -314,7 +334,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
 // in slot perm[k] and writes its hit record at that slot, so the shading
 // kernel reads every array at the same slot.
 template <bool kCount, int kFmt, bool kPerm>
-__global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
+__global__ __launch_bounds__(kRenderThreads) void k_extend(const uint32_t *__restrict__ countDev,
                                                             const float4 *__restrict__ ray,
                                                             const float4 *__restrict__ nodes,
                                                             const float4 *__restrict__ leaves, int nPrims,
@@ -323,6 +343,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
                                                             const uint32_t *__restrict__ perm) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  const uint32_t count = *countDev;  // the grid covers an upper bound of it
   TraceCounters cnt;
   if (k < count) {
     const uint32_t p = kPerm ? perm[k] : k;
@@ -374,9 +395,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend(uint32_t count,
 // the loop trips in which the wave walked.  grab: a zeroed counter.
 template <bool kCount, bool kQuant, bool kPerm>
 __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
-    uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ nodes4,
+    const uint32_t *__restrict__ countDev, const float4 *__restrict__ ray, const float4 *__restrict__ nodes4,
     const float4 *__restrict__ leaves, float4 *__restrict__ hit, unsigned long long *__restrict__ work,
-    const uint32_t *__restrict__ perm, uint32_t *grab) {
+    const uint32_t *__restrict__ perm, uint32_t *grab, const uint32_t *abortFlag) {
   __shared__ int stack[kStackLds * kRenderThreads];
   __shared__ int pend[kPend * kRenderThreads];
   int *stk = stack + threadIdx.x;
@@ -389,8 +410,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
   uint32_t p = 0;
   int st = kIdle;
   unsigned long long trips = 0;
+  const uint32_t count = *countDev;
   WaveChunk ch;
-  waveChunkInit(ch, grab);
+  waveChunkInit(ch, grab, abortFlag);
   while (true) {
     // paths loaded in the last trip start their walk
     if (st == kLoading) {
@@ -403,7 +425,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
       st = kLoading;
     }
     uint32_t item = 0;
-    if (waveClaim(ch, grab, count, st == kIdle, item)) {
+    if (waveClaim(ch, grab, count, st == kIdle, item, abortFlag)) {
       if (kPerm) {
         p = perm[item];
         st = kClaimed;
@@ -478,7 +500,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ thr,
     const uint4 *__restrict__ meta, const float4 *__restrict__ hit, float4 *__restrict__ rayOut,
     float4 *__restrict__ thrOut, uint4 *__restrict__ metaOut, float4 *__restrict__ shadow, int64_t shadowCap,
-    uint32_t *__restrict__ counters, float *__restrict__ fb, float *__restrict__ aov,
+    uint32_t *__restrict__ nextCount, uint32_t *__restrict__ bounceCnt, float *__restrict__ aov,
     uint32_t *__restrict__ qVol) {
   __shared__ uint32_t appendSh[kRenderThreads / 64 + 1];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
@@ -535,7 +557,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         atomicAdd(pa + 2, bg.z);
       }
     } else {
-      const int mi = __float_as_int(h0.w);
+      int mi = __float_as_int(h0.w);
+      if ((unsigned)mi >= (unsigned)a.nMats) {
+        atomicOr(a.err, kErrMaterial);
+        mi = 0;
+      }
       m = loadMat(a.mats, mi);
       hp = o + d * th;
       const f3 outw = xyz(h1);
@@ -543,14 +569,17 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       n = front ? outw : -outw;
       if (m.type == CPT_M_VOLUME) {
         // homogeneous volume object: walked by k_volume (it traces inside)
-        const uint32_t vs = waveAppend(counters + 2, true);
-        qVol[vs] = p;
+        const uint32_t vs = waveAppend(bounceCnt + 2, true);
+        if ((int64_t)vs < a.cap) qVol[vs] = p;
+        else atomicOr(a.err, kErrQueueCap);
       } else if (m.type == CPT_M_LIGHT) {
         // emitter (Render.cu:144-149)
         if (!(flags & kFlagNee)) {
           addRadiance(a.lbuf, ls, T * m.color);
         }
-        if (kAov && !(flags & kFlagAov)) {
+        // the reference's G-buffer (NORMAL / DEPTH display) keeps its cleared
+        // value when the first hit is an emitter (Render.cu:144-149)
+        if (kAov && !(flags & kFlagAov) && a.profile != CPT_PROFILE_NORMAL && a.profile != CPT_PROFILE_DEPTH) {
           atomicAdd(pa + 0, 1.0f);
           atomicAdd(pa + 1, 1.0f);
           atomicAdd(pa + 2, 1.0f);
@@ -595,8 +624,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         const float4 l0 = a.rec[3 * li], l1 = a.rec[3 * li + 1], l2 = a.rec[3 * li + 2];
         const f3 lc = xyz(l0), ln = xyz(l1);
         const float lw = l2.x;
-        // the reference uses the width for both extents (Render.cu:170-171)
-        const float lh = a.profile == CPT_PROFILE_REFERENCE ? l2.x : l2.y;
+        // every reference profile uses the width for both extents
+        // (Render.cu:170-171)
+        const float lh = a.profile == CPT_PROFILE_CONFIG ? l2.y : l2.x;
         f3 eu, ev;
         lightFrame(ln, eu, ev);
         const float u1 = rng.next(), u2 = rng.next();
@@ -616,8 +646,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         }
         flags |= kFlagNee;
       }
-      const uint32_t slot = blockAppend(counters + 1, want, appendSh);
-      if (want && slot < shadowCap) {
+      const uint32_t slot = blockAppend(bounceCnt + 1, want, appendSh);
+      if (want && (int64_t)slot >= shadowCap) atomicOr(a.err, kErrShadowCap);
+      if (want && (int64_t)slot < shadowCap) {
         // {origin, tmin} {direction, distance to the light} {contribution,
         // radiance slot of the path} {pixel, sample and draw count of the
         // transmittance estimate's stream, light id}
@@ -631,18 +662,25 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     }
   }
   if (vtx) {
-    // Russian roulette / depth cap (Render.cu:189-204)
+    // Russian roulette / depth cap (Render.cu:189-204): STATIC rolls 0.7 at
+    // every vertex with no cap; REAL_TIME ends after the 3rd vertex and
+    // NORMAL / DEPTH after the 2nd (`depth++ >= 2`, `depth++ >= 1`) without
+    // roulette; CONFIG caps at max_depth and rolls from rr_depth on
     const uint32_t depth = flags & kDepthMask;
     float prr = 1.0f;
     bool cont = true;
     if (a.profile == CPT_PROFILE_REFERENCE) {
       prr = 0.7f;
-      if (rng.next() > prr) cont = false;
+      if (rng.next() > kRoulette07) cont = false;
+    } else if (a.profile == CPT_PROFILE_REAL_TIME) {
+      cont = depth < 2u;
+    } else if (a.profile == CPT_PROFILE_NORMAL || a.profile == CPT_PROFILE_DEPTH) {
+      cont = depth < 1u;
     } else {
       if ((int)depth + 1 >= a.maxDepth) cont = false;
       else if ((int)depth >= a.rrDepth) {
         prr = a.rrProb;
-        if (rng.next() > prr) cont = false;
+        if (rng.next() > a.rrCmp) cont = false;
       }
     }
     if (cont) {
@@ -672,7 +710,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       }
     }
   }
-  const uint32_t slot = blockAppend(counters + 0, alive, appendSh);
+  const uint32_t slot = blockAppend(nextCount, alive, appendSh);
+  if (alive && (int64_t)slot >= a.cap) {
+    atomicOr(a.err, kErrQueueCap);
+    alive = false;
+  }
   if (alive) {
     rayOut[2 * (int64_t)slot] = make_float4(o.x, o.y, o.z, kRayTMin);
     rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
@@ -686,7 +728,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
 // medium with sigma_maj = 25, roulette 0.8 per collision, isotropic
 // scattering, ending at the far side of the object; the path then continues
 // from the last position with the last direction.  The queue length is
-// counters[2]; the grid covers an upper bound.
+// *volCount; the grid covers an upper bound.  Continuing paths are appended
+// to the next queue (*nextCount).
 __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const uint32_t *__restrict__ qVol,
                                                            uint32_t bound, const float4 *__restrict__ ray,
                                                            const float4 *__restrict__ thr,
@@ -694,11 +737,16 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
                                                            const float4 *__restrict__ hit,
                                                            float4 *__restrict__ rayOut, float4 *__restrict__ thrOut,
                                                            uint4 *__restrict__ metaOut,
-                                                           uint32_t *__restrict__ counters) {
+                                                           const uint32_t *__restrict__ volCount,
+                                                           uint32_t *__restrict__ nextCount) {
   __shared__ int stack[kStackLds * kRenderThreads];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
-  if (k >= bound || k >= counters[2]) return;
+  if (k >= bound || k >= *volCount) return;
   const uint32_t p = qVol[k];
+  if ((int64_t)p >= a.cap) {
+    atomicOr(a.err, kErrSlot);
+    return;
+  }
   const float4 o4 = ray[2 * (int64_t)p], d4 = ray[2 * (int64_t)p + 1], t4 = thr[p];
   const uint4 mt = meta[p];
   const float th = hit[2 * (int64_t)p].y;
@@ -719,7 +767,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
     f3 pos = hp + d * t;
     o = pos;
     for (int guard = 0; t < dMax && guard < kMaxCollisions; guard++) {
-      if (rng.next() > 0.8f) break;
+      if (rng.next() > kRoulette08) break;
       T = T * (__expf(-t * sig) / 0.8f);
       const float z = 1.0f - 2.0f * rng.next();
       const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
@@ -737,7 +785,11 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
       if (t > dMax || l3 < 0) break;
     }
   }
-  const uint32_t slot = waveAppend(counters + 0, true);
+  const uint32_t slot = waveAppend(nextCount, true);
+  if ((int64_t)slot >= a.cap) {
+    atomicOr(a.err, kErrQueueCap);
+    return;
+  }
   rayOut[2 * (int64_t)slot] = make_float4(o.x, o.y, o.z, kRayTMin);
   rayOut[2 * (int64_t)slot + 1] = make_float4(d.x, d.y, d.z, kRayTMax);
   thrOut[slot] = make_float4(T.x, T.y, T.z, 0.f);
@@ -760,8 +812,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_volume(RenderArgs a, const u
 // work[0], work[1] and each wave's largest visit count into work[2].
 template <bool kAtomic, int kFmt, bool kCount>
 __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const float4 *__restrict__ shadow,
-                                                            int64_t shadowCap, const uint32_t *__restrict__ counters,
-                                                            const uint32_t *__restrict__ perm,
+                                                            uint32_t ns, const uint32_t *__restrict__ perm,
                                                             unsigned long long *__restrict__ work) {
   const float4 *__restrict__ nodes = a.nodes;
   const float4 *__restrict__ leaves = a.leaves;
@@ -772,7 +823,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
   __shared__ int pend[kPend * kRenderThreads];
   const uint32_t li = blockIdx.x * kRenderThreads + threadIdx.x;
   TraceCounters cnt;
-  if ((int64_t)li < shadowCap && li < counters[1]) {
+  if (li < ns) {
     const uint32_t k = perm ? perm[li] : li;
     const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1];
     const float4 c = shadow[4 * (int64_t)k + 2];
@@ -801,7 +852,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
         tr = ratioTrack(med, seed, __float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), o, d, r0.w, tL);
       }
       const int64_t ls = __float_as_int(c.w);
-      if (kAtomic) {
+      if (ls < 0 || ls >= a.cap) {
+        atomicOr(a.err, kErrSlot);
+      } else if (kAtomic) {
         float *pl = (float *)(a.lbuf + ls);
         atomicAdd(pl + 0, c.x * tr);
         atomicAdd(pl + 1, c.y * tr);
@@ -852,14 +905,38 @@ __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ ou
   if (i < n) out[i] = acc[i] * scale;
 }
 
+// NORMAL / DEPTH display (Render.cu:239-251) from the first-vertex AOV of a
+// one-sample render: (n + 0.5) / 2, or clamp01(distance / 50) with no
+// vertex (distance 0) showing the cleared G-buffer's INF as 1.
+__global__ void k_display(const float *__restrict__ aov, float *__restrict__ out, int64_t npix, int depth) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  const float *p = aov + 7 * i;
+  if (depth) {
+    const float v = p[6] > 0.0f ? fminf(fmaxf(p[6] / 50.0f, 0.0f), 1.0f) : 1.0f;
+    out[3 * i] = v;
+    out[3 * i + 1] = v;
+    out[3 * i + 2] = v;
+  } else {
+    out[3 * i] = (p[3] + 0.5f) * 0.5f;
+    out[3 * i + 1] = (p[4] + 0.5f) * 0.5f;
+    out[3 * i + 2] = (p[5] + 0.5f) * 0.5f;
+  }
+}
+
 void renderRelease(cpt_ctx *ctx) {
   if (!ctx->render) return;
   RenderBufs *r = ctx->render;
   DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
                    &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,   &r->keysS,
-                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf};
+                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf,    &r->stage};
+  // release path: nothing to report to
   for (DevBuf *b : all)
-    if (b->p) (void)hipFree(b->p);  // release path: nothing to report to
+    if (b->p) (void)hipFree(b->p);
+  if (r->hostCnt) (void)hipHostFree(r->hostCnt);
+  for (auto &row : r->ev)
+    for (hipEvent_t &e : row)
+      if (e) (void)hipEventDestroy(e);
   delete r;
   ctx->render = nullptr;
 }
@@ -870,25 +947,56 @@ static unsigned blocksFor(int64_t n) { return (unsigned)((n + kRenderThreads - 1
 
 using namespace cpt;
 
+namespace cpt {
+
+// R.counters, u32 words: two per-bounce blocks, block (bounce & 1) holding
+// [0] the length of the queue that bounce traces, [1] its shadow rays, [2]
+// its volume-object paths, [3] the grab counter of its persistent
+// extension kernel; word 16 the error bits (kErr*); words 32..43 six u64
+// work sums of a counting render (extension walks, then shadow walks).
+constexpr int kCntBlock = 8, kCntErr = 16, kCntWork = 32, kCntReadback = 20;
+
+__global__ void k_init_counters(uint32_t *c, uint32_t count) {
+  const int t = (int)threadIdx.x;
+  if (t < 2 * kCntBlock) c[t] = t == 0 ? count : 0u;
+}
+
+static const char *renderErrText(uint32_t e) {
+  if (e & kErrShadowCap) return "shadow queue overflow";
+  if (e & kErrQueueCap) return "path queue overflow";
+  if (e & kErrMaterial) return "material index outside the table";
+  return "radiance slot or queue entry outside the batch";
+}
+
+}  // namespace cpt
+
 extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nviews, const cpt_render_params *prm,
                                 float *rgb, float *aovOut, cpt_render_stats *stats, int flags) {
   if (!ctx || !cams || !prm || !rgb || nviews <= 0) return CPT_ERR_ARG;
   if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_render", "no BVH built");
+  if (ctx->abortHost && __atomic_load_n(ctx->abortHost, __ATOMIC_ACQUIRE))
+    return fail(ctx, CPT_ERR_ABORTED, "cpt_render", "aborted by cpt_request_abort");
   const cpt_camera *cam = cams;
   if (cam->width <= 0 || cam->height <= 0 || prm->spp <= 0 || prm->sample_base < 0)
     return fail(ctx, CPT_ERR_ARG, "cpt_render", "bad image size or spp");
+  if (prm->profile < CPT_PROFILE_REFERENCE || prm->profile > CPT_PROFILE_DEPTH)
+    return fail(ctx, CPT_ERR_ARG, "cpt_render", "unknown profile");
   for (int v = 1; v < nviews; v++)
     if (cams[v].width != cam->width || cams[v].height != cam->height)
       return fail(ctx, CPT_ERR_ARG, "cpt_render_views", "views differ in size");
   if (prm->profile == CPT_PROFILE_CONFIG && (prm->max_depth <= 0 || !(prm->rr_prob > 0.0)))
     return fail(ctx, CPT_ERR_ARG, "cpt_render", "max_depth and rr_prob must be positive");
+  // NORMAL / DEPTH render one sample (Render.cu:38) and show its first vertex
+  const bool display = prm->profile == CPT_PROFILE_NORMAL || prm->profile == CPT_PROFILE_DEPTH;
+  const int spp = display ? 1 : prm->spp;
+  const bool wantAov = aovOut != nullptr || display;
   const int64_t npixView = (int64_t)cam->width * cam->height;
   const int64_t npix = npixView * nviews;  // pixels over all views
   if (npix > (int64_t)1 << 31) return fail(ctx, CPT_ERR_ARG, "cpt_render", "image too large");
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   if (!ctx->render) ctx->render = new RenderBufs();
   RenderBufs &R = *ctx->render;
-  const int64_t total = npix * prm->spp;
+  const int64_t total = npix * spp;
   const int nl = std::max(1, (int)ctx->nLights);
   int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 26);
   cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 27) / nl));
@@ -907,10 +1015,15 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       (rc = ensure(ctx, R.permB, shadowCap * 4)) || (rc = ensure(ctx, R.qVol, cap * 4)) ||
       (rc = ensure(ctx, R.keysA, shadowCap * 4)) || (rc = ensure(ctx, R.keysB, shadowCap * 4)) ||
       (rc = ensure(ctx, R.keysS, shadowCap * 4)) ||
-      (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 128)) ||
+      (rc = ensure(ctx, R.shadow, shadowCap * 64)) || (rc = ensure(ctx, R.counters, 256)) ||
       (rc = ensure(ctx, R.cams, (size_t)nviews * sizeof(CamDev))) || (rc = ensure(ctx, R.lbuf, cap * 16)) ||
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
+  if (!(flags & CPT_DEVICE_PTRS) && (rc = ensure(ctx, R.stage, npix * 40))) return rc;
+  if (!R.hostCnt) CPT_TRY_HIP(ctx, hipHostMalloc((void **)&R.hostCnt, 2 * 32 * sizeof(uint32_t), hipHostMallocDefault));
+  for (auto &row : R.ev)
+    for (hipEvent_t &e : row)
+      if (!e) CPT_TRY_HIP(ctx, hipEventCreate(&e));
   R.cap = cap;
   R.shadowCap = shadowCap;
   hipStream_t s = ctx->stream;
@@ -952,13 +1065,15 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.cams = R.cams.as<CamDev>();
   a.nviews = nviews;
   a.seed = prm->seed;
-  a.spp = prm->spp;
+  a.spp = spp;
   a.sampleBase = prm->sample_base;
   a.profile = prm->profile;
   a.maxDepth = prm->max_depth;
   a.rrDepth = prm->rr_depth;
   a.rrProb = (float)prm->rr_prob;
-  a.wantAov = aovOut ? 1 : 0;
+  a.rrCmp = (float)prm->rr_prob;
+  if ((double)a.rrCmp > prm->rr_prob) a.rrCmp = std::nextafter(a.rrCmp, -INFINITY);
+  a.wantAov = wantAov ? 1 : 0;
   a.npix = npixView;
   a.tiled = cap % npix == 0 ? 1 : 0;
   a.lbuf = R.lbuf.as<float4>();
@@ -979,9 +1094,13 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.med.nz = ctx->medN[2];
   a.med.sigma = ctx->medSigma;
   a.med.albedo = mkf3(ctx->medAlbedo[0], ctx->medAlbedo[1], ctx->medAlbedo[2]);
+  uint32_t *cnt = R.counters.as<uint32_t>();
+  a.err = cnt + kCntErr;
+  a.cap = cap;
+  a.nMats = ctx->nMats + ctx->nLights + 1;
 
   hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), npix * 3, 0.f);
-  if (aovOut)
+  if (wantAov)
     hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(), npix * 7, 0.f);
   CPT_TRY_HIP(ctx, hipGetLastError());
 
@@ -997,134 +1116,162 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
                          &perCU, (const void *)k_extend_persist<false, false, true>, kRenderThreads, 0));
     extGrid = (unsigned)ctx->numCU * (unsigned)std::max(1, perCU);
   }
-  bool shadowTimed = false;  // events 16/17 bracket a shadow pass not yet added
-  uint32_t *cnt = R.counters.as<uint32_t>();
-  // counters: [0] next queue, [1] shadow queue, [2] volume queue, [3] grab
-  // counter of the persistent extension kernel; [8..13] three u64 work sums
-  // of the extension walks, [14..19] of the shadow walks
-  unsigned long long *work = (unsigned long long *)(cnt + 8);
+  unsigned long long *work = (unsigned long long *)(cnt + kCntWork);
   const bool countWork = (prm->flags & CPT_RENDER_COUNT_WORK) != 0;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 32);
-  hipEvent_t *ev = ctx->ev;
-  CPT_TRY_HIP(ctx, hipEventRecord(ev[12], s));
-  for (int64_t base = 0; base < total; base += cap) {
-    const int64_t count = std::min(cap, total - base);
-    int cur = 0;  // state set holding the current queue
-    bool permPending = false;
-    uint32_t *queuePerm = nullptr;  // the pending order (where the sort left it)
-    a.base = base;
-    hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
-                       R.ray[cur].as<float4>(), R.thr[cur].as<float4>(), R.meta[cur].as<uint4>());
-    CPT_TRY_HIP(ctx, hipGetLastError());
-    uint32_t qn = (uint32_t)count;
-    // paths end by roulette, depth cap, miss or absorption; the cap only
-    // bounds pathological inputs (NaN geometry) so the loop always ends
-    for (int pass = 0; qn > 0 && pass < kMaxBounces; pass++) {
-      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 8);
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[13], s));
-      {
-        // with a pending ray-key order the extension kernel traces the queue
-        // in that order; hit records stay at the paths' own slots
-        const float4 *rayIn = R.ray[cur].as<float4>();
-        const uint32_t *pm = permPending ? queuePerm : nullptr;
-#define CPT_EXT(C, F, P)                                                                                      \
-  hipLaunchKernelGGL((k_extend<C, F, P>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes,     \
+  // with bounce-queue ordering the host needs each queue's length before its
+  // extension kernel; otherwise that kernel is enqueued before the length
+  // reaches the host and reads it on the device
+  const bool lagExtend = ctx->queueSortMin <= 0;
+  const uint32_t *abortDev = ctx->abortDev;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt, 64);
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[12], s));
+
+  // the extension kernel of bounce j over the queue in state set (j & 1);
+  // bound: an upper bound of its length (queues never grow)
+  auto launchExtend = [&](int j, uint32_t bound, const uint32_t *pm) -> int {
+    const int pj = j & 1;
+    const float4 *rayIn = R.ray[pj].as<float4>();
+    const uint32_t *cd = cnt + pj * kCntBlock;
+    CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pj][0], s));
+    if (bound > 0) {
+#define CPT_EXT(C, F, P)                                                                                        \
+  hipLaunchKernelGGL((k_extend<C, F, P>), dim3(blocksFor(bound)), dim3(kRenderThreads), 0, s, cd, rayIn, a.nodes, \
                      a.leaves, a.nPrims, R.hit.as<float4>(), work, a.wide, pm)
-#define CPT_EXT_F(C, P)                  \
-  switch (a.wide) {                      \
-    case 0: CPT_EXT(C, 0, P); break;     \
-    case 1: CPT_EXT(C, 1, P); break;     \
-    case 2: CPT_EXT(C, 2, P); break;     \
-    default: CPT_EXT(C, 3, P); break;    \
+#define CPT_EXT_F(C, P)               \
+  switch (a.wide) {                   \
+    case 0: CPT_EXT(C, 0, P); break;  \
+    case 1: CPT_EXT(C, 1, P); break;  \
+    case 2: CPT_EXT(C, 2, P); break;  \
+    default: CPT_EXT(C, 3, P); break; \
   }
-#define CPT_EXT_P(C, Q, P)                                                                                  \
-  hipLaunchKernelGGL((k_extend_persist<C, Q, P>), dim3(std::min<unsigned>(blocksFor(qn), extGrid)),             \
-                     dim3(kRenderThreads), 0, s, qn, rayIn, a.nodes, a.leaves, R.hit.as<float4>(), work, pm, cnt + 3)
-        if (extPersist) {
-          // counters[3] (cleared with the queue counters) is the grab counter
-          const int q = a.wide == 2 ? 1 : 0;
-          switch ((countWork ? 4 : 0) + q * 2 + (permPending ? 1 : 0)) {
-            case 0: CPT_EXT_P(false, false, false); break;
-            case 1: CPT_EXT_P(false, false, true); break;
-            case 2: CPT_EXT_P(false, true, false); break;
-            case 3: CPT_EXT_P(false, true, true); break;
-            case 4: CPT_EXT_P(true, false, false); break;
-            case 5: CPT_EXT_P(true, false, true); break;
-            case 6: CPT_EXT_P(true, true, false); break;
-            default: CPT_EXT_P(true, true, true); break;
-          }
-        } else if (countWork) {
-          if (permPending) CPT_EXT_F(true, true) else CPT_EXT_F(true, false)
-        } else {
-          if (permPending) CPT_EXT_F(false, true) else CPT_EXT_F(false, false)
+#define CPT_EXT_P(C, Q, P)                                                                                       \
+  hipLaunchKernelGGL((k_extend_persist<C, Q, P>), dim3(std::min<unsigned>(blocksFor(bound), extGrid)),             \
+                     dim3(kRenderThreads), 0, s, cd, rayIn, a.nodes, a.leaves, R.hit.as<float4>(), work, pm,       \
+                     cnt + pj * kCntBlock + 3, abortDev)
+      if (extPersist) {
+        const int q = a.wide == 2 ? 1 : 0;
+        switch ((countWork ? 4 : 0) + q * 2 + (pm ? 1 : 0)) {
+          case 0: CPT_EXT_P(false, false, false); break;
+          case 1: CPT_EXT_P(false, false, true); break;
+          case 2: CPT_EXT_P(false, true, false); break;
+          case 3: CPT_EXT_P(false, true, true); break;
+          case 4: CPT_EXT_P(true, false, false); break;
+          case 5: CPT_EXT_P(true, false, true); break;
+          case 6: CPT_EXT_P(true, true, false); break;
+          default: CPT_EXT_P(true, true, true); break;
         }
+      } else if (countWork) {
+        if (pm) CPT_EXT_F(true, true) else CPT_EXT_F(true, false)
+      } else {
+        if (pm) CPT_EXT_F(false, true) else CPT_EXT_F(false, false)
+      }
 #undef CPT_EXT_P
 #undef CPT_EXT_F
 #undef CPT_EXT
-        permPending = false;
-      }
-      const int nxt = cur ^ 1;
-      const float4 *rayC = R.ray[cur].as<float4>(), *thrC = R.thr[cur].as<float4>();
-      const uint4 *metaC = R.meta[cur].as<uint4>();
-      extRays += qn;
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[14], s));
-#define CPT_SHADE(M, V)                                                                                         \
-  hipLaunchKernelGGL((k_shade<M, V>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC,   \
-                     (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),         \
-                     R.meta[nxt].as<uint4>(), R.shadow.as<float4>(), shadowCap, cnt, R.fb.as<float>(),             \
+    }
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pj][1], s));
+    return CPT_OK;
+  };
+  // shading (and the volume-object walk) of bounce j's qn paths, then the
+  // read-back of the counters that tell the host the next queue's length
+  auto launchShade = [&](int j, uint32_t qn) -> int {
+    const int pj = j & 1, pn = pj ^ 1;
+    const float4 *rayC = R.ray[pj].as<float4>(), *thrC = R.thr[pj].as<float4>();
+    const uint4 *metaC = R.meta[pj].as<uint4>();
+    uint32_t *nextCount = cnt + pn * kCntBlock, *bounceCnt = cnt + pj * kCntBlock;
+    CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pj][2], s));
+#define CPT_SHADE(M, V)                                                                                       \
+  hipLaunchKernelGGL((k_shade<M, V>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC, \
+                     (const float4 *)R.hit.as<float4>(), R.ray[pn].as<float4>(), R.thr[pn].as<float4>(),       \
+                     R.meta[pn].as<uint4>(), R.shadow.as<float4>(), shadowCap, nextCount, bounceCnt,           \
                      R.aov.as<float>(), R.qVol.as<uint32_t>())
-      switch ((a.med.on ? 2 : 0) + (a.wantAov ? 1 : 0)) {
-        case 0: CPT_SHADE(false, false); break;
-        case 1: CPT_SHADE(false, true); break;
-        case 2: CPT_SHADE(true, false); break;
-        default: CPT_SHADE(true, true); break;
-      }
+    switch ((a.med.on ? 2 : 0) + (a.wantAov ? 1 : 0)) {
+      case 0: CPT_SHADE(false, false); break;
+      case 1: CPT_SHADE(false, true); break;
+      case 2: CPT_SHADE(true, false); break;
+      default: CPT_SHADE(true, true); break;
+    }
 #undef CPT_SHADE
-      if (ctx->hasVolumeMat)
-        hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
-                           (const uint32_t *)R.qVol.as<uint32_t>(), qn, rayC, thrC, metaC,
-                           (const float4 *)R.hit.as<float4>(), R.ray[nxt].as<float4>(), R.thr[nxt].as<float4>(),
-                           R.meta[nxt].as<uint4>(), cnt);
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[15], s));
-      uint32_t hc[2] = {0, 0};
-      CPT_TRY_HIP(ctx, hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, s));
-      CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
+    if (ctx->hasVolumeMat)
+      hipLaunchKernelGGL(k_volume, dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a,
+                         (const uint32_t *)R.qVol.as<uint32_t>(), qn, rayC, thrC, metaC,
+                         (const float4 *)R.hit.as<float4>(), R.ray[pn].as<float4>(), R.thr[pn].as<float4>(),
+                         R.meta[pn].as<uint4>(), (const uint32_t *)(bounceCnt + 2), nextCount);
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pj][3], s));
+    CPT_TRY_HIP(ctx, hipMemcpyAsync(R.hostCnt + 32 * pj, cnt, kCntReadback * sizeof(uint32_t),
+                                    hipMemcpyDeviceToHost, s));
+    CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pj][6], s));
+    return CPT_OK;
+  };
+  // a failure after work was enqueued: let it drain before returning
+  auto stop = [&](int code, const char *what, const char *detail) -> int {
+    (void)hipStreamSynchronize(s);
+    return fail(ctx, code, what, detail);
+  };
+
+  for (int64_t base = 0; base < total; base += cap) {
+    const int64_t count = std::min(cap, total - base);
+    a.base = base;
+    hipLaunchKernelGGL(k_gen, dim3(blocksFor(count)), dim3(kRenderThreads), 0, s, a, base, count,
+                       R.ray[0].as<float4>(), R.thr[0].as<float4>(), R.meta[0].as<uint4>());
+    hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(64), 0, s, cnt, (uint32_t)count);
+    CPT_TRY_HIP(ctx, hipGetLastError());
+    uint32_t qn = (uint32_t)count;  // length of bounce i's queue (known on the host)
+    if ((rc = launchExtend(0, qn, nullptr)) || (rc = launchShade(0, qn))) return stop(rc, "cpt_render", ctx->err.c_str());
+    int last = 0;
+    // paths end by roulette, depth cap, miss or absorption; the cap only
+    // bounds pathological inputs (NaN geometry) so the loop always ends
+    for (int i = 0; i < kMaxBounces; i++) {
+      const int pi = i & 1, pn = pi ^ 1;
+      last = i;
+      // the next extension kernel goes in before bounce i's counts reach the
+      // host, so the device has work while the host waits for them
+      if (lagExtend && (rc = launchExtend(i + 1, qn, nullptr))) return stop(rc, "cpt_render", ctx->err.c_str());
+      CPT_TRY_HIP(ctx, hipEventSynchronize(R.ev[pi][6]));
+      const uint32_t *h = R.hostCnt + 32 * pi;
+      const uint32_t qNext = h[pn * kCntBlock + 0], ns = h[pi * kCntBlock + 1], err = h[kCntErr];
       float e1 = 0, e2 = 0;
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e1, ev[13], ev[14]));
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e2, ev[14], ev[15]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e1, R.ev[pi][0], R.ev[pi][1]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e2, R.ev[pi][2], R.ev[pi][3]));
       extMs += e1;
       shMs += e2;
-      if (shadowTimed) {  // the previous bounce's shadow pass has finished
+      if (i > 0) {  // bounce i-1's shadow pass preceded bounce i's shading
         float e3 = 0;
-        CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[16], ev[17]));
+        CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, R.ev[pn][4], R.ev[pn][5]));
         sdMs += e3;
-        shadowTimed = false;
       }
-      const int64_t ns = std::min<int64_t>(hc[1], shadowCap);
+      if (err) return stop(CPT_ERR_STATE, "cpt_render", renderErrText(err));
+      if (ctx->abortHost && __atomic_load_n(ctx->abortHost, __ATOMIC_ACQUIRE))
+        return stop(CPT_ERR_ABORTED, "cpt_render", "aborted by cpt_request_abort");
+      if ((int64_t)ns > shadowCap || (int64_t)qNext > qn)
+        return stop(CPT_ERR_STATE, "cpt_render", "queue counts beyond their bounds");
       rays += (double)qn + (double)ns;
+      extRays += qn;
       bounces++;
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[16], s));
-      // shadow rays in the order of their origin cell (all rays of one
-      // origin cell then walk the same records towards the lights)
+      // bounce i's shadow rays, in the order of their origin cell (all rays
+      // of one origin cell then walk the same records towards the lights)
+      CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pi][4], s));
       uint32_t *shPerm = nullptr;
-      if (ctx->shadowSortMin > 0 && ns >= ctx->shadowSortMin) {
+      if (ctx->shadowSortMin > 0 && (int64_t)ns >= ctx->shadowSortMin) {
         if ((rc = radixSortPairs(ctx, R.keysS.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
                                  R.permB.as<uint32_t>(), ns, kShadowKeyBits, true, &shPerm)))
-          return rc;
+          return stop(rc, "cpt_render", ctx->err.c_str());
       }
 #define CPT_SHD(A, F, C)                                                                                     \
   hipLaunchKernelGGL((k_shadow<A, F, C>), dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,                \
-                     (const float4 *)R.shadow.as<float4>(), shadowCap, (const uint32_t *)cnt, shPerm, work + 3)
-#define CPT_SHD_F(C)                                   \
-  switch (a.wide * 2 + (nl > 1 ? 1 : 0)) {             \
-    case 0: CPT_SHD(false, 0, C); break;               \
-    case 1: CPT_SHD(true, 0, C); break;                \
-    case 2: CPT_SHD(false, 1, C); break;               \
-    case 3: CPT_SHD(true, 1, C); break;                \
-    case 4: CPT_SHD(false, 2, C); break;               \
-    case 5: CPT_SHD(true, 2, C); break;                \
-    case 6: CPT_SHD(false, 3, C); break;               \
-    default: CPT_SHD(true, 3, C); break;               \
+                     (const float4 *)R.shadow.as<float4>(), ns, shPerm, work + 3)
+#define CPT_SHD_F(C)                       \
+  switch (a.wide * 2 + (nl > 1 ? 1 : 0)) { \
+    case 0: CPT_SHD(false, 0, C); break;   \
+    case 1: CPT_SHD(true, 0, C); break;    \
+    case 2: CPT_SHD(false, 1, C); break;   \
+    case 3: CPT_SHD(true, 1, C); break;    \
+    case 4: CPT_SHD(false, 2, C); break;   \
+    case 5: CPT_SHD(true, 2, C); break;    \
+    case 6: CPT_SHD(false, 3, C); break;   \
+    default: CPT_SHD(true, 3, C); break;   \
   }
       if (ns > 0) {
         if (countWork) CPT_SHD_F(true) else CPT_SHD_F(false)
@@ -1132,44 +1279,50 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
 #undef CPT_SHD_F
 #undef CPT_SHD
       CPT_TRY_HIP(ctx, hipGetLastError());
-      CPT_TRY_HIP(ctx, hipEventRecord(ev[17], s));
-      shadowTimed = true;
-      qn = hc[0];
-      cur = nxt;
-      // incoherent bounce rays: trace the next queue in ray-key order
-      if (ctx->queueSortMin > 0 && (int64_t)qn >= ctx->queueSortMin) {
-        if ((rc = radixSortPairs(ctx, R.keysA.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
-                                 R.permB.as<uint32_t>(), qn, kQueueKeyBits, true, &queuePerm)))
-          return rc;
-        permPending = true;
+      CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pi][5], s));
+      // block pi now serves bounce i + 2
+      hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, cnt + pi * kCntBlock, kCntBlock);
+      if (qNext == 0 || i + 1 >= kMaxBounces) break;
+      if (!lagExtend) {
+        // incoherent bounce rays: trace the next queue in ray-key order
+        uint32_t *queuePerm = nullptr;
+        if ((int64_t)qNext >= ctx->queueSortMin &&
+            (rc = radixSortPairs(ctx, R.keysA.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
+                                 R.permB.as<uint32_t>(), qNext, kQueueKeyBits, true, &queuePerm)))
+          return stop(rc, "cpt_render", ctx->err.c_str());
+        if ((rc = launchExtend(i + 1, qNext, queuePerm))) return stop(rc, "cpt_render", ctx->err.c_str());
       }
+      if ((rc = launchShade(i + 1, qNext))) return stop(rc, "cpt_render", ctx->err.c_str());
+      qn = qNext;
     }
-    if (shadowTimed) {
+    {  // the batch's last shadow pass
       float e3 = 0;
-      CPT_TRY_HIP(ctx, hipEventSynchronize(ev[17]));
-      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, ev[16], ev[17]));
+      CPT_TRY_HIP(ctx, hipEventSynchronize(R.ev[last & 1][5]));
+      CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, R.ev[last & 1][4], R.ev[last & 1][5]));
       sdMs += e3;
-      shadowTimed = false;
     }
     hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
                        (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());
     CPT_TRY_HIP(ctx, hipGetLastError());
   }
-  // mean over samples
-  const float inv = 1.0f / (float)prm->spp;
+  // mean over samples (or the NORMAL / DEPTH display of the first vertex)
+  const float inv = 1.0f / (float)spp;
   float *dstRgb = rgb, *dstAov = aovOut;
-  DevBuf tmp;
   if (!(flags & CPT_DEVICE_PTRS)) {
-    if ((rc = ensure(ctx, tmp, npix * 40))) return rc;
-    dstRgb = tmp.as<float>();
-    dstAov = tmp.as<float>() + npix * 3;
+    dstRgb = R.stage.as<float>();
+    dstAov = R.stage.as<float>() + npix * 3;
   }
-  hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), dstRgb,
-                     npix * 3, inv);
+  if (display)
+    hipLaunchKernelGGL(k_display, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s, (const float *)R.aov.as<float>(),
+                       dstRgb, npix, prm->profile == CPT_PROFILE_DEPTH ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), dstRgb,
+                       npix * 3, inv);
   if (aovOut)
     hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(),
                        dstAov, npix * 7, inv);
-  CPT_TRY_HIP(ctx, hipEventRecord(ev[10], s));
+  CPT_TRY_HIP(ctx, hipGetLastError());
+  CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[10], s));
   hipError_t e = hipSuccess;
   if (!(flags & CPT_DEVICE_PTRS)) {
     e = hipMemcpyAsync(rgb, dstRgb, npix * 12, hipMemcpyDeviceToHost, s);
@@ -1177,11 +1330,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   }
   const hipError_t es = hipStreamSynchronize(s);
   if (e == hipSuccess) e = es;
-  if (tmp.p) (void)hipFree(tmp.p);
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_render", hipGetErrorString(e));
   if (stats) {
     float ms = 0;
-    CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[12], ev[10]));
+    CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev[12], ctx->ev[10]));
     stats->ms = ms;
     stats->rays = rays;
     stats->paths = (double)total;

@@ -106,7 +106,8 @@ template <bool kCount, bool kQuant>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     const float4 *__restrict__ rays, int64_t nRays, const uint32_t *__restrict__ perm,
     const float4 *__restrict__ nodes4, const float4 *__restrict__ leaves, int32_t *__restrict__ idOut,
-    int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut, uint32_t *__restrict__ work) {
+    int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut, uint32_t *__restrict__ work,
+    const uint32_t *abortFlag) {
   __shared__ int stack[kStackLds * kTraceThreads];
   int *stk = stack + threadIdx.x;
   int ovf[kStack4Total - kStackLds];
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
   int64_t ri = 0;
   int st = kIdle;
   WaveChunk ch;
-  waveChunkInit(ch, work);
+  waveChunkInit(ch, work, abortFlag);
   while (true) {
     // rays loaded in the last trip start their walk
     if (st == kLoading) {
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     }
     // idle lanes claim the next rays of the wave's chunk
     uint32_t item = 0;
-    if (waveClaim(ch, work, (uint32_t)nRays, st == kIdle, item)) {
+    if (waveClaim(ch, work, (uint32_t)nRays, st == kIdle, item, abortFlag)) {
       if (perm) {
         ri = (int64_t)perm[item];
         st = kClaimed;
@@ -218,7 +219,7 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
     const unsigned pgrid = std::min<unsigned>(grid, (unsigned)ctx->numCU * (unsigned)std::max(1, perCU));
 #define CPT_LAUNCH_P(C, Q)                                                                                         \
   hipLaunchKernelGGL((k_trace_persist<C, Q>), dim3(pgrid), dim3(kTraceThreads), 0, ctx->stream, r, n, perm, rec4, lv, \
-                     idDev, nodesDev, primsDev, work)
+                     idDev, nodesDev, primsDev, work, ctx->abortDev)
     if (nodesDev) {
       if (fmt == 2) CPT_LAUNCH_P(true, true); else CPT_LAUNCH_P(true, false);
     } else {

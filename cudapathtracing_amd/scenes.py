@@ -175,6 +175,37 @@ def cornell_box(scene: SceneData, albedo_rng: np.random.Generator, mtype=L.M_LAM
         scene.add_floor(*w, material=m)
 
 
+def box_tris(lo, hi) -> np.ndarray:
+    """12 triangles (n x 9) of an axis-aligned box, wound so that every
+    geometric normal (p1 - p0) x (p2 - p0) points out of the box (the front
+    face of a dielectric solid is its outside)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    c = (lo + hi) / 2
+    v = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for a, b, cc, d in quads:
+        for t in ((a, b, cc), (a, cc, d)):
+            p = v[list(t)]
+            if np.dot(np.cross(p[1] - p[0], p[2] - p[0]), p.mean(axis=0) - c) < 0:
+                p = p[[0, 2, 1]]
+            tris.append(p.reshape(9))
+    return np.array(tris)
+
+
+def glass_column(width=24) -> SceneData:
+    """A tall colour-1 specular-dielectric box (0.2 x 1 x 0.2) under a uniform
+    sky (0.5, 0.7, 0.9), seen from above: light entering the top is trapped
+    by total internal reflection for many vertices (a furnace test of the
+    STATIC profile's roulette with no depth cap)."""
+    s = SceneData()
+    m = s.add_material((1, 1, 1), L.M_SPECULAR_DIELECTRIC)
+    s.add_mesh(box_tris((-0.1, 0.0, -0.1), (0.1, 1.0, 0.1)), m)
+    s.camera = L.CameraDesc(L.d3((0.5, 0.7, 0.9)), L.d3((0.35, 1.6, 0.45)), L.d3((0.0, 0.6, 0.0)), 30.0, 1.0,
+                            width, 0)
+    return s
+
+
 def default_camera(width: int, height: int, fov=40.0) -> L.CameraDesc:
     return L.CameraDesc(L.d3((0, 0, 0)), L.d3((0.0, 0.0, 1.9)), L.d3((0.0, 0.0, 0.0)), fov,
                         width / height, width, 0)

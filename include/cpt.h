@@ -127,6 +127,14 @@ const char *cpt_last_error(const cpt_ctx *ctx);
  * the context's own stream. */
 int cpt_set_stream(cpt_ctx *ctx, void *hip_stream);
 int cpt_synchronize(cpt_ctx *ctx);
+/* Abort: asks the running (or next) call on ctx to stop.  The persistent
+ * traversal walks take no more rays, the path tracer stops between bounces,
+ * and the call returns CPT_ERR_ABORTED with its outputs undefined; every
+ * later call does too until cpt_clear_abort.  cpt_request_abort may be called
+ * from any thread, also while another thread is inside a call on ctx (it only
+ * stores to a word of pinned host memory that the kernels poll). */
+int cpt_request_abort(cpt_ctx *ctx);
+int cpt_clear_abort(cpt_ctx *ctx);
 /* Library build string (arch, version). */
 const char *cpt_version(void);
 /* Options.  CPT_OPT_TRAVERSAL_WIDTH: 4 (default) walks four-wide records
@@ -266,9 +274,21 @@ int cpt_last_trace_stats(const cpt_ctx *ctx, float *total_ms, float *sort_ms,
                          float *kernel_ms);
 
 /* ---- path tracing ------------------------------------------------------- */
-#define CPT_PROFILE_REFERENCE 0 /* Render.cu STATIC: RR 0.7 every bounce,   */
-                                /* no depth cap, width-for-both NEE quirk   */
+/* Render profiles: the reference's RenderType (Render.cuh:11, Render.cu:27-38,
+ * 189-204, 232-251) plus the configurable one of the workloads.  Every
+ * reference profile samples NEE over width x width of each light
+ * (Render.cu:170-171); CONFIG uses width x height. */
+#define CPT_PROFILE_REFERENCE 0 /* STATIC and REAL_TIME_NOT_SAMPLE: RR 0.7   */
+                                /* at every vertex, no depth cap             */
 #define CPT_PROFILE_CONFIG 1    /* max_depth cap, RR from rr_depth on       */
+#define CPT_PROFILE_REAL_TIME 2 /* REAL_TIME: at most 3 vertices, no RR     */
+                                /* (the reference seeds with 1000*t: pass   */
+                                /* that as the seed)                         */
+#define CPT_PROFILE_NORMAL 3    /* NORMAL: 1 spp, at most 2 vertices; rgb = */
+                                /* (n + 0.5) / 2 of the first vertex's       */
+                                /* face-forwarded normal (0 if none)         */
+#define CPT_PROFILE_DEPTH 4     /* DEPTH: 1 spp, at most 2 vertices; rgb =   */
+                                /* clamp01(distance / 50) (1 if none)        */
 
 typedef struct cpt_render_params {
   uint64_t seed;
@@ -331,6 +351,17 @@ int cpt_camera_from_desc(const cpt_camera_desc *desc, cpt_camera *out);
  * uchar(254.99*clamp01(x)); rgba8 gets h*w*4 bytes.  Host pointers. */
 int cpt_tonemap_rgba8(const float *rgb, int32_t width, int32_t height,
                       uint8_t *rgba8);
+
+/* Output files (the reference writes none; its image goes to a GL window).
+ * cpt_write_pfm: the float framebuffer rgb (h*w*3, row 0 = bottom) as a PFM
+ * ("PF", scale -1 = little-endian, rows bottom to top).
+ * cpt_write_png: the reference's displayed bytes as an 8-bit RGBA PNG (top
+ * row first): gamma 1 gives pic2RGBW / STATIC (Render.cu:462-481, 232-238),
+ * uchar(254.99*clamp01(sqrt(x))); gamma 0 the NORMAL / DEPTH display
+ * (Render.cu:239-258), uchar(254.99*clamp01(x)).  Host pointers; CPT_ERR_IO
+ * when the file cannot be written. */
+int cpt_write_pfm(const char *path, const float *rgb, int32_t width, int32_t height);
+int cpt_write_png(const char *path, const float *rgb, int32_t width, int32_t height, int32_t gamma);
 
 /* ---- heterogeneous medium (no reference counterpart; the reference has only
  * the homogeneous sigma=25 sphere hack of Render.cu:64-135) -------------- */

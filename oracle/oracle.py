@@ -65,6 +65,8 @@ RENDER_SIGS = {
     "orc_philox4x32_10": (None, [P, C.c_uint32, C.c_uint32, P]),
     "orc_render": (C.c_int, [P, P, P, P, P, C.c_int]),
     "orc_render_view": (C.c_int, [P, P, P, P, P, C.c_int, C.c_int64]),
+    "orc_render_view_cert": (C.c_int, [P, P, P, P, P, P, C.c_int, C.c_int64]),
+    "orc_volume_walk": (C.c_int, [P, P, P, C.c_double, C.c_uint64, C.c_int, P, P, P, P]),
     "orc_scene_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, P, C.c_double, P]),
     "orc_medium_transmittance": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
     "orc_medium_collide": (C.c_int, [P, P, P, C.c_double, C.c_double, C.c_uint64, C.c_int, P]),
@@ -240,17 +242,42 @@ class OracleScene:
                               cert.ctypes.data)
         return t, ids, cert.astype(bool)
 
-    def render(self, cam, params, per_sample=False, nthreads=0, pixel_offset=0):
-        """Mean radiance per pixel (h, w, 3) float64; per-sample radiance too.
-        pixel_offset: the view's first pixel index in a multi-view stream."""
+    def render(self, cam, params, per_sample=False, nthreads=0, pixel_offset=0, cert=False):
+        """Mean radiance per pixel (h, w, 3) float64 (display values for the
+        NORMAL / DEPTH profiles); per-sample radiance (npix, spp, 3) when
+        per_sample; the per-sample path certificates (npix, spp) uint8 when
+        cert (0 = certified, else the class of the first decision that was
+        not robust).  pixel_offset: the view's first pixel index in a
+        multi-view stream."""
         h, w = cam.height, cam.width
+        spp = 1 if params.profile in (3, 4) else params.spp
         rgb = np.empty((h, w, 3), np.float64)
-        ps = np.empty((h * w, params.spp, 3), np.float64) if per_sample else None
-        rc = lib().orc_render_view(self._h, C.byref(cam), C.byref(params), rgb.ctypes.data,
-                                   ps.ctypes.data if per_sample else None, nthreads, pixel_offset)
+        ps = np.empty((h * w, spp, 3), np.float64) if per_sample else None
+        ct = np.empty((h * w, spp), np.uint8) if cert else None
+        rc = lib().orc_render_view_cert(self._h, C.byref(cam), C.byref(params), rgb.ctypes.data,
+                                        ps.ctypes.data if per_sample else None,
+                                        ct.ctypes.data if cert else None, nthreads, pixel_offset)
         if rc:
             raise RuntimeError("orc_render: build the oracle BVH first")
-        return (rgb, ps) if per_sample else rgb
+        out = (rgb,)
+        if per_sample:
+            out += (ps,)
+        if cert:
+            out += (ct,)
+        return out if len(out) > 1 else rgb
+
+    def volume_walk(self, o, d, t_hit, seed, n):
+        """n walks of the volume object hit at o + t_hit d: continuing ray
+        origins (n, 3), directions (n, 3), throughput (n,), collisions (n,)."""
+        ro = np.empty((n, 3), np.float64)
+        rd = np.empty((n, 3), np.float64)
+        thr = np.empty(n, np.float64)
+        nc = np.empty(n, np.int32)
+        d3_ = C.c_double * 3
+        if lib().orc_volume_walk(self._h, d3_(*o), d3_(*d), t_hit, seed, n, ro.ctypes.data, rd.ctypes.data,
+                                 thr.ctypes.data, nc.ctypes.data):
+            raise RuntimeError("orc_volume_walk: build the oracle BVH first")
+        return ro, rd, thr, nc
 
     def medium_transmittance(self, o, d, tmin, tmax, seed, n):
         """n ratio-tracking transmittance estimates along o + t d."""

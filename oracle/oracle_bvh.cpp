@@ -373,11 +373,34 @@ int Scene::traceRef(const d3 &o, const d3 &d, double tmin, double tmax, double &
   return hitId;
 }
 
+// Robustness of tmin <= t <= tmax for a surface crossing at t: +1 robustly
+// inside, -1 robustly outside, 0 ambiguous.  The tolerance is eb relative to
+// t plus `shift`, the move of the crossing under the origin / direction
+// perturbation (see classify).
+static int rangeRobust(double t, double tmin, double tmax, double eb, double shift) {
+  const double m = std::min(t - tmin, tmax - t);
+  const double tol = eb * std::fabs(t) + shift;
+  if (m > tol) return 1;
+  if (m < -tol) return -1;
+  return 0;
+}
+
 // Robustness of one primitive's FP64 decision against small perturbations
 // (used to certify rays for FP32 comparison).  Returns +1 robust accept,
 // -1 robust reject, 0 ambiguous; tOut = plane/quadric distance.
-static int classify(const Prim &p, const d3 &o, const d3 &d, double tmin, double tmax, double eb,
-                    double &tOut) {
+// eb: relative margin of every decision.  eo, ed: absolute uncertainty of
+// the ray origin and relative uncertainty of its direction (0 for rays
+// given exactly to both sides; a path's rays start at FP32 hit points):
+// a crossing at t with unit surface normal n moves by
+// (eo + |t| |d| ed) / |d . n|, large for grazing rays, and its point on the
+// surface by |d| times that.
+static int classify(const Prim &p, const d3 &o, const d3 &d, double tmin, double tmax, double eb, double eo,
+                    double ed, double &tOut, double *shiftOut = nullptr) {
+  const double dl = Length(d);
+  auto shiftAt = [&](double t, const d3 &nUnit) {
+    const double dn = std::fabs(Dot(d, nUnit));
+    return (eo + std::fabs(t) * dl * ed) / std::max(dn, 1e-300);
+  };
   if (p.type == CPT_OBJ_TRIANGLE) {
     d3 e1 = p.p1 - p.p0, e2 = p.p2 - p.p0;
     d3 h = Cross(d, e2);
@@ -390,14 +413,22 @@ static int classify(const Prim &p, const d3 &o, const d3 &d, double tmin, double
     double v = f * Dot(d, q);
     double t = f * Dot(e2, q);
     tOut = t;
+    const d3 nrm = Cross(e1, e2);
+    const double nl = Length(nrm);
+    const double shift = nl > 0.0 ? shiftAt(t, nrm / nl) : INFINITY;
+    if (shiftOut) *shiftOut = shift;
+    // barycentric margin: eb plus the move of the crossing point over the
+    // shortest edge
+    const double emin = std::min(std::min(Length(e1), Length(e2)), Length(p.p2 - p.p1));
+    const double mb = eb + (shift * dl) / std::max(emin, 1e-300);
     double m = std::min(std::min(u, v), 1.0 - u - v);
-    double tm = std::min(t - tmin, tmax - t) / std::max(std::fabs(t), 1e-30);
+    const int rr = rangeRobust(t, tmin, tmax, eb, shift);
     // a robust barycentric or range rejection holds whatever the det test says
-    if (m < -eb || u > 1.0 + eb || tm < -eb) return -1;
-    double scale = Length(e1) * Length(e2) * Length(d);
+    if (m < -mb || u > 1.0 + mb || rr < 0) return -1;
+    double scale = Length(e1) * Length(e2) * dl;
     if (std::fabs(a) < 0.5e-8) return -1;
     if (std::fabs(a) < 2e-8 || std::fabs(a) < 1e-6 * scale) return 0;
-    if (m > eb && tm > eb) return 1;
+    if (m > mb && rr > 0) return 1;
     return 0;
   }
   if (p.type == CPT_OBJ_SPHERE) {
@@ -408,35 +439,79 @@ static int classify(const Prim &p, const d3 &o, const d3 &d, double tmin, double
     if (disc <= eb * sc) return 0;
     double s = std::sqrt(disc);
     double t0 = (-b - s) / a, t1 = (-b + s) / a;
-    auto rob = [&](double t) { return std::min(t - tmin, tmax - t) / std::max(std::fabs(t), 1e-30); };
-    if (rob(t0) > eb) { tOut = t0; return 1; }
-    if (rob(t0) >= -eb) return 0;
-    if (rob(t1) > eb) { tOut = t1; return 1; }
-    if (rob(t1) >= -eb) return 0;
+    auto rob = [&](double t) { return rangeRobust(t, tmin, tmax, eb, shiftAt(t, ((o + t * d) - p.c) / p.r)); };
+    const int r0 = rob(t0), r1 = rob(t1);
+    if (shiftOut) *shiftOut = shiftAt(r0 > 0 ? t0 : t1, ((o + (r0 > 0 ? t0 : t1) * d) - p.c) / p.r);
+    if (r0 > 0) { tOut = t0; return 1; }
+    if (r0 == 0) return 0;
+    if (r1 > 0) { tOut = t1; return 1; }
+    if (r1 == 0) return 0;
     return -1;
   }
   if (p.type == CPT_OBJ_LIGHT) {
     if (!p.visible) return -1;
     double denom = Dot(d, p.n);
-    double dl = Length(d);
     // the 1e-6 threshold is absolute, so its margin is the FP32 rounding of
-    // the dot product (a few ulp of |d|), not the relative margin eb
-    const double de = 5e-7 * dl;
+    // the dot product (a few ulp of |d|) and the direction's own uncertainty,
+    // not the relative margin eb
+    const double de = 5e-7 * dl + ed * dl;
     if (denom > -(1e-6 - de)) return -1;  // back face, parallel or near-parallel
     if (denom >= -(1e-6 + de)) return 0;
     double t = Dot(p.c - o, p.n) / denom;
     tOut = t;
+    const double shift = shiftAt(t, p.n);
+    if (shiftOut) *shiftOut = shift;
     d3 diff = (o + t * d) - p.c;
     double u = std::fabs(Dot(diff, p.eu)) - p.w / 2.0;
     double v = std::fabs(Dot(diff, p.ev)) - p.h / 2.0;
     double scale = std::max(p.w, p.h);
-    double tm = std::min(t - tmin, tmax - t) / std::max(std::fabs(t), 1e-30);
+    const int rr = rangeRobust(t, tmin, tmax, eb, shift);
     double m = std::max(u, v) / scale;
-    if (m > eb || tm < -eb) return -1;
-    if (m < -eb && tm > eb) return 1;
+    const double mb = eb + shift * dl / scale;
+    if (m > mb || rr < 0) return -1;
+    if (m < -mb && rr > 0) return 1;
     return 0;
   }
   return -1;
+}
+
+// Closest hit over all primitives (the last accepted primitive among equal
+// distances wins, as in the reference's leaf order up to ties) with a
+// certificate: ok stays true only when every primitive's accept/reject
+// decision is robust at relative margin eb, origin uncertainty eo and
+// direction uncertainty ed (classify) and the winner is separated from the
+// runner-up and from every ambiguous candidate by more than et (relative).
+// Certified rays have one FP32/FP64-stable answer.
+int Scene::bruteClosest(const d3 &o, const d3 &d, double tmin, double tmax, double eb, double et,
+                        double &tOut, bool &ok, double eo, double ed, double *tErrOut) const {
+  const int64_t np = (int64_t)prims.size();
+  double best = tmax, second = INFINITY;
+  int id = -1;
+  double amb_t = INFINITY;  // nearest ambiguous candidate
+  for (int64_t k = 0; k < np; k++) {
+    double t = 0;
+    const int c = classify(prims[k], o, d, tmin, tmax, eb, eo, ed, t);
+    if (c == 0) amb_t = std::min(amb_t, std::fabs(t) > 0 ? t : 0.0);
+    double th;
+    if (hitPrim(prims[k], o, d, tmin, id >= 0 ? best : tmax, th)) {
+      if (id >= 0) second = std::min(second, best);
+      best = th;
+      id = (int)k;
+    } else if (c == 1) {
+      second = std::min(second, t);
+    }
+  }
+  if (id >= 0) {
+    if (second < INFINITY && (second - best) <= et * std::fabs(best)) ok = false;
+    if (amb_t <= best * (1.0 + et)) ok = false;
+    double tt, sh = 0.0;
+    if (classify(prims[id], o, d, tmin, tmax, eb, eo, ed, tt, &sh) != 1) ok = false;
+    if (tErrOut) *tErrOut = eb * std::fabs(best) + sh;
+  } else if (amb_t < INFINITY) {
+    ok = false;
+  }
+  tOut = id >= 0 ? best : tmax;
+  return id;
 }
 
 }  // namespace orc
@@ -542,39 +617,14 @@ int orc_trace_ref(void *sp, const float *rays, int64_t n, double *tOut, int32_t 
 int orc_trace_brute(void *sp, const float *rays, int64_t n, double eb, double et, double *tOut,
                     int32_t *idOut, uint8_t *cert) {
   Scene *s = (Scene *)sp;
-  const int64_t np = (int64_t)s->prims.size();
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t i = 0; i < n; i++) {
     const float *r = rays + 8 * i;
     d3 o = mk(r[0], r[1], r[2]), d = mk(r[4], r[5], r[6]);
-    double tmin = r[3], tmax = r[7];
-    double best = tmax, second = INFINITY;
-    int id = -1;
     bool ok = true;
-    double amb_t = INFINITY;  // nearest ambiguous candidate
-    for (int64_t k = 0; k < np; k++) {
-      double t = 0;
-      int c = classify(s->prims[k], o, d, tmin, tmax, eb, t);
-      if (c == 0) amb_t = std::min(amb_t, std::fabs(t) > 0 ? t : 0.0);
-      double th;
-      if (hitPrim(s->prims[k], o, d, tmin, id >= 0 ? best : tmax, th)) {
-        if (id >= 0) second = std::min(second, best);
-        best = th;
-        id = (int)k;
-      } else if (c == 1) {
-        second = std::min(second, t);
-      }
-    }
-    if (id >= 0) {
-      if (second < INFINITY && (second - best) <= et * std::fabs(best)) ok = false;
-      if (amb_t <= best * (1.0 + et)) ok = false;
-      double tt;
-      int cw = classify(s->prims[id], o, d, tmin, tmax, eb, tt);
-      if (cw != 1) ok = false;
-    } else {
-      if (amb_t < INFINITY) ok = false;
-    }
-    if (tOut) tOut[i] = id >= 0 ? best : tmax;
+    double best = 0;
+    const int id = s->bruteClosest(o, d, r[3], r[7], eb, et, best, ok);
+    if (tOut) tOut[i] = best;
     if (idOut) idOut[i] = id;
     if (cert) cert[i] = ok ? 1 : 0;
   }

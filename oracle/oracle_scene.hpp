@@ -56,6 +56,12 @@ struct Scene {
   void build(const double bmin[3], const double bmax[3]);
   int traceRef(const d3 &o, const d3 &d, double tmin, double tmax, double &tBest,
                int64_t *nodesVisited) const;
+  // Brute-force closest hit; ok is cleared unless the answer is robust to
+  // relative perturbations eb (decisions) and et (hit ordering), an origin
+  // uncertainty eo (absolute) and a direction uncertainty ed (relative);
+  // tErrOut (optional): the absolute uncertainty of the winner's distance.
+  int bruteClosest(const d3 &o, const d3 &d, double tmin, double tmax, double eb, double et, double &tOut,
+                   bool &ok, double eo = 0.0, double ed = 0.0, double *tErrOut = nullptr) const;
 };
 
 bool hitTriangle(const Prim &p, const d3 &o, const d3 &d, double tmin, double tmax, double &tOut,
