@@ -21,6 +21,7 @@ CPT_ERR_OOM = -4
 CPT_ERR_IO = -5
 CPT_ERR_ABORTED = -6
 CPT_DEVICE_PTRS = 1
+NOALLOC_MAX_RAYS = 1 << 20
 
 # MaterialType (CudaPathTracing/rtmath.cuh:15) + new entries
 M_NONE, M_LIGHT, M_OPAQUE, M_SPECULAR_DIELECTRIC, M_ROUGH_DIELECTRIC, M_VOLUME, M_LAMBERT, M_GGX = range(8)
@@ -146,6 +147,7 @@ SIGNATURES = [
     ("cpt_clear_abort", C.c_int, [P]),
     ("cpt_write_pfm", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32]),
     ("cpt_write_png", C.c_int, [C.c_char_p, P, C.c_int32, C.c_int32, C.c_int32]),
+    ("cpt_trace_closest_noalloc", C.c_int, [P, P, C.c_int64, P, P]),
 ]
 
 _lib = None

@@ -782,11 +782,11 @@ int launchMorton(cpt_ctx *ctx, const double bmin[3], const double bmax[3], uint6
 static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
                       uint32_t *valsTmp, int64_t n, int bits, bool identityVals,
                       bool firstHistDone, uint64_t *keys64Out, uint32_t *orderOut,
-                      uint32_t **valsOut = nullptr) {
+                      uint32_t **valsOut = nullptr, uint32_t *hist = nullptr, uint32_t *rowTotal = nullptr) {
   const int nTiles = (int)((n + kTile - 1) / kTile);
   const int passes = (bits + 7) / 8;
-  uint32_t *hist = ctx->hist.as<uint32_t>();
-  uint32_t *rowTotal = ctx->rowTotal.as<uint32_t>();
+  if (!hist) hist = ctx->hist.as<uint32_t>();
+  if (!rowTotal) rowTotal = ctx->rowTotal.as<uint32_t>();
   uint32_t *kin = keys, *vin = vals, *kout = keysTmp, *vout = valsTmp;
   for (int p = 0; p < passes; p++) {
     const int shift = 8 * p;
@@ -822,15 +822,16 @@ static int sortPasses(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *ke
 }
 
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
-                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut) {
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut,
+                   uint32_t *hist, uint32_t *rowTotal) {
   if (valsOut) *valsOut = vals;
   if (n <= 0) return CPT_OK;
   const int nTiles = (int)((n + kTile - 1) / kTile);
   int rc;
-  if ((rc = ensure(ctx, ctx->hist, (size_t)nTiles * 256 * 4))) return rc;
-  if ((rc = ensure(ctx, ctx->rowTotal, 256 * 4))) return rc;
+  if (!hist && (rc = ensure(ctx, ctx->hist, (size_t)nTiles * 256 * 4))) return rc;
+  if (!rowTotal && (rc = ensure(ctx, ctx->rowTotal, 256 * 4))) return rc;
   return sortPasses(ctx, keys, vals, keysTmp, valsTmp, n, bits, identityVals, false, nullptr,
-                    nullptr, valsOut);
+                    nullptr, valsOut, hist, rowTotal);
 }
 
 int recordFormat(const cpt_ctx *ctx) {

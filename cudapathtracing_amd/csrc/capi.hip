@@ -136,6 +136,10 @@ int cpt_create(cpt_ctx **out, int dev) {
   ctx->abortHost = (uint32_t *)ab;
   ctx->abortDev = (const uint32_t *)abDev;
   __atomic_store_n(ctx->abortHost, 0u, __ATOMIC_RELEASE);
+  if (noAllocScratchInit(ctx) != CPT_OK) {
+    cpt_destroy(ctx);
+    return CPT_ERR_HIP;
+  }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
     ctx->numCU = cus;

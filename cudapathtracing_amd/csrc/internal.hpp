@@ -100,6 +100,8 @@ struct cpt_ctx {
   // walks poll it through abortDev, the render loop between bounces
   uint32_t *abortHost = nullptr;
   const uint32_t *abortDev = nullptr;
+  // this device's static scratch of cpt_trace_closest_noalloc
+  uint32_t *noAllocScratch = nullptr;
 
   cpt::RenderBufs *render = nullptr;
 };
@@ -115,9 +117,12 @@ void releaseAll(cpt_ctx *ctx);
 int launchMorton(cpt_ctx *ctx, const double smin[3], const double smax[3], uint64_t *keysDev);
 // Stable LSD sort of (keys, vals) in 8-bit passes; the result is in (keys,
 // vals), or, with valsOut, the sorted values are left where the last pass put
-// them (vals or valsTmp) and *valsOut points there.
+// them (vals or valsTmp) and *valsOut points there.  hist (ceil(n/4096)*256
+// words) and rowTotal (256 words): caller scratch, or null for the
+// context's own (allocated on demand).
 int radixSortPairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *vals, uint32_t *keysTmp,
-                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut = nullptr);
+                   uint32_t *valsTmp, int64_t n, int bits, bool identityVals, uint32_t **valsOut = nullptr,
+                   uint32_t *hist = nullptr, uint32_t *rowTotal = nullptr);
 int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build_stats *st);
 int refitBvh(cpt_ctx *ctx, cpt_build_stats *st);
 // Traversal record format: 0 binary nodes, 1 four-wide FP32, 2 four-wide
@@ -136,8 +141,17 @@ int packTriangles(cpt_ctx *ctx, const double *tri64Dev, const int32_t *matDev, i
 void expandBounds(const double bmin[3], const double bmax[3], double smin[3], double smax[3]);
 
 // trace.hip
+// Scratch of one traversal call: ray-order keys / permutation (n words
+// each), radix histograms (ceil(n/4096)*256 + 256 words) and the persistent
+// walk's grab counter.  Null: the context's own buffers, allocated on demand.
+struct TraceScratch {
+  uint32_t *keysA, *keysB, *valsA, *valsB, *hist, *rowTotal, *grab;
+};
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
-                int32_t *nodesDev, int32_t *primsDev);
+                int32_t *nodesDev, int32_t *primsDev, const TraceScratch *scratch = nullptr);
+// Device address of this GPU's static no-allocation scratch (set up once
+// per device by cpt_create).
+int noAllocScratchInit(cpt_ctx *ctx);
 
 // render.hip
 void renderRelease(cpt_ctx *ctx);

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "cpt_device.hpp"
 #include "internal.hpp"
@@ -174,7 +176,7 @@ __global__ void k_zero_work(uint32_t *p) {
 }
 
 int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int32_t *idDev,
-                int32_t *nodesDev, int32_t *primsDev) {
+                int32_t *nodesDev, int32_t *primsDev, const TraceScratch *sc) {
   if (n <= 0) return CPT_OK;
   Scene64 s64{ctx->tri64.as<double>(), ctx->sph64.as<double>(), ctx->lgt64.as<double>(),
               ctx->nLights, ctx->nSpheres};
@@ -184,32 +186,44 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   const int np = (int)ctx->nPrims;
   const bool sorted = ctx->raySortMin > 0 && n >= ctx->raySortMin && np > 1 && n <= 0x7fffffff;
   int rc;
-  if (sorted && ((rc = ensure(ctx, ctx->rkA, n * 4)) || (rc = ensure(ctx, ctx->rkB, n * 4)) ||
-                 (rc = ensure(ctx, ctx->rvA, n * 4)) || (rc = ensure(ctx, ctx->rvB, n * 4))))
+  if (!sc && sorted && ((rc = ensure(ctx, ctx->rkA, n * 4)) || (rc = ensure(ctx, ctx->rkB, n * 4)) ||
+                        (rc = ensure(ctx, ctx->rvA, n * 4)) || (rc = ensure(ctx, ctx->rvB, n * 4))))
     return rc;
+  uint32_t *rkA = sc ? sc->keysA : ctx->rkA.as<uint32_t>(), *rkB = sc ? sc->keysB : ctx->rkB.as<uint32_t>();
+  uint32_t *rvA = sc ? sc->valsA : ctx->rvA.as<uint32_t>(), *rvB = sc ? sc->valsB : ctx->rvB.as<uint32_t>();
   ctx->lastTraceSorted = sorted;
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[8], ctx->stream));
   const uint32_t *perm = nullptr;
   if (sorted) {
     hipLaunchKernelGGL(k_ray_keys, dim3(grid), dim3(kTraceThreads), 0, ctx->stream, r, 2, (const uint32_t *)nullptr, n,
-                       (const float *)ctx->rootBox.as<float>(), ctx->rkA.as<uint32_t>());
+                       (const float *)ctx->rootBox.as<float>(), rkA);
     CPT_TRY_HIP(ctx, hipGetLastError());
-    if ((rc = radixSortPairs(ctx, ctx->rkA.as<uint32_t>(), ctx->rvA.as<uint32_t>(), ctx->rkB.as<uint32_t>(),
-                             ctx->rvB.as<uint32_t>(), n, 30, true)))
+    if ((rc = radixSortPairs(ctx, rkA, rvA, rkB, rvB, n, 30, true, nullptr, sc ? sc->hist : nullptr,
+                             sc ? sc->rowTotal : nullptr)))
       return rc;
-    perm = ctx->rvA.as<uint32_t>();
+    perm = rvA;
     CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
   }
-  if ((rc = ensureRecords(ctx))) return rc;
   const int fmt = recordFormat(ctx);
+  if (sc) {
+    // no allocation: the records must already be current (a lazy rebuild
+    // would allocate them)
+    const bool stale = fmt != 0 && np > 1 &&
+                       (!(ctx->recValid & (1 << fmt)) || (ctx->cluster == 1 && (fmt == 1 || fmt == 2) && !ctx->clusterValid));
+    if (stale)
+      return fail(ctx, CPT_ERR_STATE, "cpt_trace_closest_noalloc",
+                  "traversal records are stale: build, refit or trace once with an allocating call");
+  } else if ((rc = ensureRecords(ctx))) {
+    return rc;
+  }
   // read after ensureRecords, which may (re)allocate the records
   const float4 *rec4 = fmt == 3   ? ctx->nodes8.as<float4>()
                        : fmt == 2 ? ctx->nodes4q.as<float4>()
                                   : ctx->nodes4.as<float4>();
   if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1 && (nodesDev || (tDev && idDev)) &&
       n < ((int64_t)1 << 32) - kChunk * 65536) {
-    if ((rc = ensure(ctx, ctx->cntA, 256))) return rc;
-    uint32_t *work = ctx->cntA.as<uint32_t>();
+    if (!sc && (rc = ensure(ctx, ctx->cntA, 256))) return rc;
+    uint32_t *work = sc ? sc->grab : ctx->cntA.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
     // one resident wave set: as many workgroups as fit on every CU at once
     int perCU = 0;
@@ -251,4 +265,58 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
   return CPT_OK;
 }
 
+// ---- scratch of the entry point that allocates nothing -------------------
+// One static array per GPU (the code object's copy on each device), held
+// until the process ends: ray-order keys and permutation ping-pong for
+// CPT_NOALLOC_MAX_RAYS rays, radix histograms for as many 4096-key tiles, the
+// row totals and the grab counter.  Host code uses the device address that
+// hipGetSymbolAddress gives (cpt_ctx::noAllocScratch), never the symbol.
+constexpr int64_t kNoAllocRays = CPT_NOALLOC_MAX_RAYS;
+constexpr int64_t kNoAllocTiles = (kNoAllocRays + 4095) / 4096;
+constexpr int64_t kNoAllocWords = 4 * kNoAllocRays + kNoAllocTiles * 256 + 256 + 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+int noAllocScratchInit(cpt_ctx *ctx) {
+  // one lookup per device (the context's device is current here)
+  static std::mutex mu;
+  static std::vector<uint32_t *> perDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)perDevice.size() <= ctx->device) perDevice.resize(ctx->device + 1, nullptr);
+  if (!perDevice[ctx->device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return fail(ctx, CPT_ERR_HIP, "hipGetSymbolAddress(scratch)", hipGetErrorString(e));
+    }
+    perDevice[ctx->device] = (uint32_t *)ptr;
+  }
+  ctx->noAllocScratch = perDevice[ctx->device];
+  return CPT_OK;
+}
+
 }  // namespace cpt
+
+using namespace cpt;
+
+extern "C" int cpt_trace_closest_noalloc(cpt_ctx *ctx, const float *rays, int64_t n, float *tOut, int32_t *idOut) {
+  if (!ctx || n < 0 || (n > 0 && (!rays || !tOut || !idOut))) return CPT_ERR_ARG;
+  if (n > kNoAllocRays)
+    return fail(ctx, CPT_ERR_ARG, "cpt_trace_closest_noalloc", "more rays than CPT_NOALLOC_MAX_RAYS");
+  if (!ctx->built) return fail(ctx, CPT_ERR_STATE, "cpt_trace_closest_noalloc", "no BVH built");
+  if (!ctx->noAllocScratch) return fail(ctx, CPT_ERR_STATE, "cpt_trace_closest_noalloc", "scratch not set up");
+  if (n == 0) return CPT_OK;
+  if (ctx->abortHost && __atomic_load_n(ctx->abortHost, __ATOMIC_ACQUIRE))
+    return fail(ctx, CPT_ERR_ABORTED, "cpt_trace_closest_noalloc", "aborted by cpt_request_abort");
+  CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
+  uint32_t *p = ctx->noAllocScratch;
+  TraceScratch sc;
+  sc.keysA = p;
+  sc.keysB = p + kNoAllocRays;
+  sc.valsA = p + 2 * kNoAllocRays;
+  sc.valsB = p + 3 * kNoAllocRays;
+  sc.hist = p + 4 * kNoAllocRays;
+  sc.rowTotal = sc.hist + kNoAllocTiles * 256;
+  sc.grab = sc.rowTotal + 256;  // cleared by traceDevice's k_zero_work before each walk
+  return traceDevice(ctx, rays, n, tOut, idOut, nullptr, nullptr, &sc);
+}

@@ -259,6 +259,19 @@ int cpt_radix_sort_pairs(cpt_ctx *ctx, uint32_t *keys, uint32_t *values,
 int cpt_trace_closest(cpt_ctx *ctx, const float *rays, int64_t n,
                       float *t_out, int32_t *id_out, int flags);
 
+/* No counterpart in the reference: cpt_trace_closest on device buffers that
+ * allocates nothing, for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Its scratch (ray-order keys and permutation, radix histograms, the
+ * persistent walk's grab counter) is a static device array of the library,
+ * one per GPU (about 17 MB), held until the process ends; the BVH and its
+ * traversal records must be current (cpt_bvh_build / cpt_bvh_refit make them;
+ * CPT_ERR_STATE otherwise).  At most CPT_NOALLOC_MAX_RAYS rays per call
+ * (CPT_ERR_ARG beyond).  Calls that overlap on one GPU, from any context,
+ * race on that array.  Outputs equal cpt_trace_closest's. */
+#define CPT_NOALLOC_MAX_RAYS (1 << 20)
+int cpt_trace_closest_noalloc(cpt_ctx *ctx, const float *rays_dev, int64_t n, float *t_dev, int32_t *id_dev);
+
 /* Same traversal with per-ray work counters (nodes visited, primitives
  * tested), used for the algorithmic-bytes model. */
 int cpt_trace_count(cpt_ctx *ctx, const float *rays, int64_t n,
