@@ -878,32 +878,21 @@ enum LaneState { kIdle = 0, kClaimed = 1, kLoading = 2, kActive = 3 };
 // its next chunk, issued when the current one was taken so that no lane
 // waits for the counter.  Every idle lane gets the next item of the chunk
 // (returns true, item set) while the chunk lasts.  `more` turns false once a
-// grab has passed n, or once the host's abort word (pinned host memory,
-// read with lane 0's grab and used at the next chunk, so no lane waits for
-// it) is non-zero: the wave then takes no more items and leaves when its
-// walks are over.  Call with the whole wave active (the caller's loop is
+// grab has passed n.  Call with the whole wave active (the caller's loop is
 // wave-uniform).
 struct WaveChunk {
   uint32_t cur = 0, end = 0;  // wave-uniform
   uint32_t ticket = 0;
-  uint32_t abortSeen = 0;     // lane 0: the abort word read with the ticket
   bool more = true;
 };
 
-__device__ __forceinline__ uint32_t readAbort(const uint32_t *abortFlag) {
-  return abortFlag ? __hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
+__device__ __forceinline__ void waveChunkInit(WaveChunk &c, uint32_t *counter) {
+  if (__lane_id() == 0) c.ticket = atomicAdd(counter, (uint32_t)kChunk);
 }
 
-__device__ __forceinline__ void waveChunkInit(WaveChunk &c, uint32_t *counter, const uint32_t *abortFlag = nullptr) {
-  if (__lane_id() == 0) {
-    c.ticket = atomicAdd(counter, (uint32_t)kChunk);
-    c.abortSeen = readAbort(abortFlag);
-  }
-}
-
-// n < 2^32 - 2^24 (the counter overshoots n by at most two grabs per wave).
-__device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint32_t n, bool idle, uint32_t &item,
-                                          const uint32_t *abortFlag = nullptr) {
+// n < 2^30 (the counter overshoots n by at most two grabs per wave, and an
+// abort moves it to kAbortGrab).
+__device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint32_t n, bool idle, uint32_t &item) {
   const uint64_t im = __ballot(idle);
   if (!im || !c.more) return false;
   // a trip that claims waits for the claimed lanes' loads before its walk
@@ -913,17 +902,13 @@ __device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint3
   const uint32_t lane = __lane_id();
   if (c.cur >= c.end) {
     const uint32_t base = __builtin_amdgcn_readfirstlane(__shfl(c.ticket, 0, 64));
-    const uint32_t ab = __builtin_amdgcn_readfirstlane(__shfl(c.abortSeen, 0, 64));
-    if (base >= n || ab != 0u) {
+    if (base >= n) {
       c.more = false;
       return false;
     }
     c.cur = base;
     c.end = n - base > (uint32_t)kChunk ? base + (uint32_t)kChunk : n;
-    if (lane == 0) {
-      c.ticket = atomicAdd(counter, (uint32_t)kChunk);
-      c.abortSeen = readAbort(abortFlag);
-    }
+    if (lane == 0) c.ticket = atomicAdd(counter, (uint32_t)kChunk);
   }
   const uint32_t avail = c.end - c.cur;
   const uint32_t nIdle = (uint32_t)__popcll(im);
@@ -932,6 +917,23 @@ __device__ __forceinline__ bool waveClaim(WaveChunk &c, uint32_t *counter, uint3
   item = c.cur + rank;
   c.cur += nIdle < avail ? nIdle : avail;
   return got;
+}
+
+// Host abort of a persistent walk.  The abort word lives in pinned host
+// memory, and a load of it crosses the host link: waiting for one per chunk
+// grab in every wave stalls the walks (loads return in order).  So only
+// wave 0 of workgroup 0 reads it, every kAbortPollTrips of its loop trips;
+// once it is set, that wave moves the shared grab counter to kAbortGrab,
+// past every n, and each wave's next grab ends its claiming: the walks in
+// flight finish and every wave leaves normally.
+constexpr uint32_t kAbortGrab = 0xC0000000u;
+constexpr uint32_t kAbortPollTrips = 64;
+
+__device__ __forceinline__ void abortRelay(const uint32_t *abortFlag, uint32_t *counter, uint32_t &trips) {
+  if (!abortFlag || blockIdx.x != 0 || threadIdx.x >= 64) return;  // wave-uniform
+  if ((++trips & (kAbortPollTrips - 1)) != 1u) return;
+  if (__lane_id() == 0 && __hip_atomic_load(abortFlag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u)
+    atomicMax(counter, kAbortGrab);
 }
 
 // Wave-aggregated fetch of work items for the lanes with `want`: returns the

@@ -411,8 +411,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
   int st = kIdle;
   unsigned long long trips = 0;
   const uint32_t count = *countDev;
+  uint32_t pollTrips = 0;
   WaveChunk ch;
-  waveChunkInit(ch, grab, abortFlag);
+  waveChunkInit(ch, grab);
   while (true) {
     // paths loaded in the last trip start their walk
     if (st == kLoading) {
@@ -425,7 +426,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_extend_persist(
       st = kLoading;
     }
     uint32_t item = 0;
-    if (waveClaim(ch, grab, count, st == kIdle, item, abortFlag)) {
+    abortRelay(abortFlag, grab, pollTrips);
+    if (waveClaim(ch, grab, count, st == kIdle, item)) {
       if (kPerm) {
         p = perm[item];
         st = kClaimed;

@@ -119,8 +119,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
   TraceCounters cnt;
   int64_t ri = 0;
   int st = kIdle;
+  uint32_t pollTrips = 0;
   WaveChunk ch;
-  waveChunkInit(ch, work, abortFlag);
+  waveChunkInit(ch, work);
   while (true) {
     // rays loaded in the last trip start their walk
     if (st == kLoading) {
@@ -135,7 +136,8 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     }
     // idle lanes claim the next rays of the wave's chunk
     uint32_t item = 0;
-    if (waveClaim(ch, work, (uint32_t)nRays, st == kIdle, item, abortFlag)) {
+    abortRelay(abortFlag, work, pollTrips);
+    if (waveClaim(ch, work, (uint32_t)nRays, st == kIdle, item)) {
       if (perm) {
         ri = (int64_t)perm[item];
         st = kClaimed;
@@ -221,7 +223,7 @@ int traceDevice(cpt_ctx *ctx, const float *raysDev, int64_t n, float *tDev, int3
                        : fmt == 2 ? ctx->nodes4q.as<float4>()
                                   : ctx->nodes4.as<float4>();
   if ((fmt == 1 || fmt == 2) && ctx->persistent && np > 1 && (nodesDev || (tDev && idDev)) &&
-      n < ((int64_t)1 << 32) - kChunk * 65536) {
+      n < (int64_t)kAbortGrab) {
     if (!sc && (rc = ensure(ctx, ctx->cntA, 256))) return rc;
     uint32_t *work = sc ? sc->grab : ctx->cntA.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(64), 0, ctx->stream, work);
