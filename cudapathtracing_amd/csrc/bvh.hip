@@ -898,6 +898,20 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
   return ensureRecords(ctx, true);
 }
 
+// The escape list of the last refit held every crossing node: its counter
+// (which also counts entries past the list's bound) is at most the bound.
+// Called after the refit has finished (the stream is idle there).
+static int checkRefitEscapes(cpt_ctx *ctx, int64_t n, const char *what) {
+  const int64_t cap = refitEscapeBound(n);
+  uint32_t cnt = 0;
+  CPT_TRY_HIP(ctx, hipMemcpy(&cnt, ctx->esc.as<int32_t>() + cap, 4, hipMemcpyDeviceToHost));
+  if ((int64_t)cnt > cap) {
+    ctx->built = false;
+    return fail(ctx, CPT_ERR_STATE, what, "refit escape list overflow (node boxes incomplete)");
+  }
+  return CPT_OK;
+}
+
 // The clustered tree of the current leaves (CPT_OPT_CLUSTER), and the wide
 // records from it when it is chosen.  Returns its time in *ms.
 int clusterStage(cpt_ctx *ctx, float *ms) {
@@ -930,6 +944,7 @@ int refitBvh(cpt_ctx *ctx, cpt_build_stats *st) {
   if (rc) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
   CPT_TRY_HIP(ctx, hipEventSynchronize(ev[1]));
+  if ((rc = checkRefitEscapes(ctx, n, "cpt_bvh_refit"))) return rc;
   float ms = 0, cms = 0;
   CPT_TRY_HIP(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (ctx->cluster == 1 && (rc = clusterStage(ctx, &cms))) return rc;
@@ -991,6 +1006,7 @@ int buildBvh(cpt_ctx *ctx, const double bmin[3], const double bmax[3], cpt_build
   if ((rc = launchRefit(ctx, n))) return rc;
   CPT_TRY_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
   CPT_TRY_HIP(ctx, hipEventSynchronize(ev[4]));
+  if ((rc = checkRefitEscapes(ctx, n, "cpt_bvh_build"))) return rc;
   float cms = 0;
   if (ctx->cluster == 1 && (rc = clusterStage(ctx, &cms))) return rc;
   if (st) {

@@ -334,14 +334,16 @@ int cpt_scene_upload(cpt_ctx *ctx, const cpt_material *materials, int32_t nm, co
   CPT_TRY_HIP(ctx, hipMemcpyAsync(ctx->lightIdx.p, lidx.data(), lidx.size() * 4, hipMemcpyHostToDevice, s));
   if (nt > 0) {
     CPT_TRY_HIP(ctx, hipMemcpyAsync(ctx->tri64.p, tri, nt * 72, hipMemcpyHostToDevice, s));
-    DevBuf matTmp;
+    // triangle material ids staged in a buffer of the context (no early
+    // return can leak it)
+    const int32_t *matDev = nullptr;
     if (triMat) {
-      if ((rc = ensure(ctx, matTmp, nt * 4))) return rc;
-      CPT_TRY_HIP(ctx, hipMemcpyAsync(matTmp.p, triMat, nt * 4, hipMemcpyHostToDevice, s));
+      if ((rc = ensure(ctx, ctx->cntB, nt * 4))) return rc;
+      CPT_TRY_HIP(ctx, hipMemcpyAsync(ctx->cntB.p, triMat, nt * 4, hipMemcpyHostToDevice, s));
+      matDev = ctx->cntB.as<int32_t>();
     }
-    rc = packTriangles(ctx, ctx->tri64.as<double>(), matTmp.as<int32_t>(), nt, small);
+    rc = packTriangles(ctx, ctx->tri64.as<double>(), matDev, nt, small);
     hipError_t e = hipStreamSynchronize(s);
-    if (matTmp.p) (void)hipFree(matTmp.p);
     if (rc) return rc;
     if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_upload", hipGetErrorString(e));
   } else {
@@ -359,8 +361,8 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri, i
   double *dst = ctx->tri64.as<double>() + 9 * first;
   CPT_TRY_HIP(ctx, hipMemcpyAsync(dst, tri, n * 72, hipMemcpyHostToDevice, s));
   // materials are kept: the pack kernel rewrites them from the old records
-  DevBuf mat;
-  int rc = ensure(ctx, mat, n * 4);
+  // (the id buffer is the context's, so no early return leaks it)
+  int rc = ensure(ctx, ctx->cntB, n * 4);
   if (rc) return rc;
   const int64_t base = (int64_t)ctx->nLights + ctx->nSpheres;
   // extract the material ids (word 11 of each 48 B record) on the host side
@@ -370,10 +372,9 @@ int cpt_scene_update_triangles(cpt_ctx *ctx, int64_t first, const double *tri, i
   CPT_TRY_HIP(ctx, hipStreamSynchronize(s));
   std::vector<int32_t> mids(n);
   for (int64_t i = 0; i < n; i++) std::memcpy(&mids[i], &recs[12 * i + 11], 4);
-  CPT_TRY_HIP(ctx, hipMemcpyAsync(mat.p, mids.data(), n * 4, hipMemcpyHostToDevice, s));
-  rc = packTriangles(ctx, dst, mat.as<int32_t>(), n, base + first);
+  CPT_TRY_HIP(ctx, hipMemcpyAsync(ctx->cntB.p, mids.data(), n * 4, hipMemcpyHostToDevice, s));
+  rc = packTriangles(ctx, dst, ctx->cntB.as<int32_t>(), n, base + first);
   hipError_t e = hipStreamSynchronize(s);
-  (void)hipFree(mat.p);
   if (rc) return rc;
   if (e != hipSuccess) return fail(ctx, CPT_ERR_HIP, "cpt_scene_update_triangles", hipGetErrorString(e));
   return CPT_OK;

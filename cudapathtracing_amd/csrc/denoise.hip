@@ -83,84 +83,144 @@ __global__ void k_prep_input(const float *__restrict__ in10, int h, int w, int H
   logRad[i * 3 + 2] = v[2];
 }
 
+// The elementwise kernels below move 8 channels (16 B) per thread; every
+// channel count is a multiple of 8 and n counts 8-channel groups.
+__device__ __forceinline__ void unpack8(const uint4 v, float f[8]) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    f[2 * j] = __uint_as_float(w[j] << 16);
+    f[2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) w[j] = (uint32_t)f2bf(f[2 * j]) | ((uint32_t)f2bf(f[2 * j + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// 2x2 average pooling; n = B*(H/2)*(W/2)*C/8
 __global__ void k_pool2(const uint16_t *__restrict__ X, int H, int W, int C, int64_t n, uint16_t *__restrict__ Y) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*(H/2)*(W/2)*C
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int c = (int)(i % C);
-  int64_t p = i / C;
+  const int C8 = C / 8;
+  const int c = (int)(i % C8) * 8;
+  int64_t p = i / C8;
   const int w2 = W / 2, h2 = H / 2;
   const int x = (int)(p % w2);
   p /= w2;
   const int y = (int)(p % h2);
   const int64_t b = p / h2;
   const int64_t base = ((b * H + 2 * y) * W + 2 * x) * C + c;
-  float s = bf2f(X[base]) + bf2f(X[base + C]) + bf2f(X[base + (int64_t)W * C]) + bf2f(X[base + (int64_t)W * C + C]);
-  Y[i] = f2bf(0.25f * s);
+  float s[8], t[8];
+  unpack8(*(const uint4 *)(X + base), s);
+  const int64_t off[3] = {C, (int64_t)W * C, (int64_t)W * C + C};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    unpack8(*(const uint4 *)(X + base + off[k]), t);
+#pragma unroll
+    for (int j = 0; j < 8; j++) s[j] += t[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) s[j] *= 0.25f;
+  *(uint4 *)(Y + i * 8) = pack8(s);
 }
 
-// dX[2y+i][2x+j] (+)= dY[y][x] / 4
+// dX[2y+i][2x+j] (+)= dY[y][x] / 4; n = B*H*W*C/8 (full resolution)
 __global__ void k_pool2_bwd(const uint16_t *__restrict__ dY, int H, int W, int C, int64_t n, uint16_t *__restrict__ dX,
                             int accumulate) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*H*W*C (full res)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int c = (int)(i % C);
-  int64_t p = i / C;
+  const int C8 = C / 8;
+  const int c = (int)(i % C8) * 8;
+  int64_t p = i / C8;
   const int x = (int)(p % W);
   p /= W;
   const int y = (int)(p % H);
   const int64_t b = p / H;
-  const float g = 0.25f * bf2f(dY[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c]);
-  dX[i] = f2bf(accumulate ? bf2f(dX[i]) + g : g);
+  float g[8];
+  unpack8(*(const uint4 *)(dY + ((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c), g);
+  if (accumulate) {
+    float d[8];
+    unpack8(*(const uint4 *)(dX + i * 8), d);
+#pragma unroll
+    for (int j = 0; j < 8; j++) g[j] = d[j] + 0.25f * g[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; j++) g[j] *= 0.25f;
+  }
+  *(uint4 *)(dX + i * 8) = pack8(g);
 }
 
-// Out[b,y,x] = [Low[b,y/2,x/2, 0:C1] | Skip[b,y,x, 0:C2]]
+// Out[b,y,x] = [Low[b,y/2,x/2, 0:C1] | Skip[b,y,x, 0:C2]]; n = B*H*W*(C1+C2)/8
 __global__ void k_up_concat(const uint16_t *__restrict__ low, int C1, const uint16_t *__restrict__ skip, int C2, int H,
                             int W, int64_t n, uint16_t *__restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*H*W*(C1+C2)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int C = C1 + C2;
-  const int c = (int)(i % C);
-  int64_t p = i / C;
+  const int C8 = (C1 + C2) / 8;
+  const int c = (int)(i % C8) * 8;
+  const int64_t p = i / C8;
   const int x = (int)(p % W);
-  int64_t q = p / W;
+  const int64_t q = p / W;
   const int y = (int)(q % H);
   const int64_t b = q / H;
-  out[i] = c < C1 ? low[((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c] : skip[p * C2 + (c - C1)];
+  const uint4 v = c < C1 ? *(const uint4 *)(low + ((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c)
+                         : *(const uint4 *)(skip + p * C2 + (c - C1));
+  *(uint4 *)(out + i * 8) = v;
 }
 
 // dLow[y][x] = sum over the 2x2 block of dOut[..., 0:C1]; dSkip = dOut[..., C1:]
+// n = nLow * (C1 + 4*C2) / 8
 __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C2, int H, int W, int64_t nLow,
                                 uint16_t *__restrict__ dLow, uint16_t *__restrict__ dSkip) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*(H/2)*(W/2)*(C1 + 4*C2)
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int C = C1 + C2;
-  const int per = C1 + 4 * C2;
+  const int per = (C1 + 4 * C2) / 8;
   if (i >= nLow * per) return;
   const int64_t lp = i / per;  // low-res pixel
-  const int k = (int)(i % per);
+  const int k = (int)(i % per) * 8;
   const int w2 = W / 2, h2 = H / 2;
   const int x = (int)(lp % w2);
   const int64_t q = lp / w2;
   const int y = (int)(q % h2);
   const int64_t b = q / h2;
   if (k < C1) {
-    float s = 0.f;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, t[8];
 #pragma unroll
     for (int dy = 0; dy < 2; dy++)
 #pragma unroll
-      for (int dx = 0; dx < 2; dx++) s += bf2f(dOut[((b * H + 2 * y + dy) * W + 2 * x + dx) * C + k]);
-    dLow[lp * C1 + k] = f2bf(s);
+      for (int dx = 0; dx < 2; dx++) {
+        unpack8(*(const uint4 *)(dOut + ((b * H + 2 * y + dy) * W + 2 * x + dx) * C + k), t);
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] += t[j];
+      }
+    *(uint4 *)(dLow + lp * C1 + k) = pack8(s);
   } else {
     const int kk = k - C1;
     const int sub = kk / C2, c = kk % C2;
     const int64_t fp = (b * H + 2 * y + (sub >> 1)) * W + 2 * x + (sub & 1);
-    dSkip[fp * C2 + c] = dOut[fp * C + C1 + c];
+    *(uint4 *)(dSkip + fp * C2 + c) = *(const uint4 *)(dOut + fp * C + C1 + c);
   }
 }
 
+// dY = 0 where Y <= 0 (ReLU); n = elements / 8
 __global__ void k_relu_mask(uint16_t *__restrict__ dY, const uint16_t *__restrict__ Y, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (!(bf2f(Y[i]) > 0.f)) dY[i] = 0;
+  const uint4 y = *(const uint4 *)(Y + i * 8);
+  uint4 d = *(const uint4 *)(dY + i * 8);
+  const uint32_t yw[4] = {y.x, y.y, y.z, y.w};
+  uint32_t dw[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    // a bf16 is > 0 when its sign bit is clear and its magnitude non-zero
+    const uint32_t lo = yw[j] & 0xffffu, hi = yw[j] >> 16;
+    const bool keepLo = !(lo & 0x8000u) && (lo & 0x7fffu) != 0 && (lo & 0x7fffu) <= 0x7f80u;
+    const bool keepHi = !(hi & 0x8000u) && (hi & 0x7fffu) != 0 && (hi & 0x7fffu) <= 0x7f80u;
+    dw[j] &= (keepLo ? 0xffffu : 0u) | (keepHi ? 0xffff0000u : 0u);
+  }
+  *(uint4 *)(dY + i * 8) = make_uint4(dw[0], dw[1], dw[2], dw[3]);
 }
 
 // L1 loss on channels 0..2 of pred (ldp channels) against log(1+target);
@@ -196,29 +256,6 @@ __global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__res
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(loss, red[0] * invCount);
-}
-
-// db[c] += sum over pixels of dY[p][c]; block = 256 threads over a pixel range
-__global__ void k_bias_grad(const uint16_t *__restrict__ dY, int C, int64_t nPix, int pixPerBlock,
-                            float *__restrict__ db) {
-  __shared__ float s[256];
-  const int64_t p0 = (int64_t)blockIdx.x * pixPerBlock;
-  const int64_t p1 = p0 + pixPerBlock < nPix ? p0 + pixPerBlock : nPix;
-  for (int c0 = 0; c0 < C; c0 += 256 / 8) {
-    // thread t: channel c0 + t%32, pixels strided by 8
-    const int c = c0 + (threadIdx.x & 31);
-    float acc = 0.f;
-    if (c < C)
-      for (int64_t p = p0 + (threadIdx.x >> 5); p < p1; p += 8) acc += bf2f(dY[p * C + c]);
-    s[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x < 32 && c < C) {
-      float t = 0.f;
-      for (int k = 0; k < 8; k++) t += s[threadIdx.x + 32 * k];
-      atomicAdd(db + c, t);
-    }
-    __syncthreads();
-  }
 }
 
 // Adam on FP32 master weights; writes the BF16 weights (OHWI) and, for the
@@ -290,6 +327,7 @@ struct cpt_denoiser {
   cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], cat[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
   cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
   cpt::DevBuf pred, logRad, dPred, loss, io;
+  cpt::DevBuf part;  // split-K partial sums of the weight gradients
   int64_t step = 0;
   float lastFwdMs = 0.f, lastStepMs = 0.f;
   double flopsFwd = 0;
@@ -303,6 +341,8 @@ static int ensureT(cpt_ctx *ctx, Tensor &t, int B, int h, int w, int c) {
   t.w = w;
   return ensure(ctx, t.buf, (size_t)B * h * w * c * 2);
 }
+
+static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerSplit);
 
 static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
   cpt_ctx *ctx = dn->ctx;
@@ -331,7 +371,37 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
   if ((rc = ensure(ctx, dn->pred, np * kOutChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
       (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)))
     return rc;
+  // weight-gradient partials: the largest layer's slices x weights
+  int64_t partFloats = 0;
+  for (int k = 0; k < kLayers; k++) {
+    int lvl = k < 2 * kLevels ? k / 2 : (k < kLayers - 1 ? kLevels - 2 - (k - 2 * kLevels) / 2 : 0);
+    const int64_t nW = (int64_t)dn->L[k].cout * 9 * dn->L[k].cin;
+    partFloats = std::max<int64_t>(partFloats, (int64_t)convBwdWSlices(dn->L[k].cin, dn->L[k].cout, B, H >> lvl,
+                                                                         W >> lvl, nullptr) * (nW + dn->L[k].cout));
+  }
+  if ((rc = ensure(ctx, dn->part, partFloats * 4))) return rc;
   return CPT_OK;
+}
+
+// One 3x3 'same' convolution (k_conv_fwd): 64 output channels per wave
+// where Cout allows, else 32 (Cout and Cin are multiples of 32).
+static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *w, const float *bias, int Cout,
+                          int B, int H, int W, uint16_t *outB, float *outF, int ldo, int mode, const float *res,
+                          int ldr) {
+  const int nt = Cout % 64 == 0 ? 64 : 32;
+  const int colTiles = (W + kTileW - 1) / kTileW, rowTiles = (H + kTileH - 1) / kTileH;
+  const int coTiles = (Cout + nt - 1) / nt;
+  // stacked tiles per workgroup: as few workgroups as still fill the chip
+  // once (about 512, two per CU), at most 8 tiles each
+  const int64_t tiles = (int64_t)colTiles * rowTiles * B * coTiles;
+  const int tpw = (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
+  const dim3 grid(colTiles, (rowTiles + tpw - 1) / tpw, B * coTiles);
+  if (nt == 64)
+    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, w, bias, Cout, H, W, outB, outF, ldo,
+                       mode, res, ldr, tpw);
+  else
+    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, w, bias, Cout, H, W, outB, outF, ldo,
+                       mode, res, ldr, tpw);
 }
 
 static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, int B, float *outF, uint16_t *outB, int mode,
@@ -341,19 +411,54 @@ static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, int B, float *
   const uint16_t *w = dn->wbf.as<uint16_t>() + dn->wbOff[layer];
   const float *bias = dn->params.as<float>() + dn->bOff[layer];
   const int H = in.h, W = in.w;
-  const int NT = Ld.cout >= 64 && Ld.cout % 64 == 0 ? 64 : (Ld.cout % 32 == 0 ? 32 : 16);
-  dim3 grid((W + kConvPix - 1) / kConvPix, H, B * ((Ld.cout + NT - 1) / NT));
-  if (NT == 64)
-    hipLaunchKernelGGL(k_conv_fwd<64>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
-                       outF, Ld.cout, mode, res, 3);
-  else if (NT == 32)
-    hipLaunchKernelGGL(k_conv_fwd<32>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
-                       outF, Ld.cout, mode, res, 3);
-  else
-    hipLaunchKernelGGL(k_conv_fwd<16>, grid, dim3(kConvThreads), 0, s, in.p(), Ld.cin, w, bias, Ld.cout, H, W, outB,
-                       outF, Ld.cout, mode, res, 3);
+  launchConvFwd(s, in.p(), Ld.cin, w, bias, Ld.cout, B, H, W, outB, outF, Ld.cout, mode, res, 3);
   dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+// Weight gradient of one conv (k_conv_bwd_w, split over pixels, then
+// k_reduce_splits into dW).  part: scratch of at least
+// convBwdWSlices(...) * Cout * 9 * Cin floats.
+static int bwdWShape(int Cin, int Cout, int &ks, int &wci, int &wco) {
+  if (Cin == 32 && Cout == 32) { ks = 4; wci = 1; wco = 1; }
+  else if (Cin == 32) { ks = 2; wci = 1; wco = 2; }
+  else if (Cout == 32) { ks = 2; wci = 2; wco = 1; }
+  else { ks = 1; wci = 2; wco = 2; }
+  return ks;
+}
+
+static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerSplit) {
+  int ks, wci, wco;
+  bwdWShape(Cin, Cout, ks, wci, wco);
+  const int tiles = ((Cin + 32 * wci - 1) / (32 * wci)) * ((Cout + 32 * wco - 1) / (32 * wco));
+  const int64_t steps = (int64_t)B * H * ((W + 31) / 32);
+  // up to about 512 workgroups, each group of waves taking at least 16
+  // K-steps (fewer slices to write and reduce)
+  int64_t s = std::max<int64_t>(1, (512 + tiles - 1) / tiles);
+  s = std::min<int64_t>(s, std::max<int64_t>(1, steps / (16 * ks)));
+  const int64_t per = (steps + s - 1) / s;
+  if (stepsPerSplit) *stepsPerSplit = (int)per;
+  return (int)((steps + per - 1) / per);
+}
+
+// dW then db (contiguous: Cout*9*Cin + Cout floats) when withBias, else dW
+static void launchConvBwdW(hipStream_t s, const uint16_t *dY, int Cout, const uint16_t *X, int Cin, int B, int H,
+                           int W, float *part, float *dW, bool withBias) {
+  int ks, wci, wco, per = 1;
+  bwdWShape(Cin, Cout, ks, wci, wco);
+  const int slices = convBwdWSlices(Cin, Cout, B, H, W, &per);
+  const dim3 grid((Cin + 32 * wci - 1) / (32 * wci), (Cout + 32 * wco - 1) / (32 * wco), slices);
+  if (ks == 4)
+    hipLaunchKernelGGL((k_conv_bwd_w<4, 1, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+  else if (wco == 2 && wci == 1)
+    hipLaunchKernelGGL((k_conv_bwd_w<2, 1, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+  else if (ks == 2)
+    hipLaunchKernelGGL((k_conv_bwd_w<2, 2, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+  else
+    hipLaunchKernelGGL((k_conv_bwd_w<1, 2, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+  const int64_t nW = (int64_t)Cout * 9 * Cin, nOut = withBias ? nW + Cout : nW;
+  hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((nOut + 255) / 256)), dim3(256), 0, s, (const float *)part,
+                     slices, nW + Cout, nOut, dW);
 }
 
 // Backward of one conv: dOut (bf16, already ReLU-masked) -> dW, db, dIn.
@@ -363,28 +468,14 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const uint16_t
   const int H = in.h, W = in.w;
   float *gw = dn->grads.as<float>() + dn->wOff[layer];
   float *gb = dn->grads.as<float>() + dn->bOff[layer];
-  const int rows = H >= 64 ? 16 : (H >= 16 ? 8 : 4);
-  const int groups = (H + rows - 1) / rows;
-  hipLaunchKernelGGL(k_conv_bwd_w, dim3((Ld.cout + 63) / 64, Ld.cin / 32, B * groups), dim3(kConvThreads), 0, s, dOut,
-                     Ld.cout, in.p(), Ld.cin, H, W, rows, gw);
-  const int64_t nPix = (int64_t)B * H * W;
-  const int ppb = 2048;
-  hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((nPix + ppb - 1) / ppb)), dim3(256), 0, s, dOut, Ld.cout, nPix, ppb, gb);
+  // weights then bias (gb == gw + Cout*9*Cin)
+  (void)gb;
+  launchConvBwdW(s, dOut, Ld.cout, in.p(), Ld.cin, B, H, W, dn->part.as<float>(), gw, true);
   if (dIn) {
     // backward data: forward kernel on dOut with the flipped transposed weights
     const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
     const int cout = Ld.cin;  // gradient channels
-    const int NT = cout >= 64 && cout % 64 == 0 ? 64 : (cout % 32 == 0 ? 32 : 16);
-    dim3 grid((W + kConvPix - 1) / kConvPix, H, B * ((cout + NT - 1) / NT));
-    if (NT == 64)
-      hipLaunchKernelGGL(k_conv_fwd<64>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
-                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
-    else if (NT == 32)
-      hipLaunchKernelGGL(k_conv_fwd<32>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
-                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
-    else
-      hipLaunchKernelGGL(k_conv_fwd<16>, grid, dim3(kConvThreads), 0, s, dOut, Ld.cout, wt, (const float *)nullptr, cout,
-                         H, W, dIn, (float *)nullptr, cout, 0, (const float *)nullptr, 3);
+    launchConvFwd(s, dOut, Ld.cout, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3);
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -398,7 +489,7 @@ static int forward(cpt_denoiser *dn, int B) {
     if (l > 0) {
       const Tensor &src = dn->e[l - 1][1];
       Tensor &dst = dn->pool[l - 1];
-      const int64_t n = (int64_t)B * dst.h * dst.w * dst.c;
+      const int64_t n = (int64_t)B * dst.h * dst.w * dst.c / 8;
       hipLaunchKernelGGL(k_pool2, dim3(nb256(n)), dim3(256), 0, s, src.p(), src.h, src.w, src.c, n, dst.p());
       cur = &dst;
     }
@@ -408,7 +499,7 @@ static int forward(cpt_denoiser *dn, int B) {
   }
   for (int l = kLevels - 2; l >= 0; l--) {
     Tensor &cat = dn->cat[l];
-    const int64_t n = (int64_t)B * cat.h * cat.w * cat.c;
+    const int64_t n = (int64_t)B * cat.h * cat.w * cat.c / 8;
     hipLaunchKernelGGL(k_up_concat, dim3(nb256(n)), dim3(256), 0, s, cur->p(), cur->c, dn->e[l][1].p(), kChan[l], cat.h,
                        cat.w, n, cat.p());
     if ((rc = convFwd(dn, layer++, cat, B, nullptr, dn->d[l][0].p(), 1, nullptr))) return rc;
@@ -419,7 +510,7 @@ static int forward(cpt_denoiser *dn, int B) {
 }
 
 static void reluMask(hipStream_t s, Tensor &g, const Tensor &y, int B) {
-  const int64_t n = (int64_t)B * y.h * y.w * y.c;
+  const int64_t n = (int64_t)B * y.h * y.w * y.c / 8;
   hipLaunchKernelGGL(k_relu_mask, dim3(nb256(n)), dim3(256), 0, s, g.p(), y.p(), n);
 }
 
@@ -439,7 +530,7 @@ static int backward(cpt_denoiser *dn, int B) {
     Tensor &low = l == kLevels - 2 ? dn->ge[kLevels - 1][1] : dn->gd[l + 1][1];
     const int C1 = low.c, C2 = kChan[l];
     const int64_t nLow = (int64_t)B * low.h * low.w;
-    hipLaunchKernelGGL(k_up_concat_bwd, dim3(nb256(nLow * (C1 + 4 * C2))), dim3(256), 0, s, dn->gcat[l].p(), C1, C2,
+    hipLaunchKernelGGL(k_up_concat_bwd, dim3(nb256(nLow * (C1 + 4 * C2) / 8)), dim3(256), 0, s, dn->gcat[l].p(), C1, C2,
                        dn->cat[l].h, dn->cat[l].w, nLow, low.p(), dn->ge[l][1].p());
   }
   // encoder, deepest first; layer index now points at enc level 4 conv 1
@@ -453,7 +544,7 @@ static int backward(cpt_denoiser *dn, int B) {
     if (l > 0) {
       // average-pool backward adds into the skip gradient of level l-1
       Tensor &g = dn->ge[l - 1][1];
-      const int64_t n = (int64_t)B * g.h * g.w * g.c;
+      const int64_t n = (int64_t)B * g.h * g.w * g.c / 8;
       hipLaunchKernelGGL(k_pool2_bwd, dim3(nb256(n)), dim3(256), 0, s, dn->gpool[l - 1].p(), g.h, g.w, g.c, n, g.p(), 1);
     }
   }
@@ -567,7 +658,7 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
   // release path: errors have nothing to report to
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
-                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf};
+                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kLevels; l++)
@@ -728,16 +819,19 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
   if (dy) chk(hipMemcpyAsync(dyb.p, dy, ny * 2, hipMemcpyHostToDevice, s));
   if (bias) chk(hipMemcpyAsync(db.p, bias, Cout * 4, hipMemcpyHostToDevice, s));
   else chk(hipMemsetAsync(db.p, 0, Cout * 4, s));
-  const unsigned gx = (W + kConvPix - 1) / kConvPix;
   if (mode == 0) {
-    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cout / 32)), dim3(kConvThreads), 0, s, dx.as<uint16_t>(), Cin,
-                       dw.as<uint16_t>(), db.as<float>(), Cout, H, W, (uint16_t *)nullptr, dout.as<float>(), Cout, 0,
-                       (const float *)nullptr, 3);
+    launchConvFwd(s, dx.as<uint16_t>(), Cin, dw.as<uint16_t>(), db.as<float>(), Cout, B, H, W, nullptr,
+                  dout.as<float>(), Cout, 0, nullptr, 3);
   } else if (mode == 1) {
-    chk(hipMemsetAsync(dout.p, 0, nout * 4, s));
-    const int rows = 4, groups = (H + rows - 1) / rows;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3((Cout + 63) / 64, Cin / 32, B * groups), dim3(kConvThreads), 0, s,
-                       dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, H, W, rows, dout.as<float>());
+    DevBuf part;
+    if ((rc = ensure(ctx, part, (size_t)convBwdWSlices(Cin, Cout, B, H, W, nullptr) * (nw + Cout) * 4))) {
+      cleanup();
+      return rc;
+    }
+    launchConvBwdW(s, dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, B, H, W, part.as<float>(), dout.as<float>(),
+                   false);
+    chk(hipStreamSynchronize(s));
+    (void)hipFree(part.p);
   } else {
     // flipped transpose on the host side of the device: reuse k_pack_weights
     // through an fp32 staging copy of w
@@ -756,9 +850,8 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
     chk(hipMemcpyAsync(wf.p, hf.data(), nw * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack_weights, dim3(nb256(nw)), dim3(256), 0, s, wf.as<float>(), Cin, Cout, dw.as<uint16_t>(),
                        wt.as<uint16_t>());
-    hipLaunchKernelGGL(k_conv_fwd<32>, dim3(gx, H, B * (Cin / 32)), dim3(kConvThreads), 0, s, dyb.as<uint16_t>(), Cout,
-                       wt.as<uint16_t>(), (const float *)nullptr, Cin, H, W, (uint16_t *)nullptr, dout.as<float>(), Cin,
-                       0, (const float *)nullptr, 3);
+    launchConvFwd(s, dyb.as<uint16_t>(), Cout, wt.as<uint16_t>(), nullptr, Cin, B, H, W, nullptr, dout.as<float>(),
+                  Cin, 0, nullptr, 3);
     chk(hipStreamSynchronize(s));
     (void)hipFree(wf.p);
   }

@@ -8,19 +8,12 @@
 // and B[8*(l/16)+j][l%16] for j = 0..7 and receives D[4*(l/16)+r][l%16],
 // r = 0..3.
 //
-// conv3x3 forward  (implicit GEMM, M = pixels, N = Cout, K = 9*Cin):
-//   a workgroup of 4 waves computes 64 consecutive pixels of one image row
-//   times NT output channels; per (ky, 32-channel chunk) it stages a 66-pixel
-//   input strip and the 3 x NT x 32 weight slab in LDS and issues
-//   3 (kx) x NT/16 MFMAs per wave.
+// conv3x3 forward  (implicit GEMM, M = Cout, N = pixels, K = 9*Cin): see
+//   k_conv_fwd.
 // conv3x3 backward-data is the forward kernel run on dY with the weights
 //   flipped and transposed (Wt[ci][2-ky][2-kx][co] = W[co][ky][kx][ci]).
-// conv3x3 backward-weight (M = Cout, N = Cin per tap, K = pixels): a
-//   workgroup owns a 64-channel Cout tile, a 32-channel Cin chunk and a run of
-//   image rows; dY and the three input rows are staged transposed
-//   (channel-major) in LDS so that each lane reads 8 consecutive pixels, and
-//   the 9 taps x 2 Cin blocks accumulate in registers before one FP32 atomic
-//   add per weight.
+// conv3x3 backward-weight (M = Cout, N = Cin per tap, K = pixels, split
+//   over K): see k_conv_bwd_w.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -45,161 +38,420 @@ constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
 // Y[b,y,x,co] = act(sum X[b,y+ky-1,x+kx-1,ci] W[co,ky,kx,ci] + bias[co]) (+res)
 // outBf (bf16, ldo channels) and/or outF (fp32, ldo channels).
 // mode bits: 1 = ReLU, 2 = add res[b,y,x,0..2] (fp32 residual, 3 channels).
-template <int NT>
-__global__ __launch_bounds__(kConvThreads) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
-                                                            const uint16_t *__restrict__ Wt,
-                                                            const float *__restrict__ bias, int Cout, int H,
-                                                            int Wd, uint16_t *__restrict__ outBf,
-                                                            float *__restrict__ outF, int ldo, int mode,
-                                                            const float *__restrict__ res, int ldr) {
-  __shared__ __attribute__((aligned(16))) uint16_t sX[(kConvPix + 2) * kLdsRow];
-  __shared__ __attribute__((aligned(16))) uint16_t sW[3 * NT * kLdsRow];
+//
+// Implicit GEMM with the output transposed, D[co][px] = W[co][tap, ci] .
+// X[tap, ci][px]: the weights are the A operand and the input pixels the B
+// operand, so each lane ends with 4 consecutive output channels of one pixel
+// (one 8 B bf16 or 16 B fp32 store).  A workgroup (4 waves) owns a tile of
+// 4 rows x 64 pixels x 16*QB output channels, wave w the row y0 + w.  Per
+// 32-channel chunk of Cin it stages in LDS the 6 x 66 input halo of the tile
+// and the 9 taps' weights (9 x 16*QB x 32), each loaded once per chunk and
+// read by all waves for all 9 taps; a wave then issues 9 x 4 x QB MFMAs.  A
+// workgroup walks `tpw` vertically stacked tiles as one sequence of (tile,
+// chunk) steps; the next step's global loads are issued before this step's
+// MFMAs and written to LDS after them, so their latency hides behind the
+// MFMAs also where Cin is a single chunk (whose weights are then loaded
+// once per workgroup).  LDS
+// rows are padded to 40 bf16 (80 B): the 16-byte reads of 8 consecutive
+// rows then cover all 32 banks.
+constexpr int kTileH = 4, kTileW = 64;
+constexpr int kHaloH = kTileH + 2, kHaloW = kTileW + 2;
+constexpr int kHaloParts = kHaloH * kHaloW * 4;  // 16 B parts of a 32-channel halo
+constexpr int kHaloIters = (kHaloParts + kConvThreads - 1) / kConvThreads;
+
+template <int QB>
+__global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
+                                                               const uint16_t *__restrict__ Wt,
+                                                               const float *__restrict__ bias, int Cout, int H,
+                                                               int Wd, uint16_t *__restrict__ outBf,
+                                                               float *__restrict__ outF, int ldo, int mode,
+                                                               const float *__restrict__ res, int ldr, int tpw) {
+  constexpr int NT = 16 * QB;
+  constexpr int kWParts = 9 * NT * 4;
+  constexpr int kWIters = (kWParts + kConvThreads - 1) / kConvThreads;
+  __shared__ __attribute__((aligned(16))) uint16_t sX[kHaloH * kHaloW * kLdsRow];
+  __shared__ __attribute__((aligned(16))) uint16_t sW[9 * NT * kLdsRow];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int x0 = blockIdx.x * kConvPix;
-  const int y = blockIdx.y;
+  const int x0 = blockIdx.x * kTileW;
   const int nCoTiles = (Cout + NT - 1) / NT;
   const int b = blockIdx.z / nCoTiles;
   const int co0 = (blockIdx.z % nCoTiles) * NT;
-  f32x4 acc[NT / 16];
-#pragma unroll
-  for (int i = 0; i < NT / 16; i++) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int64_t imgBase = (int64_t)b * H * Wd;
-  for (int ky = 0; ky < 3; ky++) {
-    const int yy = y + ky - 1;
-    const bool rowOk = yy >= 0 && yy < H;
-    for (int c0 = 0; c0 < Cin; c0 += 32) {
-      __syncthreads();
-      // input strip: 66 pixels x 32 channels, 4 x 16 B per pixel
-      for (int q = tid; q < (kConvPix + 2) * 4; q += kConvThreads) {
-        const int i = q >> 2, part = q & 3;
-        const int xx = x0 - 1 + i;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (rowOk && xx >= 0 && xx < Wd)
+  const int nChunks = Cin / 32;
+  // this workgroup's tiles: rows y0(i) = (blockIdx.y * tpw + i) * kTileH,
+  // walked as one sequence of (tile, chunk) steps
+  const int rowTiles = (H + kTileH - 1) / kTileH;
+  const int t0 = blockIdx.y * tpw;
+  const int nt = rowTiles - t0 < tpw ? rowTiles - t0 : tpw;
+  const int nSteps = nt * nChunks;
+  f32x4 acc[QB][4];
+#pragma unroll
+  for (int nb = 0; nb < QB; nb++)
+#pragma unroll
+    for (int m = 0; m < 4; m++) acc[nb][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 px[kHaloIters], pw[kWIters];
+  // global -> registers for step k: the input halo of its tile and, unless
+  // Cin is one chunk (the weights then stay in LDS), the chunk's weights
+  auto load = [&](int k) {
+    const int y0 = (t0 + k / nChunks) * kTileH, c0 = (k % nChunks) * 32;
+#pragma unroll
+    for (int i = 0; i < kHaloIters; i++) {
+      const int q = tid + i * kConvThreads;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (q < kHaloParts) {
+        const int hp = q >> 2, part = q & 3;
+        const int yy = y0 - 1 + hp / kHaloW, xx = x0 - 1 + hp % kHaloW;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
           v = *(const uint4 *)(X + (imgBase + (int64_t)yy * Wd + xx) * Cin + c0 + part * 8);
-        *(uint4 *)(sX + i * kLdsRow + part * 8) = v;
       }
-      // weights: 3 (kx) x NT (co) x 32 (ci)
-      for (int q = tid; q < 3 * NT * 4; q += kConvThreads) {
+      px[i] = v;
+    }
+    if (k > 0 && nChunks == 1) return;
+#pragma unroll
+    for (int i = 0; i < kWIters; i++) {
+      const int q = tid + i * kConvThreads;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (q < kWParts) {
         const int part = q & 3, r = q >> 2;
-        const int kx = r / NT, co = r % NT;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (co0 + co < Cout) v = *(const uint4 *)(Wt + ((((int64_t)(co0 + co) * 3 + ky) * 3 + kx) * Cin) + c0 + part * 8);
-        *(uint4 *)(sW + (kx * NT + co) * kLdsRow + part * 8) = v;
+        const int t = r / NT, co = r % NT;
+        if (co0 + co < Cout) v = *(const uint4 *)(Wt + ((int64_t)(co0 + co) * 9 + t) * Cin + c0 + part * 8);
       }
-      __syncthreads();
+      pw[i] = v;
+    }
+  };
+  // registers -> LDS
+  auto store = [&](int k) {
 #pragma unroll
-      for (int kx = 0; kx < 3; kx++) {
-        const bf16x8 a = *(const bf16x8 *)(sX + (wave * 16 + (lane & 15) + kx) * kLdsRow + (lane >> 4) * 8);
+    for (int i = 0; i < kHaloIters; i++) {
+      const int q = tid + i * kConvThreads;
+      if (q < kHaloParts) *(uint4 *)(sX + (q >> 2) * kLdsRow + (q & 3) * 8) = px[i];
+    }
+    if (k > 0 && nChunks == 1) return;
 #pragma unroll
-        for (int nb = 0; nb < NT / 16; nb++) {
-          const bf16x8 bw = *(const bf16x8 *)(sW + (kx * NT + nb * 16 + (lane & 15)) * kLdsRow + (lane >> 4) * 8);
-          acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[nb], 0, 0, 0);
+    for (int i = 0; i < kWIters; i++) {
+      const int q = tid + i * kConvThreads;
+      if (q < kWParts) *(uint4 *)(sW + (q >> 2) * kLdsRow + (q & 3) * 8) = pw[i];
+    }
+  };
+  // D[co][px]: lane holds output channels 4*(lane/16) + r of pixel lane%16
+  auto epilogue = [&](int y) {
+    if (y >= H) return;
+#pragma unroll
+    for (int nb = 0; nb < QB; nb++) {
+      const int co = co0 + nb * 16 + (lane >> 4) * 4;
+      if (co >= Cout) continue;  // Cout is a multiple of 16: all 4 channels or none
+      float bs[4] = {0.f, 0.f, 0.f, 0.f};
+      if (bias) {
+        const float4 bv = *(const float4 *)(bias + co);
+        bs[0] = bv.x;
+        bs[1] = bv.y;
+        bs[2] = bv.z;
+        bs[3] = bv.w;
+      }
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const int x = x0 + m * 16 + (lane & 15);
+        if (x >= Wd) continue;
+        const int64_t p = imgBase + (int64_t)y * Wd + x;
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          v[r] = acc[nb][m][r] + bs[r];
+          if (mode & 1) v[r] = fmaxf(v[r], 0.f);
+          if ((mode & 2) && co + r < 3) v[r] += res[p * ldr + co + r];
         }
+        if (outBf) {
+          uint2 pk;
+          pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+          pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+          *(uint2 *)(outBf + p * ldo + co) = pk;
+        }
+        if (outF) *(float4 *)(outF + p * ldo + co) = make_float4(v[0], v[1], v[2], v[3]);
       }
     }
-  }
+  };
+  if (nSteps <= 0) return;  // workgroup-uniform
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int k = 0; k < nSteps; k++) {
+    if (k + 1 < nSteps) load(k + 1);
 #pragma unroll
-  for (int nb = 0; nb < NT / 16; nb++) {
-    const int co = co0 + nb * 16 + (lane & 15);
-    if (co >= Cout) continue;
-    const float bs = bias ? bias[co] : 0.f;
+    for (int t = 0; t < 9; t++) {
+      const int ky = t / 3, kx = t % 3;
+      bf16x8 bx[4], aw[QB];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int x = x0 + wave * 16 + (lane >> 4) * 4 + r;
-      if (x >= Wd) continue;
-      const int64_t p = imgBase + (int64_t)y * Wd + x;
-      float v = acc[nb][r] + bs;
-      if (mode & 1) v = fmaxf(v, 0.f);
-      if ((mode & 2) && co < 3) v += res[p * ldr + co];
-      if (outBf) outBf[p * ldo + co] = f2bf(v);
-      if (outF) outF[p * ldo + co] = v;
+      for (int m = 0; m < 4; m++)
+        bx[m] = *(const bf16x8 *)(sX + ((wave + ky) * kHaloW + m * 16 + (lane & 15) + kx) * kLdsRow + (lane >> 4) * 8);
+#pragma unroll
+      for (int nb = 0; nb < QB; nb++)
+        aw[nb] = *(const bf16x8 *)(sW + (t * NT + nb * 16 + (lane & 15)) * kLdsRow + (lane >> 4) * 8);
+#pragma unroll
+      for (int nb = 0; nb < QB; nb++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) acc[nb][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aw[nb], bx[m], acc[nb][m], 0, 0, 0);
+    }
+    if (k % nChunks == nChunks - 1) {
+      // the tile is complete: write it and start the next one from zero
+      epilogue((t0 + k / nChunks) * kTileH + wave);
+#pragma unroll
+      for (int nb = 0; nb < QB; nb++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) acc[nb][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    if (k + 1 < nSteps) {
+      __syncthreads();  // every wave is done reading this step
+      store(k + 1);
+      __syncthreads();
     }
   }
 }
 
-// dW[co][ky][kx][ci] += sum_p dY[p][co] X[p + (ky-1, kx-1)][ci]
-// grid: (Cout/64, Cin/32, B * rowGroups); each workgroup walks `rows` rows.
-__global__ __launch_bounds__(kConvThreads) void k_conv_bwd_w(const uint16_t *__restrict__ dY, int Cout,
-                                                              const uint16_t *__restrict__ X, int Cin, int H,
-                                                              int Wd, int rows, float *__restrict__ dW) {
-  constexpr int P = kConvPix;
-  constexpr int PR = P + 8;  // padded pixel stride (bf16) of the transposed tiles
-  __shared__ __attribute__((aligned(16))) uint16_t sDy[64 * PR];           // [co][pixel]
-  __shared__ __attribute__((aligned(16))) uint16_t sX[3 * 32 * (P + 2 + 6)];  // [ky][ci][pixel]
-  constexpr int XR = P + 8;  // 66 pixels padded to 72
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int co0 = blockIdx.x * 64, c0 = blockIdx.y * 32;
-  const int groups = (H + rows - 1) / rows;
-  const int b = blockIdx.z / groups;
-  const int yBeg = (blockIdx.z % groups) * rows;
-  const int yEnd = min(H, yBeg + rows);
-  const int64_t imgBase = (int64_t)b * H * Wd;
-  // accumulators: this wave's 16 output channels x 2 input blocks x 9 taps
-  f32x4 acc[9][2];
+// Backward-weight: dW[co][ky][kx][ci] = sum_p dY[p][co] X[p + (ky-1, kx-1)][ci]
+// as a GEMM with M = Cout, N = Cin (per tap), K = pixels, split over K.
+//
+// A K-step is 32 output pixels [x0, x0+32) of one row y of one image.  Both
+// operands need the pixels contiguous per channel (a lane supplies 8
+// consecutive K), but the tensors are stored channel-contiguous (NHWC), so
+// the loader transposes: it loads two adjacent pixels' 8 channels (2 x 16 B)
+// and writes, per channel, one dword holding the two pixels (packed pairs,
+// one ds_write_b32 per 2 pixels) into
+//   sX[ky][ci][pos]  pos = pixel - (x0 - 1) in [0, 34): input rows y-1..y+1
+//   sDy[co][k]       k   = pixel - x0     in [0, 32)
+// (rows of 40 bf16 = 20 dwords).  Tap (ky, kx) needs sX positions
+// [8j + kx, 8j + kx + 8) for lane group j = lane/16: a lane reads the aligned
+// 5-dword window at 8j once per (ky, 16-channel block) and takes dwords 0..3
+// (kx = 0), the window shifted by one bf16 (kx = 1, v_alignbit) or dwords
+// 1..4 (kx = 2): 2 LDS reads for 3 taps.
+// A wave owns 32 Cout x 32 Cin x 9 taps (36 accumulator tiles): per K-step
+// 2 + 12 fragment reads for 36 MFMAs (D[co][ci] = dY^T . X).  A workgroup
+// (4 waves) is KS groups of waves, each group on its own K-steps, each
+// covering (32*WCI) Cin x (32*WCO) Cout with WCI*WCO waves (KS*WCI*WCO = 4).
+// The next K-step's global loads are issued before this one's MFMAs.  The
+// groups' sums are added in LDS and each workgroup writes its tile of one
+// slice of `part` (part[split][co][tap][ci], no atomics); k_reduce_splits
+// adds the slices in a fixed order (deterministic).
+constexpr int kBwRow = 20;  // dwords per LDS row (40 bf16)
+
+// The workgroups of the first Cin tile also sum dY over their pixels (the
+// bias gradient) from the A fragments they hold, into entries nW..nW+Cout of
+// their slice (slices are nW + Cout floats).
+template <int KS, int WCI, int WCO>
+__global__ __launch_bounds__(kConvThreads, 2) void k_conv_bwd_w(const uint16_t *__restrict__ dY, int Cout,
+                                                                 const uint16_t *__restrict__ X, int Cin, int B, int H,
+                                                                 int Wd, int stepsPerSplit, float *__restrict__ part) {
+  constexpr int CIW = 32 * WCI, COW = 32 * WCO;
+  constexpr int kGroupThreads = kConvThreads / KS;
+  constexpr int kXItems = 3 * 17 * (CIW / 8);  // (ky, pixel pair, 8-channel group)
+  constexpr int kDyItems = 16 * (COW / 8);
+  constexpr int kItems = kXItems + kDyItems;
+  constexpr int kIters = (kItems + kGroupThreads - 1) / kGroupThreads;
+  constexpr int kGroupWords = (3 * CIW + COW) * kBwRow;
+  __shared__ __attribute__((aligned(16))) uint32_t sh[KS * kGroupWords];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int g = tid / kGroupThreads, gt = tid % kGroupThreads;
+  const int wg = (tid / 64) % (WCI * WCO);  // wave within its group
+  const int wci = wg % WCI, wco = wg / WCI;
+  uint32_t *sX = sh + g * kGroupWords;
+  uint32_t *sDy = sX + 3 * CIW * kBwRow;
+  const int ci0 = blockIdx.x * CIW, co0 = blockIdx.y * COW;
+  const int nSeg = (Wd + 31) / 32;
+  const int64_t nSteps = (int64_t)B * H * nSeg;
+  const int64_t kBeg = (int64_t)blockIdx.z * stepsPerSplit;
+  const int64_t kEnd = kBeg + stepsPerSplit < nSteps ? kBeg + stepsPerSplit : nSteps;
+  const int iters = (stepsPerSplit + KS - 1) / KS;  // the same for every group: barriers stay uniform
+  f32x4 acc[9][2][2];
 #pragma unroll
-  for (int t = 0; t < 9; t++) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int y = yBeg; y < yEnd; y++) {
-    for (int x0 = 0; x0 < Wd; x0 += P) {
-      __syncthreads();
-      // dY tile transposed: 64 pixels x 64 co -> sDy[co][pixel]
-      for (int q = tid; q < P * 8; q += kConvThreads) {
-        const int px = q >> 3, part = q & 7;
-        const int x = x0 + px;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (x < Wd && co0 + part * 8 < Cout)
-          v = *(const uint4 *)(dY + (imgBase + (int64_t)y * Wd + x) * Cout + co0 + part * 8);
-        const uint16_t *h = (const uint16_t *)&v;
+  for (int t = 0; t < 9; t++)
 #pragma unroll
-        for (int j = 0; j < 8; j++) sDy[(part * 8 + j) * PR + px] = h[j];
+    for (int cb = 0; cb < 2; cb++) acc[t][cb][0] = acc[t][cb][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool biasWave = blockIdx.x == 0 && wci == 0;  // wave-uniform
+  float bsum[2] = {0.f, 0.f};
+  uint4 pa[kIters], pb[kIters];
+  // global -> registers: the two pixels of every item of K-step k (zeros
+  // outside the image or past this group's steps)
+  auto load = [&](int64_t k) {
+    const bool live = k < kEnd;
+    int b = 0, y = 0, x0 = 0;
+    if (live) {
+      b = (int)(k / ((int64_t)H * nSeg));
+      const int rem = (int)(k % ((int64_t)H * nSeg));
+      y = rem / nSeg;
+      x0 = (rem % nSeg) * 32;
+    }
+    const int64_t img = (int64_t)b * H * Wd;
+#pragma unroll
+    for (int i = 0; i < kIters; i++) {
+      const int q = gt + i * kGroupThreads;
+      uint4 a = make_uint4(0, 0, 0, 0), c = a;
+      if (live && q < kXItems) {
+        const int pr = q % 17, rest = q / 17;
+        const int cig = rest % (CIW / 8), ky = rest / (CIW / 8);
+        const int yy = y + ky - 1, xa = x0 - 1 + 2 * pr;
+        const int ci = ci0 + cig * 8;
+        if (yy >= 0 && yy < H && ci < Cin) {
+          const uint16_t *row = X + (img + (int64_t)yy * Wd) * Cin + ci;
+          if (xa >= 0 && xa < Wd) a = *(const uint4 *)(row + (int64_t)xa * Cin);
+          if (xa + 1 >= 0 && xa + 1 < Wd) c = *(const uint4 *)(row + (int64_t)(xa + 1) * Cin);
+        }
+      } else if (live && q < kItems) {
+        const int r = q - kXItems;
+        const int pr = r % 16, cog = r / 16;
+        const int xa = x0 + 2 * pr, co = co0 + cog * 8;
+        if (co < Cout) {
+          const uint16_t *row = dY + (img + (int64_t)y * Wd) * Cout + co;
+          if (xa < Wd) a = *(const uint4 *)(row + (int64_t)xa * Cout);
+          if (xa + 1 < Wd) c = *(const uint4 *)(row + (int64_t)(xa + 1) * Cout);
+        }
       }
-      // X rows y-1..y+1, pixels x0-1..x0+64, 32 channels -> sX[ky][ci][pixel]
-      for (int q = tid; q < 3 * (P + 2) * 4; q += kConvThreads) {
-        const int part = q & 3, r = q >> 2;
-        const int ky = r / (P + 2), i = r % (P + 2);
-        const int yy = y + ky - 1, xx = x0 - 1 + i;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
-          v = *(const uint4 *)(X + (imgBase + (int64_t)yy * Wd + xx) * Cin + c0 + part * 8);
-        const uint16_t *h = (const uint16_t *)&v;
+      pa[i] = a;
+      pb[i] = c;
+    }
+  };
+  // registers -> LDS, transposed: per channel one dword (pixel 2pr, 2pr + 1)
+  auto store = [&]() {
 #pragma unroll
-        for (int j = 0; j < 8; j++) sX[(ky * 32 + part * 8 + j) * XR + i] = h[j];
+    for (int i = 0; i < kIters; i++) {
+      const int q = gt + i * kGroupThreads;
+      if (q >= kItems) continue;
+      uint32_t *dst;
+      int pr;
+      if (q < kXItems) {
+        pr = q % 17;
+        const int rest = q / 17;
+        const int cig = rest % (CIW / 8), ky = rest / (CIW / 8);
+        dst = sX + (ky * CIW + cig * 8) * kBwRow + pr;
+      } else {
+        const int r = q - kXItems;
+        pr = r % 16;
+        dst = sDy + (r / 16) * 8 * kBwRow + pr;
+      }
+      const uint32_t av[4] = {pa[i].x, pa[i].y, pa[i].z, pa[i].w}, cv[4] = {pb[i].x, pb[i].y, pb[i].z, pb[i].w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        dst[(2 * j) * kBwRow] = (av[j] & 0xffffu) | (cv[j] << 16);
+        dst[(2 * j + 1) * kBwRow] = (av[j] >> 16) | (cv[j] & 0xffff0000u);
+      }
+    }
+  };
+  load(kBeg + g);
+  store();
+  __syncthreads();
+  for (int it = 0; it < iters; it++) {
+    if (it + 1 < iters) load(kBeg + g + (int64_t)(it + 1) * KS);
+    // A: dY^T, co rows wco*32 + ob*16 + lane%16, pixels 8j..8j+7
+    bf16x8 a[2];
+#pragma unroll
+    for (int ob = 0; ob < 2; ob++) {
+      a[ob] = *(const bf16x8 *)(sDy + (wco * 32 + ob * 16 + (lane & 15)) * kBwRow + (lane >> 4) * 4);
+      if (biasWave) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) bsum[ob] += (float)a[ob][j];
+      }
+    }
+#pragma unroll
+    for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+      for (int cb = 0; cb < 2; cb++) {
+        const uint32_t *src = sX + (ky * CIW + wci * 32 + cb * 16 + (lane & 15)) * kBwRow + (lane >> 4) * 4;
+        const uint4 w4 = *(const uint4 *)src;
+        const uint32_t w5 = src[4];
+        const uint32_t d[5] = {w4.x, w4.y, w4.z, w4.w, w5};
+#pragma unroll
+        for (int kx = 0; kx < 3; kx++) {
+          uint32_t v[4];
+#pragma unroll
+          for (int j = 0; j < 4; j++)
+            v[j] = kx == 0 ? d[j] : (kx == 2 ? d[j + 1] : __builtin_amdgcn_alignbit(d[j + 1], d[j], 16));
+          const bf16x8 bx = __builtin_bit_cast(bf16x8, make_uint4(v[0], v[1], v[2], v[3]));
+#pragma unroll
+          for (int ob = 0; ob < 2; ob++)
+            acc[ky * 3 + kx][cb][ob] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ob], bx, acc[ky * 3 + kx][cb][ob], 0, 0, 0);
+        }
+      }
+    }
+    if (it + 1 < iters) {
+      __syncthreads();  // every wave is done reading this K-step
+      store();
+      __syncthreads();
+    }
+  }
+  // the KS groups' sums of one tile: added into group 0's through LDS, one
+  // tap at a time (16 floats per lane, lane-fastest so that the words of a
+  // wave fall in distinct banks)
+  // bias sums: the 4 lane groups of a wave hold different pixels
+#pragma unroll
+  for (int ob = 0; ob < 2; ob++) {
+    bsum[ob] += __shfl_xor(bsum[ob], 16, 64);
+    bsum[ob] += __shfl_xor(bsum[ob], 32, 64);
+  }
+  if (KS > 1) {
+    constexpr int kLanes = WCI * WCO * 64;
+    float *red = (float *)sh;  // (KS-1) * 16 * kLanes floats, within sh
+    static_assert((KS - 1) * 16 * kLanes <= KS * kGroupWords, "reduction buffer exceeds LDS");
+    const int li = wg * 64 + lane;
+#pragma unroll
+    for (int t = 0; t < 10; t++) {  // 9 taps, then the bias sums
+      __syncthreads();
+      if (g > 0) {
+        if (t < 9) {
+#pragma unroll
+          for (int e = 0; e < 16; e++) red[((g - 1) * 16 + e) * kLanes + li] = acc[t][e >> 3][(e >> 2) & 1][e & 3];
+        } else {
+          red[((g - 1) * 16 + 0) * kLanes + li] = bsum[0];
+          red[((g - 1) * 16 + 1) * kLanes + li] = bsum[1];
+        }
       }
       __syncthreads();
+      if (g == 0) {
+        for (int gg = 1; gg < KS; gg++) {
+          if (t < 9) {
 #pragma unroll
-      for (int ks = 0; ks < P / 32; ks++) {
-        // A[i=co][k=pixel]: lane row = co (wave's block), 8 consecutive pixels
-        const bf16x8 a = *(const bf16x8 *)(sDy + (wave * 16 + (lane & 15)) * PR + ks * 32 + (lane >> 4) * 8);
-#pragma unroll
-        for (int ky = 0; ky < 3; ky++) {
-#pragma unroll
-          for (int kx = 0; kx < 3; kx++) {
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++) {
-              // B[k=pixel][n=ci]: pixel p needs input pixel p + kx - 1 -> strip index p + kx
-              const uint16_t *src = sX + (ky * 32 + nb * 16 + (lane & 15)) * XR + ks * 32 + (lane >> 4) * 8 + kx;
-              bf16x8 bv;
-#pragma unroll
-              for (int j = 0; j < 8; j++) bv[j] = __builtin_bit_cast(__bf16, src[j]);
-              acc[ky * 3 + kx][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[ky * 3 + kx][nb], 0, 0, 0);
-            }
+            for (int e = 0; e < 16; e++) acc[t][e >> 3][(e >> 2) & 1][e & 3] += red[((gg - 1) * 16 + e) * kLanes + li];
+          } else {
+            bsum[0] += red[((gg - 1) * 16 + 0) * kLanes + li];
+            bsum[1] += red[((gg - 1) * 16 + 1) * kLanes + li];
           }
         }
       }
     }
+    if (g > 0) return;
   }
-  // D[i = co][n = ci]: lane holds rows 4*(lane/16)+r, column lane%16
+  // D[co][ci]: lane holds co = 4*(lane/16) + r, ci = lane%16 of each tile
+  const int64_t nW = (int64_t)Cout * 9 * Cin;
+  float *out = part + (int64_t)blockIdx.z * (nW + Cout);
+  if (biasWave && lane < 16) {
 #pragma unroll
-  for (int t = 0; t < 9; t++) {
-#pragma unroll
-    for (int nb = 0; nb < 2; nb++) {
-      const int ci = c0 + nb * 16 + (lane & 15);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int co = co0 + wave * 16 + (lane >> 4) * 4 + r;
-        if (co < Cout && ci < Cin) atomicAdd(dW + ((int64_t)co * 9 + t) * Cin + ci, acc[t][nb][r]);
-      }
+    for (int ob = 0; ob < 2; ob++) {
+      const int co = co0 + wco * 32 + ob * 16 + lane;
+      if (co < Cout) out[nW + co] = bsum[ob];
     }
   }
+#pragma unroll
+  for (int t = 0; t < 9; t++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int ci = ci0 + wci * 32 + cb * 16 + (lane & 15);
+#pragma unroll
+      for (int ob = 0; ob < 2; ob++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int co = co0 + wco * 32 + ob * 16 + (lane >> 4) * 4 + r;
+          if (co < Cout && ci < Cin) out[((int64_t)co * 9 + t) * Cin + ci] = acc[t][cb][ob][r];
+        }
+    }
+}
+
+// out[i] = sum over the slices (stride nS floats) of part[s][i], i < nOut,
+// in slice order.
+__global__ void k_reduce_splits(const float *__restrict__ part, int slices, int64_t nS, int64_t nOut,
+                                float *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nOut) return;
+  float s = 0.f;
+#pragma unroll 8
+  for (int k = 0; k < slices; k++) s += part[(int64_t)k * nS + i];
+  out[i] = s;
 }
 
 }  // namespace cpt

@@ -90,7 +90,8 @@ struct cpt_ctx {
   int64_t shadowSortMin = (int64_t)1 << 20;  // shadow queues this long are reordered
   bool lastTraceSorted = false;
 
-  // traversal I/O staging
+  // traversal I/O staging; cntA: the persistent walks' grab counter; cntB:
+  // triangle material ids staged by the scene upload and update
   cpt::DevBuf rayBuf, tBuf, idBuf, cntA, cntB;
   cpt::DevBuf rkA, rkB, rvA, rvB;  // ray-order keys / permutation (ping-pong)
   float lastTraceMs = 0.f;
