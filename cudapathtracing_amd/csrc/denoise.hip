@@ -26,6 +26,7 @@ constexpr int kLevels = 5;
 constexpr int kChan[kLevels] = {32, 64, 64, 96, 128};
 constexpr int kInChan = 32;   // 10 used: log radiance (3), albedo (3), normal (3), log depth
 constexpr int kOutChan = 32;  // 3 used
+constexpr int kPredChan = 4;  // fp32 channels kept of the output layer (3 used)
 constexpr int kLayers = 19;
 
 struct LayerDesc {
@@ -127,9 +128,10 @@ __global__ void k_pool2(const uint16_t *__restrict__ X, int H, int W, int C, int
   *(uint4 *)(Y + i * 8) = pack8(s);
 }
 
-// dX[2y+i][2x+j] (+)= dY[y][x] / 4; n = B*H*W*C/8 (full resolution)
+// dX[2y+i][2x+j] (+)= dY[y][x] / 4; n = B*H*W*C/8 (full resolution); then,
+// with maskX, the ReLU gradient mask of maskX (the forward activation of dX)
 __global__ void k_pool2_bwd(const uint16_t *__restrict__ dY, int H, int W, int C, int64_t n, uint16_t *__restrict__ dX,
-                            int accumulate) {
+                            int accumulate, const uint16_t *__restrict__ maskX) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int C8 = C / 8;
@@ -150,30 +152,23 @@ __global__ void k_pool2_bwd(const uint16_t *__restrict__ dY, int H, int W, int C
 #pragma unroll
     for (int j = 0; j < 8; j++) g[j] *= 0.25f;
   }
-  *(uint4 *)(dX + i * 8) = pack8(g);
+  uint4 r = pack8(g);
+  if (maskX) {
+    const uint4 mv = *(const uint4 *)(maskX + i * 8);
+    r.x &= reluKeep(mv.x);
+    r.y &= reluKeep(mv.y);
+    r.z &= reluKeep(mv.z);
+    r.w &= reluKeep(mv.w);
+  }
+  *(uint4 *)(dX + i * 8) = r;
 }
 
-// Out[b,y,x] = [Low[b,y/2,x/2, 0:C1] | Skip[b,y,x, 0:C2]]; n = B*H*W*(C1+C2)/8
-__global__ void k_up_concat(const uint16_t *__restrict__ low, int C1, const uint16_t *__restrict__ skip, int C2, int H,
-                            int W, int64_t n, uint16_t *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int C8 = (C1 + C2) / 8;
-  const int c = (int)(i % C8) * 8;
-  const int64_t p = i / C8;
-  const int x = (int)(p % W);
-  const int64_t q = p / W;
-  const int y = (int)(q % H);
-  const int64_t b = q / H;
-  const uint4 v = c < C1 ? *(const uint4 *)(low + ((b * (H / 2) + y / 2) * (W / 2) + x / 2) * C1 + c)
-                         : *(const uint4 *)(skip + p * C2 + (c - C1));
-  *(uint4 *)(out + i * 8) = v;
-}
-
-// dLow[y][x] = sum over the 2x2 block of dOut[..., 0:C1]; dSkip = dOut[..., C1:]
+// dLow[y][x] = sum over the 2x2 block of dOut[..., 0:C1], ReLU-masked by
+// maskLow (the forward activation of dLow); dSkip = dOut[..., C1:]
 // n = nLow * (C1 + 4*C2) / 8
 __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C2, int H, int W, int64_t nLow,
-                                uint16_t *__restrict__ dLow, uint16_t *__restrict__ dSkip) {
+                                uint16_t *__restrict__ dLow, uint16_t *__restrict__ dSkip,
+                                const uint16_t *__restrict__ maskLow) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int C = C1 + C2;
   const int per = (C1 + 4 * C2) / 8;
@@ -195,32 +190,19 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C
 #pragma unroll
         for (int j = 0; j < 8; j++) s[j] += t[j];
       }
-    *(uint4 *)(dLow + lp * C1 + k) = pack8(s);
+    uint4 r = pack8(s);
+    const uint4 mv = *(const uint4 *)(maskLow + lp * C1 + k);
+    r.x &= reluKeep(mv.x);
+    r.y &= reluKeep(mv.y);
+    r.z &= reluKeep(mv.z);
+    r.w &= reluKeep(mv.w);
+    *(uint4 *)(dLow + lp * C1 + k) = r;
   } else {
     const int kk = k - C1;
     const int sub = kk / C2, c = kk % C2;
     const int64_t fp = (b * H + 2 * y + (sub >> 1)) * W + 2 * x + (sub & 1);
     *(uint4 *)(dSkip + fp * C2 + c) = *(const uint4 *)(dOut + fp * C + C1 + c);
   }
-}
-
-// dY = 0 where Y <= 0 (ReLU); n = elements / 8
-__global__ void k_relu_mask(uint16_t *__restrict__ dY, const uint16_t *__restrict__ Y, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint4 y = *(const uint4 *)(Y + i * 8);
-  uint4 d = *(const uint4 *)(dY + i * 8);
-  const uint32_t yw[4] = {y.x, y.y, y.z, y.w};
-  uint32_t dw[4] = {d.x, d.y, d.z, d.w};
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    // a bf16 is > 0 when its sign bit is clear and its magnitude non-zero
-    const uint32_t lo = yw[j] & 0xffffu, hi = yw[j] >> 16;
-    const bool keepLo = !(lo & 0x8000u) && (lo & 0x7fffu) != 0 && (lo & 0x7fffu) <= 0x7f80u;
-    const bool keepHi = !(hi & 0x8000u) && (hi & 0x7fffu) != 0 && (hi & 0x7fffu) <= 0x7f80u;
-    dw[j] &= (keepLo ? 0xffffu : 0u) | (keepHi ? 0xffff0000u : 0u);
-  }
-  *(uint4 *)(dY + i * 8) = make_uint4(dw[0], dw[1], dw[2], dw[3]);
 }
 
 // L1 loss on channels 0..2 of pred (ldp channels) against log(1+target);
@@ -303,6 +285,30 @@ __global__ void k_zero_f32(float *__restrict__ p, int64_t n) {
   if (i < n) p[i] = 0.f;
 }
 
+// Every layer's weights in one launch: thread i packs entry i of the
+// concatenated bf16 weights (layer l owns [wbOff[l], wbOff[l+1])).
+struct PackTable {
+  int64_t wOff[kLayers], wbOff[kLayers + 1];
+  int cin[kLayers], cout[kLayers];
+};
+
+__global__ void k_pack_all(const float *__restrict__ params, PackTable T, uint16_t *__restrict__ wbf,
+                           uint16_t *__restrict__ wtbf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.wbOff[kLayers]) return;
+  int l = 0;
+  while (i >= T.wbOff[l + 1]) l++;
+  const int64_t j = i - T.wbOff[l];
+  const int cin = T.cin[l], cout = T.cout[l];
+  const int ci = (int)(j % cin);
+  const int t = (int)((j / cin) % 9);
+  const int co = (int)(j / ((int64_t)cin * 9));
+  const uint16_t h = f2bf(params[T.wOff[l] + j]);
+  wbf[i] = h;
+  const int ky = t / 3, kx = t % 3;
+  wtbf[T.wbOff[l] + (((int64_t)ci * 3 + (2 - ky)) * 3 + (2 - kx)) * cout + co] = h;
+}
+
 static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 // ---- host-side network ------------------------------------------------------
@@ -324,7 +330,7 @@ struct cpt_denoiser {
   int maxB = 0, maxH = 0, maxW = 0;
   int B = 0, H = 0, W = 0;  // current padded shape
   // activations (and their gradients) per image batch
-  cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], cat[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
+  cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
   cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
   cpt::DevBuf pred, logRad, dPred, loss, io;
   cpt::DevBuf part;  // split-K partial sums of the weight gradients
@@ -359,8 +365,7 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
           (rc = ensureT(ctx, dn->gpool[l], B, h / 2, w / 2, kChan[l])))
         return rc;
       const int below = l == kLevels - 2 ? kChan[kLevels - 1] : kChan[l + 1];
-      if ((rc = ensureT(ctx, dn->cat[l], B, h, w, below + kChan[l])) ||
-          (rc = ensureT(ctx, dn->gcat[l], B, h, w, below + kChan[l])))
+      if ((rc = ensureT(ctx, dn->gcat[l], B, h, w, below + kChan[l])))
         return rc;
       for (int k = 0; k < 2; k++)
         if ((rc = ensureT(ctx, dn->d[l][k], B, h, w, kChan[l])) || (rc = ensureT(ctx, dn->gd[l][k], B, h, w, kChan[l])))
@@ -368,7 +373,7 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
     }
   }
   const int64_t np = (int64_t)B * H * W;
-  if ((rc = ensure(ctx, dn->pred, np * kOutChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
+  if ((rc = ensure(ctx, dn->pred, np * kPredChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
       (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)))
     return rc;
   // weight-gradient partials: the largest layer's slices x weights
@@ -385,9 +390,10 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
 
 // One 3x3 'same' convolution (k_conv_fwd): 64 output channels per wave
 // where Cout allows, else 32 (Cout and Cin are multiples of 32).
-static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *w, const float *bias, int Cout,
+static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *Xlow, int Clow, const uint16_t *w,
+                          const float *bias, int Cout,
                           int B, int H, int W, uint16_t *outB, float *outF, int ldo, int mode, const float *res,
-                          int ldr) {
+                          int ldr, const uint16_t *maskY = nullptr) {
   const int nt = Cout % 64 == 0 ? 64 : 32;
   const int colTiles = (W + kTileW - 1) / kTileW, rowTiles = (H + kTileH - 1) / kTileH;
   const int coTiles = (Cout + nt - 1) / nt;
@@ -397,21 +403,24 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
   const int tpw = (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
   const dim3 grid(colTiles, (rowTiles + tpw - 1) / tpw, B * coTiles);
   if (nt == 64)
-    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, w, bias, Cout, H, W, outB, outF, ldo,
-                       mode, res, ldr, tpw);
+    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bias, Cout, H, W, outB, outF, ldo,
+                       mode, res, ldr, maskY, tpw);
   else
-    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, w, bias, Cout, H, W, outB, outF, ldo,
-                       mode, res, ldr, tpw);
+    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bias, Cout, H, W, outB, outF, ldo,
+                       mode, res, ldr, maskY, tpw);
 }
 
-static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, int B, float *outF, uint16_t *outB, int mode,
-                   const float *res) {
+// in: the layer's input, or with low the skip half of the decoder's
+// concatenation [up2(low) | in], read in place.
+static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *low, int B, float *outF, uint16_t *outB,
+                   int mode, const float *res) {
   const LayerDesc &Ld = dn->L[layer];
   hipStream_t s = dn->ctx->stream;
   const uint16_t *w = dn->wbf.as<uint16_t>() + dn->wbOff[layer];
   const float *bias = dn->params.as<float>() + dn->bOff[layer];
   const int H = in.h, W = in.w;
-  launchConvFwd(s, in.p(), Ld.cin, w, bias, Ld.cout, B, H, W, outB, outF, Ld.cout, mode, res, 3);
+  launchConvFwd(s, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, w, bias, Ld.cout, B, H, W, outB, outF,
+                outF ? kPredChan : Ld.cout, mode, res, 3);
   dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -432,37 +441,45 @@ static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerS
   bwdWShape(Cin, Cout, ks, wci, wco);
   const int tiles = ((Cin + 32 * wci - 1) / (32 * wci)) * ((Cout + 32 * wco - 1) / (32 * wco));
   const int64_t steps = (int64_t)B * H * ((W + 31) / 32);
-  // up to about 512 workgroups, each group of waves taking at least 16
-  // K-steps (fewer slices to write and reduce)
-  int64_t s = std::max<int64_t>(1, (512 + tiles - 1) / tiles);
-  s = std::min<int64_t>(s, std::max<int64_t>(1, steps / (16 * ks)));
+  // about 256 workgroups (a K-step is latency-bound, so more of them in
+  // parallel; each slice costs a write and a read of the layer's weights),
+  // each group of waves taking at least 4 K-steps
+  int64_t s = std::max<int64_t>(1, (256 + tiles - 1) / tiles);
+  s = std::min<int64_t>(s, std::max<int64_t>(1, steps / (4 * ks)));
   const int64_t per = (steps + s - 1) / s;
   if (stepsPerSplit) *stepsPerSplit = (int)per;
   return (int)((steps + per - 1) / per);
 }
 
 // dW then db (contiguous: Cout*9*Cin + Cout floats) when withBias, else dW
-static void launchConvBwdW(hipStream_t s, const uint16_t *dY, int Cout, const uint16_t *X, int Cin, int B, int H,
-                           int W, float *part, float *dW, bool withBias) {
+static void launchConvBwdW(hipStream_t s, const uint16_t *dY, int Cout, const uint16_t *X, int Cin, const uint16_t *Xlow,
+                           int Clow, int B, int H, int W, float *part, float *dW, bool withBias) {
   int ks, wci, wco, per = 1;
   bwdWShape(Cin, Cout, ks, wci, wco);
   const int slices = convBwdWSlices(Cin, Cout, B, H, W, &per);
   const dim3 grid((Cin + 32 * wci - 1) / (32 * wci), (Cout + 32 * wco - 1) / (32 * wco), slices);
   if (ks == 4)
-    hipLaunchKernelGGL((k_conv_bwd_w<4, 1, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+    hipLaunchKernelGGL((k_conv_bwd_w<4, 1, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, Xlow, Clow, B, H, W, per,
+                       part);
   else if (wco == 2 && wci == 1)
-    hipLaunchKernelGGL((k_conv_bwd_w<2, 1, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+    hipLaunchKernelGGL((k_conv_bwd_w<2, 1, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, Xlow, Clow, B, H, W, per,
+                       part);
   else if (ks == 2)
-    hipLaunchKernelGGL((k_conv_bwd_w<2, 2, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+    hipLaunchKernelGGL((k_conv_bwd_w<2, 2, 1>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, Xlow, Clow, B, H, W, per,
+                       part);
   else
-    hipLaunchKernelGGL((k_conv_bwd_w<1, 2, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, B, H, W, per, part);
+    hipLaunchKernelGGL((k_conv_bwd_w<1, 2, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, Xlow, Clow, B, H, W, per,
+                       part);
   const int64_t nW = (int64_t)Cout * 9 * Cin, nOut = withBias ? nW + Cout : nW;
   hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((nOut + 255) / 256)), dim3(256), 0, s, (const float *)part,
                      slices, nW + Cout, nOut, dW);
 }
 
-// Backward of one conv: dOut (bf16, already ReLU-masked) -> dW, db, dIn.
-static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const uint16_t *dOut, int B, uint16_t *dIn) {
+// Backward of one conv: dOut (bf16, already ReLU-masked) -> dW, db, dIn;
+// dIn leaves masked by inAct (the forward activation of the layer's input,
+// the output of a ReLU) when inAct is given.
+static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *low, const uint16_t *dOut, int B,
+                   uint16_t *dIn, const uint16_t *inAct) {
   const LayerDesc &Ld = dn->L[layer];
   hipStream_t s = dn->ctx->stream;
   const int H = in.h, W = in.w;
@@ -470,12 +487,13 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const uint16_t
   float *gb = dn->grads.as<float>() + dn->bOff[layer];
   // weights then bias (gb == gw + Cout*9*Cin)
   (void)gb;
-  launchConvBwdW(s, dOut, Ld.cout, in.p(), Ld.cin, B, H, W, dn->part.as<float>(), gw, true);
+  launchConvBwdW(s, dOut, Ld.cout, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, B, H, W,
+                 dn->part.as<float>(), gw, true);
   if (dIn) {
     // backward data: forward kernel on dOut with the flipped transposed weights
     const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
     const int cout = Ld.cin;  // gradient channels
-    launchConvFwd(s, dOut, Ld.cout, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3);
+    launchConvFwd(s, dOut, Ld.cout, nullptr, 0, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3, inAct);
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -493,71 +511,76 @@ static int forward(cpt_denoiser *dn, int B) {
       hipLaunchKernelGGL(k_pool2, dim3(nb256(n)), dim3(256), 0, s, src.p(), src.h, src.w, src.c, n, dst.p());
       cur = &dst;
     }
-    if ((rc = convFwd(dn, layer++, *cur, B, nullptr, dn->e[l][0].p(), 1, nullptr))) return rc;
-    if ((rc = convFwd(dn, layer++, dn->e[l][0], B, nullptr, dn->e[l][1].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, *cur, nullptr, B, nullptr, dn->e[l][0].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->e[l][0], nullptr, B, nullptr, dn->e[l][1].p(), 1, nullptr))) return rc;
     cur = &dn->e[l][1];
   }
   for (int l = kLevels - 2; l >= 0; l--) {
-    Tensor &cat = dn->cat[l];
-    const int64_t n = (int64_t)B * cat.h * cat.w * cat.c / 8;
-    hipLaunchKernelGGL(k_up_concat, dim3(nb256(n)), dim3(256), 0, s, cur->p(), cur->c, dn->e[l][1].p(), kChan[l], cat.h,
-                       cat.w, n, cat.p());
-    if ((rc = convFwd(dn, layer++, cat, B, nullptr, dn->d[l][0].p(), 1, nullptr))) return rc;
-    if ((rc = convFwd(dn, layer++, dn->d[l][0], B, nullptr, dn->d[l][1].p(), 1, nullptr))) return rc;
+    // [up2(cur) | e[l][1]], read in place by the conv
+    if ((rc = convFwd(dn, layer++, dn->e[l][1], cur, B, nullptr, dn->d[l][0].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->d[l][0], nullptr, B, nullptr, dn->d[l][1].p(), 1, nullptr))) return rc;
     cur = &dn->d[l][1];
   }
-  return convFwd(dn, layer, *cur, B, dn->pred.as<float>(), nullptr, 2, dn->logRad.as<float>());
+  return convFwd(dn, layer, *cur, nullptr, B, dn->pred.as<float>(), nullptr, 2, dn->logRad.as<float>());
 }
 
-static void reluMask(hipStream_t s, Tensor &g, const Tensor &y, int B) {
-  const int64_t n = (int64_t)B * y.h * y.w * y.c / 8;
-  hipLaunchKernelGGL(k_relu_mask, dim3(nb256(n)), dim3(256), 0, s, g.p(), y.p(), n);
-}
-
+// Every ReLU gradient mask is applied by the kernel that produces the
+// gradient: the backward-data conv, the concat split or the pooling backward.
 static int backward(cpt_denoiser *dn, int B) {
   int rc;
   hipStream_t s = dn->ctx->stream;
   int layer = kLayers - 1;
   // output conv: input d[0][1], gradient dPred (already d loss / d conv output)
-  if ((rc = convBwd(dn, layer--, dn->d[0][1], dn->dPred.as<uint16_t>(), B, dn->gd[0][1].p()))) return rc;
+  if ((rc = convBwd(dn, layer--, dn->d[0][1], nullptr, dn->dPred.as<uint16_t>(), B, dn->gd[0][1].p(), dn->d[0][1].p())))
+    return rc;
   for (int l = 0; l <= kLevels - 2; l++) {
-    // decoder level l: d[l][1] = conv(d[l][0]), d[l][0] = conv(cat[l])
-    reluMask(s, dn->gd[l][1], dn->d[l][1], B);
-    if ((rc = convBwd(dn, layer--, dn->d[l][0], dn->gd[l][1].p(), B, dn->gd[l][0].p()))) return rc;
-    reluMask(s, dn->gd[l][0], dn->d[l][0], B);
-    if ((rc = convBwd(dn, layer--, dn->cat[l], dn->gd[l][0].p(), B, dn->gcat[l].p()))) return rc;
+    // decoder level l: d[l][1] = conv(d[l][0]), d[l][0] = conv(cat[l]); gd[l][1]
+    // arrives masked
+    if ((rc = convBwd(dn, layer--, dn->d[l][0], nullptr, dn->gd[l][1].p(), B, dn->gd[l][0].p(), dn->d[l][0].p())))
+      return rc;
+    const Tensor &lowIn = l == kLevels - 2 ? dn->e[kLevels - 1][1] : dn->d[l + 1][1];  // the concat's upsampled half
+    if ((rc = convBwd(dn, layer--, dn->e[l][1], &lowIn, dn->gd[l][0].p(), B, dn->gcat[l].p(), nullptr))) return rc;
     // split the concat gradient: upsampled part -> gradient of the level below
+    // (masked by that level's output), skip part -> ge[l][1] (masked once the
+    // pooling gradient is added)
     Tensor &low = l == kLevels - 2 ? dn->ge[kLevels - 1][1] : dn->gd[l + 1][1];
+    const Tensor &lowAct = l == kLevels - 2 ? dn->e[kLevels - 1][1] : dn->d[l + 1][1];
     const int C1 = low.c, C2 = kChan[l];
     const int64_t nLow = (int64_t)B * low.h * low.w;
     hipLaunchKernelGGL(k_up_concat_bwd, dim3(nb256(nLow * (C1 + 4 * C2) / 8)), dim3(256), 0, s, dn->gcat[l].p(), C1, C2,
-                       dn->cat[l].h, dn->cat[l].w, nLow, low.p(), dn->ge[l][1].p());
+                       dn->gcat[l].h, dn->gcat[l].w, nLow, low.p(), dn->ge[l][1].p(), lowAct.p());
   }
-  // encoder, deepest first; layer index now points at enc level 4 conv 1
+  // encoder, deepest first; layer index now points at enc level 4 conv 1;
+  // ge[l][1] arrives masked
   for (int l = kLevels - 1; l >= 0; l--) {
-    reluMask(s, dn->ge[l][1], dn->e[l][1], B);
-    if ((rc = convBwd(dn, layer--, dn->e[l][0], dn->ge[l][1].p(), B, dn->ge[l][0].p()))) return rc;
-    reluMask(s, dn->ge[l][0], dn->e[l][0], B);
+    if ((rc = convBwd(dn, layer--, dn->e[l][0], nullptr, dn->ge[l][1].p(), B, dn->ge[l][0].p(), dn->e[l][0].p())))
+      return rc;
     const Tensor &in = l == 0 ? dn->x0 : dn->pool[l - 1];
     uint16_t *gin = l == 0 ? nullptr : dn->gpool[l - 1].p();
-    if ((rc = convBwd(dn, layer--, in, dn->ge[l][0].p(), B, gin))) return rc;
+    if ((rc = convBwd(dn, layer--, in, nullptr, dn->ge[l][0].p(), B, gin, nullptr))) return rc;
     if (l > 0) {
-      // average-pool backward adds into the skip gradient of level l-1
+      // average-pool backward adds into the skip gradient of level l-1, then
+      // masks it by that level's output
       Tensor &g = dn->ge[l - 1][1];
       const int64_t n = (int64_t)B * g.h * g.w * g.c / 8;
-      hipLaunchKernelGGL(k_pool2_bwd, dim3(nb256(n)), dim3(256), 0, s, dn->gpool[l - 1].p(), g.h, g.w, g.c, n, g.p(), 1);
+      hipLaunchKernelGGL(k_pool2_bwd, dim3(nb256(n)), dim3(256), 0, s, dn->gpool[l - 1].p(), g.h, g.w, g.c, n, g.p(), 1,
+                         dn->e[l - 1][1].p());
     }
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
 
 static int packAll(cpt_denoiser *dn) {
-  hipStream_t s = dn->ctx->stream;
+  PackTable T;
   for (int l = 0; l < kLayers; l++) {
-    const int64_t n = (int64_t)dn->L[l].cout * 9 * dn->L[l].cin;
-    hipLaunchKernelGGL(k_pack_weights, dim3(nb256(n)), dim3(256), 0, s, dn->params.as<float>() + dn->wOff[l], dn->L[l].cin,
-                       dn->L[l].cout, dn->wbf.as<uint16_t>() + dn->wbOff[l], dn->wtbf.as<uint16_t>() + dn->wbOff[l]);
+    T.wOff[l] = dn->wOff[l];
+    T.wbOff[l] = dn->wbOff[l];
+    T.cin[l] = dn->L[l].cin;
+    T.cout[l] = dn->L[l].cout;
   }
+  T.wbOff[kLayers] = dn->wbOff[kLayers - 1] + (int64_t)dn->L[kLayers - 1].cout * 9 * dn->L[kLayers - 1].cin;
+  hipLaunchKernelGGL(k_pack_all, dim3(nb256(T.wbOff[kLayers])), dim3(256), 0, dn->ctx->stream, dn->params.as<float>(), T,
+                     dn->wbf.as<uint16_t>(), dn->wtbf.as<uint16_t>());
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
 
@@ -667,7 +690,7 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
       if (dn->ge[l][k].buf.p) (void)hipFree(dn->ge[l][k].buf.p);
     }
   for (int l = 0; l < kLevels - 1; l++) {
-    Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->cat[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
+    Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
     for (Tensor *t : ts)
       if (t->buf.p) (void)hipFree(t->buf.p);
   }
@@ -722,7 +745,7 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
   hipLaunchKernelGGL(k_zero_f32, dim3(nb256(dn->nParams)), dim3(256), 0, s, dn->grads.as<float>(), dn->nParams);
   hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)1);
   const int64_t np = (int64_t)b * dn->H * dn->W;
-  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kOutChan, dT, h, w, dn->H, dn->W, np,
+  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dT, h, w, dn->H, dn->W, np,
                      1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->loss.as<float>());
   if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
   if (lr > 0.f) {
@@ -766,7 +789,7 @@ int cpt_denoise(cpt_denoiser *dn, const float *in10, float *outRgb, int32_t b, i
   if ((rc = prepare(dn, dIn, b, h, w))) return rc;
   dn->flopsFwd = 0;
   if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
-  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kOutChan, h, w, dn->H, dn->W,
+  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, h, w, dn->H, dn->W,
                      (int64_t)npx, dOut);
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
@@ -820,7 +843,7 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
   if (bias) chk(hipMemcpyAsync(db.p, bias, Cout * 4, hipMemcpyHostToDevice, s));
   else chk(hipMemsetAsync(db.p, 0, Cout * 4, s));
   if (mode == 0) {
-    launchConvFwd(s, dx.as<uint16_t>(), Cin, dw.as<uint16_t>(), db.as<float>(), Cout, B, H, W, nullptr,
+    launchConvFwd(s, dx.as<uint16_t>(), Cin, nullptr, 0, dw.as<uint16_t>(), db.as<float>(), Cout, B, H, W, nullptr,
                   dout.as<float>(), Cout, 0, nullptr, 3);
   } else if (mode == 1) {
     DevBuf part;
@@ -828,7 +851,8 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
       cleanup();
       return rc;
     }
-    launchConvBwdW(s, dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, B, H, W, part.as<float>(), dout.as<float>(),
+    launchConvBwdW(s, dyb.as<uint16_t>(), Cout, dx.as<uint16_t>(), Cin, nullptr, 0, B, H, W, part.as<float>(),
+                   dout.as<float>(),
                    false);
     chk(hipStreamSynchronize(s));
     (void)hipFree(part.p);
@@ -850,7 +874,8 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
     chk(hipMemcpyAsync(wf.p, hf.data(), nw * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_pack_weights, dim3(nb256(nw)), dim3(256), 0, s, wf.as<float>(), Cin, Cout, dw.as<uint16_t>(),
                        wt.as<uint16_t>());
-    launchConvFwd(s, dyb.as<uint16_t>(), Cout, wt.as<uint16_t>(), nullptr, Cin, B, H, W, nullptr, dout.as<float>(),
+    launchConvFwd(s, dyb.as<uint16_t>(), Cout, nullptr, 0, wt.as<uint16_t>(), nullptr, Cin, B, H, W, nullptr,
+                  dout.as<float>(),
                   Cin, 0, nullptr, 3);
     chk(hipStreamSynchronize(s));
     (void)hipFree(wf.p);

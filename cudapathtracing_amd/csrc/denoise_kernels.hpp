@@ -31,29 +31,51 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
   return (uint16_t)(u >> 16);
 }
 
+// Per bf16 half of a dword: all ones where the value is > 0 (+inf
+// included), zero for zero, negative values and NaN -- the ReLU gradient mask.
+__device__ __forceinline__ uint32_t reluKeep(uint32_t w) {
+  const uint32_t lo = w & 0xffffu, hi = w >> 16;
+  const bool keepLo = !(lo & 0x8000u) && lo != 0 && lo <= 0x7f80u;
+  const bool keepHi = !(hi & 0x8000u) && hi != 0 && hi <= 0x7f80u;
+  return (keepLo ? 0xffffu : 0u) | (keepHi ? 0xffff0000u : 0u);
+}
+
 constexpr int kConvThreads = 256;
 constexpr int kConvPix = 64;    // output pixels per workgroup (one row segment)
 constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
 
 // Y[b,y,x,co] = act(sum X[b,y+ky-1,x+kx-1,ci] W[co,ky,kx,ci] + bias[co]) (+res)
-// outBf (bf16, ldo channels) and/or outF (fp32, ldo channels).
-// mode bits: 1 = ReLU, 2 = add res[b,y,x,0..2] (fp32 residual, 3 channels).
+// X has Cin channels; with Xlow it is the decoder's concatenation
+// [up2(Xlow) | X] read in place: channels [0, Clow) from Xlow at half
+// resolution (pixel (y/2, x/2), Clow channels per pixel), channels
+// [Clow, Cin) from X (Cin - Clow channels per pixel).
+// into outBf (bf16, ldo channels per pixel) or outF (fp32, ldo channels per
+// pixel: channels co >= ldo are not written, so a 4-channel buffer takes the
+// first 4 of a wider layer).
+// mode bits: 1 = ReLU, 2 = add res[b,y,x,0..2] (fp32 residual, 3 channels;
+// outF only).  maskY (bf16 output only, ldo channels): the value is kept where
+// maskY > 0 and zeroed elsewhere -- the ReLU gradient mask of the layer whose
+// output maskY is.
 //
 // Implicit GEMM with the output transposed, D[co][px] = W[co][tap, ci] .
 // X[tap, ci][px]: the weights are the A operand and the input pixels the B
-// operand, so each lane ends with 4 consecutive output channels of one pixel
-// (one 8 B bf16 or 16 B fp32 store).  A workgroup (4 waves) owns a tile of
-// 4 rows x 64 pixels x 16*QB output channels, wave w the row y0 + w.  Per
-// 32-channel chunk of Cin it stages in LDS the 6 x 66 input halo of the tile
-// and the 9 taps' weights (9 x 16*QB x 32), each loaded once per chunk and
-// read by all waves for all 9 taps; a wave then issues 9 x 4 x QB MFMAs.  A
-// workgroup walks `tpw` vertically stacked tiles as one sequence of (tile,
-// chunk) steps; the next step's global loads are issued before this step's
-// MFMAs and written to LDS after them, so their latency hides behind the
-// MFMAs also where Cin is a single chunk (whose weights are then loaded
-// once per workgroup).  LDS
-// rows are padded to 40 bf16 (80 B): the 16-byte reads of 8 consecutive
-// rows then cover all 32 banks.
+// operand, so each lane ends with 4 consecutive output channels of one pixel.
+// A workgroup (4 waves) owns a tile of 4 rows x 64 pixels x 16*QB output
+// channels, wave w the row y0 + w.  Per 32-channel chunk of Cin it stages in
+// LDS the 6 x 66 input halo of the tile and the 9 taps' weights
+// (9 x 16*QB x 32), each loaded once per chunk and read by all waves for all
+// 9 taps; a wave then issues 9 x 4 x QB MFMAs.  A workgroup walks `tpw`
+// vertically stacked tiles as one sequence of (tile, chunk) steps; the next
+// step's global loads are issued before this step's MFMAs and written to LDS
+// after them, so their latency hides behind the MFMAs also where Cin is a
+// single chunk (whose weights are then loaded once per workgroup).  LDS rows
+// are padded to 40 bf16 (80 B): the 16-byte reads of 8 consecutive rows then
+// cover all 32 banks.
+// A finished bf16 tile leaves through LDS: each wave writes its row (64 px x
+// 32 channels at a time) into the halo buffer, which is free after a barrier
+// at the tile's end, and reads it back as 16 B per lane, so that a store
+// instruction writes 16 whole 64 B pixel records (scattered 8 B stores per
+// lane had cost as much as the MFMAs).
 constexpr int kTileH = 4, kTileW = 64;
 constexpr int kHaloH = kTileH + 2, kHaloW = kTileW + 2;
 constexpr int kHaloParts = kHaloH * kHaloW * 4;  // 16 B parts of a 32-channel halo
@@ -61,11 +83,13 @@ constexpr int kHaloIters = (kHaloParts + kConvThreads - 1) / kConvThreads;
 
 template <int QB>
 __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
+                                                               const uint16_t *__restrict__ Xlow, int Clow,
                                                                const uint16_t *__restrict__ Wt,
                                                                const float *__restrict__ bias, int Cout, int H,
                                                                int Wd, uint16_t *__restrict__ outBf,
                                                                float *__restrict__ outF, int ldo, int mode,
-                                                               const float *__restrict__ res, int ldr, int tpw) {
+                                                               const float *__restrict__ res, int ldr,
+                                                               const uint16_t *__restrict__ maskY, int tpw) {
   constexpr int NT = 16 * QB;
   constexpr int kWParts = 9 * NT * 4;
   constexpr int kWIters = (kWParts + kConvThreads - 1) / kConvThreads;
@@ -92,8 +116,10 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
   uint4 px[kHaloIters], pw[kWIters];
   // global -> registers for step k: the input halo of its tile and, unless
   // Cin is one chunk (the weights then stay in LDS), the chunk's weights
+  const int cSkip = Cin - Clow;  // channels per pixel of X
   auto load = [&](int k) {
     const int y0 = (t0 + k / nChunks) * kTileH, c0 = (k % nChunks) * 32;
+    const bool fromLow = c0 < Clow;  // uniform: chunks never straddle Clow
 #pragma unroll
     for (int i = 0; i < kHaloIters; i++) {
       const int q = tid + i * kConvThreads;
@@ -101,8 +127,13 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
       if (q < kHaloParts) {
         const int hp = q >> 2, part = q & 3;
         const int yy = y0 - 1 + hp / kHaloW, xx = x0 - 1 + hp % kHaloW;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
-          v = *(const uint4 *)(X + (imgBase + (int64_t)yy * Wd + xx) * Cin + c0 + part * 8);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < Wd) {
+          if (fromLow)
+            v = *(const uint4 *)(Xlow + (((int64_t)b * (H >> 1) + (yy >> 1)) * (Wd >> 1) + (xx >> 1)) * Clow + c0 +
+                                 part * 8);
+          else
+            v = *(const uint4 *)(X + (imgBase + (int64_t)yy * Wd + xx) * cSkip + (c0 - Clow) + part * 8);
+        }
       }
       px[i] = v;
     }
@@ -133,13 +164,13 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
       if (q < kWParts) *(uint4 *)(sW + (q >> 2) * kLdsRow + (q & 3) * 8) = pw[i];
     }
   };
-  // D[co][px]: lane holds output channels 4*(lane/16) + r of pixel lane%16
-  auto epilogue = [&](int y) {
+  // fp32 output: lane holds output channels 4*(lane/16) + r of pixel lane%16
+  auto epilogueF = [&](int y) {
     if (y >= H) return;
 #pragma unroll
     for (int nb = 0; nb < QB; nb++) {
       const int co = co0 + nb * 16 + (lane >> 4) * 4;
-      if (co >= Cout) continue;  // Cout is a multiple of 16: all 4 channels or none
+      if (co >= Cout || co >= ldo) continue;  // channel counts are multiples of 4
       float bs[4] = {0.f, 0.f, 0.f, 0.f};
       if (bias) {
         const float4 bv = *(const float4 *)(bias + co);
@@ -160,14 +191,68 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
           if (mode & 1) v[r] = fmaxf(v[r], 0.f);
           if ((mode & 2) && co + r < 3) v[r] += res[p * ldr + co + r];
         }
-        if (outBf) {
+        *(float4 *)(outF + p * ldo + co) = make_float4(v[0], v[1], v[2], v[3]);
+      }
+    }
+  };
+  // bf16 output through LDS, 32 channels at a time (call after a barrier:
+  // the halo buffer is overwritten)
+  auto epilogueB = [&](int y) {
+    uint16_t *st = sX + wave * (kTileW * kLdsRow);
+#pragma unroll
+    for (int half = 0; half < (QB + 1) / 2; half++) {
+#pragma unroll
+      for (int nb2 = 0; nb2 < 2 && half * 2 + nb2 < QB; nb2++) {
+        const int nb = half * 2 + nb2;
+        const int col = nb2 * 16 + (lane >> 4) * 4, co = co0 + nb * 16 + (lane >> 4) * 4;
+        float bs[4] = {0.f, 0.f, 0.f, 0.f};
+        if (bias && co < Cout) {
+          const float4 bv = *(const float4 *)(bias + co);
+          bs[0] = bv.x;
+          bs[1] = bv.y;
+          bs[2] = bv.z;
+          bs[3] = bv.w;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          float v[4];
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            v[r] = acc[nb][m][r] + bs[r];
+            if (mode & 1) v[r] = fmaxf(v[r], 0.f);
+          }
           uint2 pk;
           pk.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
           pk.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
-          *(uint2 *)(outBf + p * ldo + co) = pk;
+          *(uint2 *)(st + (m * 16 + (lane & 15)) * kLdsRow + col) = pk;
         }
-        if (outF) *(float4 *)(outF + p * ldo + co) = make_float4(v[0], v[1], v[2], v[3]);
       }
+      // the row's values are in LDS: other lanes of this wave read them
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int nCh = QB - half * 2 >= 2 ? 32 : 16;  // channels staged in this half
+      if (y < H) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int px = j * 16 + (lane >> 2), part = lane & 3;
+          const int x = x0 + px, co = co0 + half * 32 + part * 8;
+          if (part * 8 >= nCh || x >= Wd || co >= Cout) continue;
+          uint4 v = *(const uint4 *)(st + px * kLdsRow + part * 8);
+          const int64_t o = (imgBase + (int64_t)y * Wd + x) * ldo + co;
+          if (maskY) {
+            const uint4 mv = *(const uint4 *)(maskY + o);
+            v.x &= reluKeep(mv.x);
+            v.y &= reluKeep(mv.y);
+            v.z &= reluKeep(mv.z);
+            v.w &= reluKeep(mv.w);
+          }
+          *(uint4 *)(outBf + o) = v;
+        }
+      }
+      // the next half overwrites the staging rows: this wave's reads of them
+      // are issued first (LDS operations of a wave complete in order)
+      __builtin_amdgcn_wave_barrier();
     }
   };
   if (nSteps <= 0) return;  // workgroup-uniform
@@ -193,7 +278,13 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
     }
     if (k % nChunks == nChunks - 1) {
       // the tile is complete: write it and start the next one from zero
-      epilogue((t0 + k / nChunks) * kTileH + wave);
+      const int y = (t0 + k / nChunks) * kTileH + wave;
+      if (outF) {
+        epilogueF(y);
+      } else {
+        __syncthreads();  // every wave is done reading the halo buffer
+        epilogueB(y);
+      }
 #pragma unroll
       for (int nb = 0; nb < QB; nb++)
 #pragma unroll
@@ -233,13 +324,18 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
 // adds the slices in a fixed order (deterministic).
 constexpr int kBwRow = 20;  // dwords per LDS row (40 bf16)
 
+// X, Xlow, Clow: the layer's input as in k_conv_fwd (with Xlow, the
+// decoder's concatenation read in place; the choice is made per 8-channel
+// item, since a 64-channel tile may straddle Clow).
 // The workgroups of the first Cin tile also sum dY over their pixels (the
 // bias gradient) from the A fragments they hold, into entries nW..nW+Cout of
 // their slice (slices are nW + Cout floats).
 template <int KS, int WCI, int WCO>
 __global__ __launch_bounds__(kConvThreads, 2) void k_conv_bwd_w(const uint16_t *__restrict__ dY, int Cout,
-                                                                 const uint16_t *__restrict__ X, int Cin, int B, int H,
-                                                                 int Wd, int stepsPerSplit, float *__restrict__ part) {
+                                                                 const uint16_t *__restrict__ X, int Cin,
+                                                                 const uint16_t *__restrict__ Xlow, int Clow, int B,
+                                                                 int H, int Wd, int stepsPerSplit,
+                                                                 float *__restrict__ part) {
   constexpr int CIW = 32 * WCI, COW = 32 * WCO;
   constexpr int kGroupThreads = kConvThreads / KS;
   constexpr int kXItems = 3 * 17 * (CIW / 8);  // (ky, pixel pair, 8-channel group)
@@ -290,9 +386,18 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_bwd_w(const uint16_t *
         const int yy = y + ky - 1, xa = x0 - 1 + 2 * pr;
         const int ci = ci0 + cig * 8;
         if (yy >= 0 && yy < H && ci < Cin) {
-          const uint16_t *row = X + (img + (int64_t)yy * Wd) * Cin + ci;
-          if (xa >= 0 && xa < Wd) a = *(const uint4 *)(row + (int64_t)xa * Cin);
-          if (xa + 1 >= 0 && xa + 1 < Wd) c = *(const uint4 *)(row + (int64_t)(xa + 1) * Cin);
+          if (ci < Clow) {
+            // the upsampled half of a concatenation: pixels xa and xa + 1
+            // (xa odd) lie in the low-resolution pixels (xa >> 1), (xa + 1) >> 1
+            const uint16_t *row = Xlow + ((int64_t)b * (H >> 1) + (yy >> 1)) * (Wd >> 1) * Clow + ci;
+            if (xa >= 0 && xa < Wd) a = *(const uint4 *)(row + (int64_t)(xa >> 1) * Clow);
+            if (xa + 1 >= 0 && xa + 1 < Wd) c = *(const uint4 *)(row + (int64_t)((xa + 1) >> 1) * Clow);
+          } else {
+            const int cs = Cin - Clow;
+            const uint16_t *row = X + (img + (int64_t)yy * Wd) * cs + (ci - Clow);
+            if (xa >= 0 && xa < Wd) a = *(const uint4 *)(row + (int64_t)xa * cs);
+            if (xa + 1 >= 0 && xa + 1 < Wd) c = *(const uint4 *)(row + (int64_t)(xa + 1) * cs);
+          }
         }
       } else if (live && q < kItems) {
         const int r = q - kXItems;
