@@ -349,6 +349,7 @@ static int ensureT(cpt_ctx *ctx, Tensor &t, int B, int h, int w, int c) {
 }
 
 static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerSplit);
+static int64_t fwdSplitFloats(int Cin, int Cout, int B, int H, int W);
 
 static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
   cpt_ctx *ctx = dn->ctx;
@@ -383,31 +384,120 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
     const int64_t nW = (int64_t)dn->L[k].cout * 9 * dn->L[k].cin;
     partFloats = std::max<int64_t>(partFloats, (int64_t)convBwdWSlices(dn->L[k].cin, dn->L[k].cout, B, H >> lvl,
                                                                          W >> lvl, nullptr) * (nW + dn->L[k].cout));
+    // split forward and backward-data convolutions of small layers
+    partFloats = std::max(partFloats, fwdSplitFloats(dn->L[k].cin, dn->L[k].cout, B, H >> lvl, W >> lvl));
+    partFloats = std::max(partFloats, fwdSplitFloats(dn->L[k].cout, dn->L[k].cin, B, H >> lvl, W >> lvl));
   }
   if ((rc = ensure(ctx, dn->part, partFloats * 4))) return rc;
   return CPT_OK;
 }
 
-// One 3x3 'same' convolution (k_conv_fwd): 64 output channels per wave
-// where Cout allows, else 32 (Cout and Cin are multiples of 32).
-static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *Xlow, int Clow, const uint16_t *w,
-                          const float *bias, int Cout,
-                          int B, int H, int W, uint16_t *outB, float *outF, int ldo, int mode, const float *res,
-                          int ldr, const uint16_t *maskY = nullptr) {
-  const int nt = Cout % 64 == 0 ? 64 : 32;
-  const int colTiles = (W + kTileW - 1) / kTileW, rowTiles = (H + kTileH - 1) / kTileH;
-  const int coTiles = (Cout + nt - 1) / nt;
+// out[p][co] (bf16, Cout channels) = act(sum over the kSplit slices of
+// part[s][p][co] + bias[co]), slices added in order; with maskY the ReLU
+// gradient mask of maskY.  n = pixels * Cout / 8.
+__global__ void k_split_sum(const float *__restrict__ part, int kSplit, int64_t sliceStride,
+                            const float *__restrict__ bias, int Cout, int relu, const uint16_t *__restrict__ maskY,
+                            int64_t n, uint16_t *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = (int)((i * 8) % Cout);
+  float v[8];
+  const float4 a = *(const float4 *)(part + i * 8), b = *(const float4 *)(part + i * 8 + 4);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  for (int s = 1; s < kSplit; s++) {
+    const float4 c0 = *(const float4 *)(part + s * sliceStride + i * 8),
+                 c1 = *(const float4 *)(part + s * sliceStride + i * 8 + 4);
+    v[0] += c0.x; v[1] += c0.y; v[2] += c0.z; v[3] += c0.w; v[4] += c1.x; v[5] += c1.y; v[6] += c1.z; v[7] += c1.w;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    if (bias) v[j] += bias[c + j];
+    if (relu) v[j] = fmaxf(v[j], 0.f);
+  }
+  uint4 r = pack8(v);
+  if (maskY) {
+    const uint4 mv = *(const uint4 *)(maskY + i * 8);
+    r.x &= reluKeep(mv.x);
+    r.y &= reluKeep(mv.y);
+    r.z &= reluKeep(mv.z);
+    r.w &= reluKeep(mv.w);
+  }
+  *(uint4 *)(out + i * 8) = r;
+}
+
+// Tiling of one 3x3 'same' convolution (k_conv_fwd): 64 output channels per
+// wave where Cout allows, else 32.  The smallest layers (under 64 tiles,
+// several Cin chunks: the deepest level) are split over Cin into up to as
+// many slices as fill 512 workgroups (fp32 partial sums, then k_split_sum):
+// there a workgroup's chain of dependent chunk steps costs more than the
+// extra launch (about 4.5 us), elsewhere it does not.
+struct FwdPlan {
+  int nt, colTiles, rowTiles, coTiles, split, tpw;
+};
+static FwdPlan fwdPlan(int Cin, int Cout, int B, int H, int W) {
+  FwdPlan P;
+  P.colTiles = (W + kTileW - 1) / kTileW;
+  P.rowTiles = (H + kTileH - 1) / kTileH;
+  P.nt = Cout % 64 == 0 ? 64 : 32;
+  P.coTiles = (Cout + P.nt - 1) / P.nt;
+  const int64_t tiles = (int64_t)P.colTiles * P.rowTiles * B * P.coTiles;
+  const int chunks = Cin / 32;
+  P.split = 1;
+  if (chunks > 1 && tiles < 64) {
+    int ks = (int)std::min<int64_t>(chunks, (512 + tiles - 1) / tiles);
+    const int per = (chunks + ks - 1) / ks;
+    P.split = (chunks + per - 1) / per;  // every slice non-empty
+  }
   // stacked tiles per workgroup: as few workgroups as still fill the chip
   // once (about 512, two per CU), at most 8 tiles each
-  const int64_t tiles = (int64_t)colTiles * rowTiles * B * coTiles;
-  const int tpw = (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
-  const dim3 grid(colTiles, (rowTiles + tpw - 1) / tpw, B * coTiles);
-  if (nt == 64)
-    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bias, Cout, H, W, outB, outF, ldo,
-                       mode, res, ldr, maskY, tpw);
+  P.tpw = P.split > 1 ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
+  return P;
+}
+// fp32 floats of partial sums a split convolution needs (0 if unsplit)
+static int64_t fwdSplitFloats(int Cin, int Cout, int B, int H, int W) {
+  const FwdPlan P = fwdPlan(Cin, Cout, B, H, W);
+  return P.split > 1 ? (int64_t)P.split * B * H * W * Cout : 0;
+}
+
+// One 3x3 'same' convolution.  splitPart: scratch for a split over Cin (at
+// least fwdSplitFloats floats); without it, or with an fp32 output, the
+// convolution is not split.
+static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *Xlow, int Clow, const uint16_t *w,
+                          const float *bias, int Cout, int B, int H, int W, uint16_t *outB, float *outF, int ldo,
+                          int mode, const float *res, int ldr, const uint16_t *maskY = nullptr,
+                          float *splitPart = nullptr) {
+  FwdPlan P = fwdPlan(Cin, Cout, B, H, W);
+  if (!splitPart || outF) {
+    P.split = 1;
+    const int64_t tiles = (int64_t)P.colTiles * P.rowTiles * B * P.coTiles;
+    P.tpw = (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
+  }
+  const dim3 grid(P.colTiles, (P.rowTiles + P.tpw - 1) / P.tpw, B * P.coTiles * P.split);
+  const int64_t sliceStride = (int64_t)B * H * W * Cout;
+  float *oF = outF;
+  uint16_t *oB = outB;
+  const float *bs = bias;
+  int md = mode;
+  const uint16_t *mk = maskY;
+  if (P.split > 1) {  // partial sums, no bias or activation: k_split_sum adds them
+    oF = splitPart;
+    oB = nullptr;
+    bs = nullptr;
+    md = 0;
+    mk = nullptr;
+    ldo = Cout;
+  }
+  if (P.nt == 64)
+    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, oF,
+                       ldo, md, res, ldr, mk, P.split, sliceStride, P.tpw);
   else
-    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bias, Cout, H, W, outB, outF, ldo,
-                       mode, res, ldr, maskY, tpw);
+    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, oF,
+                       ldo, md, res, ldr, mk, P.split, sliceStride, P.tpw);
+  if (P.split > 1) {
+    const int64_t n = sliceStride / 8;
+    hipLaunchKernelGGL(k_split_sum, dim3(nb256(n)), dim3(256), 0, s, (const float *)splitPart, P.split, sliceStride,
+                       bias, Cout, mode & 1, maskY, n, outB);
+  }
 }
 
 // in: the layer's input, or with low the skip half of the decoder's
@@ -420,7 +510,7 @@ static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *
   const float *bias = dn->params.as<float>() + dn->bOff[layer];
   const int H = in.h, W = in.w;
   launchConvFwd(s, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, w, bias, Ld.cout, B, H, W, outB, outF,
-                outF ? kPredChan : Ld.cout, mode, res, 3);
+                outF ? kPredChan : Ld.cout, mode, res, 3, nullptr, dn->part.as<float>());
   dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -493,7 +583,8 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *
     // backward data: forward kernel on dOut with the flipped transposed weights
     const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
     const int cout = Ld.cin;  // gradient channels
-    launchConvFwd(s, dOut, Ld.cout, nullptr, 0, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3, inAct);
+    launchConvFwd(s, dOut, Ld.cout, nullptr, 0, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3, inAct,
+                  dn->part.as<float>());
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
                                                                int Wd, uint16_t *__restrict__ outBf,
                                                                float *__restrict__ outF, int ldo, int mode,
                                                                const float *__restrict__ res, int ldr,
-                                                               const uint16_t *__restrict__ maskY, int tpw) {
+                                                               const uint16_t *__restrict__ maskY, int kSplit,
+                                                               int64_t sliceStride, int tpw) {
   constexpr int NT = 16 * QB;
   constexpr int kWParts = 9 * NT * 4;
   constexpr int kWIters = (kWParts + kConvThreads - 1) / kConvThreads;
@@ -98,10 +99,16 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x0 = blockIdx.x * kTileW;
   const int nCoTiles = (Cout + NT - 1) / NT;
-  const int b = blockIdx.z / nCoTiles;
-  const int co0 = (blockIdx.z % nCoTiles) * NT;
+  // split over Cin: slice s of kSplit takes chunks [cBeg, cBeg + nChunks)
+  // and writes its fp32 partial sums to outF + s * sliceStride
+  const int slice = blockIdx.z % kSplit, bz = blockIdx.z / kSplit;
+  const int b = bz / nCoTiles;
+  const int co0 = (bz % nCoTiles) * NT;
   const int64_t imgBase = (int64_t)b * H * Wd;
-  const int nChunks = Cin / 32;
+  const int chunksAll = Cin / 32, perSlice = (chunksAll + kSplit - 1) / kSplit;
+  const int cBeg = slice * perSlice;
+  const int nChunks = chunksAll - cBeg < perSlice ? chunksAll - cBeg : perSlice;
+  if (outF) outF += slice * sliceStride;
   // this workgroup's tiles: rows y0(i) = (blockIdx.y * tpw + i) * kTileH,
   // walked as one sequence of (tile, chunk) steps
   const int rowTiles = (H + kTileH - 1) / kTileH;
@@ -118,7 +125,7 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
   // Cin is one chunk (the weights then stay in LDS), the chunk's weights
   const int cSkip = Cin - Clow;  // channels per pixel of X
   auto load = [&](int k) {
-    const int y0 = (t0 + k / nChunks) * kTileH, c0 = (k % nChunks) * 32;
+    const int y0 = (t0 + k / nChunks) * kTileH, c0 = (cBeg + k % nChunks) * 32;
     const bool fromLow = c0 < Clow;  // uniform: chunks never straddle Clow
 #pragma unroll
     for (int i = 0; i < kHaloIters; i++) {
