@@ -59,6 +59,27 @@ def cores():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def median_time(fn, reps, warm=1):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
 def bench_render(cpt, L, scenes, args):
     s = scenes.displaced_icosphere(1920, 1080)
     ctx = cpt.Context(0)
@@ -90,6 +111,14 @@ def bench_render(cpt, L, scenes, args):
     ext_bytes = 64.0 * stc.extend_rays + node_bytes(args) * stc.extend_nodes + 48.0 * stc.extend_prims
     ext_bytes *= ext_rays / max(stc.extend_rays, 1.0)
     achieved = ext_bytes / (ext_ms * 1e-3) / 1e9
+    # occlusion walk: 48 B shadow record (ray, light sample) + record visits
+    # + primitive tests (the radiance update of unoccluded rays, <= 32 B, is
+    # not counted)
+    sh_ms = float(np.mean([f.shadow_ms for f in frames]))
+    sh_rays = float(np.mean([f.shadow_rays for f in frames]))
+    sh_bytes = 48.0 * stc.shadow_rays + node_bytes(args) * stc.shadow_nodes + 48.0 * stc.shadow_prims
+    sh_bytes *= sh_rays / max(stc.shadow_rays, 1.0)
+    sh_achieved = sh_bytes / (sh_ms * 1e-3) / 1e9 if sh_ms > 0 else 0.0
     res = dict(ms=ms, rays=rays, mrays=rays / (ms * 1e-3) / 1e6, ext_ms=ext_ms, ext_rays=ext_rays,
                shade_ms=float(np.mean([f.shade_ms for f in frames])),
                shadow_ms=float(np.mean([f.shadow_ms for f in frames])),
@@ -98,12 +127,17 @@ def bench_render(cpt, L, scenes, args):
                prims_per_ray=stc.extend_prims / max(stc.extend_rays, 1),
                lane_utilisation=stc.extend_nodes / max(stc.extend_wave_nodes, 1),
                achieved=achieved, build_ms=st_b.total_ms, cluster_ms=st_b.cluster_ms, tree=tree,
+               shadow_achieved=sh_achieved, shadow_rays=sh_rays,
+               shadow_nodes_per_ray=stc.shadow_nodes / max(stc.shadow_rays, 1),
+               shadow_prims_per_ray=stc.shadow_prims / max(stc.shadow_rays, 1),
+               shadow_lane_utilisation=stc.shadow_nodes / max(stc.shadow_wave_nodes, 1),
                mean=float(img.mean()))
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         o = O.OracleScene(s)
-        o.build(s.bounds_min, s.bounds_max)
+        # oracle LBVH build of the same 1.31M triangles: median of 5 after 1
+        cpu_build = median_time(lambda: o.build(s.bounds_min, s.bounds_max), 5)
         cam = cpt.camera(s.camera)
         # centred window of the frame, same camera and spp
         win = args.cpu_window
@@ -120,10 +154,27 @@ def bench_render(cpt, L, scenes, args):
         dt = time.time() - t0
         # ray count of the same window, traced on the GPU
         gw, _, gst = ctx.render(sub, args.spp, seed=3317, profile=L.PROFILE_CONFIG)
+        # config 1 (Cornell box + level-3 icosphere, 256x256, 16 spp, depth 8,
+        # roulette after 3) on the host cores and on the GPU
+        s1 = scenes.cornell_icosphere(256, 256)
+        o1 = O.OracleScene(s1)
+        o1.build(s1.bounds_min, s1.bounds_max)
+        prm1 = L.RenderParams(3317, 16, 8, 3, L.PROFILE_CONFIG, 0.7, 0, 0, 0, 0)
+        cam1 = cpt.camera(s1.camera)
+        c1 = median_time(lambda: o1.render(cam1, prm1, nthreads=cores()), 3, warm=0)
+        ctx.upload(s1)
+        ctx.build(s1.bounds_min, s1.bounds_max)
+        g1 = median_time(lambda: ctx.render(s1.camera, 16, seed=3317, profile=L.PROFILE_CONFIG), 5)
         cpu = {"value": round(gst.rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores(), "kind": "port",
+               "cpu_model": cpu_model(),
                "sample": f"FP64 oracle (reference-order unculled traversal, FP64 BSDFs, OpenMP) rendering the "
                          f"centred {win}x{win} window of the same frame at {args.spp} spp in {dt:.1f} s; "
-                         f"rays = the GPU's ray count for that window"}
+                         f"rays = the GPU's ray count for that window",
+               "build_ms_1p31m": round(cpu_build * 1e3, 1),
+               "config1": {"cpu_mpix_per_s": round(256 * 256 / c1 / 1e6, 4), "cpu_s": round(c1, 3),
+                           "gpu_mpix_per_s": round(256 * 256 / g1 / 1e6, 2), "gpu_ms": round(g1 * 1e3, 3),
+                           "spp": 16, "size": [256, 256], "max_depth": 8, "rr_depth": 3,
+                           "timing": "median of 3 (CPU) / 5 after 1 warm-up (GPU, wall time incl. readback)"}}
         log(f"cpu baseline {cpu}")
     ctx.close()
     return res, cpu, int(s.tri.shape[0])
@@ -181,33 +232,60 @@ def bench_equal_quality(cpt, L, scenes, args):
             "plain_points": [(p[0], round(p[1], 6), round(p[2], 2)) for p in eq["points"]]}
 
 
-def bench_config5(cpt, L, scenes, args):
-    """Denoised inference vs plain tracer at equal relMSE on the displaced
-    icosphere plus a grid^3 fBm medium (sigma_max 8), many views at once.
+def window_camera(L, cam, x0, y0, w, h):
+    """The w x h sub-window at pixel (x0, y0) of an expanded camera."""
+    import ctypes as C
+    sub = L.Camera()
+    C.memmove(C.byref(sub), C.byref(cam), C.sizeof(L.Camera))
+    for k in range(3):
+        sub.pixel_center[k] = cam.pixel_center[k] + x0 * cam.delta_h[k] + y0 * cam.delta_v[k]
+    sub.width, sub.height = w, h
+    return sub
 
-    Denoised: all views rendered at 4 spp with AOVs in one cpt_render_views
-    call, then the U-Net over the views (device buffers end to end; time =
-    wall time of both, synchronised).  Error: relMSE of the first
-    `c5_ref_views` views against their `c5_ref_spp` renders.  Plain: spp
-    doubled from 4 on those views until relMSE falls below the denoised
-    one (log-log interpolation); its time for all views = the measured time
-    of an all-view render at 16 spp scaled linearly to that spp."""
+
+def bench_config5(cpt, L, scenes, args):
+    """Config 5: denoised 4 spp vs the plain tracer at equal relMSE on the
+    displaced icosphere plus a grid^3 fBm medium (sigma_max 8), nv views.
+
+    Error: relMSE with the reference noise removed (two independent
+    half-spp references r1, r2: mean((x - r1)(x - r2) / (rbar^2 + 1e-2))),
+    over a centred window of each of the first `c5_score_views` views.
+    Denoised: all nv views at 4 spp with AOVs in one cpt_render_views call,
+    then the U-Net over the views (device buffers end to end), for
+    `c5_seeds` seeds; time = median wall time of the pipeline (synchronised).
+    Plain: the scored windows accumulated at spp 4, 8, 16, ... (each step
+    continues the sample stream) until the relMSE is below every bootstrap
+    resample of the denoised mean; the equal-relMSE spp is interpolated
+    between the two bracketing measurements (never extrapolated).  Plain
+    time for all nv full views at that spp = the measured time of an
+    all-view render at 64 spp x spp / 64 (frame time is linear in spp:
+    samples are independent passes).  CI: 95% bootstrap over the seeds."""
     if not os.path.exists(WEIGHTS):
         return None
     import torch
     from cudapathtracing_amd.denoiser import Denoiser
-    from cudapathtracing_amd.training import equal_quality, orbit_cameras, relmse
+    from cudapathtracing_amd.training import crossing_spp, orbit_cameras, relmse_two_ref
     w, h, nv = args.c5_width, args.c5_height, args.c5_views
+    nsv = min(args.c5_score_views, nv)
+    ww, wh = min(args.c5_window[0], w), min(args.c5_window[1], h)
+    x0, y0 = (w - ww) // 2, (h - wh) // 2
     s = scenes.displaced_icosphere_medium(w, h, grid=args.c5_grid)
     cams = orbit_cameras(nv, w, h, seed=7)
+    wins = [window_camera(L, cpt.camera(c), x0, y0, ww, wh) for c in cams[:nsv]]
     kw = dict(profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3)
     chunk = min(nv, 8)
+    half = args.c5_ref_spp // 2
     with cpt.Context(0) as ctx:
         ctx.upload(s)
         ctx.build(s.bounds_min, s.bounds_max)
         t0 = time.time()
-        refs = [ctx.render(cams[v], args.c5_ref_spp, seed=99991 + v, **kw)[0] for v in range(args.c5_ref_views)]
-        log(f"config-5 references ({args.c5_ref_views} views, {args.c5_ref_spp} spp): {time.time() - t0:.1f}s")
+        r1 = ctx.render_views(wins, half, seed=99991, **kw)[0].astype(np.float64)
+        r2 = ctx.render_views(wins, half, seed=77773, **kw)[0].astype(np.float64)
+        log(f"config-5 references ({nsv} windows {ww}x{wh}, 2 x {half} spp): {time.time() - t0:.1f}s")
+
+        def score(x):  # mean over the scored windows
+            return float(np.mean([relmse_two_ref(x[v], r1[v], r2[v]) for v in range(nsv)]))
+
         dn = Denoiser(ctx, chunk, h, w)
         dn.set_params(np.fromfile(WEIGHTS, dtype="<f4"))
         rgb = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
@@ -221,45 +299,129 @@ def bench_config5(cpt, L, scenes, args):
                 dn(x[c0:c0 + chunk], out=den[c0:c0 + chunk])
 
         denoised_pipeline(5)  # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        denoised_pipeline(7)
-        torch.cuda.synchronize()
-        t_den = (time.perf_counter() - t0) * 1e3
-        d_np = den[:args.c5_ref_views].cpu().numpy()
-        e_den = float(np.mean([relmse(d_np[v], refs[v]) for v in range(args.c5_ref_views)]))
-        e_noisy = float(np.mean([relmse(rgb[v].cpu().numpy(), refs[v]) for v in range(args.c5_ref_views)]))
-        eq = equal_quality(ctx, cams[0], refs[0], e_den, seed=11, spp_start=4, spp_max=args.c5_ref_spp // 2)
-        plain16 = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
-        ctx.render_views(cams, 16, seed=13, out=plain16, **kw)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ctx.render_views(cams, 16, seed=17, out=plain16, **kw)
-        torch.cuda.synchronize()
-        t16 = (time.perf_counter() - t0) * 1e3
-        t_plain = t16 * eq["spp"] / 16.0
+        e_den, e_noisy, t_den = [], [], []
+        for k in range(args.c5_seeds):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            denoised_pipeline(1000 + 7 * k)
+            torch.cuda.synchronize()
+            t_den.append((time.perf_counter() - t0) * 1e3)
+            crop = lambda a: a[:nsv, y0:y0 + wh, x0:x0 + ww].cpu().numpy()
+            e_den.append(score(crop(den)))
+            e_noisy.append(score(crop(rgb)))
         dn.close()
+        e_den = np.array(e_den)
+        rng = np.random.default_rng(0)
+        boot = rng.choice(e_den, size=(2000, e_den.size), replace=True).mean(axis=1)
+        lo, hi = np.quantile(boot, [0.025, 0.975])
+        target = float(e_den.mean())
+        # plain sweep on the scored windows, continuing the sample stream
+        acc = np.zeros((nsv, wh, ww, 3), np.float64)
+        pts, n = [], 0
+        step = 4
+        t0 = time.time()
+        while n + step <= args.c5_spp_max:
+            img = ctx.render_views(wins, step, seed=4242, sample_base=n, **kw)[0]
+            acc += img.astype(np.float64) * step
+            n += step
+            e = score(acc / n)
+            pts.append((n, e))
+            log(f"config-5 plain {n} spp: relMSE {e:.3e} (denoised {target:.3e}, CI [{lo:.3e}, {hi:.3e}])")
+            if e <= min(lo, target):
+                break
+            step = n  # doubles the accumulated spp
+        log(f"config-5 plain sweep: {time.time() - t0:.1f}s")
+        # all-view plain frame time at 64 spp
+        plain = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
+        ctx.render_views(cams, 16, seed=13, out=plain, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.render_views(cams, 64, seed=17, out=plain, **kw)
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) * 1e3
+    spp_eq = crossing_spp(pts, target)
+    spp_ci = [crossing_spp(pts, float(hi)), crossing_spp(pts, float(lo))]  # larger error -> fewer spp
     npix = nv * w * h
-    fit = {}
-    if not eq["reached"]:
-        # measured plain spp is only a lower bound; the a/n + b fit of relMSE
-        # (b = noise floor of the finite-spp reference) gives an estimate
-        fit = {"plain_fit_a": eq.get("fit_a"), "plain_fit_b": eq.get("fit_b")}
-        if "spp_fit" in eq:
-            t_fit = t16 * eq["spp_fit"] / 16.0
-            fit.update({"plain_spp_fit": round(eq["spp_fit"], 1), "plain_ms_fit": round(t_fit, 1),
-                        "mpix_per_s_plain_fit": round(nv * w * h / (t_fit * 1e-3) / 1e6, 4),
-                        "speedup_fit": round(t_fit / t_den, 1)})
-    return {**fit, "views": nv, "width": w, "height": h, "medium_grid": args.c5_grid, "medium_sigma_max": 8.0,
-            "reference_spp": args.c5_ref_spp, "reference_views": args.c5_ref_views, "denoised_spp": 4,
-            "relmse_noisy_4spp": e_noisy, "relmse_denoised_4spp": e_den, "denoised_ms": round(t_den, 2),
-            "plain_spp_at_equal_relmse": round(eq["spp"], 1), "plain_ms_at_equal_relmse": round(t_plain, 2),
-            "plain_ms_16spp_all_views": round(t16, 2), "reached": eq["reached"],
-            "mpix_per_s_denoised": round(npix / (t_den * 1e-3) / 1e6, 3),
-            "mpix_per_s_plain": round(npix / (t_plain * 1e-3) / 1e6, 3),
-            "speedup": round(t_plain / t_den, 2),
-            "speedup_is_lower_bound": not eq["reached"],
-            "plain_points": [(p[0], round(p[1], 6), round(p[2], 2)) for p in eq["points"]]}
+    td = float(np.median(t_den))
+    out = {"views": nv, "width": w, "height": h, "medium_grid": args.c5_grid, "medium_sigma_max": 8.0,
+           "scored_views": nsv, "scored_window": [ww, wh], "reference_spp": [half, half], "seeds": int(e_den.size),
+           "estimator": "two independent references: mean((x-r1)(x-r2)/(rbar^2+1e-2))",
+           "denoised_spp": 4, "relmse_noisy_4spp": float(np.mean(e_noisy)),
+           "relmse_denoised_4spp": target, "relmse_denoised_ci95": [float(lo), float(hi)],
+           "denoised_ms": round(td, 2), "mpix_per_s_denoised": round(npix / (td * 1e-3) / 1e6, 3),
+           "plain_ms_64spp_all_views": round(t64, 2),
+           "plain_points": [(p[0], float(f"{p[1]:.4e}")) for p in pts],
+           "reached": spp_eq is not None}
+    if spp_eq is not None:
+        t_plain = t64 * spp_eq / 64.0
+        out.update({"plain_spp_at_equal_relmse": round(spp_eq, 1),
+                    "plain_spp_ci95": [None if v is None else round(v, 1) for v in spp_ci],
+                    "plain_ms_at_equal_relmse": round(t_plain, 1),
+                    "mpix_per_s_plain": round(npix / (t_plain * 1e-3) / 1e6, 4),
+                    "speedup": round(t_plain / td, 2),
+                    "speedup_ci95": [None if v is None else round(t64 * v / 64.0 / td, 2) for v in spp_ci]})
+    else:
+        out.update({"plain_spp_at_equal_relmse": None, "speedup": None})
+    return out
+
+
+def bench_denoiser(cpt, args):
+    """BF16 U-Net on MFMA (random weights, synthetic inputs): inference of a
+    batch of 8 views at 1920x1080 (config 5's chunk of views) and one Adam
+    training step on a batch of 8 128x128 crops (the shipped weights'
+    training shape).  Times: HIP events around each pass on the library's
+    stream (medians).  FLOPs: 2 * 9 * Cin * Cout per padded pixel per
+    convolution; a training step = 3 x forward - the first layer's data
+    gradient.  Roofline: dense BF16 MFMA peak 2.5 PFLOP/s; the pass time
+    includes its elementwise kernels (input preparation, pooling, output,
+    split sums, Adam), so `frac` is a lower bound for the convolution
+    kernels themselves; `mfma_busy` is the counters' MFMA-busy share of the
+    convolution kernels (committed summary, null if the kernels changed)."""
+    import torch
+    from cudapathtracing_amd.denoiser import Denoiser
+    B, H, W = 8, 1080, 1920
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    with cpt.Context(0) as ctx:
+        dn = Denoiser(ctx, B, H, W, seed=1)
+        x = torch.rand((B, H, W, 10), generator=g, device="cuda")
+        y = torch.empty((B, H, W, 3), device="cuda")
+        for _ in range(2):
+            dn(x, out=y)
+        t = []
+        for _ in range(10):
+            dn(x, out=y)
+            t.append(dn.stats()["fwd_ms"])
+        fl = dn.stats()["fwd_flops"]
+        inf_ms = float(np.median(t))
+        rng = np.random.default_rng(0)
+        Bt, S = 8, 128
+        inp = rng.random((Bt, S, S, 10)).astype(np.float32)
+        tgt = rng.random((Bt, S, S, 3)).astype(np.float32)
+        for _ in range(2):
+            dn.train_step(inp, tgt, 1e-4)
+        ts = []
+        for _ in range(8):
+            dn.train_step(inp, tgt, 1e-4)
+            ts.append(dn.stats()["step_ms"])
+        dn(inp)
+        f1 = dn.stats()["fwd_flops"]
+        L0 = dn.layers()[0]
+        f_train = 3 * f1 - 2.0 * Bt * S * S * L0["cin"] * L0["cout"] * 9
+        tr_ms = float(np.median(ts))
+        dn.close()
+    tf_inf = fl / (inf_ms * 1e-3) / 1e12
+    tf_tr = f_train / (tr_ms * 1e-3) / 1e12
+    peak = 2500.0
+    log(f"denoiser: inference {inf_ms:.3f} ms ({tf_inf:.1f} TFLOP/s), training step {tr_ms:.3f} ms ({tf_tr:.1f})")
+    return {"inference": {"batch": B, "width": W, "height": H, "ms": round(inf_ms, 3), "gflop": round(fl / 1e9, 1),
+                          "tflops": round(tf_inf, 1), "mpix_per_s": round(B * W * H / (inf_ms * 1e-3) / 1e6, 1)},
+            "training": {"batch": Bt, "crop": S, "ms": round(tr_ms, 3), "gflop": round(f_train / 1e9, 1),
+                         "tflops": round(tf_tr, 1), "frac": round(tf_tr / peak, 4)},
+            "roofline": {"bound": "mfma", "achieved": round(tf_inf, 1), "peak": peak, "unit": "TFLOP/s",
+                         "frac": round(tf_inf / peak, 4), "traffic": None, "kernel": "k_conv_fwd",
+                         "scope": "whole inference pass",
+                         "mfma_busy": profile_value("mfma_denoiser_convs.md", "mfma_busy_k_conv_fwd", "k_conv_fwd")}}
 
 
 def bench_lbvh(cpt, scenes, args):
@@ -312,24 +474,22 @@ def bench_lbvh(cpt, scenes, args):
            "mrays_per_s": round(args.rays / (tr_avg * 1e-3) / 1e6, 1),
            "hit_fraction": round(hits, 4), "nodes_per_ray": round(n_nodes / args.rays, 2),
            "prims_per_ray": round(n_prims / args.rays, 2),
-           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(tr_kernel),
-                        "kernel": tr_kernel}}
+           "roofline": roofline_hbm(tr_kernel, achieved, k_avg)}
     if not args.no_cpu:
         from oracle import oracle as O
         o = O.OracleScene(scene)
-        tb = time.time()
-        o.build(scene.bounds_min, scene.bounds_max)
-        cb = time.time() - tb
+        cb = median_time(lambda: o.build(scene.bounds_min, scene.bounds_max), args.cpu_build_reps)
         sample = rays_np[:args.cpu_rays]
         tc = time.time()
         o.trace_ref(sample)
         dt = time.time() - tc
         out["cpu_baseline"] = {"value": round(sample.shape[0] / dt / 1e6, 4), "unit": "Mrays/s",
-                               "cores": cores(), "kind": "port",
+                               "cores": cores(), "kind": "port", "cpu_model": cpu_model(),
                                "sample": f"FP64 oracle, reference traversal order (no t-culling), first "
-                                         f"{sample.shape[0]} rays; oracle LBVH build of the same "
-                                         f"{args.tris} triangles {cb * 1e3:.0f} ms"}
+                                         f"{sample.shape[0]} of the {args.rays} rays",
+                               "build_ms": round(cb * 1e3, 1),
+                               "build_timing": f"oracle LBVH build of the same {args.tris} triangles, median of "
+                                               f"{args.cpu_build_reps} after 1 warm-up"}
     ctx.close()
     return out
 
@@ -347,17 +507,69 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": n * 4}), flush=True)
 
 
+# Sources whose code decides each profiled kernel's traffic and issue
+# counts: a committed counter summary holds the digest of these files at the
+# time it was measured, and its figures are reported only while the digest
+# still matches (null otherwise: the kernel has changed since).
+KERNEL_SOURCES = {
+    "render": ["render.hip", "bsdf.hpp", "cpt_device.hpp", "internal.hpp", "bvh.hip", "cluster.hip"],
+    "trace": ["trace.hip", "cpt_device.hpp", "internal.hpp", "bvh.hip", "cluster.hip"],
+    "denoise": ["denoise.hip", "denoise_kernels.hpp"],
+}
+KERNEL_GROUP = {"k_extend_persist": "render", "k_extend": "render", "k_shadow": "render", "k_shade": "render",
+                "k_trace_persist": "trace", "k_trace": "trace", "k_conv_fwd": "denoise", "k_conv_bwd_w": "denoise"}
+
+
+def source_digest(group):
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES[group]:
+        h.update(f.encode())
+        h.update(open(os.path.join(ROOT, "cudapathtracing_amd", "csrc", f), "rb").read())
+    h.update(open(os.path.join(ROOT, "include", "cpt.h"), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def profile_value(fname, key, kernel):
+    """`key: value` from the committed summary profiles/<fname>, when its
+    source_digest line matches the current sources of `kernel`; else None."""
+    path = os.path.join(ROOT, "profiles", fname)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.startswith(("|", " ", "`", "#")):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("source_digest") != source_digest(KERNEL_GROUP[kernel]) or key not in vals:
+        return None
+    return float(vals[key])
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by a copy of known
-    size), or None when no summary for it exists."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("hbm_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+    size), or None when none exists for the current sources."""
+    return profile_value(f"traffic_{kernel}.md", "hbm_bytes_per_launch", kernel)
+
+
+def roofline_hbm(kernel, achieved, launch_ms):
+    """HBM roofline of a traversal kernel: algorithmic bytes / launch time
+    against 8 TB/s, beside what the counters say -- the measured HBM traffic
+    and its fraction of peak, and the share of issue slots spent on VALU
+    instructions (the actual limiter of the walks, whose records are served
+    largely from L2 and the MALL)."""
+    tr = measured_traffic(kernel)
+    issue = profile_value("valu_issue_traversal_kernels.md", f"valu_issue_{kernel}", kernel)
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": kernel,
+           "avg_launch_ms": round(launch_ms, 4)}
+    if tr is not None and launch_ms > 0:
+        out["traffic_frac"] = round(tr / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if issue is not None:
+        out["valu_issue_frac"] = issue
+        out["limiter"] = "valu_issue" if issue >= 0.6 else "memory"
+    return out
 
 
 def main():
@@ -368,7 +580,8 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--tris", type=int, default=10_000_000)
     ap.add_argument("--rays", type=int, default=1 << 25)
-    ap.add_argument("--cpu-rays", type=int, default=1 << 17)
+    ap.add_argument("--cpu-rays", type=int, default=1 << 20)
+    ap.add_argument("--cpu-build-reps", type=int, default=5)
     ap.add_argument("--cpu-window", type=int, default=320)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbvh", action="store_true")
@@ -388,9 +601,13 @@ def main():
     ap.add_argument("--c5-width", type=int, default=1920)
     ap.add_argument("--c5-height", type=int, default=1080)
     ap.add_argument("--c5-grid", type=int, default=256)
-    ap.add_argument("--c5-ref-spp", type=int, default=4096)
-    ap.add_argument("--c5-ref-views", type=int, default=1)
-    ap.add_argument("--only", choices=["render", "lbvh", "eq", "c5"], default=None,
+    ap.add_argument("--c5-ref-spp", type=int, default=4096, help="total over the two independent references")
+    ap.add_argument("--c5-score-views", type=int, default=8)
+    ap.add_argument("--c5-window", type=int, nargs=2, default=[640, 360], help="scored centred window (w h)")
+    ap.add_argument("--c5-seeds", type=int, default=8)
+    ap.add_argument("--c5-spp-max", type=int, default=65536)
+    ap.add_argument("--no-dn", action="store_true", help="skip the denoiser kernels leg")
+    ap.add_argument("--only", choices=["render", "lbvh", "eq", "c5", "dn"], default=None,
                     help="run one workload and print its record (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="copy this many bytes device-to-device 5 times and exit (counter calibration)")
@@ -414,13 +631,17 @@ def main():
     if args.only == "c5":
         print(json.dumps({"config5": bench_config5(cpt, L, scenes, args)}), flush=True)
         return
+    if args.only == "dn":
+        print(json.dumps({"denoiser": bench_denoiser(cpt, args)}), flush=True)
+        return
     if args.only == "render":
-        args.no_lbvh = args.no_eq = args.no_c5 = True
+        args.no_lbvh = args.no_eq = args.no_c5 = args.no_dn = True
 
     r, cpu, ntri = bench_render(cpt, L, scenes, args)
     lb = None if args.no_lbvh else bench_lbvh(cpt, scenes, args)
     eq = None if args.no_eq else bench_equal_quality(cpt, L, scenes, args)
     c5 = None if args.no_c5 else bench_config5(cpt, L, scenes, args)
+    dnr = None if args.no_dn else bench_denoiser(cpt, args)
     ext_kernel = "k_extend_persist" if (args.width == 4 and not args.no_persistent) else "k_extend"
     out = {
         "metric": METRIC,
@@ -452,14 +673,16 @@ def main():
         "scene_cluster_ms": round(r["cluster_ms"], 3),
         "tree_cost": {"lbvh": round(r["tree"]["lbvh_cost"], 2), "clustered": round(r["tree"]["cluster_cost"], 2),
                       "clustered_depth": r["tree"]["depth"]},
-        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(ext_kernel),
-                     "kernel": ext_kernel,
-                     "avg_launch_ms": round(r["ext_ms"] / max(1, r["bounces"]), 4)},
+        "roofline": roofline_hbm(ext_kernel, r["achieved"], r["ext_ms"] / max(1, r["bounces"])),
+        "shadow": {"rays_per_frame": r["shadow_rays"], "nodes_per_ray": round(r["shadow_nodes_per_ray"], 2),
+                   "prims_per_ray": round(r["shadow_prims_per_ray"], 2),
+                   "lane_utilisation": round(r["shadow_lane_utilisation"], 3),
+                   "roofline": roofline_hbm("k_shadow", r["shadow_achieved"], r["shadow_ms"] / max(1, r["bounces"]))},
         "cpu_baseline": cpu,
         "lbvh": lb,
         "equal_relmse": eq,
         "config5": c5,
+        "denoiser": dnr,
     }
     print(json.dumps(out), flush=True)
 

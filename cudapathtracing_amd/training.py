@@ -34,6 +34,36 @@ def relmse(x: np.ndarray, ref: np.ndarray) -> float:
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
 
 
+def relmse_two_ref(x: np.ndarray, r1: np.ndarray, r2: np.ndarray) -> float:
+    """relMSE of x with the references' own noise removed: r1 and r2 are
+    independent unbiased estimates of the same image, so
+    E[(x - r1)(x - r2)] = E[(x - mu)^2] (x independent of both); normalised
+    by the mean reference, as relmse.  An unbiased estimate: it can fall
+    below zero when x is much closer to mu than the references are."""
+    x = np.asarray(x, np.float64)
+    r1 = np.asarray(r1, np.float64)
+    r2 = np.asarray(r2, np.float64)
+    rb = 0.5 * (r1 + r2)
+    return float(np.mean((x - r1) * (x - r2) / (rb ** 2 + 1e-2)))
+
+
+def crossing_spp(points, target: float):
+    """spp at which a measured relMSE curve [(spp, relMSE), ...] (spp
+    increasing) first falls to `target`, interpolating log(relMSE) linearly
+    in log(spp) between the two bracketing measurements; None when no
+    measurement is at or below the target (no extrapolation)."""
+    for k, (n, e) in enumerate(points):
+        if e <= target:
+            if k == 0:
+                return float(n)
+            n0, e0 = points[k - 1]
+            if e <= 0 or e0 <= 0 or target <= 0:
+                return float(n)
+            f = (math.log(target) - math.log(e0)) / (math.log(e) - math.log(e0))
+            return math.exp(math.log(n0) + f * (math.log(n) - math.log(n0)))
+    return None
+
+
 def random_scene(seed: int, width: int = 512, height: int = 512, medium_prob: float = 0.5) -> scenes.SceneData:
     rng = np.random.default_rng(1_000_003 * seed + 17)
     s = scenes.SceneData()
