@@ -160,11 +160,16 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
                                "(the HIP library has no CPU fallback)")
-        # torch ships its own HIP runtime.  Loading torch first lets libcpt.so
-        # bind to that same runtime (same soname): two HIP runtimes in one
-        # process cannot both open the GPU, and torch tensors are passed to
-        # this library as device pointers.
-        import torch  # noqa: F401
+        # torch (optional) ships its own HIP runtime.  Where torch is
+        # installed it is loaded first, so that libcpt.so binds to that same
+        # runtime (same soname): two HIP runtimes in one process cannot both
+        # open the GPU, and torch tensors are passed to this library as
+        # device pointers.  Without torch, libcpt.so loads the system HIP
+        # runtime it was linked against.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)

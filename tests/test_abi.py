@@ -106,3 +106,23 @@ def test_generators_deterministic_and_sized():
     assert np.allclose(np.linalg.norm(rays[:, 4:7], axis=1), 1.0, atol=1e-6)
     assert rays[:, :3].min() >= -0.5 and rays[:, :3].max() <= 1.5
     assert abs(rays[:, 4:7].mean()) < 0.02
+
+
+def test_dropin_program_links_and_reports_missing_device():
+    """The C++ drop-in binding (cudapathtracing_amd/dropin/scene_dropin.cpp)
+    is built by the library Makefile, links libcpt.so by soname through its
+    run path, and without a GPU fails cleanly at cpt_create."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    binary = os.path.join(root, "cudapathtracing_amd", "dropin", "scene_dropin")
+    assert os.path.exists(binary), "build the library first (its Makefile builds the program)"
+    dyn = subprocess.run(["readelf", "-d", binary], capture_output=True, text=True, check=True).stdout
+    assert "[libcpt.so]" in dyn and "$ORIGIN/.." in dyn
+    ini = os.path.join(root, "tests", "golden", "scene_ini", "Scene1.ini")
+    r = subprocess.run([binary], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    import torch
+    if not torch.cuda.is_available():
+        r = subprocess.run([binary, ini, "1", "16", os.devnull, os.devnull], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "no HIP device" in r.stderr
