@@ -205,10 +205,12 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C
   }
 }
 
-// L1 loss on channels 0..2 of pred (ldp channels) against log(1+target);
-// dPred (bf16, kOutChan channels) = sign(pred - t) / count.
-__global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ target, int h, int w, int H,
-                     int W, int64_t n, float invCount, uint16_t *__restrict__ dPred, float *__restrict__ loss) {
+// L1 loss on channels 0..2 of the prediction (pred, ldp channels, + the
+// residual logRad) against log(1+target); dPred (bf16, kOutChan channels) =
+// sign(prediction - t) / count.
+__global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ logRad,
+                     const float *__restrict__ target, int h, int w, int H, int W, int64_t n, float invCount,
+                     uint16_t *__restrict__ dPred, float *__restrict__ loss) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // padded pixel
   float l = 0.f;
   if (i < n) {
@@ -221,7 +223,7 @@ __global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__res
       const float *t = target + (((int64_t)b * h + y) * w + x) * 3;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const float d = pred[i * ldp + c] - log1pf(fmaxf(t[c], 0.f));
+        const float d = (pred[i * ldp + c] + logRad[i * 3 + c]) - log1pf(fmaxf(t[c], 0.f));
         l += fabsf(d);
         g[c] = f2bf(d > 0.f ? invCount : (d < 0.f ? -invCount : 0.f));
       }
@@ -269,15 +271,16 @@ __global__ void k_pack_weights(const float *__restrict__ p, int cin, int cout, u
   wt[(((int64_t)ci * 3 + (2 - ky)) * 3 + (2 - kx)) * cout + co] = h;
 }
 
-__global__ void k_output(const float *__restrict__ pred, int ldp, int h, int w, int H, int W, int64_t n,
-                         float *__restrict__ rgb) {
+// rgb = max(expm1(pred + logRad), 0) over the unpadded pixels
+__global__ void k_output(const float *__restrict__ pred, int ldp, const float *__restrict__ logRad, int h, int w,
+                         int H, int W, int64_t n, float *__restrict__ rgb) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*h*w
   if (i >= n) return;
   const int64_t per = (int64_t)h * w;
   const int b = (int)(i / per), rem = (int)(i % per), y = rem / w, x = rem % w;
   const int64_t p = ((int64_t)b * H + y) * W + x;
 #pragma unroll
-  for (int c = 0; c < 3; c++) rgb[i * 3 + c] = fmaxf(expm1f(pred[p * ldp + c]), 0.f);
+  for (int c = 0; c < 3; c++) rgb[i * 3 + c] = fmaxf(expm1f(pred[p * ldp + c] + logRad[p * 3 + c]), 0.f);
 }
 
 __global__ void k_zero_f32(float *__restrict__ p, int64_t n) {
@@ -464,8 +467,7 @@ static int64_t fwdSplitFloats(int Cin, int Cout, int B, int H, int W) {
 // convolution is not split.
 static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *Xlow, int Clow, const uint16_t *w,
                           const float *bias, int Cout, int B, int H, int W, uint16_t *outB, float *outF, int ldo,
-                          int mode, const float *res, int ldr, const uint16_t *maskY = nullptr,
-                          float *splitPart = nullptr) {
+                          int mode, const uint16_t *maskY = nullptr, float *splitPart = nullptr) {
   FwdPlan P = fwdPlan(Cin, Cout, B, H, W);
   if (!splitPart || outF) {
     P.split = 1;
@@ -487,12 +489,17 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
     mk = nullptr;
     ldo = Cout;
   }
-  if (P.nt == 64)
-    hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, oF,
-                       ldo, md, res, ldr, mk, P.split, sliceStride, P.tpw);
-  else
-    hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, oF,
-                       ldo, md, res, ldr, mk, P.split, sliceStride, P.tpw);
+#define CPT_CONV_FWD(QB, F32)                                                                                    \
+  hipLaunchKernelGGL((k_conv_fwd<QB, F32>), grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, \
+                     oF, ldo, md, mk, P.split, sliceStride, P.tpw)
+  if (P.nt == 64) {
+    if (oF) CPT_CONV_FWD(4, true);
+    else CPT_CONV_FWD(4, false);
+  } else {
+    if (oF) CPT_CONV_FWD(2, true);
+    else CPT_CONV_FWD(2, false);
+  }
+#undef CPT_CONV_FWD
   if (P.split > 1) {
     const int64_t n = sliceStride / 8;
     hipLaunchKernelGGL(k_split_sum, dim3(nb256(n)), dim3(256), 0, s, (const float *)splitPart, P.split, sliceStride,
@@ -503,14 +510,14 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
 // in: the layer's input, or with low the skip half of the decoder's
 // concatenation [up2(low) | in], read in place.
 static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *low, int B, float *outF, uint16_t *outB,
-                   int mode, const float *res) {
+                   int mode) {
   const LayerDesc &Ld = dn->L[layer];
   hipStream_t s = dn->ctx->stream;
   const uint16_t *w = dn->wbf.as<uint16_t>() + dn->wbOff[layer];
   const float *bias = dn->params.as<float>() + dn->bOff[layer];
   const int H = in.h, W = in.w;
   launchConvFwd(s, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, w, bias, Ld.cout, B, H, W, outB, outF,
-                outF ? kPredChan : Ld.cout, mode, res, 3, nullptr, dn->part.as<float>());
+                outF ? kPredChan : Ld.cout, mode, nullptr, dn->part.as<float>());
   dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -583,7 +590,7 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *
     // backward data: forward kernel on dOut with the flipped transposed weights
     const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
     const int cout = Ld.cin;  // gradient channels
-    launchConvFwd(s, dOut, Ld.cout, nullptr, 0, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, nullptr, 3, inAct,
+    launchConvFwd(s, dOut, Ld.cout, nullptr, 0, wt, nullptr, cout, B, H, W, dIn, nullptr, cout, 0, inAct,
                   dn->part.as<float>());
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
@@ -602,17 +609,19 @@ static int forward(cpt_denoiser *dn, int B) {
       hipLaunchKernelGGL(k_pool2, dim3(nb256(n)), dim3(256), 0, s, src.p(), src.h, src.w, src.c, n, dst.p());
       cur = &dst;
     }
-    if ((rc = convFwd(dn, layer++, *cur, nullptr, B, nullptr, dn->e[l][0].p(), 1, nullptr))) return rc;
-    if ((rc = convFwd(dn, layer++, dn->e[l][0], nullptr, B, nullptr, dn->e[l][1].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, *cur, nullptr, B, nullptr, dn->e[l][0].p(), 1))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->e[l][0], nullptr, B, nullptr, dn->e[l][1].p(), 1))) return rc;
     cur = &dn->e[l][1];
   }
   for (int l = kLevels - 2; l >= 0; l--) {
     // [up2(cur) | e[l][1]], read in place by the conv
-    if ((rc = convFwd(dn, layer++, dn->e[l][1], cur, B, nullptr, dn->d[l][0].p(), 1, nullptr))) return rc;
-    if ((rc = convFwd(dn, layer++, dn->d[l][0], nullptr, B, nullptr, dn->d[l][1].p(), 1, nullptr))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->e[l][1], cur, B, nullptr, dn->d[l][0].p(), 1))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->d[l][0], nullptr, B, nullptr, dn->d[l][1].p(), 1))) return rc;
     cur = &dn->d[l][1];
   }
-  return convFwd(dn, layer, *cur, nullptr, B, dn->pred.as<float>(), nullptr, 2, dn->logRad.as<float>());
+  // the output layer: conv + bias only; the residual log(1 + noisy) is added
+  // by its consumers (k_output, k_l1)
+  return convFwd(dn, layer, *cur, nullptr, B, dn->pred.as<float>(), nullptr, 0);
 }
 
 // Every ReLU gradient mask is applied by the kernel that produces the
@@ -836,7 +845,7 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
   hipLaunchKernelGGL(k_zero_f32, dim3(nb256(dn->nParams)), dim3(256), 0, s, dn->grads.as<float>(), dn->nParams);
   hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)1);
   const int64_t np = (int64_t)b * dn->H * dn->W;
-  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dT, h, w, dn->H, dn->W, np,
+  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), dT, h, w, dn->H, dn->W, np,
                      1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->loss.as<float>());
   if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
   if (lr > 0.f) {
@@ -880,7 +889,7 @@ int cpt_denoise(cpt_denoiser *dn, const float *in10, float *outRgb, int32_t b, i
   if ((rc = prepare(dn, dIn, b, h, w))) return rc;
   dn->flopsFwd = 0;
   if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
-  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, h, w, dn->H, dn->W,
+  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), h, w, dn->H, dn->W,
                      (int64_t)npx, dOut);
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
@@ -935,7 +944,7 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
   else chk(hipMemsetAsync(db.p, 0, Cout * 4, s));
   if (mode == 0) {
     launchConvFwd(s, dx.as<uint16_t>(), Cin, nullptr, 0, dw.as<uint16_t>(), db.as<float>(), Cout, B, H, W, nullptr,
-                  dout.as<float>(), Cout, 0, nullptr, 3);
+                  dout.as<float>(), Cout, 0);
   } else if (mode == 1) {
     DevBuf part;
     if ((rc = ensure(ctx, part, (size_t)convBwdWSlices(Cin, Cout, B, H, W, nullptr) * (nw + Cout) * 4))) {
@@ -966,8 +975,7 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
     hipLaunchKernelGGL(k_pack_weights, dim3(nb256(nw)), dim3(256), 0, s, wf.as<float>(), Cin, Cout, dw.as<uint16_t>(),
                        wt.as<uint16_t>());
     launchConvFwd(s, dyb.as<uint16_t>(), Cout, nullptr, 0, wt.as<uint16_t>(), nullptr, Cin, B, H, W, nullptr,
-                  dout.as<float>(),
-                  Cin, 0, nullptr, 3);
+                  dout.as<float>(), Cin, 0);
     chk(hipStreamSynchronize(s));
     (void)hipFree(wf.p);
   }

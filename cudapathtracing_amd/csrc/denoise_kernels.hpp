@@ -44,7 +44,7 @@ constexpr int kConvThreads = 256;
 constexpr int kConvPix = 64;    // output pixels per workgroup (one row segment)
 constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
 
-// Y[b,y,x,co] = act(sum X[b,y+ky-1,x+kx-1,ci] W[co,ky,kx,ci] + bias[co]) (+res)
+// Y[b,y,x,co] = act(sum X[b,y+ky-1,x+kx-1,ci] W[co,ky,kx,ci] + bias[co])
 // X has Cin channels; with Xlow it is the decoder's concatenation
 // [up2(Xlow) | X] read in place: channels [0, Clow) from Xlow at half
 // resolution (pixel (y/2, x/2), Clow channels per pixel), channels
@@ -52,8 +52,10 @@ constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
 // into outBf (bf16, ldo channels per pixel) or outF (fp32, ldo channels per
 // pixel: channels co >= ldo are not written, so a 4-channel buffer takes the
 // first 4 of a wider layer).
-// mode bits: 1 = ReLU, 2 = add res[b,y,x,0..2] (fp32 residual, 3 channels;
-// outF only).  maskY (bf16 output only, ldo channels): the value is kept where
+// mode bit 1 = ReLU.  The bias is read once per workgroup, before the loop
+// (a global load inside it, at every tile end, also waited for the
+// prefetched next step).  maskY (bf16 output only, ldo channels): the value
+// is kept where
 // maskY > 0 and zeroed elsewhere -- the ReLU gradient mask of the layer whose
 // output maskY is.
 //
@@ -81,14 +83,18 @@ constexpr int kHaloH = kTileH + 2, kHaloW = kTileW + 2;
 constexpr int kHaloParts = kHaloH * kHaloW * 4;  // 16 B parts of a 32-channel halo
 constexpr int kHaloIters = (kHaloParts + kConvThreads - 1) / kConvThreads;
 
-template <int QB>
+// kF32: fp32 output (outF); else bf16 (outBf).  Separate instantiations: the
+// fp32 path's stores, if compiled into the same loop, made the compiler wait
+// for all outstanding global loads at the top of every step (their source
+// registers are reused by the step's LDS reads), which serialised the
+// prefetch with the MFMAs.
+template <int QB, bool kF32>
 __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
                                                                const uint16_t *__restrict__ Xlow, int Clow,
                                                                const uint16_t *__restrict__ Wt,
                                                                const float *__restrict__ bias, int Cout, int H,
                                                                int Wd, uint16_t *__restrict__ outBf,
                                                                float *__restrict__ outF, int ldo, int mode,
-                                                               const float *__restrict__ res, int ldr,
                                                                const uint16_t *__restrict__ maskY, int kSplit,
                                                                int64_t sliceStride, int tpw) {
   constexpr int NT = 16 * QB;
@@ -120,55 +126,79 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
   for (int nb = 0; nb < QB; nb++)
 #pragma unroll
     for (int m = 0; m < 4; m++) acc[nb][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // the bias of the 4 output channels this lane holds in each 16-channel block
+  float bsv[QB][4];
+#pragma unroll
+  for (int nb = 0; nb < QB; nb++) {
+    const int co = co0 + nb * 16 + (lane >> 4) * 4;
+    float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (bias && co < Cout) bv = *(const float4 *)(bias + co);
+    bsv[nb][0] = bv.x;
+    bsv[nb][1] = bv.y;
+    bsv[nb][2] = bv.z;
+    bsv[nb][3] = bv.w;
+  }
   uint4 px[kHaloIters], pw[kWIters];
+  // Loads are unconditional, from addresses clamped into the tensors; bit i
+  // of pxOk / pwOk says whether item i lies inside (else it is stored as
+  // zero).  Masked (branching) loads made the compiler wait for them right
+  // after issue, before this step's MFMAs; with the choice deferred to the
+  // LDS store their latency overlaps the MFMAs.
+  uint32_t pxOk = 0, pwOk = 0;
   // global -> registers for step k: the input halo of its tile and, unless
   // Cin is one chunk (the weights then stay in LDS), the chunk's weights
   const int cSkip = Cin - Clow;  // channels per pixel of X
   auto load = [&](int k) {
     const int y0 = (t0 + k / nChunks) * kTileH, c0 = (cBeg + k % nChunks) * 32;
     const bool fromLow = c0 < Clow;  // uniform: chunks never straddle Clow
+    pxOk = 0;
 #pragma unroll
     for (int i = 0; i < kHaloIters; i++) {
       const int q = tid + i * kConvThreads;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (q < kHaloParts) {
-        const int hp = q >> 2, part = q & 3;
-        const int yy = y0 - 1 + hp / kHaloW, xx = x0 - 1 + hp % kHaloW;
-        if (yy >= 0 && yy < H && xx >= 0 && xx < Wd) {
-          if (fromLow)
-            v = *(const uint4 *)(Xlow + (((int64_t)b * (H >> 1) + (yy >> 1)) * (Wd >> 1) + (xx >> 1)) * Clow + c0 +
-                                 part * 8);
-          else
-            v = *(const uint4 *)(X + (imgBase + (int64_t)yy * Wd + xx) * cSkip + (c0 - Clow) + part * 8);
-        }
-      }
-      px[i] = v;
+      const int qq = q < kHaloParts ? q : kHaloParts - 1;
+      const int hp = qq >> 2, part = qq & 3;
+      const int yy = y0 - 1 + hp / kHaloW, xx = x0 - 1 + hp % kHaloW;
+      const bool in = q < kHaloParts && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+      const int yc = yy < 0 ? 0 : (yy >= H ? H - 1 : yy), xc = xx < 0 ? 0 : (xx >= Wd ? Wd - 1 : xx);
+      const uint16_t *src =
+          fromLow ? Xlow + (((int64_t)b * (H >> 1) + (yc >> 1)) * (Wd >> 1) + (xc >> 1)) * Clow + c0 + part * 8
+                  : X + (imgBase + (int64_t)yc * Wd + xc) * cSkip + (c0 - Clow) + part * 8;
+      px[i] = *(const uint4 *)src;
+      pxOk |= (in ? 1u : 0u) << i;
     }
     if (k > 0 && nChunks == 1) return;
+    pwOk = 0;
 #pragma unroll
     for (int i = 0; i < kWIters; i++) {
       const int q = tid + i * kConvThreads;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (q < kWParts) {
-        const int part = q & 3, r = q >> 2;
-        const int t = r / NT, co = r % NT;
-        if (co0 + co < Cout) v = *(const uint4 *)(Wt + ((int64_t)(co0 + co) * 9 + t) * Cin + c0 + part * 8);
-      }
-      pw[i] = v;
+      const int qq = q < kWParts ? q : kWParts - 1;
+      const int part = qq & 3, r = qq >> 2;
+      const int t = r / NT, co = co0 + r % NT;
+      const int cc = co < Cout ? co : Cout - 1;
+      pw[i] = *(const uint4 *)(Wt + ((int64_t)cc * 9 + t) * Cin + c0 + part * 8);
+      pwOk |= (q < kWParts && co < Cout ? 1u : 0u) << i;
     }
   };
   // registers -> LDS
+  auto masked = [](uint4 v, uint32_t ok) {  // v, or zero where ok is 0 (no select of addresses)
+    const uint32_t m = 0u - (ok & 1u);
+    v.x &= m;
+    v.y &= m;
+    v.z &= m;
+    v.w &= m;
+    return v;
+  };
   auto store = [&](int k) {
 #pragma unroll
     for (int i = 0; i < kHaloIters; i++) {
       const int q = tid + i * kConvThreads;
-      if (q < kHaloParts) *(uint4 *)(sX + (q >> 2) * kLdsRow + (q & 3) * 8) = px[i];
+      if (q < kHaloParts) *(uint4 *)(sX + (q >> 2) * kLdsRow + (q & 3) * 8) = masked(px[i], pxOk >> i);
     }
     if (k > 0 && nChunks == 1) return;
 #pragma unroll
     for (int i = 0; i < kWIters; i++) {
       const int q = tid + i * kConvThreads;
-      if (q < kWParts) *(uint4 *)(sW + (q >> 2) * kLdsRow + (q & 3) * 8) = pw[i];
+      if (q < kWParts) *(uint4 *)(sW + (q >> 2) * kLdsRow + (q & 3) * 8) = masked(pw[i], pwOk >> i);
     }
   };
   // fp32 output: lane holds output channels 4*(lane/16) + r of pixel lane%16
@@ -178,14 +208,6 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
     for (int nb = 0; nb < QB; nb++) {
       const int co = co0 + nb * 16 + (lane >> 4) * 4;
       if (co >= Cout || co >= ldo) continue;  // channel counts are multiples of 4
-      float bs[4] = {0.f, 0.f, 0.f, 0.f};
-      if (bias) {
-        const float4 bv = *(const float4 *)(bias + co);
-        bs[0] = bv.x;
-        bs[1] = bv.y;
-        bs[2] = bv.z;
-        bs[3] = bv.w;
-      }
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const int x = x0 + m * 16 + (lane & 15);
@@ -194,9 +216,8 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
         float v[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          v[r] = acc[nb][m][r] + bs[r];
+          v[r] = acc[nb][m][r] + bsv[nb][r];
           if (mode & 1) v[r] = fmaxf(v[r], 0.f);
-          if ((mode & 2) && co + r < 3) v[r] += res[p * ldr + co + r];
         }
         *(float4 *)(outF + p * ldo + co) = make_float4(v[0], v[1], v[2], v[3]);
       }
@@ -211,21 +232,13 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
 #pragma unroll
       for (int nb2 = 0; nb2 < 2 && half * 2 + nb2 < QB; nb2++) {
         const int nb = half * 2 + nb2;
-        const int col = nb2 * 16 + (lane >> 4) * 4, co = co0 + nb * 16 + (lane >> 4) * 4;
-        float bs[4] = {0.f, 0.f, 0.f, 0.f};
-        if (bias && co < Cout) {
-          const float4 bv = *(const float4 *)(bias + co);
-          bs[0] = bv.x;
-          bs[1] = bv.y;
-          bs[2] = bv.z;
-          bs[3] = bv.w;
-        }
+        const int col = nb2 * 16 + (lane >> 4) * 4;
 #pragma unroll
         for (int m = 0; m < 4; m++) {
           float v[4];
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            v[r] = acc[nb][m][r] + bs[r];
+            v[r] = acc[nb][m][r] + bsv[nb][r];
             if (mode & 1) v[r] = fmaxf(v[r], 0.f);
           }
           uint2 pk;
@@ -286,7 +299,7 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
     if (k % nChunks == nChunks - 1) {
       // the tile is complete: write it and start the next one from zero
       const int y = (t0 + k / nChunks) * kTileH + wave;
-      if (outF) {
+      if constexpr (kF32) {
         epilogueF(y);
       } else {
         __syncthreads();  // every wave is done reading the halo buffer
