@@ -336,7 +336,10 @@ struct cpt_denoiser {
   cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
   cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
   cpt::DevBuf pred, logRad, dPred, loss, io;
-  cpt::DevBuf part;  // split-K partial sums of the weight gradients
+  cpt::DevBuf part;   // fp32 partial sums of convolutions split over Cin
+  cpt::DevBuf wpart;  // every layer's weight-gradient slices (reduced after the backward pass)
+  int64_t wpOff[cpt::kLayers] = {};  // float offset of layer k's slices in wpart
+  int wpSlices[cpt::kLayers] = {};
   int64_t step = 0;
   float lastFwdMs = 0.f, lastStepMs = 0.f;
   double flopsFwd = 0;
@@ -380,18 +383,21 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
   if ((rc = ensure(ctx, dn->pred, np * kPredChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
       (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)))
     return rc;
-  // weight-gradient partials: the largest layer's slices x weights
-  int64_t partFloats = 0;
+  // weight-gradient slices of every layer, one region each (reduced by one
+  // launch after the backward pass); split partials of the forward and
+  // backward-data convolutions of small layers (the largest layer's)
+  int64_t partFloats = 0, wpFloats = 0;
   for (int k = 0; k < kLayers; k++) {
     int lvl = k < 2 * kLevels ? k / 2 : (k < kLayers - 1 ? kLevels - 2 - (k - 2 * kLevels) / 2 : 0);
     const int64_t nW = (int64_t)dn->L[k].cout * 9 * dn->L[k].cin;
-    partFloats = std::max<int64_t>(partFloats, (int64_t)convBwdWSlices(dn->L[k].cin, dn->L[k].cout, B, H >> lvl,
-                                                                         W >> lvl, nullptr) * (nW + dn->L[k].cout));
-    // split forward and backward-data convolutions of small layers
+    dn->wpSlices[k] = convBwdWSlices(dn->L[k].cin, dn->L[k].cout, B, H >> lvl, W >> lvl, nullptr);
+    dn->wpOff[k] = wpFloats;
+    wpFloats += (int64_t)dn->wpSlices[k] * (nW + dn->L[k].cout);
     partFloats = std::max(partFloats, fwdSplitFloats(dn->L[k].cin, dn->L[k].cout, B, H >> lvl, W >> lvl));
     partFloats = std::max(partFloats, fwdSplitFloats(dn->L[k].cout, dn->L[k].cin, B, H >> lvl, W >> lvl));
   }
-  if ((rc = ensure(ctx, dn->part, partFloats * 4))) return rc;
+  if ((rc = ensure(ctx, dn->part, std::max<int64_t>(partFloats, 1) * 4)) || (rc = ensure(ctx, dn->wpart, wpFloats * 4)))
+    return rc;
   return CPT_OK;
 }
 
@@ -548,7 +554,8 @@ static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerS
   return (int)((steps + per - 1) / per);
 }
 
-// dW then db (contiguous: Cout*9*Cin + Cout floats) when withBias, else dW
+// dW then db (contiguous: Cout*9*Cin + Cout floats) when withBias, else dW;
+// with dW null only the slices are written (reduced later, k_reduce_layers)
 static void launchConvBwdW(hipStream_t s, const uint16_t *dY, int Cout, const uint16_t *X, int Cin, const uint16_t *Xlow,
                            int Clow, int B, int H, int W, float *part, float *dW, bool withBias) {
   int ks, wci, wco, per = 1;
@@ -567,6 +574,7 @@ static void launchConvBwdW(hipStream_t s, const uint16_t *dY, int Cout, const ui
   else
     hipLaunchKernelGGL((k_conv_bwd_w<1, 2, 2>), grid, dim3(kConvThreads), 0, s, dY, Cout, X, Cin, Xlow, Clow, B, H, W, per,
                        part);
+  if (!dW) return;
   const int64_t nW = (int64_t)Cout * 9 * Cin, nOut = withBias ? nW + Cout : nW;
   hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((nOut + 255) / 256)), dim3(256), 0, s, (const float *)part,
                      slices, nW + Cout, nOut, dW);
@@ -580,12 +588,9 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *
   const LayerDesc &Ld = dn->L[layer];
   hipStream_t s = dn->ctx->stream;
   const int H = in.h, W = in.w;
-  float *gw = dn->grads.as<float>() + dn->wOff[layer];
-  float *gb = dn->grads.as<float>() + dn->bOff[layer];
-  // weights then bias (gb == gw + Cout*9*Cin)
-  (void)gb;
+  // slices only: every layer's are reduced into the gradients after the pass
   launchConvBwdW(s, dOut, Ld.cout, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, B, H, W,
-                 dn->part.as<float>(), gw, true);
+                 dn->wpart.as<float>() + dn->wpOff[layer], nullptr, true);
   if (dIn) {
     // backward data: forward kernel on dOut with the flipped transposed weights
     const uint16_t *wt = dn->wtbf.as<uint16_t>() + dn->wbOff[layer];
@@ -668,6 +673,44 @@ static int backward(cpt_denoiser *dn, int B) {
     }
   }
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
+}
+
+// The weight gradients of every layer: grads[gOff[l] + i] = sum over layer
+// l's slices (in slice order) of part[pOff[l] + s * nS[l] + i], i < nS[l]
+// (weights then biases).
+struct ReduceTable {
+  int64_t pOff[kLayers], gOff[kLayers], nS[kLayers], end[kLayers + 1];
+  int slices[kLayers];
+};
+
+__global__ void k_reduce_layers(const float *__restrict__ part, ReduceTable T, float *__restrict__ grads) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= T.end[kLayers]) return;
+  int l = 0;
+  while (i >= T.end[l + 1]) l++;
+  const int64_t j = i - T.end[l], nS = T.nS[l];
+  const float *p = part + T.pOff[l] + j;
+  float s = 0.f;
+  const int ns = T.slices[l];
+#pragma unroll 8
+  for (int k = 0; k < ns; k++) s += p[(int64_t)k * nS];
+  grads[T.gOff[l] + j] = s;
+}
+
+static void reduceGrads(cpt_denoiser *dn) {
+  ReduceTable T;
+  int64_t n = 0;
+  for (int l = 0; l < kLayers; l++) {
+    T.pOff[l] = dn->wpOff[l];
+    T.gOff[l] = dn->wOff[l];
+    T.nS[l] = (int64_t)dn->L[l].cout * 9 * dn->L[l].cin + dn->L[l].cout;
+    T.slices[l] = dn->wpSlices[l];
+    T.end[l] = n;
+    n += T.nS[l];
+  }
+  T.end[kLayers] = n;
+  hipLaunchKernelGGL(k_reduce_layers, dim3(nb256(n)), dim3(256), 0, dn->ctx->stream, dn->wpart.as<float>(), T,
+                     dn->grads.as<float>());
 }
 
 static int packAll(cpt_denoiser *dn) {
@@ -781,7 +824,7 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
   // release path: errors have nothing to report to
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
-                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part};
+                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kLevels; l++)
@@ -842,12 +885,12 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
   if ((rc = prepare(dn, dIn, b, h, w))) return rc;
   dn->flopsFwd = 0;
   if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
-  hipLaunchKernelGGL(k_zero_f32, dim3(nb256(dn->nParams)), dim3(256), 0, s, dn->grads.as<float>(), dn->nParams);
   hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)1);
   const int64_t np = (int64_t)b * dn->H * dn->W;
   hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), dT, h, w, dn->H, dn->W, np,
                      1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->loss.as<float>());
   if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
+  reduceGrads(dn);  // every parameter's gradient (weights and biases of every layer)
   if (lr > 0.f) {
     dn->step++;
     const float b1 = 0.9f, b2 = 0.999f;
