@@ -368,8 +368,9 @@ def bench_config5(cpt, L, scenes, args):
 def bench_denoiser(cpt, args):
     """BF16 U-Net on MFMA (random weights, synthetic inputs): inference of a
     batch of 8 views at 1920x1080 (config 5's chunk of views) and one Adam
-    training step on a batch of 8 128x128 crops (the shipped weights'
-    training shape).  Times: HIP events around each pass on the library's
+    training step on a batch of 8 512x512 inputs (config 4), beside the step
+    on 8 128x128 crops (the shipped weights' training shape).  Times: HIP
+    events around each pass on the library's
     stream (medians).  FLOPs: 2 * 9 * Cin * Cout per padded pixel per
     convolution; a training step = 3 x forward - the first layer's data
     gradient.  Roofline: dense BF16 MFMA peak 2.5 PFLOP/s; the pass time
@@ -395,29 +396,34 @@ def bench_denoiser(cpt, args):
         fl = dn.stats()["fwd_flops"]
         inf_ms = float(np.median(t))
         rng = np.random.default_rng(0)
-        Bt, S = 8, 128
-        inp = rng.random((Bt, S, S, 10)).astype(np.float32)
-        tgt = rng.random((Bt, S, S, 3)).astype(np.float32)
-        for _ in range(2):
-            dn.train_step(inp, tgt, 1e-4)
-        ts = []
-        for _ in range(8):
-            dn.train_step(inp, tgt, 1e-4)
-            ts.append(dn.stats()["step_ms"])
-        dn(inp)
-        f1 = dn.stats()["fwd_flops"]
         L0 = dn.layers()[0]
-        f_train = 3 * f1 - 2.0 * Bt * S * S * L0["cin"] * L0["cout"] * 9
-        tr_ms = float(np.median(ts))
+        peak = 2500.0
+        train = {}
+        for S in (512, 128):
+            Bt = 8
+            inp = torch.rand((Bt, S, S, 10), generator=g, device="cuda")
+            tgt = torch.rand((Bt, S, S, 3), generator=g, device="cuda")
+            for _ in range(2):
+                dn.train_step(inp, tgt, 1e-4)
+            ts = []
+            for _ in range(8):
+                dn.train_step(inp, tgt, 1e-4)
+                ts.append(dn.stats()["step_ms"])
+            dn(inp)
+            f1 = dn.stats()["fwd_flops"]
+            f_train = 3 * f1 - 2.0 * Bt * S * S * L0["cin"] * L0["cout"] * 9
+            tr_ms = float(np.median(ts))
+            tf_tr = f_train / (tr_ms * 1e-3) / 1e12
+            train[S] = {"batch": Bt, "size": S, "ms": round(tr_ms, 3), "gflop": round(f_train / 1e9, 1),
+                        "tflops": round(tf_tr, 1), "frac": round(tf_tr / peak, 4)}
         dn.close()
     tf_inf = fl / (inf_ms * 1e-3) / 1e12
-    tf_tr = f_train / (tr_ms * 1e-3) / 1e12
-    peak = 2500.0
-    log(f"denoiser: inference {inf_ms:.3f} ms ({tf_inf:.1f} TFLOP/s), training step {tr_ms:.3f} ms ({tf_tr:.1f})")
+    log(f"denoiser: inference {inf_ms:.3f} ms ({tf_inf:.1f} TFLOP/s), training step 512^2 {train[512]['ms']} ms "
+        f"({train[512]['tflops']}), 128^2 {train[128]['ms']} ms ({train[128]['tflops']})")
     return {"inference": {"batch": B, "width": W, "height": H, "ms": round(inf_ms, 3), "gflop": round(fl / 1e9, 1),
                           "tflops": round(tf_inf, 1), "mpix_per_s": round(B * W * H / (inf_ms * 1e-3) / 1e6, 1)},
-            "training": {"batch": Bt, "crop": S, "ms": round(tr_ms, 3), "gflop": round(f_train / 1e9, 1),
-                         "tflops": round(tf_tr, 1), "frac": round(tf_tr / peak, 4)},
+            "training": {**train[512], "workload": "config 4: batch 8 of 512x512 inputs"},
+            "training_128_crops": train[128],
             "roofline": {"bound": "mfma", "achieved": round(tf_inf, 1), "peak": peak, "unit": "TFLOP/s",
                          "frac": round(tf_inf / peak, 4), "traffic": None, "kernel": "k_conv_fwd",
                          "scope": "whole inference pass",

@@ -81,11 +81,18 @@ class Denoiser:
             in10 = np.ascontiguousarray(in10, np.float32)
             target = np.ascontiguousarray(target, np.float32)
         loss = C.c_float()
-        g = np.empty(self.n_params, np.float32) if want_grads else None
+        g = None
+        if want_grads:
+            if dev:  # device inputs: the gradients land in a device buffer too
+                import torch
+                g = torch.empty(self.n_params, dtype=torch.float32, device=in10.device)
+            else:
+                g = np.empty(self.n_params, np.float32)
         self._check(self._lib.cpt_denoiser_train_step(self._h, _ptr(in10), _ptr(target), b, h, w, lr,
-                                                      C.byref(loss), g.ctypes.data if want_grads else None,
-                                                      L.CPT_DEVICE_PTRS if dev else 0))
-        return (float(loss.value), g) if want_grads else float(loss.value)
+                                                      C.byref(loss), _ptr(g), L.CPT_DEVICE_PTRS if dev else 0))
+        if not want_grads:
+            return float(loss.value)
+        return float(loss.value), (g.cpu().numpy() if dev else g)
 
     def __call__(self, in10, out=None):
         b, h, w = in10.shape[:3]

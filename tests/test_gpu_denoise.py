@@ -90,3 +90,20 @@ def test_training_reduces_loss(ctx):
     st = dn.stats()
     assert st["fwd_flops"] > 0 and st["fwd_ms"] > 0
     dn.close()
+
+
+def test_device_tensor_training_step_equals_host(ctx):
+    """A training step on device tensors (CPT_DEVICE_PTRS, gradients returned
+    in a device buffer) gives exactly the loss and gradients of the same step
+    on host arrays."""
+    import torch
+    print("[gpu] denoiser device-tensor training step", flush=True)
+    in10, target = batch(3, 2, 48, 48)
+    dn = Denoiser(ctx, 2, 48, 48, seed=9)
+    try:
+        lh, gh = dn.train_step(in10, target, 0.0, want_grads=True)
+        ld, gd = dn.train_step(torch.from_numpy(in10).cuda(), torch.from_numpy(target).cuda(), 0.0, want_grads=True)
+    finally:
+        dn.close()
+    assert lh == ld
+    assert np.array_equal(gh, gd)
