@@ -207,10 +207,13 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C
 
 // L1 loss on channels 0..2 of the prediction (pred, ldp channels, + the
 // residual logRad) against log(1+target); dPred (bf16, kOutChan channels) =
-// sign(prediction - t) / count.
+// sign(prediction - t) / count.  The loss is deterministic: each block writes
+// its partial sum to part[block]; the last block to finish (counted in
+// *counter, zero on entry) adds the partials in a fixed order into *loss.
 __global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ logRad,
                      const float *__restrict__ target, int h, int w, int H, int W, int64_t n, float invCount,
-                     uint16_t *__restrict__ dPred, float *__restrict__ loss) {
+                     uint16_t *__restrict__ dPred, float *__restrict__ part, unsigned *__restrict__ counter,
+                     float *__restrict__ loss) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // padded pixel
   float l = 0.f;
   if (i < n) {
@@ -233,13 +236,30 @@ __global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__res
   }
   // block reduction of the loss
   __shared__ float red[256];
+  __shared__ bool last;
   red[threadIdx.x] = l;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(loss, red[0] * invCount);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = red[0] * invCount;
+    __threadfence();  // the partial is visible before the count says so
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  float s = 0.f;
+  for (unsigned k = threadIdx.x; k < gridDim.x; k += blockDim.x) s += part[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = red[0];
 }
 
 // Adam on FP32 master weights; writes the BF16 weights (OHWI) and, for the
@@ -335,7 +355,7 @@ struct cpt_denoiser {
   // activations (and their gradients) per image batch
   cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
   cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
-  cpt::DevBuf pred, logRad, dPred, loss, io;
+  cpt::DevBuf pred, logRad, dPred, loss, lossPart, io;
   cpt::DevBuf part;   // fp32 partial sums of convolutions split over Cin
   cpt::DevBuf wpart;  // every layer's weight-gradient slices (reduced after the backward pass)
   int64_t wpOff[cpt::kLayers] = {};  // float offset of layer k's slices in wpart
@@ -381,7 +401,8 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
   }
   const int64_t np = (int64_t)B * H * W;
   if ((rc = ensure(ctx, dn->pred, np * kPredChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
-      (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)))
+      (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)) ||
+      (rc = ensure(ctx, dn->lossPart, (size_t)nb256(np) * 4)))
     return rc;
   // weight-gradient slices of every layer, one region each (reduced by one
   // launch after the backward pass); split partials of the forward and
@@ -544,10 +565,11 @@ static int convBwdWSlices(int Cin, int Cout, int B, int H, int W, int *stepsPerS
   bwdWShape(Cin, Cout, ks, wci, wco);
   const int tiles = ((Cin + 32 * wci - 1) / (32 * wci)) * ((Cout + 32 * wco - 1) / (32 * wco));
   const int64_t steps = (int64_t)B * H * ((W + 31) / 32);
-  // about 256 workgroups (a K-step is latency-bound, so more of them in
-  // parallel; each slice costs a write and a read of the layer's weights),
-  // each group of waves taking at least 4 K-steps
-  int64_t s = std::max<int64_t>(1, (256 + tiles - 1) / tiles);
+  // about 512 workgroups, two per CU, as many as the registers allow (a
+  // K-step is latency-bound, so as many as possible in parallel; each slice
+  // costs a write and a read of the layer's weights), each group of waves
+  // taking at least 4 K-steps
+  int64_t s = std::max<int64_t>(1, (512 + tiles - 1) / tiles);
   s = std::min<int64_t>(s, std::max<int64_t>(1, steps / (4 * ks)));
   const int64_t per = (steps + s - 1) / s;
   if (stepsPerSplit) *stepsPerSplit = (int)per;
@@ -824,7 +846,7 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
   // release path: errors have nothing to report to
   (void)hipStreamSynchronize(ctx->stream);
   DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
-                    &dn->loss, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
+                    &dn->loss, &dn->lossPart, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (int l = 0; l < kLevels; l++)
@@ -885,10 +907,12 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
   if ((rc = prepare(dn, dIn, b, h, w))) return rc;
   dn->flopsFwd = 0;
   if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
-  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)1);
+  // loss and the block counter of k_l1 (loss[1], as bits)
+  hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)2);
   const int64_t np = (int64_t)b * dn->H * dn->W;
   hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), dT, h, w, dn->H, dn->W, np,
-                     1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->loss.as<float>());
+                     1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->lossPart.as<float>(),
+                     (unsigned *)dn->loss.as<float>() + 1, dn->loss.as<float>());
   if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
   reduceGrads(dn);  // every parameter's gradient (weights and biases of every layer)
   if (lr > 0.f) {
