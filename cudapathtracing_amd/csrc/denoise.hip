@@ -52,7 +52,7 @@ static void layerTable(LayerDesc L[kLayers]) {
 // ---- elementwise kernels ----------------------------------------------------
 
 __global__ void k_prep_input(const float *__restrict__ in10, int h, int w, int H, int W, int64_t n,
-                             uint16_t *__restrict__ X, float *__restrict__ logRad) {
+                             uint16_t *__restrict__ X) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // padded pixel index
   if (i >= n) return;
   const int64_t per = (int64_t)H * W;
@@ -79,9 +79,6 @@ __global__ void k_prep_input(const float *__restrict__ in10, int h, int w, int H
     for (int j = 0; j < 8; j++) ph[j] = f2bf(v[c + j]);
     *(uint4 *)(d + c) = pk;
   }
-  logRad[i * 3 + 0] = v[0];
-  logRad[i * 3 + 1] = v[1];
-  logRad[i * 3 + 2] = v[2];
 }
 
 // The elementwise kernels below move 8 channels (16 B) per thread; every
@@ -206,11 +203,12 @@ __global__ void k_up_concat_bwd(const uint16_t *__restrict__ dOut, int C1, int C
 }
 
 // L1 loss on channels 0..2 of the prediction (pred, ldp channels, + the
-// residual logRad) against log(1+target); dPred (bf16, kOutChan channels) =
+// residual log(1 + max(noisy radiance, 0)) from the input in10) against
+// log(1+target); dPred (bf16, kOutChan channels) =
 // sign(prediction - t) / count.  The loss is deterministic: each block writes
 // its partial sum to part[block]; the last block to finish (counted in
 // *counter, zero on entry) adds the partials in a fixed order into *loss.
-__global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ logRad,
+__global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__restrict__ in10,
                      const float *__restrict__ target, int h, int w, int H, int W, int64_t n, float invCount,
                      uint16_t *__restrict__ dPred, float *__restrict__ part, unsigned *__restrict__ counter,
                      float *__restrict__ loss) {
@@ -223,10 +221,11 @@ __global__ void k_l1(const float *__restrict__ pred, int ldp, const float *__res
 #pragma unroll
     for (int c = 0; c < kOutChan; c++) g[c] = 0;
     if (y < h && x < w) {
-      const float *t = target + (((int64_t)b * h + y) * w + x) * 3;
+      const int64_t u = ((int64_t)b * h + y) * w + x;  // unpadded pixel
+      const float *t = target + u * 3;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const float d = (pred[i * ldp + c] + logRad[i * 3 + c]) - log1pf(fmaxf(t[c], 0.f));
+        const float d = (pred[i * ldp + c] + log1pf(fmaxf(in10[u * 10 + c], 0.f))) - log1pf(fmaxf(t[c], 0.f));
         l += fabsf(d);
         g[c] = f2bf(d > 0.f ? invCount : (d < 0.f ? -invCount : 0.f));
       }
@@ -291,8 +290,9 @@ __global__ void k_pack_weights(const float *__restrict__ p, int cin, int cout, u
   wt[(((int64_t)ci * 3 + (2 - ky)) * 3 + (2 - kx)) * cout + co] = h;
 }
 
-// rgb = max(expm1(pred + logRad), 0) over the unpadded pixels
-__global__ void k_output(const float *__restrict__ pred, int ldp, const float *__restrict__ logRad, int h, int w,
+// rgb = max(expm1(pred + log(1 + max(noisy, 0))), 0) over the unpadded
+// pixels (noisy: the radiance channels of the input in10)
+__global__ void k_output(const float *__restrict__ pred, int ldp, const float *__restrict__ in10, int h, int w,
                          int H, int W, int64_t n, float *__restrict__ rgb) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over B*h*w
   if (i >= n) return;
@@ -300,7 +300,8 @@ __global__ void k_output(const float *__restrict__ pred, int ldp, const float *_
   const int b = (int)(i / per), rem = (int)(i % per), y = rem / w, x = rem % w;
   const int64_t p = ((int64_t)b * H + y) * W + x;
 #pragma unroll
-  for (int c = 0; c < 3; c++) rgb[i * 3 + c] = fmaxf(expm1f(pred[p * ldp + c] + logRad[p * 3 + c]), 0.f);
+  for (int c = 0; c < 3; c++)
+    rgb[i * 3 + c] = fmaxf(expm1f(pred[p * ldp + c] + log1pf(fmaxf(in10[i * 10 + c], 0.f))), 0.f);
 }
 
 __global__ void k_zero_f32(float *__restrict__ p, int64_t n) {
@@ -355,7 +356,7 @@ struct cpt_denoiser {
   // activations (and their gradients) per image batch
   cpt::Tensor x0, e[cpt::kLevels][2], pool[cpt::kLevels - 1], d[cpt::kLevels - 1][2];
   cpt::Tensor ge[cpt::kLevels][2], gpool[cpt::kLevels - 1], gcat[cpt::kLevels - 1], gd[cpt::kLevels - 1][2], gx0;
-  cpt::DevBuf pred, logRad, dPred, loss, lossPart, io;
+  cpt::DevBuf pred, dPred, loss, lossPart, io;
   cpt::DevBuf part;   // fp32 partial sums of convolutions split over Cin
   cpt::DevBuf wpart;  // every layer's weight-gradient slices (reduced after the backward pass)
   int64_t wpOff[cpt::kLayers] = {};  // float offset of layer k's slices in wpart
@@ -400,7 +401,7 @@ static int allocActs(cpt_denoiser *dn, int B, int H, int W) {
     }
   }
   const int64_t np = (int64_t)B * H * W;
-  if ((rc = ensure(ctx, dn->pred, np * kPredChan * 4)) || (rc = ensure(ctx, dn->logRad, np * 12)) ||
+  if ((rc = ensure(ctx, dn->pred, np * kPredChan * 4)) ||
       (rc = ensure(ctx, dn->dPred, np * kOutChan * 2)) || (rc = ensure(ctx, dn->loss, 64)) ||
       (rc = ensure(ctx, dn->lossPart, (size_t)nb256(np) * 4)))
     return rc;
@@ -758,8 +759,7 @@ static int prepare(cpt_denoiser *dn, const float *in10Dev, int b, int h, int w) 
   dn->H = H;
   dn->W = W;
   const int64_t n = (int64_t)b * H * W;
-  hipLaunchKernelGGL(k_prep_input, dim3(nb256(n)), dim3(256), 0, dn->ctx->stream, in10Dev, h, w, H, W, n, dn->x0.p(),
-                     dn->logRad.as<float>());
+  hipLaunchKernelGGL(k_prep_input, dim3(nb256(n)), dim3(256), 0, dn->ctx->stream, in10Dev, h, w, H, W, n, dn->x0.p());
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
 
@@ -845,7 +845,7 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
   cpt_ctx *ctx = dn->ctx;
   // release path: errors have nothing to report to
   (void)hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->logRad, &dn->dPred,
+  DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->dPred,
                     &dn->loss, &dn->lossPart, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
   for (DevBuf *b : bufs)
     if (b->p) (void)hipFree(b->p);
@@ -910,7 +910,7 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
   // loss and the block counter of k_l1 (loss[1], as bits)
   hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(256), 0, s, dn->loss.as<float>(), (int64_t)2);
   const int64_t np = (int64_t)b * dn->H * dn->W;
-  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), dT, h, w, dn->H, dn->W, np,
+  hipLaunchKernelGGL(k_l1, dim3(nb256(np)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dIn, dT, h, w, dn->H, dn->W, np,
                      1.0f / (float)(npx * 3), dn->dPred.as<uint16_t>(), dn->lossPart.as<float>(),
                      (unsigned *)dn->loss.as<float>() + 1, dn->loss.as<float>());
   if ((rc = backward(dn, b))) return fail(ctx, rc, "denoiser backward", "launch");
@@ -956,7 +956,7 @@ int cpt_denoise(cpt_denoiser *dn, const float *in10, float *outRgb, int32_t b, i
   if ((rc = prepare(dn, dIn, b, h, w))) return rc;
   dn->flopsFwd = 0;
   if ((rc = forward(dn, b))) return fail(ctx, rc, "denoiser forward", "launch");
-  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dn->logRad.as<float>(), h, w, dn->H, dn->W,
+  hipLaunchKernelGGL(k_output, dim3(nb256(npx)), dim3(256), 0, s, dn->pred.as<float>(), kPredChan, dIn, h, w, dn->H, dn->W,
                      (int64_t)npx, dOut);
   CPT_TRY_HIP(ctx, hipGetLastError());
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], s));
