@@ -522,7 +522,8 @@ KERNEL_SOURCES = {
     "trace": ["trace.hip", "cpt_device.hpp", "internal.hpp", "bvh.hip", "cluster.hip"],
     "denoise": ["denoise.hip", "denoise_kernels.hpp"],
 }
-KERNEL_GROUP = {"k_extend_persist": "render", "k_extend": "render", "k_shadow": "render", "k_shade": "render",
+KERNEL_GROUP = {"k_extend_persist": "render", "k_extend": "render", "k_shadow": "render", "k_shadow_persist": "render",
+                "k_shade": "render",
                 "k_trace_persist": "trace", "k_trace": "trace", "k_conv_fwd": "denoise", "k_conv_bwd_w": "denoise"}
 
 
@@ -649,6 +650,7 @@ def main():
     c5 = None if args.no_c5 else bench_config5(cpt, L, scenes, args)
     dnr = None if args.no_dn else bench_denoiser(cpt, args)
     ext_kernel = "k_extend_persist" if (args.width == 4 and not args.no_persistent) else "k_extend"
+    sh_kernel = "k_shadow_persist" if (args.width == 4 and not args.no_persistent) else "k_shadow"
     out = {
         "metric": METRIC,
         "value": round(r["mrays"], 1),
@@ -683,7 +685,7 @@ def main():
         "shadow": {"rays_per_frame": r["shadow_rays"], "nodes_per_ray": round(r["shadow_nodes_per_ray"], 2),
                    "prims_per_ray": round(r["shadow_prims_per_ray"], 2),
                    "lane_utilisation": round(r["shadow_lane_utilisation"], 3),
-                   "roofline": roofline_hbm("k_shadow", r["shadow_achieved"], r["shadow_ms"] / max(1, r["bounces"]))},
+                   "roofline": roofline_hbm(sh_kernel, r["shadow_achieved"], r["shadow_ms"] / max(1, r["bounces"]))},
         "cpu_baseline": cpu,
         "lbvh": lb,
         "equal_relmse": eq,

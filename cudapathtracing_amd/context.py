@@ -82,6 +82,12 @@ class Context:
         """Persistent lane-refilling traversal kernels (default on)."""
         self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))
 
+    def set_persistent_shadow(self, on: bool):
+        """cpt_render's four-wide occlusion walks as persistent lane-refilling
+        kernels, in append order (default on); off: one lane per shadow ray,
+        ordered by origin cell for long queues."""
+        self._check(self._lib.cpt_set_option(self._h, 8, 1 if on else 0))
+
     def set_queue_sort_min(self, n: int):
         """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never, the default)."""
         self._check(self._lib.cpt_set_option(self._h, 5, n))

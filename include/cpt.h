@@ -149,6 +149,12 @@ const char *cpt_version(void);
  * traversals as persistent kernels whose lanes take a new ray when theirs is
  * finished; 0 launches one lane per ray.  Results are identical. */
 #define CPT_OPT_PERSISTENT 3
+/* CPT_OPT_PERSISTENT_SHADOW: 1 (default) runs cpt_render's four-wide
+ * occlusion walks as persistent lane-refilling kernels too (with
+ * CPT_OPT_PERSISTENT 1), in append order (CPT_OPT_SHADOW_SORT_MIN then
+ * applies only to the one-lane-per-ray kernel); 0 launches one lane per
+ * shadow ray.  Results are identical. */
+#define CPT_OPT_PERSISTENT_SHADOW 8
 /* CPT_OPT_QUANTIZED: 1 (default) makes four-wide walks read 64 B records
  * whose child boxes are 8-bit planes on a power-of-two grid, rounded outward
  * (each contains the FP32 box); 0 reads the 128 B FP32 records.  Hits are

@@ -87,17 +87,20 @@ def test_sky_over_floor_exact(ctx):
     assert st.rays > 0 and st.bounces >= 2
 
 
-@pytest.mark.parametrize("width,quant,qsort,persist", [(4, False, 0, 1), (4, False, 1, 1), (4, True, 1, 1),
-                                                       (4, False, 1, 0), (2, False, 0, 1), (4, True, 0, 0),
-                                                       (8, False, 0, 1)])
-def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
+@pytest.mark.parametrize("width,quant,qsort,persist,pshadow", [(4, False, 0, 1, 1), (4, False, 1, 1, 1),
+                                                               (4, True, 1, 1, 1), (4, False, 1, 0, 1),
+                                                               (2, False, 0, 1, 1), (4, True, 0, 0, 1),
+                                                               (8, False, 0, 1, 1), (4, True, 1, 1, 0),
+                                                               (4, True, 0, 1, 0)])
+def test_cornell_lambert_paths(ctx, width, quant, qsort, persist, pshadow):
     print(f"[gpu] render cornell paths (width {width}, quantized {quant}, queue sort {qsort}, "
-          f"persistent {persist})", flush=True)
+          f"persistent {persist}, persistent shadows {pshadow})", flush=True)
     s = scenes.cornell_icosphere()
     d = scenes.default_camera(48, 48)
     ctx.set_traversal_width(width)
     ctx.set_quantized(quant)
     ctx.set_persistent(bool(persist))
+    ctx.set_persistent_shadow(bool(pshadow))
     if qsort:  # bounce and shadow queues both traced in key order
         ctx.set_queue_sort_min(1)
         ctx.set_shadow_sort_min(1)
@@ -107,8 +110,34 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist):
         ctx.set_traversal_width(4)
         ctx.set_quantized(True)
         ctx.set_persistent(True)
+        ctx.set_persistent_shadow(True)
         ctx.set_queue_sort_min(0)
         ctx.set_shadow_sort_min(1 << 20)
+
+
+def test_occlusion_kernels_agree_bit_for_bit(ctx):
+    """The persistent occlusion walk (append order) and the one-lane-per-ray
+    kernel (append order, and origin-cell order) decide every shadow ray the
+    same way and add each unoccluded contribution to its path's own slot: one
+    light, so no atomic adds, and the images are equal bit for bit."""
+    print("[gpu] occlusion kernels bit for bit", flush=True)
+    s = scenes.displaced_icosphere(40, 40, level=5)
+    ctx.upload(s)
+    ctx.build(s.bounds_min, s.bounds_max)
+    kw = dict(seed=5, profile=L.PROFILE_CONFIG)
+    try:
+        ctx.set_persistent_shadow(True)
+        a, _, sa = ctx.render(s.camera, 8, **kw)
+        ctx.set_persistent_shadow(False)
+        ctx.set_shadow_sort_min(0)
+        b, _, _ = ctx.render(s.camera, 8, **kw)
+        ctx.set_shadow_sort_min(1)
+        c, _, _ = ctx.render(s.camera, 8, **kw)
+    finally:
+        ctx.set_persistent_shadow(True)
+        ctx.set_shadow_sort_min(1 << 20)
+    assert sa.rays > 0 and a.mean() > 0
+    assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
 @pytest.mark.parametrize("ppb", [37 * 29, 500])
