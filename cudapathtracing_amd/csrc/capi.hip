@@ -112,7 +112,12 @@ int cpt_create(cpt_ctx **out, int dev) {
   if (!ctx) return CPT_ERR_OOM;
   ctx->device = dev;
   hipError_t e = hipSetDevice(dev);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->ownStream, hipStreamNonBlocking);
+  // a blocking stream: ordered with the device's null stream both ways, so
+  // device buffers written by null-stream work (PyTorch's default stream
+  // has handle 0, which cpt_set_stream maps to this stream) are complete
+  // before the library reads them, and the library's outputs before
+  // null-stream work reads them
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->ownStream, hipStreamDefault);
   for (int i = 0; i < 18 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -152,6 +157,9 @@ void cpt_destroy(cpt_ctx *ctx) {
   // release path: errors have nothing to report to
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  // denoisers destroyed after their context only release their handles
+  for (cpt_denoiser *dn : ctx->denoisers) cpt::denoiserRelease(dn);
+  ctx->denoisers.clear();
   releaseAll(ctx);
   for (int i = 0; i < 18; i++)
     if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);

@@ -231,6 +231,16 @@ struct TraceCounters {
   int nodes = 0, prims = 0;
 };
 
+// Whether a hit at t (t <= tbest, as the hit tests return) from leaf li
+// replaces the best hit so far (leaf `hit`, -1 for none): when nearer or, at
+// an equal distance, from a lower leaf index (no hit counts as the highest),
+// so that a closest hit depends on the ray alone, not on the order in which
+// its leaves are tested (which the persistent walks' queued leaf tests vary
+// with the other lanes of the wave).
+__device__ __forceinline__ bool takesHit(float t, float tbest, int li, int hit) {
+  return t < tbest || (uint32_t)li < (uint32_t)hit;
+}
+
 // kAny: return at the first primitive hit in [tmin, tbest] (occlusion).
 // Closest hit over the LBVH.  Starts at the root's children (the root box
 // is not tested, as in traverseIterative, BVH.cuh:293-301), tests both child
@@ -282,7 +292,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
     if (hA && ca < 0) {
       float t;
       if (kCount) cnt.prims++;
-      if (hitLeaf(leaves, ~ca, o, d, tmin, tbest, t)) {
+      if (hitLeaf(leaves, ~ca, o, d, tmin, tbest, t) && (kAny || takesHit(t, tbest, ~ca, hit))) {
         tbest = t;
         hit = ~ca;
         if (kAny) return hit;
@@ -292,7 +302,7 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
     if (hB && cb < 0) {
       float t;
       if (kCount) cnt.prims++;
-      if (hitLeaf(leaves, ~cb, o, d, tmin, tbest, t)) {
+      if (hitLeaf(leaves, ~cb, o, d, tmin, tbest, t) && (kAny || takesHit(t, tbest, ~cb, hit))) {
         tbest = t;
         hit = ~cb;
         if (kAny) return hit;
@@ -569,7 +579,7 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
       const int li = ~(k == 0 ? ref[0] : k == 1 ? ref[1] : k == 2 ? ref[2] : ref[3]);
       float t;
       if (kCount) cnt.prims++;
-      if (hitLeaf(leaves, li, o, d, tmin, tbest, t)) {
+      if (hitLeaf(leaves, li, o, d, tmin, tbest, t) && (kAny || takesHit(t, tbest, li, hit))) {
         tbest = t;
         hit = li;
         if (kAny) return hit;
@@ -662,7 +672,7 @@ __device__ __forceinline__ int traceClosest8(const float4 *__restrict__ nodes8,
       for (int j = 1; j < 8; j++) r = k == j ? ref[j] : r;
       float t;
       if (kCount) cnt.prims++;
-      if (hitLeaf(leaves, ~r, o, d, tmin, tbest, t)) {
+      if (hitLeaf(leaves, ~r, o, d, tmin, tbest, t) && (kAny || takesHit(t, tbest, ~r, hit))) {
         tbest = t;
         hit = ~r;
         if (kAny) return hit;
@@ -769,14 +779,14 @@ __device__ __forceinline__ bool walk4Leaf(Walk4 &w, const float4 *__restrict__ l
   if (kCount) cnt.prims++;
   if (kPayload) {
     const float4 q0 = leaves[3 * li + 0], q1 = leaves[3 * li + 1], q2 = leaves[3 * li + 2];
-    if (hitRecord(q0, q1, q2, w.o, w.d, w.tmin, w.tbest, t)) {
+    if (hitRecord(q0, q1, q2, w.o, w.d, w.tmin, w.tbest, t) && (kAny || takesHit(t, w.tbest, li, w.hit))) {
       w.tbest = t;
       w.hit = li;
       const int type = __float_as_int(q1.w);
       w.tm = (__float_as_int(q2.w) << 3) | type;
       w.nrm = outwardNormal(q0, q1, q2, type, w.o + w.d * t);
     }
-  } else if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t)) {
+  } else if (hitLeaf(leaves, li, w.o, w.d, w.tmin, w.tbest, t) && (kAny || takesHit(t, w.tbest, li, w.hit))) {
     w.tbest = t;
     w.hit = li;
     if (kAny) return true;

@@ -14,6 +14,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "cpt_device.hpp"
@@ -365,6 +366,32 @@ struct cpt_denoiser {
   float lastFwdMs = 0.f, lastStepMs = 0.f;
   double flopsFwd = 0;
 };
+
+namespace cpt {
+
+void denoiserRelease(cpt_denoiser *dn) {
+  cpt_ctx *ctx = dn->ctx;
+  if (!ctx) return;
+  // release path: errors have nothing to report to
+  (void)hipStreamSynchronize(ctx->stream);
+  DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->dPred,
+                    &dn->loss, &dn->lossPart, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
+  for (DevBuf *b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (int l = 0; l < kLevels; l++)
+    for (int k = 0; k < 2; k++) {
+      if (dn->e[l][k].buf.p) (void)hipFree(dn->e[l][k].buf.p);
+      if (dn->ge[l][k].buf.p) (void)hipFree(dn->ge[l][k].buf.p);
+    }
+  for (int l = 0; l < kLevels - 1; l++) {
+    Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
+    for (Tensor *t : ts)
+      if (t->buf.p) (void)hipFree(t->buf.p);
+  }
+  dn->ctx = nullptr;
+}
+
+}  // namespace cpt
 
 namespace cpt {
 
@@ -836,28 +863,17 @@ int cpt_denoiser_create(cpt_ctx *ctx, int32_t maxB, int32_t maxH, int32_t maxW, 
     cpt_denoiser_destroy(dn);
     return fail(ctx, CPT_ERR_HIP, "cpt_denoiser_create", "upload failed");
   }
+  ctx->denoisers.push_back(dn);
   *out = dn;
   return CPT_OK;
 }
 
 void cpt_denoiser_destroy(cpt_denoiser *dn) {
   if (!dn) return;
-  cpt_ctx *ctx = dn->ctx;
-  // release path: errors have nothing to report to
-  (void)hipStreamSynchronize(ctx->stream);
-  DevBuf *bufs[] = {&dn->params, &dn->grads, &dn->m, &dn->v, &dn->wbf, &dn->wtbf, &dn->pred, &dn->dPred,
-                    &dn->loss, &dn->lossPart, &dn->io, &dn->x0.buf, &dn->gx0.buf, &dn->part, &dn->wpart};
-  for (DevBuf *b : bufs)
-    if (b->p) (void)hipFree(b->p);
-  for (int l = 0; l < kLevels; l++)
-    for (int k = 0; k < 2; k++) {
-      if (dn->e[l][k].buf.p) (void)hipFree(dn->e[l][k].buf.p);
-      if (dn->ge[l][k].buf.p) (void)hipFree(dn->ge[l][k].buf.p);
-    }
-  for (int l = 0; l < kLevels - 1; l++) {
-    Tensor *ts[] = {&dn->pool[l], &dn->gpool[l], &dn->gcat[l], &dn->d[l][0], &dn->d[l][1], &dn->gd[l][0], &dn->gd[l][1]};
-    for (Tensor *t : ts)
-      if (t->buf.p) (void)hipFree(t->buf.p);
+  if (cpt_ctx *ctx = dn->ctx) {
+    auto &v = ctx->denoisers;
+    v.erase(std::remove(v.begin(), v.end(), dn), v.end());
+    cpt::denoiserRelease(dn);
   }
   delete dn;
 }
@@ -865,14 +881,14 @@ void cpt_denoiser_destroy(cpt_denoiser *dn) {
 int64_t cpt_denoiser_num_params(const cpt_denoiser *dn) { return dn ? dn->nParams : -1; }
 
 int cpt_denoiser_get_params(cpt_denoiser *dn, float *out) {
-  if (!dn || !out) return CPT_ERR_ARG;
+  if (!dn || !out || !dn->ctx) return CPT_ERR_ARG;
   CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(out, dn->params.p, dn->nParams * 4, hipMemcpyDeviceToHost, dn->ctx->stream));
   CPT_TRY_HIP(dn->ctx, hipStreamSynchronize(dn->ctx->stream));
   return CPT_OK;
 }
 
 int cpt_denoiser_set_params(cpt_denoiser *dn, const float *in) {
-  if (!dn || !in) return CPT_ERR_ARG;
+  if (!dn || !in || !dn->ctx) return CPT_ERR_ARG;
   CPT_TRY_HIP(dn->ctx, hipMemcpyAsync(dn->params.p, in, dn->nParams * 4, hipMemcpyHostToDevice, dn->ctx->stream));
   int rc = packAll(dn);
   if (rc) return fail(dn->ctx, rc, "cpt_denoiser_set_params", "pack");
@@ -895,7 +911,7 @@ int cpt_denoiser_layers(const cpt_denoiser *dn, int32_t *cin, int32_t *cout, int
 // gradient of every parameter; with lr > 0 an Adam step follows.
 int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *target, int32_t b, int32_t h, int32_t w,
                             float lr, float *loss_out, float *grads_out, int flags) {
-  if (!dn || !in10 || !target || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
+  if (!dn || !dn->ctx || !in10 || !target || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
   cpt_ctx *ctx = dn->ctx;
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
@@ -942,7 +958,7 @@ int cpt_denoiser_train_step(cpt_denoiser *dn, const float *in10, const float *ta
 // Denoised radiance for b images: in10 = B*h*w*10 floats (rgb, albedo,
 // normal, depth), out_rgb = B*h*w*3.
 int cpt_denoise(cpt_denoiser *dn, const float *in10, float *outRgb, int32_t b, int32_t h, int32_t w, int flags) {
-  if (!dn || !in10 || !outRgb || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
+  if (!dn || !dn->ctx || !in10 || !outRgb || b <= 0 || h <= 0 || w <= 0) return CPT_ERR_ARG;
   cpt_ctx *ctx = dn->ctx;
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;

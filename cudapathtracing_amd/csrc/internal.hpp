@@ -30,6 +30,10 @@ struct MatDev {
 
 struct RenderBufs;
 
+// Frees a denoiser's device memory and detaches it from its context (its
+// handle stays valid for cpt_denoiser_destroy; other calls on it fail).
+void denoiserRelease(cpt_denoiser *dn);
+
 }  // namespace cpt
 
 struct cpt_ctx {
@@ -38,6 +42,7 @@ struct cpt_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::vector<cpt::DevBuf *> bufs;
+  std::vector<cpt_denoiser *> denoisers;  // live denoisers: released by cpt_destroy
 
   // scene (insertion order)
   int64_t nPrims = 0, nTri = 0;

@@ -53,7 +53,8 @@ class Denoiser:
 
     def _check(self, rc):
         if rc:
-            raise CptError(rc, self._lib.cpt_last_error(self.ctx._h).decode())
+            h = getattr(self.ctx, "_h", None)
+            raise CptError(rc, self._lib.cpt_last_error(h).decode() if h else "the context is closed")
 
     def layers(self):
         n = 32

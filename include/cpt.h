@@ -121,10 +121,17 @@ typedef struct cpt_camera {
 typedef struct cpt_ctx cpt_ctx;
 
 int cpt_create(cpt_ctx **out, int hip_device);
+/* Also releases the device memory of the context's denoisers: one destroyed
+ * afterwards only frees its handle (calls on it meanwhile return
+ * CPT_ERR_ARG). */
 void cpt_destroy(cpt_ctx *ctx);
 const char *cpt_last_error(const cpt_ctx *ctx);
 /* Use an external hipStream_t (e.g. torch's current stream); NULL restores
- * the context's own stream. */
+ * the context's own stream.  The own stream is a blocking stream: work on
+ * it is ordered with work on the device's null stream in both directions
+ * (so callers whose work runs on the null stream, such as PyTorch's default
+ * stream, whose handle is NULL, need no extra synchronisation); work on
+ * other non-blocking streams is not ordered with it. */
 int cpt_set_stream(cpt_ctx *ctx, void *hip_stream);
 int cpt_synchronize(cpt_ctx *ctx);
 /* Abort: asks the running (or next) call on ctx to stop.  The persistent

@@ -107,3 +107,54 @@ def test_device_tensor_training_step_equals_host(ctx):
         dn.close()
     assert lh == ld
     assert np.array_equal(gh, gd)
+
+
+def test_device_calls_ordered_with_torch_default_stream(ctx):
+    """Inputs written by PyTorch's default stream and read by the denoiser at
+    once, and its output read by PyTorch at once (no synchronisation between
+    the three), give exactly the synchronised result: the library's own
+    stream is ordered with the null stream.  The producer runs a few large
+    matmuls before it writes the input, so an unordered read would see the
+    NaNs the input buffer held before."""
+    import torch
+    print("[gpu] denoiser ordered with torch's default stream", flush=True)
+    in10, _ = batch(4, 4, 256, 256)
+    dn = Denoiser(ctx, 4, 256, 256, seed=4)
+    try:
+        src = torch.from_numpy(in10).cuda()
+        ref = torch.empty((4, 256, 256, 3), device="cuda")
+        torch.cuda.synchronize()
+        dn(src, out=ref)
+        torch.cuda.synchronize()
+        assert torch.isfinite(ref).all()
+        busy = torch.randn(4096, 4096, device="cuda")
+        for _ in range(3):
+            x = torch.full_like(src, float("nan"))
+            out = torch.full_like(ref, -1.0)
+            torch.cuda.synchronize()
+            for _ in range(6):
+                busy = (busy @ busy) * (1.0 / 64)
+            x.copy_(src)
+            dn(x, out=out)
+            got = out.clone()
+            torch.cuda.synchronize()
+            assert torch.equal(got, ref)
+    finally:
+        dn.close()
+
+
+def test_denoiser_outliving_its_context():
+    """A denoiser destroyed after its context only releases its handle (the
+    context has freed its device memory); calls on it meanwhile fail
+    cleanly."""
+    import cudapathtracing_amd as cpt
+    from cudapathtracing_amd.context import CptError
+    print("[gpu] denoiser outliving its context", flush=True)
+    ctx = cpt.Context(0)
+    dn = Denoiser(ctx, 1, 32, 32, seed=2)
+    x = np.zeros((1, 32, 32, 10), np.float32)
+    assert np.isfinite(dn(x)).all()
+    ctx.close()
+    with pytest.raises(CptError):
+        dn(x)
+    dn.close()

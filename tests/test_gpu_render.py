@@ -115,19 +115,27 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist, pshadow):
         ctx.set_shadow_sort_min(1 << 20)
 
 
-def test_occlusion_kernels_agree_bit_for_bit(ctx):
+@pytest.mark.parametrize("medium", [False, True])
+def test_occlusion_kernels_agree_bit_for_bit(ctx, medium):
     """The persistent occlusion walk (append order) and the one-lane-per-ray
     kernel (append order, and origin-cell order) decide every shadow ray the
     same way and add each unoccluded contribution to its path's own slot: one
-    light, so no atomic adds, and the images are equal bit for bit."""
-    print("[gpu] occlusion kernels bit for bit", flush=True)
-    s = scenes.displaced_icosphere(40, 40, level=5)
+    light, so no atomic adds, and the images are equal bit for bit; so are
+    repeats of the persistent render, whose walks pair rays with lanes by
+    timing (closest-hit ties go to the lowest leaf whatever the test order).
+    With the fBm medium the shadow rays are ratio-tracked."""
+    print(f"[gpu] occlusion kernels bit for bit (medium {medium})", flush=True)
+    if medium:
+        s = scenes.displaced_icosphere_medium(64, 64, level=6, grid=32)
+    else:
+        s = scenes.displaced_icosphere(40, 40, level=5)
     ctx.upload(s)
     ctx.build(s.bounds_min, s.bounds_max)
     kw = dict(seed=5, profile=L.PROFILE_CONFIG)
     try:
         ctx.set_persistent_shadow(True)
         a, _, sa = ctx.render(s.camera, 8, **kw)
+        a2, _, _ = ctx.render(s.camera, 8, **kw)
         ctx.set_persistent_shadow(False)
         ctx.set_shadow_sort_min(0)
         b, _, _ = ctx.render(s.camera, 8, **kw)
@@ -136,7 +144,9 @@ def test_occlusion_kernels_agree_bit_for_bit(ctx):
     finally:
         ctx.set_persistent_shadow(True)
         ctx.set_shadow_sort_min(1 << 20)
+        ctx.set_medium(None)
     assert sa.rays > 0 and a.mean() > 0
+    assert np.array_equal(a, a2)
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 

@@ -246,6 +246,46 @@ def test_cluster_tree_coincident_primitives(ctx):
     assert copies(i0).mean() > 0.01
 
 
+def test_exact_ties_resolve_to_one_primitive(ctx):
+    """Three exact copies of every triangle of a soup: each hit on the soup is
+    a three-way tie at one distance, which every walk resolves to the copy in
+    the lowest leaf, so binary, four-wide (FP32 and quantized, one lane per
+    ray and persistent) and eight-wide walks over one tree return the same
+    ids, and so does every repeat of the persistent walks, whose queued leaf
+    tests run in an order set by the other lanes of the wave."""
+    print("[gpu] trace exact ties", flush=True)
+    n = 30_000
+    s = scenes.SceneData()
+    m = s.add_material((0.5, 0.5, 0.5), L.M_LAMBERT)
+    soup = scenes.triangle_soup(n, seed=12, mu=-3.5)
+    s.add_mesh(np.concatenate([soup, soup, soup]), m)
+    ctx.upload(s)
+    ctx.set_cluster(0)
+    rays = scenes.random_rays(200_000, seed=13, lo=-0.2, hi=1.2)
+    ref = None
+    try:
+        ctx.build(s.bounds_min, s.bounds_max)
+        for width, quant, persistent in ((2, False, False), (4, False, False), (4, True, False),
+                                         (4, True, True), (4, False, True), (8, False, False)):
+            ctx.set_traversal_width(width)
+            ctx.set_quantized(quant)
+            ctx.set_persistent(persistent)
+            for _ in range(3 if persistent else 1):
+                t, ids = ctx.trace(rays)
+                if ref is None:
+                    ref = (t, ids)
+                    continue
+                cfg = (width, quant, persistent)
+                assert np.array_equal(ids, ref[1]), (cfg, np.where(ids != ref[1])[0][:5])
+                assert np.array_equal(t, ref[0]), cfg
+    finally:
+        ctx.set_traversal_width(4)
+        ctx.set_quantized(True)
+        ctx.set_persistent(True)
+        ctx.set_cluster(2)
+    assert np.mean(ref[1] >= 0) > 0.2
+
+
 def test_device_pointer_trace(ctx):
     import torch
     print("[gpu] trace with device pointers", flush=True)
