@@ -685,6 +685,8 @@ def main():
         "shadow": {"rays_per_frame": r["shadow_rays"], "nodes_per_ray": round(r["shadow_nodes_per_ray"], 2),
                    "prims_per_ray": round(r["shadow_prims_per_ray"], 2),
                    "lane_utilisation": round(r["shadow_lane_utilisation"], 3),
+                   "kernels": "k_shadow_persist for a bounce with at least 2^22 shadow rays (no medium), "
+                              "k_shadow below (origin-cell order from 2^20 rays)",
                    "roofline": roofline_hbm(sh_kernel, r["shadow_achieved"], r["shadow_ms"] / max(1, r["bounces"]))},
         "cpu_baseline": cpu,
         "lbvh": lb,

@@ -22,6 +22,7 @@ CPT_ERR_IO = -5
 CPT_ERR_ABORTED = -6
 CPT_DEVICE_PTRS = 1
 NOALLOC_MAX_RAYS = 1 << 20
+PERSISTENT_SHADOW_MIN_DEFAULT = 1 << 22
 
 # MaterialType (CudaPathTracing/rtmath.cuh:15) + new entries
 M_NONE, M_LIGHT, M_OPAQUE, M_SPECULAR_DIELECTRIC, M_ROUGH_DIELECTRIC, M_VOLUME, M_LAMBERT, M_GGX = range(8)

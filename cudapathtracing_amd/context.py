@@ -83,10 +83,17 @@ class Context:
         self._check(self._lib.cpt_set_option(self._h, 3, 1 if on else 0))
 
     def set_persistent_shadow(self, on: bool):
-        """cpt_render's four-wide occlusion walks as persistent lane-refilling
-        kernels, in append order (default on); off: one lane per shadow ray,
-        ordered by origin cell for long queues."""
-        self._check(self._lib.cpt_set_option(self._h, 8, 1 if on else 0))
+        """cpt_render's four-wide occlusion walks always as persistent
+        lane-refilling kernels in append order (on) or never (off: one lane
+        per shadow ray, ordered by origin cell for long queues);
+        set_persistent_shadow_min sets the automatic choice."""
+        self.set_persistent_shadow_min(1 if on else 0)
+
+    def set_persistent_shadow_min(self, n: int = L.PERSISTENT_SHADOW_MIN_DEFAULT):
+        """The persistent occlusion walk for a bounce with at least n shadow
+        rays in a scene without a medium, the one-lane kernel otherwise (1:
+        always, 0: never; without an argument the default, 2^22)."""
+        self._check(self._lib.cpt_set_option(self._h, 8, int(n)))
 
     def set_queue_sort_min(self, n: int):
         """cpt_render bounce queues of at least n paths are traced in ray-key order (0: never, the default)."""

@@ -223,8 +223,8 @@ int cpt_set_option(cpt_ctx *ctx, int32_t option, int32_t value) {
     ctx->quantized = value != 0;
     return CPT_OK;
   }
-  if (option == CPT_OPT_PERSISTENT_SHADOW && (value == 0 || value == 1)) {
-    ctx->persistentShadow = value != 0;
+  if (option == CPT_OPT_PERSISTENT_SHADOW && value >= 0) {
+    ctx->persistentShadowMin = value;
     return CPT_OK;
   }
   if (option == CPT_OPT_PERSISTENT && (value == 0 || value == 1)) {

@@ -93,7 +93,7 @@ struct cpt_ctx {
   // keeps neighbouring pixels' paths together)
   int64_t queueSortMin = 0;
   int64_t shadowSortMin = (int64_t)1 << 20;  // shadow queues this long are reordered
-  bool persistentShadow = true;                // occlusion walks as persistent kernels
+  int64_t persistentShadowMin = CPT_PERSISTENT_SHADOW_MIN_DEFAULT;  // shadow rays for the persistent walk
   bool lastTraceSorted = false;
 
   // traversal I/O staging; cntA: the persistent walks' grab counter; cntB:

@@ -1370,8 +1370,14 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       CPT_TRY_HIP(ctx, hipEventRecord(R.ev[pi][4], s));
       // the persistent walk refills its lanes in append order: on the
       // displaced icosphere it took 31.9 ms a frame unsorted, 40.9 sorted
-      // (sort included), the one-lane-per-ray kernel 37.3 sorted, 47.7 not
-      const bool shPersist = extPersist && ctx->persistentShadow;
+      // (sort included), the one-lane-per-ray kernel 37.3 sorted, 47.7 not.
+      // Below about 2^22 rays a wave's fixed chunks leave the chip part idle
+      // and end in a tail (960x540 at 4 spp: 3.7 against 2.2 ms), and with a
+      // medium each lane ratio-tracks as its walk ends while the rest of the
+      // wave waits (32 views at 4 spp: 119 against 104 ms), so the automatic
+      // choice (a minimum above 1) takes the one-lane kernel there
+      const int64_t shMin = ctx->persistentShadowMin;
+      const bool shPersist = extPersist && shMin > 0 && (int64_t)ns >= shMin && (shMin == 1 || !a.med.on);
       uint32_t *shPerm = nullptr;
       if (!shPersist && ctx->shadowSortMin > 0 && (int64_t)ns >= ctx->shadowSortMin) {
         if ((rc = radixSortPairs(ctx, R.keysS.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),

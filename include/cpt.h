@@ -156,12 +156,15 @@ const char *cpt_version(void);
  * traversals as persistent kernels whose lanes take a new ray when theirs is
  * finished; 0 launches one lane per ray.  Results are identical. */
 #define CPT_OPT_PERSISTENT 3
-/* CPT_OPT_PERSISTENT_SHADOW: 1 (default) runs cpt_render's four-wide
- * occlusion walks as persistent lane-refilling kernels too (with
- * CPT_OPT_PERSISTENT 1), in append order (CPT_OPT_SHADOW_SORT_MIN then
- * applies only to the one-lane-per-ray kernel); 0 launches one lane per
- * shadow ray.  Results are identical. */
+/* CPT_OPT_PERSISTENT_SHADOW: cpt_render walks a bounce's shadow rays with
+ * the persistent lane-refilling four-wide kernel (with CPT_OPT_PERSISTENT 1),
+ * in append order, when there are at least this many of them and the scene
+ * has no participating medium (default CPT_PERSISTENT_SHADOW_MIN_DEFAULT);
+ * 1: always, also with a medium; 0: never.  Otherwise one lane per shadow
+ * ray (ordered by origin cell from CPT_OPT_SHADOW_SORT_MIN rays).  Results
+ * are identical. */
 #define CPT_OPT_PERSISTENT_SHADOW 8
+#define CPT_PERSISTENT_SHADOW_MIN_DEFAULT (1 << 22)
 /* CPT_OPT_QUANTIZED: 1 (default) makes four-wide walks read 64 B records
  * whose child boxes are 8-bit planes on a power-of-two grid, rounded outward
  * (each contains the FP32 box); 0 reads the 128 B FP32 records.  Hits are

@@ -30,6 +30,30 @@ def test_every_declared_symbol_is_exported():
     assert set(names) <= bound, set(names) - bound
 
 
+def header_defines():
+    src = open(os.path.join(ROOT, "include", "cpt.h")).read()
+    out = {}
+    for name, val in re.findall(r"^#define\s+(CPT_[A-Z0-9_]+)\s+(.+?)\s*(?:/\*.*)?$", src, flags=re.M):
+        try:
+            out[name] = eval(val, {})
+        except Exception:
+            pass
+    return out
+
+
+def test_python_constants_match_header():
+    """The Python binding's copies of the header's constants (status codes,
+    flags, option defaults and limits) hold the header's values."""
+    d = header_defines()
+    pairs = {"CPT_ERR_ARG": L.CPT_ERR_ARG, "CPT_ERR_HIP": L.CPT_ERR_HIP, "CPT_ERR_STATE": L.CPT_ERR_STATE,
+             "CPT_ERR_OOM": L.CPT_ERR_OOM, "CPT_ERR_IO": L.CPT_ERR_IO, "CPT_ERR_ABORTED": L.CPT_ERR_ABORTED,
+             "CPT_DEVICE_PTRS": L.CPT_DEVICE_PTRS, "CPT_NOALLOC_MAX_RAYS": L.NOALLOC_MAX_RAYS,
+             "CPT_PERSISTENT_SHADOW_MIN_DEFAULT": L.PERSISTENT_SHADOW_MIN_DEFAULT,
+             "CPT_M_LAMBERT": L.M_LAMBERT, "CPT_M_GGX": L.M_GGX, "CPT_OBJ_TRIANGLE": L.OBJ_TRIANGLE}
+    for k, v in pairs.items():
+        assert d[k] == v, (k, d[k], v)
+
+
 def test_struct_layouts_match_header(tmp_path):
     """ctypes layouts equal what a C compiler makes of include/cpt.h."""
     import shutil

@@ -110,7 +110,7 @@ def test_cornell_lambert_paths(ctx, width, quant, qsort, persist, pshadow):
         ctx.set_traversal_width(4)
         ctx.set_quantized(True)
         ctx.set_persistent(True)
-        ctx.set_persistent_shadow(True)
+        ctx.set_persistent_shadow_min()
         ctx.set_queue_sort_min(0)
         ctx.set_shadow_sort_min(1 << 20)
 
@@ -142,7 +142,7 @@ def test_occlusion_kernels_agree_bit_for_bit(ctx, medium):
         ctx.set_shadow_sort_min(1)
         c, _, _ = ctx.render(s.camera, 8, **kw)
     finally:
-        ctx.set_persistent_shadow(True)
+        ctx.set_persistent_shadow_min()
         ctx.set_shadow_sort_min(1 << 20)
         ctx.set_medium(None)
     assert sa.rays > 0 and a.mean() > 0
