@@ -502,7 +502,7 @@ static FwdPlan fwdPlan(int Cin, int Cout, int B, int H, int W) {
     P.split = (chunks + per - 1) / per;  // every slice non-empty
   }
   // stacked tiles per workgroup: as few workgroups as still fill the chip
-  // once (about 512, two per CU), at most 8 tiles each
+  // twice (about 512, one per CU at a time), at most 8 tiles each
   P.tpw = P.split > 1 ? 1 : (int)std::max<int64_t>(1, std::min<int64_t>(8, (tiles + 511) / 512));
   return P;
 }
@@ -539,16 +539,25 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
     mk = nullptr;
     ldo = Cout;
   }
-#define CPT_CONV_FWD(QB, F32)                                                                                    \
-  hipLaunchKernelGGL((k_conv_fwd<QB, F32>), grid, dim3(kConvThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, W, oB, \
-                     oF, ldo, md, mk, P.split, sliceStride, P.tpw)
-  if (P.nt == 64) {
-    if (oF) CPT_CONV_FWD(4, true);
-    else CPT_CONV_FWD(4, false);
-  } else {
-    if (oF) CPT_CONV_FWD(2, true);
-    else CPT_CONV_FWD(2, false);
+  // one 32-channel chunk per slice: the weights are copied into LDS once
+  const int chunks = Cin / 32, perSlice = (chunks + P.split - 1) / P.split;
+#define CPT_CONV_FWD(QB, F32, ONE)                                                                             \
+  hipLaunchKernelGGL((k_conv_fwd<QB, F32, ONE>), grid, dim3(kFwdThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, \
+                     W, oB, oF, ldo, md, mk, P.split, sliceStride, P.tpw)
+#define CPT_CONV_FWD_QB(QB)                        \
+  if (perSlice == 1) {                             \
+    if (oF) CPT_CONV_FWD(QB, true, true);          \
+    else CPT_CONV_FWD(QB, false, true);            \
+  } else {                                         \
+    if (oF) CPT_CONV_FWD(QB, true, false);         \
+    else CPT_CONV_FWD(QB, false, false);           \
   }
+  if (P.nt == 64) {
+    CPT_CONV_FWD_QB(4)
+  } else {
+    CPT_CONV_FWD_QB(2)
+  }
+#undef CPT_CONV_FWD_QB
 #undef CPT_CONV_FWD
   if (P.split > 1) {
     const int64_t n = sliceStride / 8;
@@ -814,6 +823,10 @@ extern "C" {
 
 int cpt_denoiser_create(cpt_ctx *ctx, int32_t maxB, int32_t maxH, int32_t maxW, uint64_t seed, cpt_denoiser **out) {
   if (!ctx || !out || maxB <= 0 || maxH <= 0 || maxW <= 0) return CPT_ERR_ARG;
+  // the convolution kernels address one image of a tensor with 32-bit
+  // offsets: the widest per-image tensor is 32 bf16 channels at full size
+  if ((int64_t)maxH * maxW * 64 >= (1ll << 31))
+    return fail(ctx, CPT_ERR_ARG, "cpt_denoiser_create", "image too large (H*W*64 B must stay below 2^31)");
   *out = nullptr;
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   cpt_denoiser *dn = new cpt_denoiser();
@@ -994,6 +1007,8 @@ int cpt_conv3x3(cpt_ctx *ctx, int mode, const uint16_t *x, const uint16_t *w, co
                 float *out, int32_t B, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
   if (!ctx || B <= 0 || H <= 0 || W <= 0 || Cin <= 0 || Cout <= 0 || Cin % 32 || Cout % 32 || mode < 0 || mode > 2)
     return fail(ctx, CPT_ERR_ARG, "cpt_conv3x3", "channels must be multiples of 32");
+  if ((int64_t)H * W * std::max(Cin, Cout) * 4 >= (1ll << 31))
+    return fail(ctx, CPT_ERR_ARG, "cpt_conv3x3", "image too large (H*W*channels*4 B must stay below 2^31)");
   CPT_TRY_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const size_t nx = (size_t)B * H * W * Cin, ny = (size_t)B * H * W * Cout, nw = (size_t)Cout * 9 * Cin;

@@ -22,6 +22,7 @@ namespace cpt {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((vector_size(16)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
@@ -41,8 +42,84 @@ __device__ __forceinline__ uint32_t reluKeep(uint32_t w) {
 }
 
 constexpr int kConvThreads = 256;
-constexpr int kConvPix = 64;    // output pixels per workgroup (one row segment)
 constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
+
+// ---- buffer resources (gfx9 raw buffers) -------------------------------------
+// A raw buffer resource over `bytes` bytes at p: loads at offsets at or past
+// `bytes` return zero (and an LDS-direct load writes zero), stores there are
+// dropped.  Offsets are 32-bit; kOob is past every buffer used here.
+constexpr uint32_t kOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufRsrc(const void *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+typedef __attribute__((address_space(3))) void lds_void;
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+// The same resource as four words, for the LDS copy below.
+__device__ __forceinline__ i32x4 bufDesc(const void *p, int64_t bytes) {
+  const uint64_t a = (uint64_t)p;
+  i32x4 d;
+  d.x = (int32_t)(uint32_t)a;
+  d.y = (int32_t)((uint32_t)(a >> 32) & 0xffffu);  // stride 0
+  d.z = (int32_t)bytes;
+  d.w = 0x00020000;
+  return d;
+}
+// 16 B per lane straight into LDS: lane l writes dst + 16 l (dst
+// wave-uniform).  The instruction writes no register (its data goes to the
+// LDS address in M0), so an asm block that returns at issue leaves no
+// register in flight.  The compiler's builtin for it makes every later LDS
+// read wait for all copies in flight (it cannot tell which LDS they write),
+// which would serialise the two stages; here the waits are vmWait's.
+// M0 is declared clobbered (the compiler keeps no value there across it);
+// clang notes that it does not preserve reserved registers for the block,
+// which is all this needs.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void loadToLds16(i32x4 r, void *dst, uint32_t off) {
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void *)dst);
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds"
+               :
+               : "s"(m0), "v"(off), "s"(r)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+// Waits until at most N of this wave's vector memory operations are
+// outstanding (they complete in issue order).
+template <int N>
+__device__ __forceinline__ void vmWait() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// vmWait with a wave-uniform count (an immediate per case); counts above 47
+// wait for everything
+__device__ __forceinline__ void vmWaitDyn(int n) {
+#define CPT_VMW(N) \
+  case N:          \
+    vmWait<N>();   \
+    break;
+#define CPT_VMW8(B) CPT_VMW(B) CPT_VMW(B + 1) CPT_VMW(B + 2) CPT_VMW(B + 3) CPT_VMW(B + 4) CPT_VMW(B + 5) \
+    CPT_VMW(B + 6) CPT_VMW(B + 7)
+  switch (n) {
+    CPT_VMW8(0) CPT_VMW8(8) CPT_VMW8(16) CPT_VMW8(24) CPT_VMW8(32) CPT_VMW8(40)
+    default:
+      vmWait<0>();
+  }
+#undef CPT_VMW8
+#undef CPT_VMW
+}
+
+// Workgroup barrier for LDS traffic only: this wave's LDS reads and writes
+// are complete (lgkmcnt 0) and nobody passes before every wave is here.  The
+// LDS copies (buffer_load ... lds) are waited for separately with vmWait;
+// __syncthreads' workgroup fence would also wait for every outstanding global
+// store and copy (vmcnt 0).  The wave-scope fences keep the compiler from
+// moving memory operations across it.
+__device__ __forceinline__ void ldsBarrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Y[b,y,x,co] = act(sum X[b,y+ky-1,x+kx-1,ci] W[co,ky,kx,ci] + bias[co])
 // X has Cin channels; with Xlow it is the decoder's concatenation
@@ -52,75 +129,99 @@ constexpr int kLdsRow = 40;     // bf16 per LDS row (32 + 8 padding)
 // into outBf (bf16, ldo channels per pixel) or outF (fp32, ldo channels per
 // pixel: channels co >= ldo are not written, so a 4-channel buffer takes the
 // first 4 of a wider layer).
-// mode bit 1 = ReLU.  The bias is read once per workgroup, before the loop
-// (a global load inside it, at every tile end, also waited for the
-// prefetched next step).  maskY (bf16 output only, ldo channels): the value
-// is kept where
-// maskY > 0 and zeroed elsewhere -- the ReLU gradient mask of the layer whose
-// output maskY is.
+// mode bit 1 = ReLU.  maskY (bf16 output only, ldo channels): the value is
+// kept where maskY > 0 and zeroed elsewhere -- the ReLU gradient mask of the
+// layer whose output maskY is.  Split over Cin (kSplit > 1): slice s takes
+// its chunks and writes fp32 partial sums to outF + s * sliceStride.
 //
 // Implicit GEMM with the output transposed, D[co][px] = W[co][tap, ci] .
 // X[tap, ci][px]: the weights are the A operand and the input pixels the B
 // operand, so each lane ends with 4 consecutive output channels of one pixel.
-// A workgroup (4 waves) owns a tile of 4 rows x 64 pixels x 16*QB output
-// channels, wave w the row y0 + w.  Per 32-channel chunk of Cin it stages in
-// LDS the 6 x 66 input halo of the tile and the 9 taps' weights
-// (9 x 16*QB x 32), each loaded once per chunk and read by all waves for all
-// 9 taps; a wave then issues 9 x 4 x QB MFMAs.  A workgroup walks `tpw`
-// vertically stacked tiles as one sequence of (tile, chunk) steps; the next
-// step's global loads are issued before this step's MFMAs and written to LDS
-// after them, so their latency hides behind the MFMAs also where Cin is a
-// single chunk (whose weights are then loaded once per workgroup).  LDS rows
-// are padded to 40 bf16 (80 B): the 16-byte reads of 8 consecutive rows then
-// cover all 32 banks.
-// A finished bf16 tile leaves through LDS: each wave writes its row (64 px x
-// 32 channels at a time) into the halo buffer, which is free after a barrier
-// at the tile's end, and reads it back as 16 B per lane, so that a store
-// instruction writes 16 whole 64 B pixel records (scattered 8 B stores per
-// lane had cost as much as the MFMAs).
-constexpr int kTileH = 4, kTileW = 64;
+// A workgroup (8 waves) owns a tile of 8 rows x 64 pixels x 16*QB output
+// channels, wave w the row y0 + w, and walks `tpw` vertically stacked tiles
+// as one sequence of (tile, 32-channel chunk) steps.
+//
+// Data movement: a step needs the 10 x 66 input halo of its tile (32
+// channels, 64 B per pixel) and the chunk's 9 taps of weights (9 x 16*QB
+// rows of 64 B).  Both are copied from HBM straight into LDS by
+// buffer_load ... lds (16 B per lane, a wave moves 16 pixel records per
+// instruction), no registers in between, into a ring of NS stages: step
+// k+NS-1's copies are issued right after the barrier that opens step k and
+// land while steps k .. k+NS-2 run their MFMAs (NS = 3 where the weights
+// stay in LDS, else 2).  Pixels outside the image (and weight rows past Cout)
+// get an offset past the buffer, which the copy turns into zeros.  Records
+// are 64 B without padding; the 16 B part p of record r sits at part
+// p ^ ((r >> 1) & 3), so the 16 B fragment reads of 8 consecutive records
+// (one per lane of a quarter wave) cover all 32 banks.  kOneChunk (Cin of one
+// chunk per slice): the weights are copied once and stay.
+// Waits: a wave's vector memory operations complete in issue order, so
+// before step k it waits until only the operations it issued after step k's
+// copies (later copies, finished tiles' stores) may still be outstanding;
+// every store goes through a buffer resource (out-of-range lanes get kOob)
+// so that their number per tile is fixed.  A finished bf16 tile leaves through LDS (the
+// step's halo stage, free after a barrier): each wave writes its row there,
+// 32 channels at a time, and reads it back as 16 B per lane, so that a store
+// instruction writes 16 whole 64 B pixel records.
+constexpr int kFwdWaves = 8, kFwdThreads = 64 * kFwdWaves;
+constexpr int kTileH = kFwdWaves, kTileW = 64;
 constexpr int kHaloH = kTileH + 2, kHaloW = kTileW + 2;
-constexpr int kHaloParts = kHaloH * kHaloW * 4;  // 16 B parts of a 32-channel halo
-constexpr int kHaloIters = (kHaloParts + kConvThreads - 1) / kConvThreads;
+constexpr int kHaloPix = kHaloH * kHaloW;                                    // 660
+constexpr int kHaloBytes = kHaloPix * 64;                                    // 42,240
+constexpr int kHaloLoads = (kHaloPix + 15) / 16;                             // 42 (the last: 4 pixels)
+constexpr int kHaloLoadsPerWave = (kHaloLoads + kFwdWaves - 1) / kFwdWaves;  // 6
+static_assert(kFwdWaves * kTileW * kLdsRow * 2 <= kHaloBytes, "output staging exceeds a halo stage");
 
-// kF32: fp32 output (outF); else bf16 (outBf).  Separate instantiations: the
-// fp32 path's stores, if compiled into the same loop, made the compiler wait
-// for all outstanding global loads at the top of every step (their source
-// registers are reused by the step's LDS reads), which serialised the
-// prefetch with the MFMAs.
-template <int QB, bool kF32>
-__global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
-                                                               const uint16_t *__restrict__ Xlow, int Clow,
-                                                               const uint16_t *__restrict__ Wt,
-                                                               const float *__restrict__ bias, int Cout, int H,
-                                                               int Wd, uint16_t *__restrict__ outBf,
-                                                               float *__restrict__ outF, int ldo, int mode,
-                                                               const uint16_t *__restrict__ maskY, int kSplit,
-                                                               int64_t sliceStride, int tpw) {
+// byte offset of 16 B part p of 64 B record r in a swizzled LDS stage
+__device__ __forceinline__ uint32_t swzOff(int r, int p) {
+  return (uint32_t)(r * 64 + ((p ^ ((r >> 1) & 3)) << 4));
+}
+
+template <int QB, bool kF32, bool kOneChunk>
+__global__ __launch_bounds__(kFwdThreads, 1) void k_conv_fwd(const uint16_t *__restrict__ X, int Cin,
+                                                              const uint16_t *__restrict__ Xlow, int Clow,
+                                                              const uint16_t *__restrict__ Wt,
+                                                              const float *__restrict__ bias, int Cout, int H,
+                                                              int Wd, uint16_t *__restrict__ outBf,
+                                                              float *__restrict__ outF, int ldo, int mode,
+                                                              const uint16_t *__restrict__ maskY, int kSplit,
+                                                              int64_t sliceStride, int tpw) {
   constexpr int NT = 16 * QB;
-  constexpr int kWParts = 9 * NT * 4;
-  constexpr int kWIters = (kWParts + kConvThreads - 1) / kConvThreads;
-  __shared__ __attribute__((aligned(16))) uint16_t sX[kHaloH * kHaloW * kLdsRow];
-  __shared__ __attribute__((aligned(16))) uint16_t sW[9 * NT * kLdsRow];
+  constexpr int kWRows = 9 * NT;
+  constexpr int kWBytes = kWRows * 64;
+  constexpr int kWLoads = kWRows / 16;
+  constexpr int kWLoadsPerWave = (kWLoads + kFwdWaves - 1) / kFwdWaves;
+  // stores per wave per finished tile: 4 per 32 staged channels (bf16), 4
+  // per 16-channel block (fp32)
+  constexpr int kEpiStores = kF32 ? 4 * QB : 4 * ((QB + 1) / 2);
+  // stages: three where the weights stay (they and three halos fit the 160
+  // KB of LDS), else two of halo and weights
+  constexpr int NS = kOneChunk ? 3 : 2;
+  constexpr int NW = kOneChunk ? 1 : NS;
+  static_assert(NS * kHaloBytes + NW * kWBytes <= 160 * 1024, "stages exceed LDS");
+  __shared__ __attribute__((aligned(16))) uint8_t sH[NS][kHaloBytes];
+  __shared__ __attribute__((aligned(16))) uint8_t sW[NW][kWBytes];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x0 = blockIdx.x * kTileW;
   const int nCoTiles = (Cout + NT - 1) / NT;
-  // split over Cin: slice s of kSplit takes chunks [cBeg, cBeg + nChunks)
-  // and writes its fp32 partial sums to outF + s * sliceStride
   const int slice = blockIdx.z % kSplit, bz = blockIdx.z / kSplit;
   const int b = bz / nCoTiles;
   const int co0 = (bz % nCoTiles) * NT;
-  const int64_t imgBase = (int64_t)b * H * Wd;
   const int chunksAll = Cin / 32, perSlice = (chunksAll + kSplit - 1) / kSplit;
   const int cBeg = slice * perSlice;
   const int nChunks = chunksAll - cBeg < perSlice ? chunksAll - cBeg : perSlice;
   if (outF) outF += slice * sliceStride;
-  // this workgroup's tiles: rows y0(i) = (blockIdx.y * tpw + i) * kTileH,
-  // walked as one sequence of (tile, chunk) steps
   const int rowTiles = (H + kTileH - 1) / kTileH;
   const int t0 = blockIdx.y * tpw;
   const int nt = rowTiles - t0 < tpw ? rowTiles - t0 : tpw;
   const int nSteps = nt * nChunks;
+  if (nSteps <= 0) return;  // workgroup-uniform
+  const int cSkip = Cin - Clow;  // channels per pixel of X
+  // per-image resources: every offset below stays under 2^31
+  const int64_t pixImg = (int64_t)H * Wd;
+  const i32x4 rX = bufDesc(X + b * pixImg * cSkip, pixImg * cSkip * 2);
+  const int64_t lowImg = (int64_t)(H >> 1) * (Wd >> 1);
+  const i32x4 rL = bufDesc(Clow ? Xlow + b * lowImg * Clow : X, Clow ? lowImg * Clow * 2 : 0);
+  const i32x4 rW = bufDesc(Wt, (int64_t)Cout * 9 * Cin * 2);
   f32x4 acc[QB][4];
 #pragma unroll
   for (int nb = 0; nb < QB; nb++)
@@ -138,95 +239,89 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
     bsv[nb][2] = bv.z;
     bsv[nb][3] = bv.w;
   }
-  uint4 px[kHaloIters], pw[kWIters];
-  // Loads are unconditional, from addresses clamped into the tensors; bit i
-  // of pxOk / pwOk says whether item i lies inside (else it is stored as
-  // zero).  Masked (branching) loads made the compiler wait for them right
-  // after issue, before this step's MFMAs; with the choice deferred to the
-  // LDS store their latency overlaps the MFMAs.
-  uint32_t pxOk = 0, pwOk = 0;
-  // global -> registers for step k: the input halo of its tile and, unless
-  // Cin is one chunk (the weights then stay in LDS), the chunk's weights
-  const int cSkip = Cin - Clow;  // channels per pixel of X
-  auto load = [&](int k) {
+  // the bias is in registers before any copy is issued: a later first use
+  // would otherwise wait for every copy in flight
+  vmWait<0>();
+  // this wave's copies per step: halo records, and weight rows where they
+  // are copied (kOneChunk: at step 0 only)
+  const int hLoads = (kHaloLoads - wave + kFwdWaves - 1) / kFwdWaves;
+  const int wLoads = (kWLoads - wave + kFwdWaves - 1) / kFwdWaves;
+  // copies of step k into stage k % NS
+  auto issue = [&](int k) __attribute__((always_inline)) {
     const int y0 = (t0 + k / nChunks) * kTileH, c0 = (cBeg + k % nChunks) * 32;
     const bool fromLow = c0 < Clow;  // uniform: chunks never straddle Clow
-    pxOk = 0;
+    uint8_t *hb = sH[k % NS];
 #pragma unroll
-    for (int i = 0; i < kHaloIters; i++) {
-      const int q = tid + i * kConvThreads;
-      const int qq = q < kHaloParts ? q : kHaloParts - 1;
-      const int hp = qq >> 2, part = qq & 3;
+    for (int i = 0; i < kHaloLoadsPerWave; i++) {
+      const int j = wave + kFwdWaves * i;  // wave-uniform
+      if (j >= kHaloLoads) break;
+      const int hp = 16 * j + (lane >> 2), lp = (lane & 3) ^ ((hp >> 1) & 3);
       const int yy = y0 - 1 + hp / kHaloW, xx = x0 - 1 + hp % kHaloW;
-      const bool in = q < kHaloParts && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
-      const int yc = yy < 0 ? 0 : (yy >= H ? H - 1 : yy), xc = xx < 0 ? 0 : (xx >= Wd ? Wd - 1 : xx);
-      const uint16_t *src =
-          fromLow ? Xlow + (((int64_t)b * (H >> 1) + (yc >> 1)) * (Wd >> 1) + (xc >> 1)) * Clow + c0 + part * 8
-                  : X + (imgBase + (int64_t)yc * Wd + xc) * cSkip + (c0 - Clow) + part * 8;
-      px[i] = *(const uint4 *)src;
-      pxOk |= (in ? 1u : 0u) << i;
+      const bool in = hp < kHaloPix && yy >= 0 && yy < H && xx >= 0 && xx < Wd;
+      const int yc = in ? yy : 0, xc = in ? xx : 0;
+      const uint32_t off =
+          fromLow ? (uint32_t)((((yc >> 1) * (Wd >> 1) + (xc >> 1)) * Clow + c0 + lp * 8) * 2)
+                  : (uint32_t)((((int64_t)yc * Wd + xc) * cSkip + (c0 - Clow) + lp * 8) * 2);
+      const i32x4 r = fromLow ? rL : rX;
+      if (j == kHaloLoads - 1) {
+        if (lane < (kHaloPix - 16 * j) * 4) loadToLds16(r, hb + j * 1024, in ? off : kOob);
+      } else {
+        loadToLds16(r, hb + j * 1024, in ? off : kOob);
+      }
     }
-    if (k > 0 && nChunks == 1) return;
-    pwOk = 0;
+    if (kOneChunk && k > 0) return;
+    uint8_t *wb = sW[kOneChunk ? 0 : k % NS];
 #pragma unroll
-    for (int i = 0; i < kWIters; i++) {
-      const int q = tid + i * kConvThreads;
-      const int qq = q < kWParts ? q : kWParts - 1;
-      const int part = qq & 3, r = qq >> 2;
+    for (int i = 0; i < kWLoadsPerWave; i++) {
+      const int j = wave + kFwdWaves * i;
+      if (j >= kWLoads) break;
+      const int r = 16 * j + (lane >> 2), lp = (lane & 3) ^ ((r >> 1) & 3);
       const int t = r / NT, co = co0 + r % NT;
-      const int cc = co < Cout ? co : Cout - 1;
-      pw[i] = *(const uint4 *)(Wt + ((int64_t)cc * 9 + t) * Cin + c0 + part * 8);
-      pwOk |= (q < kWParts && co < Cout ? 1u : 0u) << i;
+      const uint32_t off = co < Cout ? (uint32_t)((((int64_t)co * 9 + t) * Cin + c0 + lp * 8) * 2) : kOob;
+      loadToLds16(rW, wb + j * 1024, off);
     }
   };
-  // registers -> LDS
-  auto masked = [](uint4 v, uint32_t ok) {  // v, or zero where ok is 0 (no select of addresses)
-    const uint32_t m = 0u - (ok & 1u);
-    v.x &= m;
-    v.y &= m;
-    v.z &= m;
-    v.w &= m;
-    return v;
-  };
-  auto store = [&](int k) {
+  // fragment offsets: B (halo) of tap t for pixel block 0, A (weights) of
+  // row lane%16; blocks of 16 pixels or rows keep the swizzle (+1024 B)
+  uint32_t hOff[9];
 #pragma unroll
-    for (int i = 0; i < kHaloIters; i++) {
-      const int q = tid + i * kConvThreads;
-      if (q < kHaloParts) *(uint4 *)(sX + (q >> 2) * kLdsRow + (q & 3) * 8) = masked(px[i], pxOk >> i);
-    }
-    if (k > 0 && nChunks == 1) return;
-#pragma unroll
-    for (int i = 0; i < kWIters; i++) {
-      const int q = tid + i * kConvThreads;
-      if (q < kWParts) *(uint4 *)(sW + (q >> 2) * kLdsRow + (q & 3) * 8) = masked(pw[i], pwOk >> i);
-    }
-  };
+  for (int t = 0; t < 9; t++)
+    hOff[t] = swzOff((wave + t / 3) * kHaloW + (lane & 15) + t % 3, lane >> 4);
+  const uint32_t aOff = swzOff(lane & 15, lane >> 4);
   // fp32 output: lane holds output channels 4*(lane/16) + r of pixel lane%16
-  auto epilogueF = [&](int y) {
-    if (y >= H) return;
+  auto epilogueF = [&](int y) __attribute__((always_inline)) {
+    float *base = outF + (int64_t)b * pixImg * ldo;
+    const __amdgpu_buffer_rsrc_t rO = bufRsrc(base, pixImg * ldo * 4);
 #pragma unroll
     for (int nb = 0; nb < QB; nb++) {
       const int co = co0 + nb * 16 + (lane >> 4) * 4;
-      if (co >= Cout || co >= ldo) continue;  // channel counts are multiples of 4
 #pragma unroll
       for (int m = 0; m < 4; m++) {
         const int x = x0 + m * 16 + (lane & 15);
-        if (x >= Wd) continue;
-        const int64_t p = imgBase + (int64_t)y * Wd + x;
-        float v[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          v[r] = acc[nb][m][r] + bsv[nb][r];
-          if (mode & 1) v[r] = fmaxf(v[r], 0.f);
+        // channel counts are multiples of 4: a lane's 4 channels are all in or out
+        const bool ok = y < H && x < Wd && co < Cout && co < ldo;
+        const uint32_t o = ok ? (uint32_t)((((int64_t)y * Wd + x) * ldo + co) * 4) : kOob;
+        float4 v;
+        v.x = acc[nb][m][0] + bsv[nb][0];
+        v.y = acc[nb][m][1] + bsv[nb][1];
+        v.z = acc[nb][m][2] + bsv[nb][2];
+        v.w = acc[nb][m][3] + bsv[nb][3];
+        if (mode & 1) {
+          v.x = fmaxf(v.x, 0.f);
+          v.y = fmaxf(v.y, 0.f);
+          v.z = fmaxf(v.z, 0.f);
+          v.w = fmaxf(v.w, 0.f);
         }
-        *(float4 *)(outF + p * ldo + co) = make_float4(v[0], v[1], v[2], v[3]);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rO, o, 0, 0);
       }
     }
   };
-  // bf16 output through LDS, 32 channels at a time (call after a barrier:
-  // the halo buffer is overwritten)
-  auto epilogueB = [&](int y) {
-    uint16_t *st = sX + wave * (kTileW * kLdsRow);
+  // bf16 output through LDS (stg: a free halo stage, after a barrier)
+  auto epilogueB = [&](int y, uint8_t *stg) __attribute__((always_inline)) {
+    uint16_t *st = (uint16_t *)stg + wave * (kTileW * kLdsRow);
+    const int64_t outImg = (int64_t)b * pixImg * ldo;
+    const __amdgpu_buffer_rsrc_t rO = bufRsrc(outBf + outImg, pixImg * ldo * 2);
+    const __amdgpu_buffer_rsrc_t rM = bufRsrc(maskY ? maskY + outImg : outBf, maskY ? pixImg * ldo * 2 : 0);
 #pragma unroll
     for (int half = 0; half < (QB + 1) / 2; half++) {
 #pragma unroll
@@ -252,45 +347,53 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int nCh = QB - half * 2 >= 2 ? 32 : 16;  // channels staged in this half
-      if (y < H) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int px = j * 16 + (lane >> 2), part = lane & 3;
-          const int x = x0 + px, co = co0 + half * 32 + part * 8;
-          if (part * 8 >= nCh || x >= Wd || co >= Cout) continue;
-          uint4 v = *(const uint4 *)(st + px * kLdsRow + part * 8);
-          const int64_t o = (imgBase + (int64_t)y * Wd + x) * ldo + co;
-          if (maskY) {
-            const uint4 mv = *(const uint4 *)(maskY + o);
-            v.x &= reluKeep(mv.x);
-            v.y &= reluKeep(mv.y);
-            v.z &= reluKeep(mv.z);
-            v.w &= reluKeep(mv.w);
-          }
-          *(uint4 *)(outBf + o) = v;
+      for (int j = 0; j < 4; j++) {
+        const int px = j * 16 + (lane >> 2), part = lane & 3;
+        const int x = x0 + px, co = co0 + half * 32 + part * 8;
+        const bool ok = y < H && part * 8 < nCh && x < Wd && co < Cout;
+        const uint32_t o = ok ? (uint32_t)((((int64_t)y * Wd + x) * ldo + co) * 2) : kOob;
+        uint4 v = *(const uint4 *)(st + px * kLdsRow + part * 8);
+        if (maskY) {
+          const uint4 mv = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rM, o, 0, 0));
+          v.x &= reluKeep(mv.x);
+          v.y &= reluKeep(mv.y);
+          v.z &= reluKeep(mv.z);
+          v.w &= reluKeep(mv.w);
         }
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rO, o, 0, 0);
       }
       // the next half overwrites the staging rows: this wave's reads of them
       // are issued first (LDS operations of a wave complete in order)
       __builtin_amdgcn_wave_barrier();
     }
   };
-  if (nSteps <= 0) return;  // workgroup-uniform
-  load(0);
-  store(0);
-  __syncthreads();
+  // copies (L) and stores (E) this wave issues for step j
+  auto L = [&](int j) { return j < nSteps ? hLoads + ((!kOneChunk || j == 0) ? wLoads : 0) : 0; };
+  auto E = [&](int j) { return j >= 0 && j % nChunks == nChunks - 1 ? kEpiStores : 0; };
+#pragma unroll
+  for (int j = 0; j < NS - 1; j++)
+    if (j < nSteps) issue(j);
   for (int k = 0; k < nSteps; k++) {
-    if (k + 1 < nSteps) load(k + 1);
+    // Step j's copies are issued at the top of step j - NS + 1 and its
+    // stores at the end of step j.  Operations complete in issue order, so
+    // step k's copies are in once no more than the younger ones remain:
+    // the stores of steps k-NS+1 .. k-1 and the copies of steps k+1 ..
+    // k+NS-2.
+    int younger = E(k - 1);
+    if (NS == 3) younger += E(k - 2) + L(k + 1);
+    vmWaitDyn(younger);
+    ldsBarrier();  // every wave's copies of step k are in LDS; step k-1 is read
+    if (k + NS - 1 < nSteps) issue(k + NS - 1);
+    const uint8_t *hb = sH[k % NS];
+    const uint8_t *wb = sW[kOneChunk ? 0 : k % NS];
 #pragma unroll
     for (int t = 0; t < 9; t++) {
-      const int ky = t / 3, kx = t % 3;
       bf16x8 bx[4], aw[QB];
 #pragma unroll
-      for (int m = 0; m < 4; m++)
-        bx[m] = *(const bf16x8 *)(sX + ((wave + ky) * kHaloW + m * 16 + (lane & 15) + kx) * kLdsRow + (lane >> 4) * 8);
+      for (int m = 0; m < 4; m++) bx[m] = *(const bf16x8 *)(hb + hOff[t] + m * 1024);
 #pragma unroll
-      for (int nb = 0; nb < QB; nb++)
-        aw[nb] = *(const bf16x8 *)(sW + (t * NT + nb * 16 + (lane & 15)) * kLdsRow + (lane >> 4) * 8);
+      for (int nb = 0; nb < QB; nb++) aw[nb] = *(const bf16x8 *)(wb + aOff + (t * NT + nb * 16) * 64);
 #pragma unroll
       for (int nb = 0; nb < QB; nb++)
 #pragma unroll
@@ -302,18 +405,13 @@ __global__ __launch_bounds__(kConvThreads, 2) void k_conv_fwd(const uint16_t *__
       if constexpr (kF32) {
         epilogueF(y);
       } else {
-        __syncthreads();  // every wave is done reading the halo buffer
-        epilogueB(y);
+        ldsBarrier();  // every wave is done reading this step's halo stage
+        epilogueB(y, sH[k % NS]);
       }
 #pragma unroll
       for (int nb = 0; nb < QB; nb++)
 #pragma unroll
         for (int m = 0; m < 4; m++) acc[nb][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
-    if (k + 1 < nSteps) {
-      __syncthreads();  // every wave is done reading this step
-      store(k + 1);
-      __syncthreads();
     }
   }
 }

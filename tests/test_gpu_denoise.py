@@ -19,7 +19,8 @@ def bf(a):
     return h, from_bf16_bits(h).astype(np.float64)
 
 
-@pytest.mark.parametrize("B,H,W,Cin,Cout", [(1, 3, 5, 32, 32), (2, 5, 70, 64, 96), (1, 9, 130, 96, 64)])
+@pytest.mark.parametrize("B,H,W,Cin,Cout", [(1, 3, 5, 32, 32), (2, 5, 70, 64, 96), (1, 9, 130, 96, 64),
+                                             (1, 600, 700, 32, 32), (1, 300, 520, 64, 64)])
 def test_conv_kernels(ctx, B, H, W, Cin, Cout):
     print(f"[gpu] conv3x3 B={B} H={H} W={W} Cin={Cin} Cout={Cout}", flush=True)
     rng = np.random.default_rng(B * 1000 + W)
