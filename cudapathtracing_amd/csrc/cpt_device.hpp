@@ -394,9 +394,40 @@ __device__ __forceinline__ void loadRecord4(const float4 *__restrict__ recs, int
 // slot order finds occluders sooner) and only the others are sorted.  The
 // others are pushed farthest first, so pops come nearest first.  Returns
 // false when no internal child was hit.
-template <bool kAny>
+// Entry distances kept beside the LDS stack entries of closest-hit walks
+// (kCullPop): a popped entry whose box the ray enters beyond the best hit so
+// far is skipped without a visit.  Stored as fp16 rounded toward zero, i.e.
+// down for the non-negative entry distances (t >= tmin >= 0), so a stored
+// value never exceeds the true one and a skipped entry is one the visit
+// would have found empty; an entry at exactly the best distance is kept
+// (equal-distance hits from lower leaves still win, see takesHit).  Entries
+// past the LDS part of the stack carry no distance and are always visited.
+__device__ __forceinline__ uint16_t entryKey(float t) {
+  return (uint16_t)(__builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(t, t)) & 0xffffu);
+}
+__device__ __forceinline__ float entryOf(uint16_t k) { return (float)__builtin_bit_cast(_Float16, k); }
+
+// Pops the next entry of a walk's stack into node (false when it is empty);
+// kCullPop: entries whose stored entry distance exceeds tbest are dropped.
+template <bool kCullPop>
+__device__ __forceinline__ bool popEntry(const int *stk, const uint16_t *stkT, int stride, const int *ovf, int &sp,
+                                         int &node, float tbest) {
+  while (sp > 0) {
+    sp--;
+    if (sp < kStackLds) {
+      if (kCullPop && entryOf(stkT[sp * stride]) > tbest) continue;
+      node = stk[sp * stride];
+    } else {
+      node = ovf[sp - kStackLds];
+    }
+    return true;
+  }
+  return false;
+}
+
+template <bool kAny, bool kCullPop = false>
 __device__ __forceinline__ bool descend4(const bool h[4], const float tn[4], const int ref[4], int *stk, int stride,
-                                         int *ovf, int &sp, int &node) {
+                                         int *ovf, int &sp, int &node, uint16_t *stkT = nullptr) {
   const int m = (int)h[0] + (int)h[1] + (int)h[2] + (int)h[3];
   if (m == 0) return false;
   int best = 0;
@@ -437,13 +468,22 @@ __device__ __forceinline__ bool descend4(const bool h[4], const float tn[4], con
     stk[sp * stride] = pr[0];
     stk[(sp + 1) * stride] = pr[1];
     stk[(sp + 2) * stride] = pr[2];
+    if (kCullPop) {
+      stkT[sp * stride] = entryKey(pt[0]);
+      stkT[(sp + 1) * stride] = entryKey(pt[1]);
+      stkT[(sp + 2) * stride] = entryKey(pt[2]);
+    }
     sp += c;
   } else {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       if (k < c) {
-        if (sp < kStackLds) stk[sp * stride] = pr[k];
-        else if (sp < kStack4Total) ovf[sp - kStackLds] = pr[k];
+        if (sp < kStackLds) {
+          stk[sp * stride] = pr[k];
+          if (kCullPop) stkT[sp * stride] = entryKey(pt[k]);
+        } else if (sp < kStack4Total) {
+          ovf[sp - kStackLds] = pr[k];
+        }
         sp = sp < kStack4Total ? sp + 1 : sp;
       }
     }
@@ -526,11 +566,12 @@ __device__ __forceinline__ void recordSlabs4(const float4 *__restrict__ recs, in
   }
 }
 
-template <bool kCount, bool kAny = false, bool kQuant = false>
+template <bool kCount, bool kAny = false, bool kQuant = false, bool kCullPop = false>
 __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
                                              const float4 *__restrict__ leaves, int n, f3 o, f3 d,
                                              float tmin, float &tbest, int *stk, int stride,
-                                             TraceCounters &cnt) {
+                                             TraceCounters &cnt, uint16_t *stkT = nullptr) {
+  static_assert(!(kAny && kCullPop), "occlusion walks keep tmax: nothing to cull");
   int hit = -1;
   if (n <= 0) return -1;
   if (n == 1) {
@@ -586,11 +627,9 @@ __device__ __forceinline__ int traceClosest4(const float4 *__restrict__ nodes4,
       }
     }
     // internal children: continue with the nearest, push the others
-    if (!descend4<kAny>(h, tn, ref, stk, stride, ovf, sp, node)) {
-      if (sp == 0) break;
-      sp--;
-      node = sp < kStackLds ? stk[sp * stride] : ovf[sp - kStackLds];
-    }
+    if (!descend4<kAny, kCullPop>(h, tn, ref, stk, stride, ovf, sp, node, stkT) &&
+        !popEntry<kCullPop>(stk, stkT, stride, ovf, sp, node, tbest))
+      break;
   }
   return hit;
 }
@@ -808,10 +847,13 @@ constexpr int kPend = 6;  // queued leaf tests per lane (kPostpone)
 // which every lane with queued leaves tests one, once half of them have
 // some or none can visit (no record left, or no room for four more leaves);
 // otherwise a visit trip.  *visit says which (wave-uniform).
-template <bool kCount, bool kAny, bool kQuant = false, bool kPayload = false, bool kPostpone = false>
+template <bool kCount, bool kAny, bool kQuant = false, bool kPayload = false, bool kPostpone = false,
+          bool kCullPop = false>
 __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ nodes4,
                                           const float4 *__restrict__ leaves, int *stk, int stride, int *ovf,
-                                          TraceCounters &cnt, int *pnd = nullptr, bool *visit = nullptr) {
+                                          TraceCounters &cnt, int *pnd = nullptr, bool *visit = nullptr,
+                                          uint16_t *stkT = nullptr) {
+  static_assert(!(kAny && kCullPop), "occlusion walks keep tmax: nothing to cull");
   if (kPostpone) {
     const bool canVisit = w.node >= 0 && w.npend <= kPend - 4;
     const int nWalk = __popcll(__ballot(true));
@@ -865,14 +907,11 @@ __device__ __forceinline__ bool walk4Step(Walk4 &w, const float4 *__restrict__ n
       if (walk4Leaf<kCount, kAny, kPayload>(w, leaves, li, cnt)) return false;
     }
   }
-  if (!descend4<kAny>(h, tn, ref, stk, stride, ovf, w.sp, w.node)) {
-    if (w.sp == 0) {
-      if (!kPostpone) return false;
-      w.node = -1;
-      return w.npend > 0;
-    }
-    w.sp--;
-    w.node = w.sp < kStackLds ? stk[w.sp * stride] : ovf[w.sp - kStackLds];
+  if (!descend4<kAny, kCullPop>(h, tn, ref, stk, stride, ovf, w.sp, w.node, stkT) &&
+      !popEntry<kCullPop>(stk, stkT, stride, ovf, w.sp, w.node, w.tbest)) {
+    if (!kPostpone) return false;
+    w.node = -1;
+    return w.npend > 0;
   }
   return true;
 }

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
     int nPrims, Scene64 s64, float *__restrict__ tOut, int32_t *__restrict__ idOut,
     int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut) {
   __shared__ int stack[kStackLds * kTraceThreads];
+  __shared__ uint16_t stackT[kStackLds * kTraceThreads];
   const int64_t li = (int64_t)blockIdx.x * kTraceThreads + threadIdx.x;
   if (li >= nRays) return;
   const int64_t i = perm ? (int64_t)perm[li] : li;
@@ -70,10 +71,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace(
   const int leaf =
       kFmt == 3 ? traceClosest8<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads,
                                         cnt)
-      : kFmt == 2 ? traceClosest4<kCount, false, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x,
-                                                      kTraceThreads, cnt)
-      : kFmt == 1
-          ? traceClosest4<kCount>(nodes4, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt)
+      : kFmt == 2 ? traceClosest4<kCount, false, true, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest,
+                                                            stack + threadIdx.x, kTraceThreads, cnt,
+                                                            stackT + threadIdx.x)
+      : kFmt == 1 ? traceClosest4<kCount, false, false, true>(nodes4, leaves, nPrims, o, d, r0.w, tbest,
+                                                             stack + threadIdx.x, kTraceThreads, cnt,
+                                                             stackT + threadIdx.x)
           : traceClosest<kCount>(nodes, leaves, nPrims, o, d, r0.w, tbest, stack + threadIdx.x, kTraceThreads, cnt);
   if (kCount) {
     nodesOut[i] = cnt.nodes;
@@ -111,7 +114,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     int32_t *__restrict__ nodesOut, int32_t *__restrict__ primsOut, uint32_t *__restrict__ work,
     const uint32_t *abortFlag) {
   __shared__ int stack[kStackLds * kTraceThreads];
+  __shared__ uint16_t stackT[kStackLds * kTraceThreads];
   int *stk = stack + threadIdx.x;
+  uint16_t *stkT = stackT + threadIdx.x;
   int ovf[kStack4Total - kStackLds];
   // a lane's ray is loaded straight into its (idle) walk state
   Walk4 w;
@@ -149,7 +154,9 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persist(
     }
     if (!ch.more && __ballot(st != kIdle) == 0) break;
     // one record visit of every walking lane
-    if (st == kActive && !walk4Step<kCount, false, kQuant>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt)) {
+    if (st == kActive &&
+        !walk4Step<kCount, false, kQuant, false, false, true>(w, nodes4, leaves, stk, kTraceThreads, ovf, cnt, nullptr,
+                                                               nullptr, stkT)) {
       st = kIdle;
       if (kCount) {
         nodesOut[ri] = cnt.nodes;
