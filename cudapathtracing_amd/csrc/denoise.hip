@@ -515,9 +515,11 @@ static int64_t fwdSplitFloats(int Cin, int Cout, int B, int H, int W) {
 // One 3x3 'same' convolution.  splitPart: scratch for a split over Cin (at
 // least fwdSplitFloats floats); without it, or with an fp32 output, the
 // convolution is not split.
+// poolOut (bf16 output): also its 2x2 average pooling, (H/2) x (W/2) x ldo.
 static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint16_t *Xlow, int Clow, const uint16_t *w,
                           const float *bias, int Cout, int B, int H, int W, uint16_t *outB, float *outF, int ldo,
-                          int mode, const uint16_t *maskY = nullptr, float *splitPart = nullptr) {
+                          int mode, const uint16_t *maskY = nullptr, float *splitPart = nullptr,
+                          uint16_t *poolOut = nullptr) {
   FwdPlan P = fwdPlan(Cin, Cout, B, H, W);
   if (!splitPart || outF) {
     P.split = 1;
@@ -541,9 +543,10 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
   }
   // one 32-channel chunk per slice: the weights are copied into LDS once
   const int chunks = Cin / 32, perSlice = (chunks + P.split - 1) / P.split;
+  uint16_t *pk = P.split > 1 ? nullptr : poolOut;  // a split convolution pools after k_split_sum
 #define CPT_CONV_FWD(QB, F32, ONE)                                                                             \
   hipLaunchKernelGGL((k_conv_fwd<QB, F32, ONE>), grid, dim3(kFwdThreads), 0, s, X, Cin, Xlow, Clow, w, bs, Cout, H, \
-                     W, oB, oF, ldo, md, mk, P.split, sliceStride, P.tpw)
+                     W, oB, oF, ldo, md, mk, P.split, sliceStride, P.tpw, F32 ? nullptr : pk)
 #define CPT_CONV_FWD_QB(QB)                        \
   if (perSlice == 1) {                             \
     if (oF) CPT_CONV_FWD(QB, true, true);          \
@@ -563,20 +566,24 @@ static void launchConvFwd(hipStream_t s, const uint16_t *X, int Cin, const uint1
     const int64_t n = sliceStride / 8;
     hipLaunchKernelGGL(k_split_sum, dim3(nb256(n)), dim3(256), 0, s, (const float *)splitPart, P.split, sliceStride,
                        bias, Cout, mode & 1, maskY, n, outB);
+    if (poolOut) {
+      const int64_t np = (int64_t)B * (H / 2) * (W / 2) * Cout / 8;
+      hipLaunchKernelGGL(k_pool2, dim3(nb256(np)), dim3(256), 0, s, outB, H, W, Cout, np, poolOut);
+    }
   }
 }
 
 // in: the layer's input, or with low the skip half of the decoder's
 // concatenation [up2(low) | in], read in place.
 static int convFwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *low, int B, float *outF, uint16_t *outB,
-                   int mode) {
+                   int mode, uint16_t *poolOut = nullptr) {
   const LayerDesc &Ld = dn->L[layer];
   hipStream_t s = dn->ctx->stream;
   const uint16_t *w = dn->wbf.as<uint16_t>() + dn->wbOff[layer];
   const float *bias = dn->params.as<float>() + dn->bOff[layer];
   const int H = in.h, W = in.w;
   launchConvFwd(s, in.p(), Ld.cin, low ? low->p() : nullptr, low ? low->c : 0, w, bias, Ld.cout, B, H, W, outB, outF,
-                outF ? kPredChan : Ld.cout, mode, nullptr, dn->part.as<float>());
+                outF ? kPredChan : Ld.cout, mode, nullptr, dn->part.as<float>(), poolOut);
   dn->flopsFwd += 2.0 * B * H * W * (double)Ld.cin * Ld.cout * 9;
   return hipGetLastError() == hipSuccess ? CPT_OK : CPT_ERR_HIP;
 }
@@ -657,19 +664,14 @@ static int convBwd(cpt_denoiser *dn, int layer, const Tensor &in, const Tensor *
 
 static int forward(cpt_denoiser *dn, int B) {
   int rc = 0;
-  hipStream_t s = dn->ctx->stream;
   int layer = 0;
   const Tensor *cur = &dn->x0;
   for (int l = 0; l < kLevels; l++) {
-    if (l > 0) {
-      const Tensor &src = dn->e[l - 1][1];
-      Tensor &dst = dn->pool[l - 1];
-      const int64_t n = (int64_t)B * dst.h * dst.w * dst.c / 8;
-      hipLaunchKernelGGL(k_pool2, dim3(nb256(n)), dim3(256), 0, s, src.p(), src.h, src.w, src.c, n, dst.p());
-      cur = &dst;
-    }
+    if (l > 0) cur = &dn->pool[l - 1];  // pooled by the previous level's second convolution
     if ((rc = convFwd(dn, layer++, *cur, nullptr, B, nullptr, dn->e[l][0].p(), 1))) return rc;
-    if ((rc = convFwd(dn, layer++, dn->e[l][0], nullptr, B, nullptr, dn->e[l][1].p(), 1))) return rc;
+    if ((rc = convFwd(dn, layer++, dn->e[l][0], nullptr, B, nullptr, dn->e[l][1].p(), 1,
+                      l < kLevels - 1 ? dn->pool[l].p() : nullptr)))
+      return rc;
     cur = &dn->e[l][1];
   }
   for (int l = kLevels - 2; l >= 0; l--) {

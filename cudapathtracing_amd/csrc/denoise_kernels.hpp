@@ -131,7 +131,10 @@ __device__ __forceinline__ void ldsBarrier() {
 // first 4 of a wider layer).
 // mode bit 1 = ReLU.  maskY (bf16 output only, ldo channels): the value is
 // kept where maskY > 0 and zeroed elsewhere -- the ReLU gradient mask of the
-// layer whose output maskY is.  Split over Cin (kSplit > 1): slice s takes
+// layer whose output maskY is.  poolOut (bf16 output only, H and W even):
+// also the 2x2 average of the stored bf16 outputs at (y/2, x/2), ldo
+// channels per pixel, summed as ((x00 + x01) + x10) + x11 and scaled by 1/4
+// in FP32 -- the encoder's pooling, without reading the output back.  Split over Cin (kSplit > 1): slice s takes
 // its chunks and writes fp32 partial sums to outF + s * sliceStride.
 //
 // Implicit GEMM with the output transposed, D[co][px] = W[co][tap, ci] .
@@ -184,7 +187,8 @@ __global__ __launch_bounds__(kFwdThreads, 1) void k_conv_fwd(const uint16_t *__r
                                                               int Wd, uint16_t *__restrict__ outBf,
                                                               float *__restrict__ outF, int ldo, int mode,
                                                               const uint16_t *__restrict__ maskY, int kSplit,
-                                                              int64_t sliceStride, int tpw) {
+                                                              int64_t sliceStride, int tpw,
+                                                              uint16_t *__restrict__ poolOut) {
   constexpr int NT = 16 * QB;
   constexpr int kWRows = 9 * NT;
   constexpr int kWBytes = kWRows * 64;
@@ -363,9 +367,49 @@ __global__ __launch_bounds__(kFwdThreads, 1) void k_conv_fwd(const uint16_t *__r
         }
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), rO, o, 0, 0);
       }
-      // the next half overwrites the staging rows: this wave's reads of them
-      // are issued first (LDS operations of a wave complete in order)
-      __builtin_amdgcn_wave_barrier();
+      if (poolOut) {  // uniform
+        // rows y0 + w and y0 + w + 1 are waves w and w + 1: the even waves
+        // pool once every wave's row is staged
+        ldsBarrier();
+        if (!(wave & 1)) {
+          const uint16_t *r0 = (const uint16_t *)stg + wave * (kTileW * kLdsRow), *r1 = r0 + kTileW * kLdsRow;
+          const int H2 = H >> 1, W2 = Wd >> 1;
+          const __amdgpu_buffer_rsrc_t rP =
+              bufRsrc(poolOut + (int64_t)b * H2 * W2 * ldo, (int64_t)H2 * W2 * ldo * 2);
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            const int q = j * 64 + lane, px = q >> 2, part = q & 3;  // 32 pooled pixels x 4 parts
+            const int xo = (x0 >> 1) + px, co = co0 + half * 32 + part * 8;
+            const bool ok = y < H && part * 8 < nCh && 2 * xo < Wd && co < Cout;
+            const uint32_t o = ok ? (uint32_t)((((int64_t)(y >> 1) * W2 + xo) * ldo + co) * 2) : kOob;
+            const uint4 a00 = *(const uint4 *)(r0 + (2 * px) * kLdsRow + part * 8),
+                        a01 = *(const uint4 *)(r0 + (2 * px + 1) * kLdsRow + part * 8),
+                        a10 = *(const uint4 *)(r1 + (2 * px) * kLdsRow + part * 8),
+                        a11 = *(const uint4 *)(r1 + (2 * px + 1) * kLdsRow + part * 8);
+            const uint32_t w00[4] = {a00.x, a00.y, a00.z, a00.w}, w01[4] = {a01.x, a01.y, a01.z, a01.w},
+                           w10[4] = {a10.x, a10.y, a10.z, a10.w}, w11[4] = {a11.x, a11.y, a11.z, a11.w};
+            uint32_t pk[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+              float lo = __uint_as_float(w00[e] << 16), hi = __uint_as_float(w00[e] & 0xffff0000u);
+              lo += __uint_as_float(w01[e] << 16);
+              hi += __uint_as_float(w01[e] & 0xffff0000u);
+              lo += __uint_as_float(w10[e] << 16);
+              hi += __uint_as_float(w10[e] & 0xffff0000u);
+              lo += __uint_as_float(w11[e] << 16);
+              hi += __uint_as_float(w11[e] & 0xffff0000u);
+              pk[e] = (uint32_t)f2bf(lo * 0.25f) | ((uint32_t)f2bf(hi * 0.25f) << 16);
+            }
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_uint4(pk[0], pk[1], pk[2], pk[3])),
+                                                   rP, o, 0, 0);
+          }
+        }
+        ldsBarrier();  // the staged rows are read before the next half overwrites them
+      } else {
+        // the next half overwrites the staging rows: this wave's reads of them
+        // are issued first (LDS operations of a wave complete in order)
+        __builtin_amdgcn_wave_barrier();
+      }
     }
   };
   // copies (L) and stores (E) this wave issues for step j
