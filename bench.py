@@ -288,13 +288,13 @@ def bench_config5(cpt, L, scenes, args):
 
         dn = Denoiser(ctx, chunk, h, w)
         dn.set_params(np.fromfile(WEIGHTS, dtype="<f4"))
-        rgb = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
-        aov = torch.empty((nv, h, w, 7), dtype=torch.float32, device="cuda")
+        # the render writes the denoiser's input layout (radiance + 7 AOVs per pixel)
+        x = torch.empty((nv, h, w, 10), dtype=torch.float32, device="cuda")
+        rgb = x[..., :3]
         den = torch.empty((nv, h, w, 3), dtype=torch.float32, device="cuda")
 
         def denoised_pipeline(seed):
-            ctx.render_views(cams, 4, seed=seed, aov=True, out=rgb, aov_out=aov, **kw)
-            x = torch.cat([rgb, aov], dim=-1).contiguous()
+            ctx.render_views(cams, 4, seed=seed, aov=True, pack=True, out=x, **kw)
             for c0 in range(0, nv, chunk):
                 dn(x[c0:c0 + chunk], out=den[c0:c0 + chunk])
 
@@ -567,7 +567,9 @@ def roofline_hbm(kernel, achieved, launch_ms):
     instructions (the actual limiter of the walks, whose records are served
     largely from L2 and the MALL)."""
     tr = measured_traffic(kernel)
-    issue = profile_value("valu_issue_traversal_kernels.md", f"valu_issue_{kernel}", kernel)
+    issue = profile_value(f"valu_issue_{kernel}.md", f"valu_issue_{kernel}", kernel)
+    if issue is None:
+        issue = profile_value("valu_issue_traversal_kernels.md", f"valu_issue_{kernel}", kernel)
     out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": kernel,
            "avg_launch_ms": round(launch_ms, 4)}

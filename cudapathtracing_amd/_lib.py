@@ -35,6 +35,7 @@ PROFILE_REAL_TIME = 2
 PROFILE_NORMAL = 3
 PROFILE_DEPTH = 4
 RENDER_COUNT_WORK = 1
+RENDER_PACK_AOV = 2
 
 D3 = C.c_double * 3
 

@@ -321,29 +321,35 @@ class Context:
 
 
 def _render_views_impl(self, cams, spp, seed=3317, profile=L.PROFILE_CONFIG, max_depth=8, rr_depth=3,
-                       rr_prob=0.7, sample_base=0, aov=False, paths_per_batch=0, out=None, aov_out=None):
+                       rr_prob=0.7, sample_base=0, aov=False, paths_per_batch=0, out=None, aov_out=None,
+                       pack=False):
     cl = [c if isinstance(c, L.Camera) else camera(c) for c in cams]
     n = len(cl)
     arr = (L.Camera * n)(*cl)
+    if pack and (not aov or aov_out is not None):
+        raise ValueError("pack=True writes the AOVs into out: it needs aov=True and no aov_out")
     p = L.RenderParams(seed, spp, max_depth, rr_depth, profile, rr_prob, sample_base, 1 if aov else 0,
-                       paths_per_batch, 0)
+                       paths_per_batch, L.RENDER_PACK_AOV if pack else 0)
     h, w = cl[0].height, cl[0].width
     dev = out is not None and _is_torch(out) and out.is_cuda
     if dev:
         self._torch_stream()
     if out is None:
-        out = np.empty((n, h, w, 3), np.float32)
-    if aov and aov_out is None:
+        out = np.empty((n, h, w, 10 if pack else 3), np.float32)
+    if aov and not pack and aov_out is None:
         aov_out = np.empty((n, h, w, 7), np.float32)
     st = L.RenderStats()
-    self._check(self._lib.cpt_render_views(self._h, arr, n, C.byref(p), _ptr(out), _ptr(aov_out) if aov else None,
+    self._check(self._lib.cpt_render_views(self._h, arr, n, C.byref(p), _ptr(out),
+                                           _ptr(aov_out) if aov and not pack else None,
                                            C.byref(st), L.CPT_DEVICE_PTRS if dev else 0))
-    return out, (aov_out if aov else None), st
+    return out, (aov_out if aov and not pack else None), st
 
 
 Context.render_views = _render_views_impl
 Context.render_views.__doc__ = """Renders several same-size views in one wavefront pipeline;
-returns (nviews, h, w, 3) radiance, optional AOVs and the stats."""
+returns (nviews, h, w, 3) radiance, optional AOVs and the stats.  pack=True
+(with aov=True) writes (nviews, h, w, 10) into out: radiance then the 7 AOVs
+per pixel, the learned denoiser's input layout."""
 
 
 def camera(desc: L.CameraDesc) -> L.Camera:

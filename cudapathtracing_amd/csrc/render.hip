@@ -55,7 +55,7 @@ struct RenderBufs {
   // path state, double-buffered and indexed by queue slot: k_shade writes
   // the survivors compacted into the other set
   DevBuf ray[2], thr[2], meta[2];
-  DevBuf hit, permA, permB, qVol, keysA, keysB, keysS, shadow, counters, fb, aov, cams, lbuf;
+  DevBuf hit, permA, permB, qVol, keysA, keysB, keysS, shadow, counters, fb, aov, cams, lbuf, apath;
   DevBuf stage;                   // device copy of the outputs for host-pointer calls
   uint32_t *hostCnt = nullptr;    // pinned: per bounce parity, the counters read back
   hipEvent_t ev[2][7] = {};       // per bounce parity: extend, shade, shadow pairs, readback
@@ -144,6 +144,11 @@ struct RenderArgs {
   MediumDev med;
   int64_t base;  // first path of the batch (path g = (sample - sampleBase) * npix * nviews + pixel)
   float4 *lbuf;  // radiance per path of the batch, index g - base
+  // first-vertex AOVs per path of the batch (wantAov), two float4 at index
+  // 2 (g - base): {albedo rgb, normal x} {normal y, normal z, depth, 0};
+  // each path writes its own once, so they are summed per pixel in sample
+  // order like the radiance (deterministic, no atomics)
+  float4 *apath;
   // sort keys written with each next-queue and shadow ray (renderKey)
   const float *rootBox;
   uint32_t *qkeys, *skeys;
@@ -325,6 +330,10 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
   a.lbuf[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (a.apath) {
+    a.apath[2 * p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.apath[2 * p + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 
@@ -502,8 +511,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     RenderArgs a, uint32_t count, const float4 *__restrict__ ray, const float4 *__restrict__ thr,
     const uint4 *__restrict__ meta, const float4 *__restrict__ hit, float4 *__restrict__ rayOut,
     float4 *__restrict__ thrOut, uint4 *__restrict__ metaOut, float4 *__restrict__ shadow, int64_t shadowCap,
-    uint32_t *__restrict__ nextCount, uint32_t *__restrict__ bounceCnt, float *__restrict__ aov,
-    uint32_t *__restrict__ qVol) {
+    uint32_t *__restrict__ nextCount, uint32_t *__restrict__ bounceCnt, uint32_t *__restrict__ qVol) {
   __shared__ uint32_t appendSh[kRenderThreads / 64 + 1];
   const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
   const bool active = k < count;
@@ -530,7 +538,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   f3 T = mkf3(t4.x, t4.y, t4.z);
   f3 o = xyz(o4), d = xyz(d4);
   const int64_t ls = pathSlot(a, mt.x, mt.y);
-  float *pa = kAov ? aov + 7 * (int64_t)mt.x : nullptr;
+  // the path's first-vertex AOV record (written once: the first event sets
+  // kFlagAov or ends the path)
+  auto writeAov = [&](f3 alb3, f3 nrm, float depth) {
+    a.apath[2 * ls] = make_float4(alb3.x, alb3.y, alb3.z, nrm.x);
+    a.apath[2 * ls + 1] = make_float4(nrm.y, nrm.z, depth, 0.f);
+  };
 
   // scattering vertex: a surface with a BSDF, or a real collision in the medium
   bool vtx = false, inMedium = false;
@@ -553,11 +566,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
       const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
       const f3 bg = mkf3(cv.bg[0], cv.bg[1], cv.bg[2]);
       addRadiance(a.lbuf, ls, T * bg);
-      if (kAov && !(flags & kFlagAov)) {
-        atomicAdd(pa + 0, bg.x);
-        atomicAdd(pa + 1, bg.y);
-        atomicAdd(pa + 2, bg.z);
-      }
+      if (kAov && !(flags & kFlagAov)) writeAov(bg, mkf3(0.f, 0.f, 0.f), 0.f);
     } else {
       int mi = __float_as_int(h0.w);
       if ((unsigned)mi >= (unsigned)a.nMats) {
@@ -581,15 +590,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
         }
         // the reference's G-buffer (NORMAL / DEPTH display) keeps its cleared
         // value when the first hit is an emitter (Render.cu:144-149)
-        if (kAov && !(flags & kFlagAov) && a.profile != CPT_PROFILE_NORMAL && a.profile != CPT_PROFILE_DEPTH) {
-          atomicAdd(pa + 0, 1.0f);
-          atomicAdd(pa + 1, 1.0f);
-          atomicAdd(pa + 2, 1.0f);
-          atomicAdd(pa + 3, n.x);
-          atomicAdd(pa + 4, n.y);
-          atomicAdd(pa + 5, n.z);
-          atomicAdd(pa + 6, th * len3(d));
-        }
+        if (kAov && !(flags & kFlagAov) && a.profile != CPT_PROFILE_NORMAL && a.profile != CPT_PROFILE_DEPTH)
+          writeAov(mkf3(1.f, 1.f, 1.f), n, th * len3(d));
       } else {
         vtx = true;
       }
@@ -598,14 +600,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   const f3 alb = inMedium ? a.med.albedo : m.color;
   if (vtx) {
     if (kAov && !(flags & kFlagAov)) {
-      atomicAdd(pa + 0, alb.x);
-      atomicAdd(pa + 1, alb.y);
-      atomicAdd(pa + 2, alb.z);
-      atomicAdd(pa + 3, n.x);
-      atomicAdd(pa + 4, n.y);
-      atomicAdd(pa + 5, n.z);
       const CamDev &cv = a.cams[mt.x / (uint32_t)a.npix];
-      atomicAdd(pa + 6, len3(mkf3((float)cv.lf[0], (float)cv.lf[1], (float)cv.lf[2]) - hp));
+      writeAov(alb, n, len3(mkf3((float)cv.lf[0], (float)cv.lf[1], (float)cv.lf[2]) - hp));
     }
     flags |= kFlagAov;
   }
@@ -1010,9 +1006,43 @@ __global__ void k_gather(const float4 *__restrict__ lbuf, int64_t base, int64_t 
   fb[3 * px + 2] += b;
 }
 
+// The batch's AOV records summed per pixel in sample order into aov
+// (npixAll x 7), as k_gather does the radiance.
+__global__ void k_gather_aov(const float4 *__restrict__ apath, int64_t base, int64_t count, int64_t npixAll,
+                             float *__restrict__ aov) {
+  const int64_t px = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npixAll) return;
+  int64_t s = base / npixAll;
+  if (s * npixAll + px < base) s++;
+  float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int64_t gi = s * npixAll + px; gi < base + count; gi += npixAll) {
+    const float4 p0 = apath[2 * (gi - base)], p1 = apath[2 * (gi - base) + 1];
+    v[0] += p0.x;
+    v[1] += p0.y;
+    v[2] += p0.z;
+    v[3] += p0.w;
+    v[4] += p1.x;
+    v[5] += p1.y;
+    v[6] += p1.z;
+  }
+#pragma unroll
+  for (int c = 0; c < 7; c++) aov[7 * px + c] += v[c];
+}
+
 __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ out, int64_t n, float scale) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = acc[i] * scale;
+}
+
+// CPT_RENDER_PACK_AOV: out[10 px + c] = the means of radiance (c < 3) and of
+// the AOVs (3 <= c < 10); i over npix * 10
+__global__ void k_finalize_pack(const float *__restrict__ rgb, const float *__restrict__ aov, float *__restrict__ out,
+                                int64_t n, float scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t px = i / 10;
+  const int c = (int)(i - px * 10);
+  out[i] = (c < 3 ? rgb[3 * px + c] : aov[7 * px + c - 3]) * scale;
 }
 
 // NORMAL / DEPTH display (Render.cu:239-251) from the first-vertex AOV of a
@@ -1039,7 +1069,8 @@ void renderRelease(cpt_ctx *ctx) {
   RenderBufs *r = ctx->render;
   DevBuf *all[] = {&r->ray[0], &r->ray[1], &r->thr[0],  &r->thr[1],    &r->meta[0], &r->meta[1],
                    &r->hit,    &r->permA,  &r->permB,   &r->qVol,      &r->keysA,   &r->keysB,   &r->keysS,
-                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf,    &r->stage};
+                   &r->shadow, &r->counters, &r->fb,    &r->aov,       &r->cams,    &r->lbuf,    &r->stage,
+                   &r->apath};
   // release path: nothing to report to
   for (DevBuf *b : all)
     if (b->p) (void)hipFree(b->p);
@@ -1100,7 +1131,10 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   // NORMAL / DEPTH render one sample (Render.cu:38) and show its first vertex
   const bool display = prm->profile == CPT_PROFILE_NORMAL || prm->profile == CPT_PROFILE_DEPTH;
   const int spp = display ? 1 : prm->spp;
-  const bool wantAov = aovOut != nullptr || display;
+  const bool packAov = (prm->flags & CPT_RENDER_PACK_AOV) != 0;
+  if (packAov && (aovOut || !prm->want_aov || display))
+    return fail(ctx, CPT_ERR_ARG, "cpt_render", "CPT_RENDER_PACK_AOV needs want_aov, aov == NULL and a radiance profile");
+  const bool wantAov = aovOut != nullptr || display || packAov;
   const int64_t npixView = (int64_t)cam->width * cam->height;
   const int64_t npix = npixView * nviews;  // pixels over all views
   if (npix > (int64_t)1 << 31) return fail(ctx, CPT_ERR_ARG, "cpt_render", "image too large");
@@ -1131,6 +1165,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       (rc = ensure(ctx, R.fb, npix * 12)) || (rc = ensure(ctx, R.aov, npix * 28)))
     return rc;
   if (!(flags & CPT_DEVICE_PTRS) && (rc = ensure(ctx, R.stage, npix * 40))) return rc;
+  if (wantAov && (rc = ensure(ctx, R.apath, cap * 32))) return rc;
   if (!R.hostCnt) CPT_TRY_HIP(ctx, hipHostMalloc((void **)&R.hostCnt, 2 * 32 * sizeof(uint32_t), hipHostMallocDefault));
   for (auto &row : R.ev)
     for (hipEvent_t &e : row)
@@ -1188,6 +1223,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.npix = npixView;
   a.tiled = cap % npix == 0 ? 1 : 0;
   a.lbuf = R.lbuf.as<float4>();
+  a.apath = wantAov ? R.apath.as<float4>() : nullptr;
   a.base = 0;
   a.rootBox = ctx->rootBox.as<float>();
   // bounce-queue keys only where a queue may be reordered
@@ -1300,7 +1336,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   hipLaunchKernelGGL((k_shade<M, V>), dim3(blocksFor(qn)), dim3(kRenderThreads), 0, s, a, qn, rayC, thrC, metaC, \
                      (const float4 *)R.hit.as<float4>(), R.ray[pn].as<float4>(), R.thr[pn].as<float4>(),       \
                      R.meta[pn].as<uint4>(), R.shadow.as<float4>(), shadowCap, nextCount, bounceCnt,           \
-                     R.aov.as<float>(), R.qVol.as<uint32_t>())
+                     R.qVol.as<uint32_t>())
     switch ((a.med.on ? 2 : 0) + (a.wantAov ? 1 : 0)) {
       case 0: CPT_SHADE(false, false); break;
       case 1: CPT_SHADE(false, true); break;
@@ -1454,6 +1490,9 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     }
     hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
                        (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());
+    if (wantAov)
+      hipLaunchKernelGGL(k_gather_aov, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
+                         (const float4 *)R.apath.as<float4>(), base, count, npix, R.aov.as<float>());
     CPT_TRY_HIP(ctx, hipGetLastError());
   }
   // mean over samples (or the NORMAL / DEPTH display of the first vertex)
@@ -1466,6 +1505,9 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   if (display)
     hipLaunchKernelGGL(k_display, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s, (const float *)R.aov.as<float>(),
                        dstRgb, npix, prm->profile == CPT_PROFILE_DEPTH ? 1 : 0);
+  else if (packAov)
+    hipLaunchKernelGGL(k_finalize_pack, dim3(blocksFor(npix * 10)), dim3(kRenderThreads), 0, s,
+                       (const float *)R.fb.as<float>(), (const float *)R.aov.as<float>(), dstRgb, npix * 10, inv);
   else
     hipLaunchKernelGGL(k_finalize, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), dstRgb,
                        npix * 3, inv);
@@ -1476,7 +1518,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   CPT_TRY_HIP(ctx, hipEventRecord(ctx->ev[10], s));
   hipError_t e = hipSuccess;
   if (!(flags & CPT_DEVICE_PTRS)) {
-    e = hipMemcpyAsync(rgb, dstRgb, npix * 12, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(rgb, dstRgb, npix * (packAov ? 40 : 12), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess && aovOut) e = hipMemcpyAsync(aovOut, dstAov, npix * 28, hipMemcpyDeviceToHost, s);
   }
   const hipError_t es = hipStreamSynchronize(s);

@@ -334,6 +334,10 @@ typedef struct cpt_render_params {
 } cpt_render_params;
 
 #define CPT_RENDER_COUNT_WORK 1
+/* params.flags bit (with want_aov): rgb receives h*w*10 floats per view,
+ * per pixel the radiance and then the 7 AOVs (albedo rgb, normal xyz,
+ * depth) -- the learned denoiser's input layout -- and aov must be NULL. */
+#define CPT_RENDER_PACK_AOV 2
 
 typedef struct cpt_render_stats {
   double ms;

@@ -52,9 +52,20 @@ def test_views_with_medium_paths_and_inference(ctx):
                 certified_match(g[v], r, cert, f"view {v} sample {b}", 0.8)
         # denoised 4 spp: the network on the GPU against the oracle's forward pass
         rgb, aov, _ = ctx.render_views(cams, 4, seed=3317, profile=L.PROFILE_CONFIG, aov=True)
+        # the packed output (the denoiser's input layout, CPT_RENDER_PACK_AOV) is
+        # the same numbers, to host memory and into a device tensor
+        packed, none, _ = ctx.render_views(cams, 4, seed=3317, profile=L.PROFILE_CONFIG, aov=True, pack=True)
+        assert none is None and packed.shape == (2, H, W, 10)
+        import torch
+        dev = torch.empty((2, H, W, 10), dtype=torch.float32, device="cuda")
+        ctx.render_views(cams, 4, seed=3317, profile=L.PROFILE_CONFIG, aov=True, pack=True, out=dev)
+        torch.cuda.synchronize()
+        ctx.set_stream(None)
     finally:
         ctx.set_medium(None)
     in10 = np.stack([pack_input(rgb[v], aov[v]) for v in range(2)])
+    assert np.array_equal(packed, in10)
+    assert np.array_equal(dev.cpu().numpy(), in10)
     p = np.fromfile(WEIGHTS, dtype="<f4")
     dn = Denoiser(ctx, 2, H, W, seed=1)
     try:
