@@ -527,13 +527,24 @@ KERNEL_GROUP = {"k_extend_persist": "render", "k_extend": "render", "k_shadow": 
                 "k_trace_persist": "trace", "k_trace": "trace", "k_conv_fwd": "denoise", "k_conv_bwd_w": "denoise"}
 
 
+def _code_only(path):
+    """Source text without comments and blank lines, so that a comment edit
+    keeps the digest."""
+    import re
+    t = open(path, encoding="utf-8").read()
+    t = re.sub(r"/\*.*?\*/", "", t, flags=re.S)
+    t = re.sub(r"//[^\n]*", "", t)
+    return "\n".join(l.rstrip() for l in t.splitlines() if l.strip()).encode()
+
+
 def source_digest(group):
+    """Digest of the code (comments removed) that decides a group's kernels."""
     import hashlib
     h = hashlib.sha256()
     for f in KERNEL_SOURCES[group]:
         h.update(f.encode())
-        h.update(open(os.path.join(ROOT, "cudapathtracing_amd", "csrc", f), "rb").read())
-    h.update(open(os.path.join(ROOT, "include", "cpt.h"), "rb").read())
+        h.update(_code_only(os.path.join(ROOT, "cudapathtracing_amd", "csrc", f)))
+    h.update(_code_only(os.path.join(ROOT, "include", "cpt.h")))
     return h.hexdigest()[:16]
 
 

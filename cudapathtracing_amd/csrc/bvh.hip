@@ -470,9 +470,6 @@ __global__ __launch_bounds__(kRefitThreads) void k_refit_cross(const int32_t *__
   }
 }
 
-// Four-wide record of every internal node (layout in cpt_device.hpp): the
-// node's children with each internal child replaced by its two children,
-// and a free slot filled from the larger-area grandchild.
 // One axis of a quantized record: origin = the lowest child plane, grid
 // step 2^e >= extent / 255, planes rounded outward to the grid and checked in
 // FP32 with the decode the traversal uses (fmaf(q, 2^e, origin)); e grows if
@@ -529,10 +526,11 @@ __device__ __forceinline__ void quantAxis(const float lo[N], const float hi[N], 
   }
 }
 
-__global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__restrict__ nodes4,
-                            float4 *__restrict__ nodes4q) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
+// Four-wide entries of binary node i: its children, each internal child
+// replaced by its own two children; when one child is a leaf, the
+// larger-area internal grandchild is opened into the free slot.  ref[] holds
+// binary node indices (>= 0), ~leaf or kEmpty4.
+__device__ __forceinline__ void collapse4(const float4 *__restrict__ nodes, int64_t i, float b[6][4], int ref[4]) {
   // Every slot index below is a compile-time constant (selects, no running
   // counter), so the slot arrays stay in registers.
   const float4 a0 = nodes[4 * (int64_t)i], a1 = nodes[4 * (int64_t)i + 1], a2 = nodes[4 * (int64_t)i + 2],
@@ -570,8 +568,6 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
   }
   // Slots in order: child A's entries (itself if a leaf, else its two
   // children), then child B's; unused slots are empty.
-  float b[6][4];
-  int ref[4];
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     b[q][0] = in0 ? g0[0][q] : cb0[q];
@@ -617,8 +613,13 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
       ref[3] = __float_as_int(g3v.y);
     }
   }
+}
+
+// Record `slot` in the FP32 (nodes4) and/or quantized (nodes4q) format.
+__device__ __forceinline__ void storeRecord4(float4 *__restrict__ nodes4, float4 *__restrict__ nodes4q, int64_t slot,
+                                             const float b[6][4], const int ref[4]) {
   if (nodes4) {
-    float4 *o = nodes4 + 8 * (int64_t)i;
+    float4 *o = nodes4 + 8 * slot;
     // SoA: minx maxx miny maxy minz maxz refs
     o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
     o[1] = make_float4(b[3][0], b[3][1], b[3][2], b[3][3]);
@@ -638,12 +639,133 @@ __global__ void k_collapse4(int n, const float4 *__restrict__ nodes, float4 *__r
   uint32_t eb[3], ql[3][1], qh[3][1];
 #pragma unroll
   for (int ax = 0; ax < 3; ax++) quantAxis<4>(b[ax], b[ax + 3], valid, org[ax], eb[ax], ql[ax], qh[ax]);
-  float4 *oq = nodes4q + 4 * (int64_t)i;
+  float4 *oq = nodes4q + 4 * slot;
   oq[0] = make_float4(org[0], org[1], org[2], __uint_as_float(eb[0] | (eb[1] << 8) | (eb[2] << 16)));
   oq[1] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
   oq[2] = make_float4(__uint_as_float(ql[0][0]), __uint_as_float(qh[0][0]), __uint_as_float(ql[1][0]),
                       __uint_as_float(qh[1][0]));
   oq[3] = make_float4(__uint_as_float(ql[2][0]), __uint_as_float(qh[2][0]), 0.f, 0.f);
+}
+
+// Four-wide records in level order from the root.  Only the records a walk
+// can reach are written: every internal entry lies two or three binary
+// levels below its record's node, so they are fewer than half of the N-1
+// binary nodes (0.46 of them on random triangle soups), and the internal
+// entries of one record get consecutive slots, so sibling records share
+// 128 B lines.  lev[2L] is the first slot of
+// record level L and lev[2L+1] its width.  Processing level L collapses node
+// recNode[s] for each of the level's slots s, takes slots for the record's
+// internal entries at the end of level L + 1 and writes the record with
+// those slots as its references.  The order of slots within a level follows
+// the order in which workgroups take them; no walk depends on it (walks
+// compare entry distances and slot positions within a record, never record
+// indices).
+constexpr int kRecLevels = 32;  // an entry of level L lies at binary depth >= 2L; internal nodes at <= 63
+constexpr int kRecTop = 5;      // levels 0..4 (at most 4^4 = 256 records each) in one workgroup
+
+template <bool kQuant>
+__device__ __forceinline__ void storeRecordFmt(float4 *__restrict__ out, int64_t slot, const float b[6][4],
+                                               const int ref[4]) {
+  storeRecord4(kQuant ? nullptr : out, kQuant ? out : nullptr, slot, b, ref);
+}
+
+// First slot of this thread's k (0..4) internal entries: workgroup-wide
+// exclusive prefix over the 256 threads, plus one atomic on *width for the
+// whole workgroup (none when width is null: the caller is the only
+// workgroup).  Returns the workgroup's total in *wgTotal.
+__device__ __forceinline__ uint32_t takeSlots(uint32_t k, uint32_t *__restrict__ width, uint32_t next,
+                                              uint32_t *sh, uint32_t *wgTotal) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t pre = 0, total = 0;
+#pragma unroll
+  for (int bit = 0; bit < 3; bit++) {
+    const uint64_t m = __ballot((k >> bit) & 1u);
+    pre += (uint32_t)__popcll(m & below) << bit;
+    total += (uint32_t)__popcll(m) << bit;
+  }
+  __syncthreads();  // sh is free: the previous call's readers are past it
+  if (lane == 0) sh[wave] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < 4; w++) {
+      const uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[4] = width && run ? atomicAdd(width, run) : 0u;
+    sh[5] = run;
+  }
+  __syncthreads();
+  *wgTotal = sh[5];
+  return next + sh[4] + sh[wave] + pre;
+}
+
+// Collapses the record of slot `slot0` (when act), takes its entries' slots
+// and writes it.  Every thread of the workgroup calls it (the slot scan).
+template <bool kQuant>
+__device__ __forceinline__ uint32_t collapseSlot(const float4 *__restrict__ nodes, float4 *__restrict__ out,
+                                                 int32_t *__restrict__ recNode, uint32_t slot0, bool act,
+                                                 int level, uint32_t *__restrict__ width, uint32_t next,
+                                                 uint32_t *sh) {
+  float b[6][4];
+  int ref[4] = {kEmpty4, kEmpty4, kEmpty4, kEmpty4};
+  if (act) collapse4(nodes, recNode[slot0], b, ref);
+  const uint32_t k = (uint32_t)(ref[0] >= 0) + (ref[1] >= 0) + (ref[2] >= 0) + (ref[3] >= 0);
+  uint32_t wgTotal;
+  uint32_t slot = takeSlots(k, width, next, sh, &wgTotal);
+  if (act) {
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+      if (ref[s] >= 0) {
+        if (level + 1 < kRecLevels) {
+          recNode[slot] = ref[s];
+          ref[s] = (int)slot++;
+        } else {
+          ref[s] = kEmpty4;  // beyond the depth bound: never reached
+        }
+      }
+    storeRecordFmt<kQuant>(out, slot0, b, ref);
+  }
+  return wgTotal;
+}
+
+// Levels 0 .. kRecTop-1 (at most 4^L <= 256 records each: one trip of the
+// one workgroup); then lev[] for the level launches.
+template <bool kQuant>
+__global__ __launch_bounds__(256) void k_collapse4_top(const float4 *__restrict__ nodes, float4 *__restrict__ out,
+                                                       int32_t *__restrict__ recNode, uint32_t *__restrict__ lev) {
+  __shared__ uint32_t sh[6];
+  if (threadIdx.x == 0) recNode[0] = 0;  // the root
+  __syncthreads();
+  uint32_t first = 0, width = 1;
+  for (int level = 0; level < kRecTop; level++) {
+    const uint32_t next = first + width;
+    const uint32_t total = collapseSlot<kQuant>(nodes, out, recNode, first + threadIdx.x, threadIdx.x < width,
+                                                level, nullptr, next, sh);
+    __syncthreads();  // recNode of the next level written by this workgroup
+    first = next;
+    width = total;
+  }
+  if (threadIdx.x < 2 * kRecLevels + 2)
+    lev[threadIdx.x] = threadIdx.x == 2 * kRecTop ? first : threadIdx.x == 2 * kRecTop + 1 ? width : 0u;
+}
+
+// One level L >= kRecTop; the grid strides over the level's slots.
+template <bool kQuant>
+__global__ __launch_bounds__(256) void k_collapse4_level(const float4 *__restrict__ nodes, float4 *__restrict__ out,
+                                                         int32_t *__restrict__ recNode, uint32_t *__restrict__ lev,
+                                                         int level) {
+  __shared__ uint32_t sh[6];
+  const uint32_t first = lev[2 * level], width = lev[2 * level + 1];
+  const uint32_t next = first + width;
+  if (blockIdx.x == 0 && threadIdx.x == 0) lev[2 * level + 2] = next;
+  // the bound is uniform per workgroup: every thread runs every trip
+  for (uint32_t base = blockIdx.x * blockDim.x; base < width; base += gridDim.x * blockDim.x) {
+    const uint32_t t = base + threadIdx.x;
+    collapseSlot<kQuant>(nodes, out, recNode, first + t, t < width, level, &lev[2 * level + 3], next, sh);
+  }
 }
 
 // Eight-wide quantized record of every internal node (layout in
@@ -857,10 +979,23 @@ int ensureRecords(cpt_ctx *ctx, bool lbvhOnly) {
   const unsigned grid = (unsigned)((n - 1 + 255) / 256);
   if (fmt == 1 || fmt == 2) {
     DevBuf &dst = fmt == 1 ? ctx->nodes4 : ctx->nodes4q;
-    if ((rc = ensure(ctx, dst, nn * (fmt == 1 ? 128 : 64)))) return rc;
-    hipLaunchKernelGGL(k_collapse4, dim3(grid), dim3(256), 0, ctx->stream, (int)n, bin,
-                       fmt == 1 ? dst.as<float4>() : (float4 *)nullptr,
-                       fmt == 2 ? dst.as<float4>() : (float4 *)nullptr);
+    if ((rc = ensure(ctx, dst, nn * (fmt == 1 ? 128 : 64))) || (rc = ensure(ctx, ctx->recNode, nn * 4)) ||
+        (rc = ensure(ctx, ctx->recLev, (2 * kRecLevels + 2) * 4)))
+      return rc;
+    float4 *out = dst.as<float4>();
+    int32_t *recNode = ctx->recNode.as<int32_t>();
+    uint32_t *lev = ctx->recLev.as<uint32_t>();
+    if (fmt == 2) hipLaunchKernelGGL(k_collapse4_top<true>, dim3(1), dim3(256), 0, ctx->stream, bin, out, recNode, lev);
+    else hipLaunchKernelGGL(k_collapse4_top<false>, dim3(1), dim3(256), 0, ctx->stream, bin, out, recNode, lev);
+    // level L holds at most min(4^L, N-1) records
+    int64_t widest = (int64_t)1 << (2 * kRecTop);
+    for (int L = kRecTop; L < kRecLevels; L++, widest = std::min<int64_t>(widest * 4, (int64_t)nn)) {
+      const unsigned g = (unsigned)std::min<int64_t>((std::min<int64_t>(widest, (int64_t)nn) + 255) / 256, 2048);
+      if (fmt == 2)
+        hipLaunchKernelGGL(k_collapse4_level<true>, dim3(g), dim3(256), 0, ctx->stream, bin, out, recNode, lev, L);
+      else
+        hipLaunchKernelGGL(k_collapse4_level<false>, dim3(g), dim3(256), 0, ctx->stream, bin, out, recNode, lev, L);
+    }
   } else {
     if ((rc = ensure(ctx, ctx->nodes8, nn * 128))) return rc;
     hipLaunchKernelGGL(k_collapse8, dim3(grid), dim3(256), 0, ctx->stream, (int)n,

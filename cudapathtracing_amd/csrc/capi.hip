@@ -61,7 +61,8 @@ void releaseAll(cpt_ctx *ctx) {
                    &ctx->rkA,      &ctx->rkB,     &ctx->rvA,        &ctx->rvB,        &ctx->nodes4q,
                    &ctx->nodes8,   &ctx->plC[0],  &ctx->plC[1],     &ctx->plB[0],     &ctx->plB[1],
                    &ctx->plD[0],   &ctx->plD[1],  &ctx->plNN,       &ctx->plSum,      &ctx->nodesP,
-                   &ctx->plS[0],   &ctx->plS[1],  &ctx->plCtr,      &ctx->plTmp,      &ctx->plSah};
+                   &ctx->plS[0],   &ctx->plS[1],  &ctx->plCtr,      &ctx->plTmp,      &ctx->plSah,
+                   &ctx->recNode,  &ctx->recLev};
   for (DevBuf *b : all) freeBuf(*b);
   renderRelease(ctx);
 }

@@ -335,8 +335,9 @@ __device__ __forceinline__ int traceClosest(const float4 *__restrict__ nodes,
 // leaf, the larger-area internal grandchild is opened into the fourth slot.
 // Every internal entry lies two or three binary levels below i.  SoA
 // {minx[4]} {maxx[4]} {miny[4]} {maxy[4]} {minz[4]} {maxz[4]} {ref[4]}
-// {unused}; ref >= 0 is a binary node index (whose record continues the
-// walk), ~k is leaf k, kEmpty4 an unused slot.
+// {unused}; ref >= 0 is the index of the record that continues the walk
+// (records are stored in level order from the root, record 0, and only
+// those a walk can reach), ~k is leaf k, kEmpty4 an unused slot.
 constexpr int kEmpty4 = (int)0x80000000;
 constexpr int kStack4Total = 96;  // <= 3 pushes per >= 2 binary levels, depth <= 64
 

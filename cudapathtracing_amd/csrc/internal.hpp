@@ -61,8 +61,10 @@ struct cpt_ctx {
   bool built = false;
   cpt::DevBuf codesA, codesB, valsA, valsB, hist, rowTotal;
   cpt::DevBuf keys64, order, nodes, parentInt, parentLeaf, flags, leaves, rootBox;
-  cpt::DevBuf nodes4;   // (N-1) x 128 B four-wide records (grandchildren of each node)
-  cpt::DevBuf nodes4q;  // (N-1) x 64 B quantized four-wide records
+  cpt::DevBuf nodes4;   // <= N-1 four-wide records of 128 B, level order from the root
+  cpt::DevBuf nodes4q;  // the same records quantized, 64 B each
+  cpt::DevBuf recNode;  // binary node of each four-wide record (build scratch)
+  cpt::DevBuf recLev;   // first slot and width of each record level (build scratch)
   cpt::DevBuf nodes8;   // (N-1) x 128 B quantized eight-wide records
   int recValid = 0;     // bit f: wide records of format f are current
   // clustered (PLOC) binary tree over the same leaves, for the wide records
