@@ -886,9 +886,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow(RenderArgs a, const f
 // its lanes busy on the displaced icosphere).  Same results as
 // k_shadow<kAtomic, 1 or 2, kCount>; counting sums the walks and, in
 // work[2], the loop trips in which the wave walked.  countDev: the number of
-// shadow rays; grab: a zeroed counter.
-template <bool kAtomic, bool kQuant, bool kPerm, bool kCount>
-__global__ __launch_bounds__(kRenderThreads) void k_shadow_persist(RenderArgs a, const float4 *__restrict__ shadow,
+// shadow rays; grab: a zeroed counter.  kDefer (a medium is on): an
+// unoccluded ray is only marked (its record's last word set to -1) and
+// k_transmit ratio-tracks it afterwards with full waves, instead of one lane
+// tracking while the rest of its wave waits.
+template <bool kAtomic, bool kQuant, bool kPerm, bool kCount, bool kDefer>
+__global__ __launch_bounds__(kRenderThreads) void k_shadow_persist(RenderArgs a, float4 *__restrict__ shadow,
                                                                     const uint32_t *__restrict__ countDev,
                                                                     const uint32_t *__restrict__ perm, uint32_t *grab,
                                                                     const uint32_t *abortFlag,
@@ -949,7 +952,9 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow_persist(RenderArgs a,
     bool visit = false;
     if (st == kActive &&
         !walk4Step<kCount, true, kQuant, false, true>(w, nodes, leaves, stk, kRenderThreads, ovf, cnt, pnd, &visit)) {
-      if (w.hit < 0) {
+      if (w.hit < 0 && kDefer) {
+        shadow[4 * (int64_t)k + 3].w = __int_as_float(-1);
+      } else if (w.hit < 0) {
         float tr = 1.0f;
         if (med.on) {
           const float4 l = shadow[4 * (int64_t)k + 3];
@@ -984,6 +989,72 @@ __global__ __launch_bounds__(kRenderThreads) void k_shadow_persist(RenderArgs a,
       atomicAdd(work + 2, trips);
     }
   }
+}
+
+// The medium transmittance of the shadow rays that k_shadow_persist<...,
+// kDefer> found unoccluded (last word of the record -1; k_shade writes the
+// light id, >= 0, there), times their contribution, added to the path's
+// radiance slot as k_shadow adds it: the same numbers in the same slots.
+template <bool kAtomic>
+__global__ __launch_bounds__(kRenderThreads) void k_transmit(RenderArgs a, const float4 *__restrict__ shadow,
+                                                              uint32_t ns) {
+  const uint32_t k = blockIdx.x * kRenderThreads + threadIdx.x;
+  if (k >= ns) return;
+  const float4 l = shadow[4 * (int64_t)k + 3];
+  if (__float_as_int(l.w) >= 0) return;
+  const float4 r0 = shadow[4 * (int64_t)k], r1 = shadow[4 * (int64_t)k + 1], c = shadow[4 * (int64_t)k + 2];
+  const float tr = ratioTrack(a.med, a.seed, __float_as_uint(l.x), __float_as_uint(l.y), __float_as_uint(l.z), xyz(r0),
+                              xyz(r1), r0.w, r1.w);
+  const int64_t ls = __float_as_int(c.w);
+  if (ls < 0 || ls >= a.cap) {
+    atomicOr(a.err, kErrSlot);
+  } else if (kAtomic) {
+    float *pl = (float *)(a.lbuf + ls);
+    atomicAdd(pl + 0, c.x * tr);
+    atomicAdd(pl + 1, c.y * tr);
+    atomicAdd(pl + 2, c.z * tr);
+  } else {
+    addRadiance(a.lbuf, ls, mkf3(c.x * tr, c.y * tr, c.z * tr));
+  }
+}
+
+// k_shadow_persist with its template flags chosen at run time.
+template <bool A, bool Q, bool P, bool C>
+static void shadowPersistT(bool defer, dim3 grid, hipStream_t s, const RenderArgs &a, float4 *shadow,
+                           const uint32_t *countDev, const uint32_t *perm, uint32_t *grab, const uint32_t *abortFlag,
+                           unsigned long long *work) {
+  if (defer)
+    hipLaunchKernelGGL((k_shadow_persist<A, Q, P, C, true>), grid, dim3(kRenderThreads), 0, s, a, shadow, countDev,
+                       perm, grab, abortFlag, work);
+  else
+    hipLaunchKernelGGL((k_shadow_persist<A, Q, P, C, false>), grid, dim3(kRenderThreads), 0, s, a, shadow, countDev,
+                       perm, grab, abortFlag, work);
+}
+
+static void launchShadowPersist(bool atomic, bool quant, bool perm, bool count, bool defer, dim3 grid,
+                                hipStream_t s, const RenderArgs &a, float4 *shadow, const uint32_t *countDev,
+                                const uint32_t *permDev, uint32_t *grab, const uint32_t *abortFlag,
+                                unsigned long long *work) {
+#define CPT_SHP(A, Q, P, C) shadowPersistT<A, Q, P, C>(defer, grid, s, a, shadow, countDev, permDev, grab, abortFlag, work)
+  switch ((count ? 8 : 0) + (atomic ? 4 : 0) + (quant ? 2 : 0) + (perm ? 1 : 0)) {
+    case 0: CPT_SHP(false, false, false, false); break;
+    case 1: CPT_SHP(false, false, true, false); break;
+    case 2: CPT_SHP(false, true, false, false); break;
+    case 3: CPT_SHP(false, true, true, false); break;
+    case 4: CPT_SHP(true, false, false, false); break;
+    case 5: CPT_SHP(true, false, true, false); break;
+    case 6: CPT_SHP(true, true, false, false); break;
+    case 7: CPT_SHP(true, true, true, false); break;
+    case 8: CPT_SHP(false, false, false, true); break;
+    case 9: CPT_SHP(false, false, true, true); break;
+    case 10: CPT_SHP(false, true, false, true); break;
+    case 11: CPT_SHP(false, true, true, true); break;
+    case 12: CPT_SHP(true, false, false, true); break;
+    case 13: CPT_SHP(true, false, true, true); break;
+    case 14: CPT_SHP(true, true, false, true); break;
+    default: CPT_SHP(true, true, true, true); break;
+  }
+#undef CPT_SHP
 }
 
 // Adds the batch's path radiance to the frame: thread = pixel (over all
@@ -1264,7 +1335,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     extGrid = (unsigned)ctx->numCU * (unsigned)std::max(1, perCU);
     perCU = 0;
     CPT_TRY_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &perCU, (const void *)k_shadow_persist<false, true, true, false>, kRenderThreads, 0));
+                         &perCU, (const void *)k_shadow_persist<false, true, true, false, false>, kRenderThreads, 0));
     shGrid = (unsigned)ctx->numCU * (unsigned)std::max(1, perCU);
   }
   unsigned long long *work = (unsigned long long *)(cnt + kCntWork);
@@ -1408,12 +1479,12 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       // displaced icosphere it took 31.9 ms a frame unsorted, 40.9 sorted
       // (sort included), the one-lane-per-ray kernel 37.3 sorted, 47.7 not.
       // Below about 2^22 rays a wave's fixed chunks leave the chip part idle
-      // and end in a tail (960x540 at 4 spp: 3.7 against 2.2 ms), and with a
-      // medium each lane ratio-tracks as its walk ends while the rest of the
-      // wave waits (32 views at 4 spp: 119 against 104 ms), so the automatic
-      // choice (a minimum above 1) takes the one-lane kernel there
+      // and end in a tail (960x540 at 4 spp: 3.7 against 2.2 ms).  With a
+      // medium it only marks the unoccluded rays and k_transmit ratio-tracks
+      // them with full waves (tracking in the walk's lanes as each walk ended
+      // left the rest of the wave waiting: slower than the one-lane kernel)
       const int64_t shMin = ctx->persistentShadowMin;
-      const bool shPersist = extPersist && shMin > 0 && (int64_t)ns >= shMin && (shMin == 1 || !a.med.on);
+      const bool shPersist = extPersist && shMin > 0 && (int64_t)ns >= shMin;
       uint32_t *shPerm = nullptr;
       if (!shPersist && ctx->shadowSortMin > 0 && (int64_t)ns >= ctx->shadowSortMin) {
         if ((rc = radixSortPairs(ctx, R.keysS.as<uint32_t>(), R.permA.as<uint32_t>(), R.keysB.as<uint32_t>(),
@@ -1434,35 +1505,22 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
     case 6: CPT_SHD(false, 3, C); break;   \
     default: CPT_SHD(true, 3, C); break;   \
   }
-#define CPT_SHD_P(A, Q, P, C)                                                                                 \
-  hipLaunchKernelGGL((k_shadow_persist<A, Q, P, C>), dim3(std::min<unsigned>(blocksFor(ns), shGrid)),          \
-                     dim3(kRenderThreads), 0, s, a, (const float4 *)R.shadow.as<float4>(), cnt + pi * kCntBlock + 1, \
-                     shPerm, cnt + pi * kCntBlock + 4, abortDev, work + 3)
       if (ns > 0 && shPersist) {
         // the same lane-refilling walk as the extension kernel's
-        const int sel = (countWork ? 8 : 0) + (nl > 1 ? 4 : 0) + (a.wide == 2 ? 2 : 0) + (shPerm ? 1 : 0);
-        switch (sel) {
-          case 0: CPT_SHD_P(false, false, false, false); break;
-          case 1: CPT_SHD_P(false, false, true, false); break;
-          case 2: CPT_SHD_P(false, true, false, false); break;
-          case 3: CPT_SHD_P(false, true, true, false); break;
-          case 4: CPT_SHD_P(true, false, false, false); break;
-          case 5: CPT_SHD_P(true, false, true, false); break;
-          case 6: CPT_SHD_P(true, true, false, false); break;
-          case 7: CPT_SHD_P(true, true, true, false); break;
-          case 8: CPT_SHD_P(false, false, false, true); break;
-          case 9: CPT_SHD_P(false, false, true, true); break;
-          case 10: CPT_SHD_P(false, true, false, true); break;
-          case 11: CPT_SHD_P(false, true, true, true); break;
-          case 12: CPT_SHD_P(true, false, false, true); break;
-          case 13: CPT_SHD_P(true, false, true, true); break;
-          case 14: CPT_SHD_P(true, true, false, true); break;
-          default: CPT_SHD_P(true, true, true, true); break;
+        launchShadowPersist(nl > 1, a.wide == 2, shPerm != nullptr, countWork, a.med.on != 0,
+                            dim3(std::min<unsigned>(blocksFor(ns), shGrid)), s, a, R.shadow.as<float4>(),
+                            cnt + pi * kCntBlock + 1, shPerm, cnt + pi * kCntBlock + 4, abortDev, work + 3);
+        if (a.med.on) {
+          if (nl > 1)
+            hipLaunchKernelGGL(k_transmit<true>, dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,
+                               (const float4 *)R.shadow.as<float4>(), ns);
+          else
+            hipLaunchKernelGGL(k_transmit<false>, dim3(blocksFor(ns)), dim3(kRenderThreads), 0, s, a,
+                               (const float4 *)R.shadow.as<float4>(), ns);
         }
       } else if (ns > 0) {
         if (countWork) CPT_SHD_F(true) else CPT_SHD_F(false)
       }
-#undef CPT_SHD_P
 #undef CPT_SHD_F
 #undef CPT_SHD
       CPT_TRY_HIP(ctx, hipGetLastError());

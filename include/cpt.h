@@ -158,11 +158,11 @@ const char *cpt_version(void);
 #define CPT_OPT_PERSISTENT 3
 /* CPT_OPT_PERSISTENT_SHADOW: cpt_render walks a bounce's shadow rays with
  * the persistent lane-refilling four-wide kernel (with CPT_OPT_PERSISTENT 1),
- * in append order, when there are at least this many of them and the scene
- * has no participating medium (default CPT_PERSISTENT_SHADOW_MIN_DEFAULT);
- * 1: always, also with a medium; 0: never.  Otherwise one lane per shadow
- * ray (ordered by origin cell from CPT_OPT_SHADOW_SORT_MIN rays).  Results
- * are identical. */
+ * in append order, when there are at least this many of them (default
+ * CPT_PERSISTENT_SHADOW_MIN_DEFAULT; 1: always; 0: never); with a
+ * participating medium the unoccluded rays are then ratio-tracked by a
+ * second kernel.  Otherwise one lane per shadow ray (ordered by origin cell
+ * from CPT_OPT_SHADOW_SORT_MIN rays).  Results are identical. */
 #define CPT_OPT_PERSISTENT_SHADOW 8
 #define CPT_PERSISTENT_SHADOW_MIN_DEFAULT (1 << 22)
 /* CPT_OPT_QUANTIZED: 1 (default) makes four-wide walks read 64 B records
