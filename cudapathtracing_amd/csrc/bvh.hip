@@ -349,18 +349,42 @@ __device__ __forceinline__ void storeChildBox(float *__restrict__ nf, int64_t no
 // workgroup-scope counters, every internal node whose leaf range lies inside
 // the chunk.  Each child of a node that crosses a chunk boundary is written
 // to global memory and its parent appended to an escape list.  Pass 2
-// (k_refit_cross) climbs from the escapes through the crossing nodes with
-// agent-scope counters.  Internal node i of a Karras tree is the lowest
-// common ancestor of leaves split(i), split(i)+1, so the crossing nodes are
-// the ancestors of the chunk-boundary splits: at most 64 per boundary
-// (depth <= 64 for unique 64-bit keys), two escapes each.
+// climbs from the escapes through the crossing nodes, in chunks of
+// kRefitMid[0] and then kRefitMid[1] leaves (k_refit_mid: one workgroup per
+// chunk, workgroup-scope counters) and last (k_refit_cross) over the nodes
+// that cross those chunks' boundaries too, with agent-scope counters (each
+// of which writes back and invalidates the L2 on this multi-XCD GPU:
+// climbing every crossing node that way took 0.63 ms of a 10M build).
+// Internal node i of a Karras tree is the lowest common ancestor of leaves
+// split(i), split(i)+1, so the crossing nodes of a chunk size are the
+// ancestors of the chunk-boundary splits: at most 64 per boundary (depth <=
+// 64 for unique 64-bit keys), two escapes each.
 constexpr int kRefitThreads = 256;  // k_refit_cross
 constexpr int kRefitChunk = 1024;   // leaves (and threads) per k_refit_local workgroup
+constexpr int64_t kRefitMid[2] = {int64_t(1) << 16, int64_t(1) << 22};
 
-int64_t refitEscapeBound(int64_t n) {
-  const int64_t chunks = (n + kRefitChunk - 1) / kRefitChunk;
+static int64_t escapeBound(int64_t n, int64_t chunk) {
+  const int64_t chunks = (n + chunk - 1) / chunk;
   return std::min<int64_t>(2 * n, 128 * chunks + 2);
 }
+
+// escape lists of the three chunk sizes, then their counters
+struct EscapeLists {
+  int64_t cap[3], off[3], counters;
+};
+static EscapeLists escapeLists(int64_t n) {
+  EscapeLists e;
+  e.cap[0] = escapeBound(n, kRefitChunk);
+  e.cap[1] = escapeBound(n, kRefitMid[0]);
+  e.cap[2] = escapeBound(n, kRefitMid[1]);
+  e.off[0] = 0;
+  e.off[1] = e.cap[0];
+  e.off[2] = e.cap[0] + e.cap[1];
+  e.counters = e.off[2] + e.cap[2];
+  return e;
+}
+
+int64_t refitEscapeBound(int64_t n) { return escapeLists(n).counters + 64; }
 
 __global__ __launch_bounds__(kRefitChunk) void k_refit_local(
     int n, const uint32_t *__restrict__ order, const float4 *__restrict__ rec, const float *__restrict__ box32,
@@ -369,10 +393,23 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit_local(
     int32_t *__restrict__ esc, uint32_t *__restrict__ escCount, int64_t escCap) {
   __shared__ float cbox[kRefitChunk * 12];  // [node - c0][slot][minx miny minz maxx maxy maxz]
   __shared__ int cnt[kRefitChunk];
+  // leaf range and parent link of internal nodes c0 .. c1: a node whose
+  // range lies inside the chunk has its index there (node i of a Karras tree
+  // is an end of its own range), so the climbs read no global memory
+  __shared__ int2 rng[kRefitChunk];
+  __shared__ int parS[kRefitChunk];
   const int c0 = blockIdx.x * kRefitChunk;
   const int c1 = min(c0 + kRefitChunk, n) - 1;  // last leaf of the chunk
   cnt[threadIdx.x] = 0;
-  if (c0 + (int)threadIdx.x < n - 1) flags[c0 + threadIdx.x] = 0;  // counters of pass 2
+  {
+    const int j = c0 + (int)threadIdx.x;
+    if (j < n - 1) {
+      flags[j] = 0;  // counters of pass 2
+      const float4 rg = nodes[4 * (int64_t)j + 3];  // z, w: first and last leaf
+      rng[threadIdx.x] = make_int2(__float_as_int(rg.z), __float_as_int(rg.w));
+      parS[threadIdx.x] = parentInt[j];
+    }
+  }
   __syncthreads();
   const int i = c0 + (int)threadIdx.x;
   if (i > c1) return;
@@ -410,14 +447,15 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit_local(
   while (true) {
     const int node = par >> 1, slot = par & 1;
     storeChildBox(nf, node, slot, bx);
-    // the node's leaf range and its parent, loaded together (parentInt[0] = -1)
-    const float4 rg = nodes[4 * (int64_t)node + 3];  // z, w: first and last leaf
-    const int parNext = parentInt[node];
-    if (__float_as_int(rg.z) < c0 || __float_as_int(rg.w) > c1) {
+    // a node outside c0 .. c1 or with a leaf outside the chunk crosses its
+    // boundary (parentInt[0] = -1)
+    const bool inside = node >= c0 && node <= c1 && rng[node - c0].x >= c0 && rng[node - c0].y <= c1;
+    if (!inside) {
       const uint32_t e = atomicAdd(escCount, 1u);
       if ((int64_t)e < escCap) esc[e] = node;
       return;
     }
+    const int parNext = parS[node - c0];
     float *s = cbox + (node - c0) * 12;
 #pragma unroll
     for (int k = 0; k < 6; k++) s[slot * 6 + k] = bx[k];
@@ -434,6 +472,55 @@ __global__ __launch_bounds__(kRefitChunk) void k_refit_local(
       return;
     }
     par = parNext;
+  }
+}
+
+// One workgroup per chunk of S leaves: the escapes of the previous pass
+// whose node index lies in the chunk (a node's index is an end of its own
+// leaf range, so each escape has one such chunk).  A node whose range lies
+// inside the chunk is finished by the second of its two arrivals (escapes
+// and climbs of this workgroup only: both children lie inside the chunk
+// too); a node crossing the chunk's boundary goes to the next list.
+__global__ __launch_bounds__(kRefitChunk) void k_refit_mid(int n, int64_t S, const int32_t *__restrict__ escIn,
+                                                           const uint32_t *__restrict__ cntIn, int64_t capIn,
+                                                           int32_t *__restrict__ escOut, uint32_t *__restrict__ cntOut,
+                                                           int64_t capOut, float4 *nodes,
+                                                           const int32_t *__restrict__ parentInt, int32_t *flags,
+                                                           float *__restrict__ rootBox) {
+  const int64_t c0 = (int64_t)blockIdx.x * S, c1 = std::min<int64_t>(c0 + S, n) - 1;
+  const int64_t m = std::min<int64_t>(*cntIn, capIn);
+  float *nf = (float *)nodes;
+  for (int64_t e = threadIdx.x; e < m; e += blockDim.x) {
+    int node = escIn[e];
+    if ((int64_t)node / S != (int64_t)blockIdx.x) continue;
+    while (true) {
+      const float4 rg = nodes[4 * (int64_t)node + 3];  // z, w: first and last leaf
+      if (__float_as_int(rg.z) < c0 || __float_as_int(rg.w) > c1) {
+        const uint32_t k = atomicAdd(cntOut, 1u);
+        if ((int64_t)k < capOut) escOut[k] = node;
+        break;
+      }
+      const int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;  // the other child is not finished: its arrival continues
+      const float4 a = nodes[4 * (int64_t)node + 0];
+      const float4 b = nodes[4 * (int64_t)node + 1];
+      const float4 c = nodes[4 * (int64_t)node + 2];
+      float bx[6];
+      bx[0] = fminf(a.x, b.x);
+      bx[3] = fmaxf(a.y, b.y);
+      bx[1] = fminf(a.z, b.z);
+      bx[4] = fmaxf(a.w, b.w);
+      bx[2] = fminf(c.x, c.z);
+      bx[5] = fmaxf(c.y, c.w);
+      if (node == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) rootBox[k] = bx[k];
+        break;
+      }
+      const int par = parentInt[node];
+      node = par >> 1;
+      storeChildBox(nf, node, par & 1, bx);
+    }
   }
 }
 
@@ -1009,23 +1096,29 @@ int ensureRecords(cpt_ctx *ctx, bool lbvhOnly) {
 // Node boxes (two refit passes), sorted leaf records and the wide records of
 // the current traversal format.
 static int launchRefit(cpt_ctx *ctx, int64_t n) {
-  const int64_t cap = refitEscapeBound(n);
-  int rc = ensure(ctx, ctx->esc, (size_t)(cap + 64) * 4);
+  const EscapeLists L = escapeLists(n);
+  int rc = ensure(ctx, ctx->esc, (size_t)refitEscapeBound(n) * 4);
   if (rc) return rc;
   int32_t *esc = ctx->esc.as<int32_t>();
-  uint32_t *escCount = (uint32_t *)(esc + cap);
-  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)escCount, (int64_t)1);
+  uint32_t *escCount = (uint32_t *)(esc + L.counters);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)escCount, (int64_t)3);
   const unsigned chunks = (unsigned)((n + kRefitChunk - 1) / kRefitChunk);
   hipLaunchKernelGGL(k_refit_local, dim3(chunks), dim3(kRefitChunk), 0, ctx->stream, (int)n,
                      ctx->order.as<uint32_t>(), ctx->rec.as<float4>(), ctx->box32.as<float>(),
                      ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->parentLeaf.as<int32_t>(),
-                     ctx->flags.as<int32_t>(), ctx->leaves.as<float4>(), ctx->rootBox.as<float>(), esc, escCount,
-                     cap);
+                     ctx->flags.as<int32_t>(), ctx->leaves.as<float4>(), ctx->rootBox.as<float>(), esc + L.off[0],
+                     escCount, L.cap[0]);
   if (n > 1) {
-    hipLaunchKernelGGL(k_refit_cross, dim3((unsigned)((cap + kRefitThreads - 1) / kRefitThreads)),
-                       dim3(kRefitThreads), 0, ctx->stream, (const int32_t *)esc, (const uint32_t *)escCount, cap,
-                       ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(), ctx->flags.as<int32_t>(),
-                       ctx->rootBox.as<float>());
+    for (int k = 0; k < 2; k++)
+      hipLaunchKernelGGL(k_refit_mid, dim3((unsigned)((n + kRefitMid[k] - 1) / kRefitMid[k])), dim3(kRefitChunk), 0,
+                         ctx->stream, (int)n, kRefitMid[k], (const int32_t *)(esc + L.off[k]),
+                         (const uint32_t *)(escCount + k), L.cap[k], esc + L.off[k + 1], escCount + k + 1,
+                         L.cap[k + 1], ctx->nodes.as<float4>(), ctx->parentInt.as<int32_t>(),
+                         ctx->flags.as<int32_t>(), ctx->rootBox.as<float>());
+    hipLaunchKernelGGL(k_refit_cross, dim3((unsigned)((L.cap[2] + kRefitThreads - 1) / kRefitThreads)),
+                       dim3(kRefitThreads), 0, ctx->stream, (const int32_t *)(esc + L.off[2]),
+                       (const uint32_t *)(escCount + 2), L.cap[2], ctx->nodes.as<float4>(),
+                       ctx->parentInt.as<int32_t>(), ctx->flags.as<int32_t>(), ctx->rootBox.as<float>());
   }
   CPT_TRY_HIP(ctx, hipGetLastError());
   ctx->recValid = 0;
@@ -1033,14 +1126,14 @@ static int launchRefit(cpt_ctx *ctx, int64_t n) {
   return ensureRecords(ctx, true);
 }
 
-// The escape list of the last refit held every crossing node: its counter
-// (which also counts entries past the list's bound) is at most the bound.
+// The escape lists of the last refit held every crossing node: each counter
+// (which also counts entries past its list's bound) is at most the bound.
 // Called after the refit has finished (the stream is idle there).
 static int checkRefitEscapes(cpt_ctx *ctx, int64_t n, const char *what) {
-  const int64_t cap = refitEscapeBound(n);
-  uint32_t cnt = 0;
-  CPT_TRY_HIP(ctx, hipMemcpy(&cnt, ctx->esc.as<int32_t>() + cap, 4, hipMemcpyDeviceToHost));
-  if ((int64_t)cnt > cap) {
+  const EscapeLists L = escapeLists(n);
+  uint32_t cnt[3] = {0, 0, 0};
+  CPT_TRY_HIP(ctx, hipMemcpy(cnt, ctx->esc.as<int32_t>() + L.counters, 12, hipMemcpyDeviceToHost));
+  if ((int64_t)cnt[0] > L.cap[0] || (int64_t)cnt[1] > L.cap[1] || (int64_t)cnt[2] > L.cap[2]) {
     ctx->built = false;
     return fail(ctx, CPT_ERR_STATE, what, "refit escape list overflow (node boxes incomplete)");
   }

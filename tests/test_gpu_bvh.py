@@ -43,6 +43,9 @@ CASES = {
     "tile_edge": lambda: soup(4097, seed=3),
     "duplicates": lambda: soup(20000, seed=4, dup=2),
     "soup_1m": lambda: soup(1_000_000, seed=1),
+    # above 2^22 leaves: node boxes finished by all three refit stages
+    # (1024-leaf chunks, 64K- and 4M-leaf chunks, the rest across them)
+    "soup_5m": lambda: soup(5_000_000, seed=5),
 }
 
 
