@@ -89,7 +89,7 @@ struct cpt_ctx {
   bool quantized = true;  // four-wide walks read the 64 B quantized records
   bool persistent = true;  // four-wide traces as persistent, lane-refilling kernels
   int numCU = 256;
-  int64_t raySortMin = (int64_t)1 << 20;  // trace batches this large are reordered
+  int64_t raySortMin = 0;  // trace batches this large are reordered (0: never)
   // path-tracer bounce queues this long are reordered (0: never, the
   // default: the walks are faster on the unsorted queue, whose append order
   // keeps neighbouring pixels' paths together)

@@ -326,6 +326,6 @@ def test_ray_ordering_keeps_results(ctx):
         n1, p1 = ctx.trace_count(rays)
         assert np.array_equal(i0, i1) and np.array_equal(t0, t1), smin
         assert np.array_equal(n0, n1) and np.array_equal(p0, p1), smin
-    ctx.set_ray_sort_min(1 << 20)
+    ctx.set_ray_sort_min(0)
     ctx.set_persistent(True)
     assert np.mean(i0 >= 0) > 0.1
