@@ -12,8 +12,8 @@
 //   k_volume  the homogeneous volume object walk (Render.cu:64-135)
 //   k_shadow  closest hit of each shadow ray; adds its contribution when the
 //             light itself is the closest hit (Render.cu:180-185)
-// Radiance is summed per pixel with FP32 atomics and divided by spp at the
-// end (Render.cu:220-222).  Random numbers: the counter-based stream of
+// Radiance is kept per path and summed per pixel in sample order at the end
+// of each batch, then divided by spp (Render.cu:220-222).  Random numbers: the counter-based stream of
 // cpt_device.hpp, consumed in the reference's order per sample (jitter x,y;
 // per vertex: 2 per light for NEE, 1 for roulette, 3 for the BSDF).
 #include <hip/hip_runtime.h>
@@ -1214,12 +1214,16 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   RenderBufs &R = *ctx->render;
   const int64_t total = npix * spp;
   const int nl = std::max(1, (int)ctx->nLights);
-  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 26);
+  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 27);
   cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 27) / nl));
   cap = std::min(cap, total);
   // whole samples per batch where a batch holds at least one (k_gen then
-  // orders each sample's pixels in tiles)
-  if (cap >= npix) cap -= cap % npix;
+  // orders each sample's pixels in tiles), spread evenly over the fewest
+  // batches so that no batch is a small remainder
+  if (cap >= npix) {
+    const int64_t nb = (spp + cap / npix - 1) / (cap / npix);
+    cap = (spp + nb - 1) / nb * npix;
+  }
   const int64_t shadowCap = cap * nl;
   int rc;
   for (int b = 0; b < 2; b++)
