@@ -1214,8 +1214,8 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   RenderBufs &R = *ctx->render;
   const int64_t total = npix * spp;
   const int nl = std::max(1, (int)ctx->nLights);
-  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 27);
-  cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 27) / nl));
+  int64_t cap = prm->paths_per_batch > 0 ? prm->paths_per_batch : ((int64_t)1 << 28);
+  cap = std::min<int64_t>(cap, std::max<int64_t>(65536, ((int64_t)1 << 28) / nl));
   cap = std::min(cap, total);
   // whole samples per batch where a batch holds at least one (k_gen then
   // orders each sample's pixels in tiles), spread evenly over the fewest

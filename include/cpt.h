@@ -332,7 +332,9 @@ typedef struct cpt_render_params {
   double rr_prob;       /* survival probability, 0.7 in Render.cu:202 */
   int32_t sample_base;  /* index of the first sample (progressive)    */
   int32_t want_aov;     /* fill albedo/normal/depth outputs           */
-  int32_t paths_per_batch; /* 0 = automatic                          */
+  int32_t paths_per_batch; /* 0 = automatic: up to 2^28 / lights, whole
+                            * samples spread evenly over the fewest batches
+                            * (about 290 B of device memory per path)    */
   int32_t flags;           /* CPT_RENDER_COUNT_WORK: count extension-  */
                            /* kernel node visits and primitive tests   */
 } cpt_render_params;
