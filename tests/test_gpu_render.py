@@ -150,11 +150,12 @@ def test_occlusion_kernels_agree_bit_for_bit(ctx, medium):
     assert np.array_equal(a, b) and np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("ppb", [37 * 29, 500])
+@pytest.mark.parametrize("ppb", [37 * 29, 2 * 37 * 29 + 5, 500])
 def test_odd_size_multi_batch(ctx, ppb):
     # 37 x 29 (not a multiple of the 8 x 8 path tiles) at 3 spp: one whole
-    # sample per batch in tile order (ppb = w*h), or batches that split a
-    # sample with untiled paths (ppb = 500)
+    # sample per batch in tile order (ppb = w*h), whole samples spread over
+    # the fewest batches (room for 2 samples: batches of 2 and 1), or batches
+    # that split a sample with untiled paths (ppb = 500)
     print(f"[gpu] render 37x29 x 3 spp, {ppb} paths per batch", flush=True)
     s = scenes.cornell_icosphere()
     # height = (int)(37 / 1.27) = 29 (Camera.cu:12)
