@@ -1100,6 +1100,33 @@ __global__ void k_gather_aov(const float4 *__restrict__ apath, int64_t base, int
   for (int c = 0; c < 7; c++) aov[7 * px + c] += v[c];
 }
 
+// CPT_RENDER_PACK_AOV with every sample in one batch (paths 0 .. count-1):
+// per pixel, the radiance and the AOVs of its samples summed in sample order
+// and scaled, written in the packed layout -- k_gather, k_gather_aov and
+// k_finalize_pack in one pass, with the same sums.
+__global__ void k_gather_pack(const float4 *__restrict__ lbuf, const float4 *__restrict__ apath, int64_t count,
+                              int64_t npixAll, float *__restrict__ out, float scale) {
+  const int64_t px = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npixAll) return;
+  float v[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int64_t gi = px; gi < count; gi += npixAll) {
+    const float4 l = lbuf[gi], p0 = apath[2 * gi], p1 = apath[2 * gi + 1];
+    v[0] += l.x;
+    v[1] += l.y;
+    v[2] += l.z;
+    v[3] += p0.x;
+    v[4] += p0.y;
+    v[5] += p0.z;
+    v[6] += p0.w;
+    v[7] += p1.x;
+    v[8] += p1.y;
+    v[9] += p1.z;
+  }
+  float2 *o = (float2 *)(out + 10 * px);  // 40 B records: 8 B aligned
+#pragma unroll
+  for (int c = 0; c < 5; c++) o[c] = make_float2(v[2 * c] * scale, v[2 * c + 1] * scale);
+}
+
 __global__ void k_finalize(const float *__restrict__ acc, float *__restrict__ out, int64_t n, float scale) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = acc[i] * scale;
@@ -1321,9 +1348,21 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.cap = cap;
   a.nMats = ctx->nMats + ctx->nLights + 1;
 
-  hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), npix * 3, 0.f);
-  if (wantAov)
-    hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(), npix * 7, 0.f);
+  // one batch with the packed output: the frame is written straight from the
+  // per-path sums (no accumulation buffers)
+  const bool fusedPack = packAov && cap >= total;
+  const float inv = 1.0f / (float)spp;
+  float *dstRgb = rgb, *dstAov = aovOut;
+  if (!(flags & CPT_DEVICE_PTRS)) {
+    dstRgb = R.stage.as<float>();
+    dstAov = R.stage.as<float>() + npix * 3;
+  }
+  if (!fusedPack) {
+    hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 3)), dim3(kRenderThreads), 0, s, R.fb.as<float>(), npix * 3, 0.f);
+    if (wantAov)
+      hipLaunchKernelGGL(k_fill, dim3(blocksFor(npix * 7)), dim3(kRenderThreads), 0, s, R.aov.as<float>(), npix * 7,
+                         0.f);
+  }
   CPT_TRY_HIP(ctx, hipGetLastError());
 
   double rays = 0, extMs = 0, shMs = 0, sdMs = 0, extRays = 0;
@@ -1550,21 +1589,22 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
       CPT_TRY_HIP(ctx, hipEventElapsedTime(&e3, R.ev[last & 1][4], R.ev[last & 1][5]));
       sdMs += e3;
     }
-    hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
-                       (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());
-    if (wantAov)
-      hipLaunchKernelGGL(k_gather_aov, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
-                         (const float4 *)R.apath.as<float4>(), base, count, npix, R.aov.as<float>());
+    if (fusedPack) {
+      hipLaunchKernelGGL(k_gather_pack, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
+                         (const float4 *)R.lbuf.as<float4>(), (const float4 *)R.apath.as<float4>(), count, npix,
+                         dstRgb, inv);
+    } else {
+      hipLaunchKernelGGL(k_gather, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
+                         (const float4 *)R.lbuf.as<float4>(), base, count, npix, R.fb.as<float>());
+      if (wantAov)
+        hipLaunchKernelGGL(k_gather_aov, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s,
+                           (const float4 *)R.apath.as<float4>(), base, count, npix, R.aov.as<float>());
+    }
     CPT_TRY_HIP(ctx, hipGetLastError());
   }
   // mean over samples (or the NORMAL / DEPTH display of the first vertex)
-  const float inv = 1.0f / (float)spp;
-  float *dstRgb = rgb, *dstAov = aovOut;
-  if (!(flags & CPT_DEVICE_PTRS)) {
-    dstRgb = R.stage.as<float>();
-    dstAov = R.stage.as<float>() + npix * 3;
-  }
-  if (display)
+  if (fusedPack) {
+  } else if (display)
     hipLaunchKernelGGL(k_display, dim3(blocksFor(npix)), dim3(kRenderThreads), 0, s, (const float *)R.aov.as<float>(),
                        dstRgb, npix, prm->profile == CPT_PROFILE_DEPTH ? 1 : 0);
   else if (packAov)
