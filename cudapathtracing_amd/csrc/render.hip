@@ -149,6 +149,7 @@ struct RenderArgs {
   // each path writes its own once, so they are summed per pixel in sample
   // order like the radiance (deterministic, no atomics)
   float4 *apath;
+  int firstBounce;  // this shading launch is bounce 0 (every path of the batch)
   // sort keys written with each next-queue and shadow ray (renderKey)
   const float *rootBox;
   uint32_t *qkeys, *skeys;
@@ -330,10 +331,8 @@ __global__ __launch_bounds__(kRenderThreads) void k_gen(RenderArgs a, int64_t ba
   thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   meta[p] = make_uint4(pixel, sample, rng.cnt, 0u);
   a.lbuf[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (a.apath) {
-    a.apath[2 * p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    a.apath[2 * p + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  // the AOV record is written by the path's first shading (zeros where that
+  // vertex sets none), so it is not cleared here
 }
 
 
@@ -539,10 +538,12 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
   f3 o = xyz(o4), d = xyz(d4);
   const int64_t ls = pathSlot(a, mt.x, mt.y);
   // the path's first-vertex AOV record (written once: the first event sets
-  // kFlagAov or ends the path)
+  // kFlagAov or ends the path; bounce 0 writes zeros where it sets none)
+  bool aovWritten = false;
   auto writeAov = [&](f3 alb3, f3 nrm, float depth) {
     a.apath[2 * ls] = make_float4(alb3.x, alb3.y, alb3.z, nrm.x);
     a.apath[2 * ls + 1] = make_float4(nrm.y, nrm.z, depth, 0.f);
+    aovWritten = true;
   };
 
   // scattering vertex: a surface with a BSDF, or a real collision in the medium
@@ -605,6 +606,7 @@ __global__ __launch_bounds__(kRenderThreads) void k_shade(
     }
     flags |= kFlagAov;
   }
+  if (kAov && active && a.firstBounce && !aovWritten) writeAov(mkf3(0.f, 0.f, 0.f), mkf3(0.f, 0.f, 0.f), 0.f);
   const f3 ud = unit3(d);
   const f3 V = -ud;
   // NEE to every light (Render.cu:161-187): opaque-type surfaces (Render.cu:166)
@@ -1326,6 +1328,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   a.tiled = cap % npix == 0 ? 1 : 0;
   a.lbuf = R.lbuf.as<float4>();
   a.apath = wantAov ? R.apath.as<float4>() : nullptr;
+  a.firstBounce = 0;
   a.base = 0;
   a.rootBox = ctx->rootBox.as<float>();
   // bounce-queue keys only where a queue may be reordered
@@ -1442,6 +1445,7 @@ extern "C" int cpt_render_views(cpt_ctx *ctx, const cpt_camera *cams, int32_t nv
   // read-back of the counters that tell the host the next queue's length
   auto launchShade = [&](int j, uint32_t qn) -> int {
     const int pj = j & 1, pn = pj ^ 1;
+    a.firstBounce = j == 0 ? 1 : 0;
     const float4 *rayC = R.ray[pj].as<float4>(), *thrC = R.thr[pj].as<float4>();
     const uint4 *metaC = R.meta[pj].as<uint4>();
     uint32_t *nextCount = cnt + pn * kCntBlock, *bounceCnt = cnt + pj * kCntBlock;
