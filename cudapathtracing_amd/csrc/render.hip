@@ -3,19 +3,24 @@
 // Replaces the megakernel render<<<>>> (CudaPathTracing/Render.cu:16-261):
 // one path per (pixel, sample), state in HBM (SoA), index queues between
 //   k_gen     camera ray (Camera::getRandomSampleRay, Camera.cuh:37-43) in FP64
-//   k_extend  closest hit (traverseIterative) for the active queue
+//   k_extend_persist (k_extend)  closest hit (traverseIterative) of the
+//             active queue, lanes refilled as their walks end
 //   k_shade   miss/sky, emitter, heterogeneous medium (delta tracking), NEE
 //             to every quad light (Render.cu:161-187), Russian roulette
 //             (Render.cu:189-204), BSDF sampling (Render.cu:206-218); pushes
 //             survivors to the next queue, shadow rays to the shadow queue and
 //             volume-object hits to the volume queue
 //   k_volume  the homogeneous volume object walk (Render.cu:64-135)
-//   k_shadow  closest hit of each shadow ray; adds its contribution when the
-//             light itself is the closest hit (Render.cu:180-185)
+//   k_shadow_persist (k_shadow)  occlusion query of each shadow ray up to
+//             its light (the reference's "the light is the closest hit",
+//             Render.cu:180-185); adds the contribution of an unoccluded ray,
+//             or with a medium marks it for k_transmit (ratio tracking)
+//   k_gather, k_gather_aov, k_gather_pack  per-pixel sums of the batch
 // Radiance is kept per path and summed per pixel in sample order at the end
-// of each batch, then divided by spp (Render.cu:220-222).  Random numbers: the counter-based stream of
-// cpt_device.hpp, consumed in the reference's order per sample (jitter x,y;
-// per vertex: 2 per light for NEE, 1 for roulette, 3 for the BSDF).
+// of each batch, then divided by spp (Render.cu:220-222).  Random numbers:
+// the counter-based stream of cpt_device.hpp, consumed in the reference's
+// order per sample (jitter x,y; per vertex: 2 per light for NEE, 1 for
+// roulette, 3 for the BSDF).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
